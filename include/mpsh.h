@@ -1,0 +1,143 @@
+/*
+ * mpsh.h -- C ABI of the MI355X block-sparse U(1) contraction engine.
+ *
+ * The entry points replace the Julia hot path of amiragha/mps:
+ *   SymTensors.contract / permutelegs / fuselegs / splitleg / SymMatrix,
+ *   TensorNetAlgs._applymps2site / _mpsupdateleft / _mpsupdateright and the
+ *   block-wise vector operations KrylovKit's eigsolve calls on SymTensors.
+ * Plain pointers and sizes only.  Device pointers are HIP device addresses;
+ * `stream` is a hipStream_t (NULL = default stream).  Tensor data is the
+ * concatenation of all blocks of a structure in column-major sector order,
+ * each block column-major (reference src/symtensor.jl:12-30, src/sector.jl:30-36).
+ *
+ * Errors: every call returns an mpsh_status; mpsh_last_error() gives a
+ * thread-local message.  Plans and structures are immutable once created;
+ * one plan may be executed concurrently only on distinct workspaces.
+ */
+#ifndef MPSH_H
+#define MPSH_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef enum {
+  MPSH_OK = 0,
+  MPSH_ERR_INVALID = 1,         /* malformed arguments (index sets, ranks, NULLs)            */
+  MPSH_ERR_SPACE_MISMATCH = 2,  /* "not contractible!"  (src/contract.jl:237-238)              */
+  MPSH_ERR_SECTOR_MISMATCH = 3, /* block table inconsistent with the space (src/symtensor.jl:20-26) */
+  MPSH_ERR_NOT_SIMILAR = 4,     /* axpy!/dot on non-similar tensors (src/symtensor.jl:291,316) */
+  MPSH_ERR_NONFINITE = 5,
+  MPSH_ERR_DEVICE = 6,          /* a HIP runtime call failed                                  */
+  MPSH_ERR_OOM = 7,
+  MPSH_ERR_UNSUPPORTED = 8      /* e.g. rank-1 outputs of contract (src/contract.jl:24-42)   */
+} mpsh_status;
+
+typedef enum {
+  MPSH_F64 = 0,  /* double storage, FP64 MFMA                                    */
+  MPSH_F32 = 1,  /* float storage, FP32 MFMA                                     */
+  MPSH_BF16 = 2  /* float storage, BF16 MFMA operands, FP32 accumulation         */
+} mpsh_dtype;
+
+const char* mpsh_last_error(void);
+const char* mpsh_version(void);
+
+/* ------------------------------------------------------------------------ */
+/* Structures: a tensor charge plus U(1) legs, with every allowed sector.    */
+/* Replaces SymTensor's sector bookkeeping: VectorSpace (src/vspace.jl:5-18), */
+/* _allsectorsandsizes (src/sector.jl:69-90), the SortedDict key table        */
+/* (src/sorteddict.jl:1-99).                                                  */
+/* ------------------------------------------------------------------------ */
+typedef struct mpsh_struct_s* mpsh_struct;
+
+/* legs: isdual[rank]; ncharges[rank]; charges/dims concatenated leg by leg.  */
+mpsh_status mpsh_struct_create(int64_t charge, int32_t rank, const uint8_t* isdual, const int32_t* ncharges,
+                               const int64_t* charges, const int64_t* dims, mpsh_struct* out);
+void mpsh_struct_destroy(mpsh_struct s);
+int32_t mpsh_struct_rank(mpsh_struct s);
+int64_t mpsh_struct_charge(mpsh_struct s);
+int64_t mpsh_struct_nblocks(mpsh_struct s);
+int64_t mpsh_struct_numel(mpsh_struct s);
+int32_t mpsh_struct_leg_ncharges(mpsh_struct s, int32_t leg); /* leg is 0-based */
+mpsh_status mpsh_struct_leg(mpsh_struct s, int32_t leg, uint8_t* isdual, int64_t* charges, int64_t* dims);
+/* sectors[nblocks*rank], sizes[nblocks*rank], offsets[nblocks+1] (elements). */
+mpsh_status mpsh_struct_blocks(mpsh_struct s, int64_t* sectors, int64_t* sizes, int64_t* offsets);
+/* block index of a sector, -1 when absent */
+int64_t mpsh_struct_find(mpsh_struct s, const int64_t* sector);
+
+/* ------------------------------------------------------------------------ */
+/* Re-legging: permutelegs (src/symreleg.jl:34-51), fuselegs (:79-146),      */
+/* splitleg (:161-214), SymMatrix (src/contract.jl:112-171).  Leg numbers and */
+/* permutations are 1-based as in the reference.                              */
+/* ------------------------------------------------------------------------ */
+typedef struct mpsh_releg_s* mpsh_releg;
+
+mpsh_status mpsh_permutelegs_plan(mpsh_struct A, const int32_t* perm, mpsh_releg* plan, mpsh_struct* out);
+mpsh_status mpsh_fuselegs_plan(mpsh_struct A, int32_t l, int32_t n, int32_t dual, mpsh_releg* plan,
+                               mpsh_struct* out);
+mpsh_status mpsh_splitleg_plan(mpsh_struct A, int32_t l, int32_t nspaces, const uint8_t* isdual,
+                               const int32_t* ncharges, const int64_t* charges, const int64_t* dims,
+                               mpsh_releg* plan, mpsh_struct* out);
+mpsh_status mpsh_symmatrix_plan(mpsh_struct A, int32_t nrows, const int32_t* rows, int32_t ncols,
+                                const int32_t* cols, mpsh_releg* plan, mpsh_struct* out);
+mpsh_status mpsh_releg_execute(mpsh_releg plan, mpsh_dtype dt, const void* src, void* dst, void* stream);
+void mpsh_releg_destroy(mpsh_releg plan);
+
+/* ------------------------------------------------------------------------ */
+/* contract(A, idxA, B, idxB)  (src/contract.jl:11-47): negative labels are   */
+/* contracted (ordered by |label|), positive labels are output slots 1..K.    */
+/* Plain path: gather-fuse A and B, one grouped GEMM over all fused charges,  */
+/* scatter-split-permute C -- the reference algorithm on the device.          */
+/* ------------------------------------------------------------------------ */
+typedef struct mpsh_contract_s* mpsh_contract;
+
+mpsh_status mpsh_contract_plan(mpsh_struct A, const int32_t* idxA, mpsh_struct B, const int32_t* idxB,
+                               mpsh_contract* plan, mpsh_struct* C);
+size_t mpsh_contract_workspace(mpsh_contract plan, mpsh_dtype dt); /* bytes */
+int64_t mpsh_contract_flops(mpsh_contract plan);                   /* sum 2*M*N*K of matched blocks */
+mpsh_status mpsh_contract_execute(mpsh_contract plan, mpsh_dtype dt, const void* A, const void* B, void* C,
+                                  void* workspace, void* stream);
+void mpsh_contract_destroy(mpsh_contract plan);
+
+/* ------------------------------------------------------------------------ */
+/* Two-site effective Hamiltonian  _applymps2site(v, envL, envR, WL, WR)      */
+/* (src/apply.jl:93-105).  Fused path: step-1 grouped GEMM (v^T envL), one    */
+/* combined-MPO pass (WL x WR over the middle bond), step-4 grouped GEMM with */
+/* K segments over (mpo_r, ket_r).  Output structure and block layout equal   */
+/* the reference's four-contract chain.                                       */
+/* ------------------------------------------------------------------------ */
+typedef struct mpsh_heff_s* mpsh_heff;
+
+mpsh_status mpsh_heff_plan(mpsh_struct v, mpsh_struct envL, mpsh_struct envR, mpsh_struct WL, mpsh_struct WR,
+                           mpsh_heff* plan, mpsh_struct* out);
+size_t mpsh_heff_workspace(mpsh_heff plan, mpsh_dtype dt); /* bytes */
+/* stats[0..9]: reference-order FLOPs, reference-order elements moved, step FLOPs x4,
+ * step elements x4 (the reference's own four contracts, src/apply.jl:99-103). */
+mpsh_status mpsh_heff_stats(mpsh_heff plan, int64_t* stats);
+mpsh_status mpsh_heff_apply(mpsh_heff plan, mpsh_dtype dt, const void* v, const void* envL, const void* envR,
+                            const void* WL, const void* WR, void* out, void* workspace, void* stream);
+void mpsh_heff_destroy(mpsh_heff plan);
+
+/* ------------------------------------------------------------------------ */
+/* Block-wise vector operations on packed buffers (KrylovKit's interface on   */
+/* SymTensors, src/symtensor.jl:266-323).  n = number of elements.            */
+/* ------------------------------------------------------------------------ */
+#define MPSH_DOT_PARTIALS 1024
+/* result (device, one double) = sum x*y; partials: device scratch of MPSH_DOT_PARTIALS doubles */
+mpsh_status mpsh_dot(mpsh_dtype dt, int64_t n, const void* x, const void* y, double* result, double* partials,
+                     void* stream);
+/* y = a*x + b*y   (axpby!, src/symtensor.jl:299-308) */
+mpsh_status mpsh_axpby(mpsh_dtype dt, int64_t n, double a, const void* x, double b, void* y, void* stream);
+/* x = a*x   (rmul!, src/symtensor.jl:280-285) */
+mpsh_status mpsh_scal(mpsh_dtype dt, int64_t n, double a, void* x, void* stream);
+/* dst = (dst type) src; dtypes MPSH_F64 / MPSH_F32 */
+mpsh_status mpsh_convert(mpsh_dtype src_dt, const void* src, mpsh_dtype dst_dt, void* dst, int64_t n,
+                         void* stream);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* MPSH_H */

@@ -1,0 +1,48 @@
+"""Build libmpsh.so in-tree for gfx950 (MI355X).
+
+The host planner is plain C++ (g++); kernels and the C ABI are HIP (hipcc).
+Packed-FP32 instruction selection is switched off for all device code.
+Usage: ``python -m mps_amd.build`` (or ``__graft_entry__.build()``).
+"""
+from __future__ import annotations
+
+import os
+import subprocess
+import sys
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+CSRC = os.path.join(HERE, "csrc")
+BUILD = os.path.join(HERE, "_build")
+LIB = os.path.join(HERE, "libmpsh.so")
+ARCH = os.environ.get("MPSH_OFFLOAD_ARCH", "gfx950")
+
+HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
+HIP_FLAGS = ["-O3", "-fPIC", "-std=c++17", f"--offload-arch={ARCH}",
+             "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
+             "-Wno-unused-command-line-argument"]
+
+
+def _run(cmd):
+    print("+", " ".join(cmd), flush=True)
+    subprocess.run(cmd, check=True)
+
+
+def build(verbose: bool = True) -> str:
+    os.makedirs(BUILD, exist_ok=True)
+    objs = []
+    o = os.path.join(BUILD, "planner.o")
+    _run(["g++", "-std=c++17", "-O2", "-fPIC", "-Wall", "-c", os.path.join(CSRC, "planner.cpp"), "-o", o])
+    objs.append(o)
+    for src in ("kernels.hip", "api.hip"):
+        o = os.path.join(BUILD, src.replace(".hip", ".o"))
+        _run([HIPCC, *HIP_FLAGS, "-c", os.path.join(CSRC, src), "-o", o])
+        objs.append(o)
+    tmp = LIB + ".tmp"
+    _run([HIPCC, f"--offload-arch={ARCH}", "-shared", "-fPIC", *objs, "-o", tmp])
+    os.replace(tmp, LIB)
+    return LIB
+
+
+if __name__ == "__main__":
+    build()
+    sys.exit(0)

@@ -1,0 +1,458 @@
+// C ABI (include/mpsh.h): plan objects around the host planner, lazy
+// per-device upload of their descriptor tables, and the execute calls that
+// enqueue the HIP kernels on the caller's stream.  No call here allocates
+// device memory except the one-time table upload of a plan.
+#include <hip/hip_runtime.h>
+
+#include <cstring>
+#include <map>
+#include <memory>
+#include <mutex>
+#include <string>
+
+#include "../../include/mpsh.h"
+#include "kernels.hpp"
+#include "planner.hpp"
+
+using namespace mpsh;
+
+static thread_local std::string g_err;
+
+static mpsh_status fail(int code, const std::string& m) {
+  g_err = m;
+  return (mpsh_status)code;
+}
+
+#define API_TRY try {
+#define API_CATCH                                                      \
+  }                                                                    \
+  catch (const Error& e) { return fail(e.code, e.what()); }            \
+  catch (const std::bad_alloc&) { return fail(ERR_OOM, "host OOM"); } \
+  catch (const std::exception& e) { return fail(ERR_INVALID, e.what()); }
+
+static void check(hipError_t e, const char* what) {
+  if (e != hipSuccess)
+    throw Error(e == hipErrorOutOfMemory ? ERR_OOM : ERR_DEVICE,
+                std::string(what) + ": " + hipGetErrorString(e));
+}
+
+namespace {
+
+struct DevBuf {
+  void* p = nullptr;
+  DevBuf() = default;
+  DevBuf(const DevBuf&) = delete;
+  ~DevBuf() {
+    if (p) (void)hipFree(p);
+  }
+};
+
+struct DevTables {
+  std::vector<std::unique_ptr<DevBuf>> bufs;
+  template <class T>
+  const T* up(const std::vector<T>& v) {
+    if (v.empty()) return nullptr;
+    auto b = std::make_unique<DevBuf>();
+    check(hipMalloc(&b->p, v.size() * sizeof(T)), "hipMalloc(plan tables)");
+    check(hipMemcpy(b->p, v.data(), v.size() * sizeof(T), hipMemcpyHostToDevice), "hipMemcpy(plan tables)");
+    const T* r = (const T*)b->p;
+    bufs.push_back(std::move(b));
+    return r;
+  }
+  DevCopy copy(const CopyPlan& c) {
+    DevCopy d;
+    d.descs = up(c.descs);
+    d.tile_start = up(c.tile_start);
+    d.ndesc = (int)c.descs.size();
+    d.ntiles = c.ntiles;
+    return d;
+  }
+  DevGemm gemm(const GemmPlan& g) {
+    DevGemm d;
+    std::vector<GemmTile> t = g.tiles(64, 64);
+    d.probs = up(g.probs);
+    d.segs = up(g.segs);
+    d.tiles = up(t);
+    d.ntiles = (int)t.size();
+    return d;
+  }
+};
+
+int current_device() {
+  int d = 0;
+  check(hipGetDevice(&d), "hipGetDevice");
+  return d;
+}
+
+int elem_size(mpsh_dtype dt) { return dt == MPSH_F64 ? 8 : 4; }
+int gemm_mode(mpsh_dtype dt) { return dt == MPSH_F64 ? 0 : (dt == MPSH_F32 ? 1 : 2); }
+
+void check_dtype(int dt) {
+  if (dt != MPSH_F64 && dt != MPSH_F32 && dt != MPSH_BF16) throw Error(ERR_INVALID, "unknown dtype");
+}
+
+std::vector<Leg> parse_legs(int32_t rank, const uint8_t* isdual, const int32_t* nch, const int64_t* ch,
+                            const int64_t* dims) {
+  if (rank <= 0 || !isdual || !nch || !ch || !dims) throw Error(ERR_INVALID, "bad leg arrays");
+  std::vector<Leg> legs;
+  int64_t p = 0;
+  for (int i = 0; i < rank; ++i) {
+    std::vector<std::pair<int64_t, int64_t>> pairs;
+    if (nch[i] <= 0) throw Error(ERR_INVALID, "leg with no charges");
+    for (int j = 0; j < nch[i]; ++j, ++p) pairs.push_back({ch[p], dims[p]});
+    legs.push_back(make_leg(isdual[i] != 0, pairs));
+  }
+  return legs;
+}
+
+}  // namespace
+
+struct mpsh_struct_s {
+  Struct s;
+};
+
+static mpsh_struct wrap(const Struct& s) {
+  mpsh_struct r = new mpsh_struct_s;
+  r->s = s;
+  return r;
+}
+
+struct mpsh_releg_s {
+  RelegPlan p;
+  std::mutex mu;
+  std::map<int, std::pair<std::unique_ptr<DevTables>, DevCopy>> dev;
+  const DevCopy& get() {
+    std::lock_guard<std::mutex> g(mu);
+    int d = current_device();
+    auto it = dev.find(d);
+    if (it == dev.end()) {
+      auto t = std::make_unique<DevTables>();
+      DevCopy c = t->copy(p.copy);
+      it = dev.emplace(d, std::make_pair(std::move(t), c)).first;
+    }
+    return it->second.second;
+  }
+};
+
+struct ContractDev {
+  std::unique_ptr<DevTables> t;
+  DevCopy ga, gb, sc;
+  DevGemm gm;
+};
+
+struct mpsh_contract_s {
+  ContractPlan p;
+  std::mutex mu;
+  std::map<int, ContractDev> dev;
+  const ContractDev& get() {
+    std::lock_guard<std::mutex> g(mu);
+    int d = current_device();
+    auto it = dev.find(d);
+    if (it == dev.end()) {
+      ContractDev c;
+      c.t = std::make_unique<DevTables>();
+      c.ga = c.t->copy(p.FA.gather);
+      c.gb = c.t->copy(p.FB.gather);
+      c.sc = c.t->copy(p.scatter);
+      c.gm = c.t->gemm(p.gemm);
+      it = dev.emplace(d, std::move(c)).first;
+    }
+    return it->second;
+  }
+};
+
+struct HeffDev {
+  std::unique_ptr<DevTables> t;
+  DevGemm g1, g4;
+  DevMpo mpo;
+};
+
+struct mpsh_heff_s {
+  HeffPlan p;
+  std::mutex mu;
+  std::map<int, HeffDev> dev;
+  const HeffDev& get() {
+    std::lock_guard<std::mutex> g(mu);
+    int d = current_device();
+    auto it = dev.find(d);
+    if (it == dev.end()) {
+      HeffDev h;
+      h.t = std::make_unique<DevTables>();
+      h.g1 = h.t->gemm(p.g1);
+      h.g4 = h.t->gemm(p.g4);
+      h.mpo.sigs = h.t->up(p.sigs);
+      h.mpo.terms = h.t->up(p.terms);
+      h.mpo.pairs = h.t->up(p.pairs);
+      h.mpo.groups = h.t->up(p.groups);
+      h.mpo.inrows = h.t->up(p.inrows);
+      h.mpo.outrows = h.t->up(p.outrows);
+      h.mpo.work = h.t->up(p.work);
+      h.mpo.nterms = (int)p.terms.size();
+      h.mpo.nwork = (int)p.work.size();
+      h.mpo.cmax = p.maxC <= 32 ? 32 : 64;
+      it = dev.emplace(d, std::move(h)).first;
+    }
+    return it->second;
+  }
+};
+
+extern "C" {
+
+const char* mpsh_last_error(void) { return g_err.c_str(); }
+const char* mpsh_version(void) { return "mpsh 0.1 gfx950"; }
+
+// ---------------------------------------------------------------------------
+// structures
+// ---------------------------------------------------------------------------
+mpsh_status mpsh_struct_create(int64_t charge, int32_t rank, const uint8_t* isdual, const int32_t* ncharges,
+                               const int64_t* charges, const int64_t* dims, mpsh_struct* out) {
+  API_TRY
+  if (!out) throw Error(ERR_INVALID, "null out");
+  *out = wrap(make_struct(charge, parse_legs(rank, isdual, ncharges, charges, dims)));
+  return MPSH_OK;
+  API_CATCH
+}
+
+void mpsh_struct_destroy(mpsh_struct s) { delete s; }
+int32_t mpsh_struct_rank(mpsh_struct s) { return s ? s->s.rank() : -1; }
+int64_t mpsh_struct_charge(mpsh_struct s) { return s ? s->s.charge : 0; }
+int64_t mpsh_struct_nblocks(mpsh_struct s) { return s ? s->s.nblocks : -1; }
+int64_t mpsh_struct_numel(mpsh_struct s) { return s ? s->s.numel() : -1; }
+
+int32_t mpsh_struct_leg_ncharges(mpsh_struct s, int32_t leg) {
+  if (!s || leg < 0 || leg >= s->s.rank()) return -1;
+  return (int32_t)s->s.legs[leg].charges.size();
+}
+
+mpsh_status mpsh_struct_leg(mpsh_struct s, int32_t leg, uint8_t* isdual, int64_t* charges, int64_t* dims) {
+  if (!s || leg < 0 || leg >= s->s.rank()) return fail(ERR_INVALID, "bad leg");
+  const Leg& l = s->s.legs[leg];
+  if (isdual) *isdual = l.isdual ? 1 : 0;
+  if (charges) std::memcpy(charges, l.charges.data(), l.charges.size() * 8);
+  if (dims) std::memcpy(dims, l.dims.data(), l.dims.size() * 8);
+  return MPSH_OK;
+}
+
+mpsh_status mpsh_struct_blocks(mpsh_struct s, int64_t* sectors, int64_t* sizes, int64_t* offsets) {
+  if (!s) return fail(ERR_INVALID, "null structure");
+  if (sectors) std::memcpy(sectors, s->s.sectors.data(), s->s.sectors.size() * 8);
+  if (sizes) std::memcpy(sizes, s->s.sizes.data(), s->s.sizes.size() * 8);
+  if (offsets) std::memcpy(offsets, s->s.offsets.data(), s->s.offsets.size() * 8);
+  return MPSH_OK;
+}
+
+int64_t mpsh_struct_find(mpsh_struct s, const int64_t* sector) {
+  if (!s || !sector) return -1;
+  return s->s.find(std::vector<int64_t>(sector, sector + s->s.rank()));
+}
+
+// ---------------------------------------------------------------------------
+// re-legging
+// ---------------------------------------------------------------------------
+static mpsh_status finish_releg(RelegPlan&& p, mpsh_releg* plan, mpsh_struct* out) {
+  if (!plan || !out) return fail(ERR_INVALID, "null out");
+  mpsh_releg r = new mpsh_releg_s;
+  r->p = std::move(p);
+  *out = wrap(r->p.out);
+  *plan = r;
+  return MPSH_OK;
+}
+
+mpsh_status mpsh_permutelegs_plan(mpsh_struct A, const int32_t* perm, mpsh_releg* plan, mpsh_struct* out) {
+  API_TRY
+  if (!A || !perm) throw Error(ERR_INVALID, "null argument");
+  return finish_releg(permutelegs_plan(A->s, std::vector<int>(perm, perm + A->s.rank())), plan, out);
+  API_CATCH
+}
+
+mpsh_status mpsh_fuselegs_plan(mpsh_struct A, int32_t l, int32_t n, int32_t dual, mpsh_releg* plan,
+                               mpsh_struct* out) {
+  API_TRY
+  if (!A) throw Error(ERR_INVALID, "null argument");
+  return finish_releg(fuselegs_plan(A->s, l, n, dual != 0), plan, out);
+  API_CATCH
+}
+
+mpsh_status mpsh_splitleg_plan(mpsh_struct A, int32_t l, int32_t nspaces, const uint8_t* isdual,
+                               const int32_t* ncharges, const int64_t* charges, const int64_t* dims,
+                               mpsh_releg* plan, mpsh_struct* out) {
+  API_TRY
+  if (!A) throw Error(ERR_INVALID, "null argument");
+  return finish_releg(splitleg_plan(A->s, l, parse_legs(nspaces, isdual, ncharges, charges, dims)), plan, out);
+  API_CATCH
+}
+
+mpsh_status mpsh_symmatrix_plan(mpsh_struct A, int32_t nrows, const int32_t* rows, int32_t ncols,
+                                const int32_t* cols, mpsh_releg* plan, mpsh_struct* out) {
+  API_TRY
+  if (!A || (nrows && !rows) || (ncols && !cols)) throw Error(ERR_INVALID, "null argument");
+  return finish_releg(symmatrix_releg(A->s, std::vector<int>(rows, rows + nrows), std::vector<int>(cols, cols + ncols)),
+                      plan, out);
+  API_CATCH
+}
+
+mpsh_status mpsh_releg_execute(mpsh_releg plan, mpsh_dtype dt, const void* src, void* dst, void* stream) {
+  API_TRY
+  check_dtype(dt);
+  if (!plan || (!src && plan->p.in.numel()) || (!dst && plan->p.out.numel())) throw Error(ERR_INVALID, "null argument");
+  const DevCopy& c = plan->get();
+  check(launch_copy(c, elem_size(dt), src, dst, (hipStream_t)stream), "copy kernel");
+  return MPSH_OK;
+  API_CATCH
+}
+
+void mpsh_releg_destroy(mpsh_releg plan) { delete plan; }
+
+// ---------------------------------------------------------------------------
+// contract
+// ---------------------------------------------------------------------------
+mpsh_status mpsh_contract_plan(mpsh_struct A, const int32_t* idxA, mpsh_struct B, const int32_t* idxB,
+                               mpsh_contract* plan, mpsh_struct* C) {
+  API_TRY
+  if (!A || !B || !idxA || !idxB || !plan || !C) throw Error(ERR_INVALID, "null argument");
+  mpsh_contract p = new mpsh_contract_s;
+  try {
+    p->p = contract_plan(A->s, std::vector<int>(idxA, idxA + A->s.rank()), B->s,
+                         std::vector<int>(idxB, idxB + B->s.rank()));
+  } catch (...) {
+    delete p;
+    throw;
+  }
+  *C = wrap(p->p.C);
+  *plan = p;
+  return MPSH_OK;
+  API_CATCH
+}
+
+size_t mpsh_contract_workspace(mpsh_contract plan, mpsh_dtype dt) {
+  if (!plan) return 0;
+  return (size_t)plan->p.ws_elems * (size_t)elem_size(dt);
+}
+
+int64_t mpsh_contract_flops(mpsh_contract plan) { return plan ? plan->p.gemm.flops : 0; }
+
+mpsh_status mpsh_contract_execute(mpsh_contract plan, mpsh_dtype dt, const void* A, const void* B, void* C,
+                                  void* workspace, void* stream) {
+  API_TRY
+  check_dtype(dt);
+  if (!plan || !A || !B || !C || (!workspace && plan->p.ws_elems)) throw Error(ERR_INVALID, "null argument");
+  const ContractDev& d = plan->get();
+  hipStream_t s = (hipStream_t)stream;
+  const int es = elem_size(dt);
+  char* ws = (char*)workspace;
+  check(launch_copy(d.ga, es, A, ws, s), "gather A");
+  check(launch_copy(d.gb, es, B, ws, s), "gather B");
+  check(launch_gemm(d.gm, gemm_mode(dt), ws + plan->p.fa_off * es, ws + plan->p.fb_off * es,
+                    ws + plan->p.fc_off * es, s),
+        "grouped gemm");
+  check(launch_copy(d.sc, es, ws, C, s), "scatter C");
+  return MPSH_OK;
+  API_CATCH
+}
+
+void mpsh_contract_destroy(mpsh_contract plan) { delete plan; }
+
+// ---------------------------------------------------------------------------
+// H_eff
+// ---------------------------------------------------------------------------
+mpsh_status mpsh_heff_plan(mpsh_struct v, mpsh_struct envL, mpsh_struct envR, mpsh_struct WL, mpsh_struct WR,
+                           mpsh_heff* plan, mpsh_struct* out) {
+  API_TRY
+  if (!v || !envL || !envR || !WL || !WR || !plan || !out) throw Error(ERR_INVALID, "null argument");
+  mpsh_heff h = new mpsh_heff_s;
+  try {
+    h->p = heff_plan(v->s, envL->s, envR->s, WL->s, WR->s);
+    if (h->p.maxR > 64 || h->p.maxC > 64)
+      throw Error(ERR_UNSUPPORTED, "combined MPO group wider than 64 channels");
+  } catch (...) {
+    delete h;
+    throw;
+  }
+  *out = wrap(h->p.out);
+  *plan = h;
+  return MPSH_OK;
+  API_CATCH
+}
+
+size_t mpsh_heff_workspace(mpsh_heff plan, mpsh_dtype dt) {
+  if (!plan) return 0;
+  return (size_t)plan->p.ws_elems * (size_t)elem_size(dt);
+}
+
+mpsh_status mpsh_heff_stats(mpsh_heff plan, int64_t* stats) {
+  if (!plan || !stats) return fail(ERR_INVALID, "null argument");
+  const HeffPlan& p = plan->p;
+  stats[0] = p.ref_flops;
+  stats[1] = p.ref_elems;
+  for (int i = 0; i < 4; ++i) {
+    stats[2 + i] = p.step_flops[i];
+    stats[6 + i] = p.step_elems[i];
+  }
+  return MPSH_OK;
+}
+
+mpsh_status mpsh_heff_apply(mpsh_heff plan, mpsh_dtype dt, const void* v, const void* envL, const void* envR,
+                            const void* WL, const void* WR, void* out, void* workspace, void* stream) {
+  API_TRY
+  check_dtype(dt);
+  if (!plan || !v || !envL || !envR || !WL || !WR || !out || !workspace) throw Error(ERR_INVALID, "null argument");
+  const HeffDev& d = plan->get();
+  const HeffPlan& p = plan->p;
+  hipStream_t s = (hipStream_t)stream;
+  const int es = elem_size(dt);
+  char* ws = (char*)workspace;
+  void* T1 = ws + p.t1_off * es;
+  void* T3 = ws + p.t3_off * es;
+  void* W = ws + p.w_off * es;
+  check(launch_gemm(d.g1, gemm_mode(dt), v, envL, T1, s), "step-1 gemm");
+  check(launch_mpo(d.mpo, es, WL, WR, W, T1, T3, s), "mpo");
+  check(launch_gemm(d.g4, gemm_mode(dt), T3, envR, out, s), "step-4 gemm");
+  return MPSH_OK;
+  API_CATCH
+}
+
+void mpsh_heff_destroy(mpsh_heff plan) { delete plan; }
+
+// ---------------------------------------------------------------------------
+// vector ops
+// ---------------------------------------------------------------------------
+mpsh_status mpsh_dot(mpsh_dtype dt, int64_t n, const void* x, const void* y, double* result, double* partials,
+                     void* stream) {
+  API_TRY
+  check_dtype(dt);
+  if (n < 0 || !result || !partials || (n && (!x || !y))) throw Error(ERR_INVALID, "bad dot arguments");
+  check(launch_dot(elem_size(dt), n, x, y, result, partials, MPSH_DOT_PARTIALS, (hipStream_t)stream), "dot");
+  return MPSH_OK;
+  API_CATCH
+}
+
+mpsh_status mpsh_axpby(mpsh_dtype dt, int64_t n, double a, const void* x, double b, void* y, void* stream) {
+  API_TRY
+  check_dtype(dt);
+  if (n < 0 || (n && (!x || !y))) throw Error(ERR_INVALID, "bad axpby arguments");
+  check(launch_axpby(elem_size(dt), n, a, x, b, y, (hipStream_t)stream), "axpby");
+  return MPSH_OK;
+  API_CATCH
+}
+
+mpsh_status mpsh_scal(mpsh_dtype dt, int64_t n, double a, void* x, void* stream) {
+  API_TRY
+  check_dtype(dt);
+  if (n < 0 || (n && !x)) throw Error(ERR_INVALID, "bad scal arguments");
+  check(launch_scal(elem_size(dt), n, a, x, (hipStream_t)stream), "scal");
+  return MPSH_OK;
+  API_CATCH
+}
+
+mpsh_status mpsh_convert(mpsh_dtype src_dt, const void* src, mpsh_dtype dst_dt, void* dst, int64_t n,
+                         void* stream) {
+  API_TRY
+  check_dtype(src_dt);
+  check_dtype(dst_dt);
+  if (n < 0 || (n && (!src || !dst))) throw Error(ERR_INVALID, "bad convert arguments");
+  check(launch_convert(elem_size(src_dt), src, elem_size(dst_dt), dst, n, (hipStream_t)stream), "convert");
+  return MPSH_OK;
+  API_CATCH
+}
+
+}  // extern "C"

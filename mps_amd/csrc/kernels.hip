@@ -1,0 +1,526 @@
+// HIP kernels for gfx950 (MI355X / CDNA4) and their launchers.
+//
+//  * copy_kernel     -- batched strided N-d copy; 64x64 LDS-tiled transpose when
+//                       the source- and destination-contiguous dims differ.
+//                       Implements fuse / split / permute (SymMatrix,
+//                       SymTensor(::SymMatrix), permutelegs, fuselegs, splitleg).
+//  * gemm_grouped    -- persistent grouped GEMM over uneven block problems with
+//                       K segments; MFMA 16x16x4 f64 / 16x16x4 f32 / 16x16x32 bf16.
+//  * mpo_combine     -- builds the combined two-site MPO matrices W_group.
+//  * mpo_apply       -- T3 = W_group . T1 per (bra_l, ket_r) pixel group.
+//  * dot/axpby/scal/convert -- block-wise vector ops on packed buffers.
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+
+#include "kernels.hpp"
+
+namespace mpsh {
+
+typedef float floatx4 __attribute__((ext_vector_type(4)));
+typedef double doublex4 __attribute__((ext_vector_type(4)));
+typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
+
+// ---------------------------------------------------------------------------
+// copy
+// ---------------------------------------------------------------------------
+template <typename T>
+__global__ __launch_bounds__(256) void copy_kernel(const CopyDesc* __restrict__ descs,
+                                                   const int64_t* __restrict__ ts, int ndesc,
+                                                   const T* __restrict__ src, T* __restrict__ dst) {
+  const int64_t bid = blockIdx.x;
+  int lo = 0, hi = ndesc - 1;
+  while (lo < hi) {
+    const int mid = (lo + hi + 1) >> 1;
+    if (ts[mid] <= bid)
+      lo = mid;
+    else
+      hi = mid - 1;
+  }
+  const CopyDesc* d = descs + lo;
+  int64_t t = bid - ts[lo];
+  const int ndim = d->ndim;
+  const int64_t d0 = d->dims[0], d1 = ndim > 1 ? d->dims[1] : 1;
+  const int64_t s0 = d->src_st[0], s1 = ndim > 1 ? d->src_st[1] : 0;
+  const int64_t w0 = d->dst_st[0], w1 = ndim > 1 ? d->dst_st[1] : 0;
+  int64_t sb = d->src_off, db = d->dst_off;
+  if (d->transpose) {
+    __shared__ T tile[64][65];
+    const int64_t nx = (d0 + 63) >> 6, ny = (d1 + 63) >> 6;
+    const int64_t tx = t % nx;
+    t /= nx;
+    const int64_t ty = t % ny;
+    t /= ny;
+    for (int k = 2; k < ndim; ++k) {
+      const int64_t i = t % d->dims[k];
+      t /= d->dims[k];
+      sb += i * d->src_st[k];
+      db += i * d->dst_st[k];
+    }
+    const int lx = threadIdx.x & 63, ly = threadIdx.x >> 6;
+    for (int i = ly; i < 64; i += 4) {
+      const int64_t x = tx * 64 + i, y = ty * 64 + lx;
+      if (x < d0 && y < d1) tile[i][lx] = src[sb + x * s0 + y * s1];
+    }
+    __syncthreads();
+    for (int j = ly; j < 64; j += 4) {
+      const int64_t x = tx * 64 + lx, y = ty * 64 + j;
+      if (x < d0 && y < d1) dst[db + x * w0 + y * w1] = tile[lx][j];
+    }
+  } else {
+    const int64_t n01 = d0 * d1;
+    const int64_t nt = (n01 + 4095) >> 12;
+    const int64_t tt = t % nt;
+    t /= nt;
+    for (int k = 2; k < ndim; ++k) {
+      const int64_t i = t % d->dims[k];
+      t /= d->dims[k];
+      sb += i * d->src_st[k];
+      db += i * d->dst_st[k];
+    }
+    const uint32_t ud0 = (uint32_t)d0;
+    for (int i = threadIdx.x; i < 4096; i += 256) {
+      const int64_t e = tt * 4096 + i;
+      if (e >= n01) break;
+      const uint32_t ue = (uint32_t)e;
+      const uint32_t y = ue / ud0, x = ue - y * ud0;
+      dst[db + x * w0 + y * w1] = src[sb + x * s0 + y * s1];
+    }
+  }
+}
+
+hipError_t launch_copy(const DevCopy& c, int elem_size, const void* src, void* dst, hipStream_t s) {
+  if (c.ntiles == 0) return hipSuccess;
+  dim3 grid((unsigned)c.ntiles), block(256);
+  if (elem_size == 8)
+    hipLaunchKernelGGL(copy_kernel<double>, grid, block, 0, s, c.descs, c.tile_start, c.ndesc,
+                       (const double*)src, (double*)dst);
+  else
+    hipLaunchKernelGGL(copy_kernel<float>, grid, block, 0, s, c.descs, c.tile_start, c.ndesc,
+                       (const float*)src, (float*)dst);
+  return hipGetLastError();
+}
+
+// ---------------------------------------------------------------------------
+// grouped GEMM
+//   C[M x N] (col-major, ldc) = sum_s A_s^T B_s, A_s(k,m) = A[a_off + k + m*lda],
+//   B_s(k,n) = B[b_off + k + n*ldb].  Tile 64x64, 4 waves in 2x2, each wave
+//   2x2 MFMA 16x16 sub-tiles.  Operands are staged K-contiguous in LDS; each
+//   lane reads a contiguous K-run and feeds successive MFMAs (the K order
+//   inside a chunk is permuted identically for A and B).
+// ---------------------------------------------------------------------------
+template <int MODE>
+struct GT;
+template <>
+struct GT<0> {  // FP64
+  typedef double T;
+  typedef double L;
+  typedef doublex4 Acc;
+  static constexpr int BK = 16, PAD = 2;
+};
+template <>
+struct GT<1> {  // FP32
+  typedef float T;
+  typedef float L;
+  typedef floatx4 Acc;
+  static constexpr int BK = 16, PAD = 4;
+};
+template <>
+struct GT<2> {  // BF16 operands, FP32 accumulate/storage
+  typedef float T;
+  typedef __bf16 L;
+  typedef floatx4 Acc;
+  static constexpr int BK = 32, PAD = 8;
+};
+
+constexpr int GBM = 64, GBN = 64;
+
+template <int MODE>
+__device__ __forceinline__ void mfma_chunk(typename GT<MODE>::Acc& c, const typename GT<MODE>::L* a,
+                                           const typename GT<MODE>::L* b);
+
+template <>
+__device__ __forceinline__ void mfma_chunk<0>(doublex4& c, const double* a, const double* b) {
+  const double2 a01 = *(const double2*)a, a23 = *(const double2*)(a + 2);
+  const double2 b01 = *(const double2*)b, b23 = *(const double2*)(b + 2);
+  c = __builtin_amdgcn_mfma_f64_16x16x4f64(a01.x, b01.x, c, 0, 0, 0);
+  c = __builtin_amdgcn_mfma_f64_16x16x4f64(a01.y, b01.y, c, 0, 0, 0);
+  c = __builtin_amdgcn_mfma_f64_16x16x4f64(a23.x, b23.x, c, 0, 0, 0);
+  c = __builtin_amdgcn_mfma_f64_16x16x4f64(a23.y, b23.y, c, 0, 0, 0);
+}
+template <>
+__device__ __forceinline__ void mfma_chunk<1>(floatx4& c, const float* a, const float* b) {
+  const floatx4 av = *(const floatx4*)a, bv = *(const floatx4*)b;
+  c = __builtin_amdgcn_mfma_f32_16x16x4f32(av[0], bv[0], c, 0, 0, 0);
+  c = __builtin_amdgcn_mfma_f32_16x16x4f32(av[1], bv[1], c, 0, 0, 0);
+  c = __builtin_amdgcn_mfma_f32_16x16x4f32(av[2], bv[2], c, 0, 0, 0);
+  c = __builtin_amdgcn_mfma_f32_16x16x4f32(av[3], bv[3], c, 0, 0, 0);
+}
+template <>
+__device__ __forceinline__ void mfma_chunk<2>(floatx4& c, const __bf16* a, const __bf16* b) {
+  const bf16x8 av = *(const bf16x8*)a, bv = *(const bf16x8*)b;
+  c = __builtin_amdgcn_mfma_f32_16x16x32_bf16(av, bv, c, 0, 0, 0);
+}
+
+template <int MODE>
+__global__ __launch_bounds__(256) void gemm_grouped(const GemmProb* __restrict__ probs,
+                                                    const GemmSeg* __restrict__ segs,
+                                                    const GemmTile* __restrict__ tiles, int ntiles,
+                                                    const typename GT<MODE>::T* __restrict__ A,
+                                                    const typename GT<MODE>::T* __restrict__ B,
+                                                    typename GT<MODE>::T* __restrict__ C) {
+  typedef typename GT<MODE>::T T;
+  typedef typename GT<MODE>::L L;
+  typedef typename GT<MODE>::Acc Acc;
+  constexpr int BK = GT<MODE>::BK, LD = GT<MODE>::BK + GT<MODE>::PAD;
+  constexpr int PER = GBM * BK / 256;   // elements per thread per operand
+  constexpr int RSTEP = 256 / BK;       // rows covered per pass
+  constexpr int KL = (MODE == 2) ? 8 : 4;  // K run read per lane per chunk step
+  __shared__ __attribute__((aligned(16))) L As[2][GBM][LD];
+  __shared__ __attribute__((aligned(16))) L Bs[2][GBN][LD];
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const int wm = wave >> 1, wn = wave & 1;
+  const int kk = tid % BK, mm = tid / BK;
+  const int g = lane >> 4, l16 = lane & 15;
+
+  for (int t = blockIdx.x; t < ntiles; t += gridDim.x) {
+    const GemmTile tile = tiles[t];
+    const GemmProb p = probs[tile.prob];
+    Acc acc[2][2];
+#pragma unroll
+    for (int i = 0; i < 2; ++i)
+#pragma unroll
+      for (int j = 0; j < 2; ++j)
+#pragma unroll
+        for (int r = 0; r < 4; ++r) acc[i][j][r] = 0;
+
+    // chunk iterator over (segment, k0)
+    int seg = 0;
+    int64_t k0 = 0;
+    int64_t K = 0, a_off = 0, lda = 0, b_off = 0, ldb = 0;
+    bool have = p.nseg > 0;
+    if (have) {
+      const GemmSeg s = segs[p.seg0];
+      K = s.K; a_off = s.a_off; lda = s.lda; b_off = s.b_off; ldb = s.ldb;
+    }
+    T ra[PER], rb[PER];
+    auto load = [&]() {
+#pragma unroll
+      for (int i = 0; i < PER; ++i) {
+        const int row = mm + i * RSTEP;
+        const int64_t k = k0 + kk;
+        const int m = tile.m0 + row, n = tile.n0 + row;
+        ra[i] = (k < K && m < p.M) ? A[a_off + k + (int64_t)m * lda] : T(0);
+        rb[i] = (k < K && n < p.N) ? B[b_off + k + (int64_t)n * ldb] : T(0);
+      }
+    };
+    auto advance = [&]() {
+      k0 += BK;
+      if (k0 >= K) {
+        ++seg;
+        k0 = 0;
+        if (seg < p.nseg) {
+          const GemmSeg s = segs[p.seg0 + seg];
+          K = s.K; a_off = s.a_off; lda = s.lda; b_off = s.b_off; ldb = s.ldb;
+        } else {
+          have = false;
+        }
+      }
+    };
+    if (have) load();
+    int buf = 0;
+    while (have) {
+#pragma unroll
+      for (int i = 0; i < PER; ++i) {
+        As[buf][mm + i * RSTEP][kk] = (L)ra[i];
+        Bs[buf][mm + i * RSTEP][kk] = (L)rb[i];
+      }
+      __syncthreads();
+      advance();
+      if (have) load();
+#pragma unroll
+      for (int kc = 0; kc < BK; kc += 4 * KL) {
+#pragma unroll
+        for (int i = 0; i < 2; ++i) {
+          const L* ap = &As[buf][wm * 32 + i * 16 + l16][kc + KL * g];
+#pragma unroll
+          for (int j = 0; j < 2; ++j) {
+            const L* bp = &Bs[buf][wn * 32 + j * 16 + l16][kc + KL * g];
+            mfma_chunk<MODE>(acc[i][j], ap, bp);
+          }
+        }
+      }
+      buf ^= 1;
+    }
+    // epilogue: lane holds D[4*g + r][l16] of each 16x16 sub-tile
+#pragma unroll
+    for (int i = 0; i < 2; ++i)
+#pragma unroll
+      for (int j = 0; j < 2; ++j) {
+        const int n = tile.n0 + wn * 32 + j * 16 + l16;
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+          const int m = tile.m0 + wm * 32 + i * 16 + 4 * g + r;
+          if (m < p.M && n < p.N) C[p.c_off + m + (int64_t)n * p.ldc] = (T)acc[i][j][r];
+        }
+      }
+    __syncthreads();  // LDS reuse by the next tile
+  }
+}
+
+static int g_num_cu = 0;
+static int num_cu() {
+  if (g_num_cu == 0) {
+    int dev = 0, n = 0;
+    if (hipGetDevice(&dev) == hipSuccess &&
+        hipDeviceGetAttribute(&n, hipDeviceAttributeMultiprocessorCount, dev) == hipSuccess && n > 0)
+      g_num_cu = n;
+    else
+      g_num_cu = 256;
+  }
+  return g_num_cu;
+}
+
+hipError_t launch_gemm(const DevGemm& gm, int mode, const void* A, const void* B, void* C, hipStream_t s) {
+  if (gm.ntiles == 0) return hipSuccess;
+  const int maxgrid = num_cu() * 8;
+  dim3 grid((unsigned)(gm.ntiles < maxgrid ? gm.ntiles : maxgrid)), block(256);
+  if (mode == 0)
+    hipLaunchKernelGGL(gemm_grouped<0>, grid, block, 0, s, gm.probs, gm.segs, gm.tiles, gm.ntiles,
+                       (const double*)A, (const double*)B, (double*)C);
+  else if (mode == 1)
+    hipLaunchKernelGGL(gemm_grouped<1>, grid, block, 0, s, gm.probs, gm.segs, gm.tiles, gm.ntiles,
+                       (const float*)A, (const float*)B, (float*)C);
+  else
+    hipLaunchKernelGGL(gemm_grouped<2>, grid, block, 0, s, gm.probs, gm.segs, gm.tiles, gm.ntiles,
+                       (const float*)A, (const float*)B, (float*)C);
+  return hipGetLastError();
+}
+
+// ---------------------------------------------------------------------------
+// combined two-site MPO
+// W_sig[(row0+r)*C + col0 + c] = sum_pairs sum_jm WL[jw,i1o,jm,i1] * WR[jm,i2o,jx,i2]
+//   r = jx + Dx*(i1o + d1o*i2o),  c = jw + Dw*(i1 + d1i*i2)
+// ---------------------------------------------------------------------------
+template <typename T>
+__global__ __launch_bounds__(256) void mpo_combine(const MpoSig* __restrict__ sigs,
+                                                   const MpoTerm* __restrict__ terms,
+                                                   const MpoPair* __restrict__ pairs,
+                                                   const T* __restrict__ WL, const T* __restrict__ WR,
+                                                   T* __restrict__ W) {
+  const MpoTerm tm = terms[blockIdx.x];
+  const MpoSig sg = sigs[tm.sig];
+  const int n = tm.R * tm.C;
+  for (int idx = threadIdx.x; idx < n; idx += blockDim.x) {
+    const int r = idx / tm.C, c = idx - r * tm.C;
+    const int jx = r % tm.Dx, i1o = (r / tm.Dx) % tm.d1o, i2o = r / (tm.Dx * tm.d1o);
+    const int jw = c % tm.Dw, i1 = (c / tm.Dw) % tm.d1i, i2 = c / (tm.Dw * tm.d1i);
+    T acc = 0;
+    for (int q = 0; q < tm.npair; ++q) {
+      const MpoPair pr = pairs[tm.pair0 + q];
+      const int Dm = pr.Dm;
+      const int64_t lbase = pr.wl_off + jw + (int64_t)tm.Dw * (i1o + (int64_t)tm.d1o * ((int64_t)Dm * i1));
+      const int64_t lstep = (int64_t)tm.Dw * tm.d1o;
+      const int64_t rbase = pr.wr_off + (int64_t)Dm * (i2o + (int64_t)tm.d2o * (jx + (int64_t)tm.Dx * i2));
+      for (int jm = 0; jm < Dm; ++jm) acc += WL[lbase + jm * lstep] * WR[rbase + jm];
+    }
+    W[sg.w_off + (int64_t)(tm.row0 + r) * sg.C + tm.col0 + c] = acc;
+  }
+}
+
+// out[r][pix] = sum_c W[r][c] * in[c][pix], pix = n + dR*m
+template <typename T, int CMAX>
+__global__ __launch_bounds__(256) void mpo_apply(const MpoSig* __restrict__ sigs,
+                                                 const MpoGroup* __restrict__ groups,
+                                                 const MpoInRow* __restrict__ inrows,
+                                                 const MpoOutRow* __restrict__ outrows,
+                                                 const MpoWork* __restrict__ work, const T* __restrict__ W,
+                                                 const T* __restrict__ T1, T* __restrict__ T3) {
+  __shared__ __attribute__((aligned(16))) T Ws[64 * CMAX];
+  __shared__ int64_t ib[CMAX], ins[CMAX], im[CMAX];
+  __shared__ int64_t ob[64], om[64];
+  const MpoWork wk = work[blockIdx.x];
+  const MpoGroup g = groups[wk.group];
+  const MpoSig sg = sigs[g.sig];
+  const int R = sg.R, C = sg.C;
+  for (int i = threadIdx.x; i < R * CMAX; i += blockDim.x) {
+    const int r = i / CMAX, c = i - r * CMAX;
+    Ws[i] = c < C ? W[sg.w_off + (int64_t)r * C + c] : T(0);
+  }
+  for (int i = threadIdx.x; i < CMAX; i += blockDim.x) {
+    if (i < C) {
+      const MpoInRow q = inrows[g.in0 + i];
+      ib[i] = q.base;
+      ins[i] = q.nstride;
+      im[i] = q.mstride;
+    } else {
+      ib[i] = 0;
+      ins[i] = 0;
+      im[i] = 0;
+    }
+  }
+  for (int i = threadIdx.x; i < R; i += blockDim.x) {
+    const MpoOutRow q = outrows[g.out0 + i];
+    ob[i] = q.base;
+    om[i] = q.mstride;
+  }
+  __syncthreads();
+  const int64_t p = (int64_t)wk.pix0 + threadIdx.x;
+  if (p >= g.npix) return;
+  const int64_t m = p / g.dR, n = p - m * g.dR;
+  T v[CMAX];
+#pragma unroll
+  for (int c = 0; c < CMAX; ++c) v[c] = c < C ? T1[ib[c] + n * ins[c] + m * im[c]] : T(0);
+  for (int r = 0; r < R; ++r) {
+    const T* wr = Ws + r * CMAX;
+    T acc = 0;
+#pragma unroll
+    for (int c = 0; c < CMAX; ++c) acc += wr[c] * v[c];
+    T3[ob[r] + n + m * om[r]] = acc;
+  }
+}
+
+hipError_t launch_mpo(const DevMpo& d, int elem_size, const void* WL, const void* WR, void* W, const void* T1,
+                      void* T3, hipStream_t s) {
+  if (d.nterms > 0) {
+    if (elem_size == 8)
+      hipLaunchKernelGGL(mpo_combine<double>, dim3(d.nterms), dim3(256), 0, s, d.sigs, d.terms, d.pairs,
+                         (const double*)WL, (const double*)WR, (double*)W);
+    else
+      hipLaunchKernelGGL(mpo_combine<float>, dim3(d.nterms), dim3(256), 0, s, d.sigs, d.terms, d.pairs,
+                         (const float*)WL, (const float*)WR, (float*)W);
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return e;
+  }
+  if (d.nwork == 0) return hipSuccess;
+  dim3 grid(d.nwork), block(256);
+  if (elem_size == 8) {
+    if (d.cmax <= 32)
+      hipLaunchKernelGGL((mpo_apply<double, 32>), grid, block, 0, s, d.sigs, d.groups, d.inrows, d.outrows,
+                         d.work, (const double*)W, (const double*)T1, (double*)T3);
+    else
+      hipLaunchKernelGGL((mpo_apply<double, 64>), grid, block, 0, s, d.sigs, d.groups, d.inrows, d.outrows,
+                         d.work, (const double*)W, (const double*)T1, (double*)T3);
+  } else {
+    if (d.cmax <= 32)
+      hipLaunchKernelGGL((mpo_apply<float, 32>), grid, block, 0, s, d.sigs, d.groups, d.inrows, d.outrows,
+                         d.work, (const float*)W, (const float*)T1, (float*)T3);
+    else
+      hipLaunchKernelGGL((mpo_apply<float, 64>), grid, block, 0, s, d.sigs, d.groups, d.inrows, d.outrows,
+                         d.work, (const float*)W, (const float*)T1, (float*)T3);
+  }
+  return hipGetLastError();
+}
+
+// ---------------------------------------------------------------------------
+// vector ops
+// ---------------------------------------------------------------------------
+template <typename T>
+__global__ __launch_bounds__(256) void dot_partial(int64_t n, const T* __restrict__ x, const T* __restrict__ y,
+                                                   double* __restrict__ partials) {
+  __shared__ double sh[256];
+  double acc = 0.0;
+  for (int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x; i < n; i += (int64_t)gridDim.x * 256)
+    acc += (double)x[i] * (double)y[i];
+  sh[threadIdx.x] = acc;
+  __syncthreads();
+  for (int w = 128; w > 0; w >>= 1) {
+    if (threadIdx.x < w) sh[threadIdx.x] += sh[threadIdx.x + w];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) partials[blockIdx.x] = sh[0];
+}
+
+__global__ __launch_bounds__(256) void dot_final(int np, const double* __restrict__ partials,
+                                                 double* __restrict__ out) {
+  __shared__ double sh[256];
+  double acc = 0.0;
+  for (int i = threadIdx.x; i < np; i += 256) acc += partials[i];
+  sh[threadIdx.x] = acc;
+  __syncthreads();
+  for (int w = 128; w > 0; w >>= 1) {
+    if (threadIdx.x < w) sh[threadIdx.x] += sh[threadIdx.x + w];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) out[0] = sh[0];
+}
+
+hipError_t launch_dot(int elem_size, int64_t n, const void* x, const void* y, double* out, double* partials,
+                      int max_partials, hipStream_t s) {
+  int64_t nb = (n + 2047) / 2048;
+  if (nb < 1) nb = 1;
+  if (nb > max_partials) nb = max_partials;
+  if (elem_size == 8)
+    hipLaunchKernelGGL(dot_partial<double>, dim3((unsigned)nb), dim3(256), 0, s, n, (const double*)x,
+                       (const double*)y, partials);
+  else
+    hipLaunchKernelGGL(dot_partial<float>, dim3((unsigned)nb), dim3(256), 0, s, n, (const float*)x,
+                       (const float*)y, partials);
+  hipLaunchKernelGGL(dot_final, dim3(1), dim3(256), 0, s, (int)nb, partials, out);
+  return hipGetLastError();
+}
+
+template <typename T>
+__global__ __launch_bounds__(256) void axpby_kernel(int64_t n, double a, const T* __restrict__ x, double b,
+                                                    T* __restrict__ y) {
+  for (int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x; i < n; i += (int64_t)gridDim.x * 256)
+    y[i] = (T)(a * (double)x[i] + b * (double)y[i]);
+}
+
+template <typename T>
+__global__ __launch_bounds__(256) void scal_kernel(int64_t n, double a, T* __restrict__ x) {
+  for (int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x; i < n; i += (int64_t)gridDim.x * 256)
+    x[i] = (T)(a * (double)x[i]);
+}
+
+template <typename S, typename D>
+__global__ __launch_bounds__(256) void convert_kernel(int64_t n, const S* __restrict__ x, D* __restrict__ y) {
+  for (int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x; i < n; i += (int64_t)gridDim.x * 256)
+    y[i] = (D)x[i];
+}
+
+static unsigned elt_grid(int64_t n) {
+  int64_t g = (n + 255) / 256;
+  if (g > 4096) g = 4096;
+  if (g < 1) g = 1;
+  return (unsigned)g;
+}
+
+hipError_t launch_axpby(int elem_size, int64_t n, double a, const void* x, double b, void* y, hipStream_t s) {
+  if (n == 0) return hipSuccess;
+  if (elem_size == 8)
+    hipLaunchKernelGGL(axpby_kernel<double>, dim3(elt_grid(n)), dim3(256), 0, s, n, a, (const double*)x, b,
+                       (double*)y);
+  else
+    hipLaunchKernelGGL(axpby_kernel<float>, dim3(elt_grid(n)), dim3(256), 0, s, n, a, (const float*)x, b,
+                       (float*)y);
+  return hipGetLastError();
+}
+
+hipError_t launch_scal(int elem_size, int64_t n, double a, void* x, hipStream_t s) {
+  if (n == 0) return hipSuccess;
+  if (elem_size == 8)
+    hipLaunchKernelGGL(scal_kernel<double>, dim3(elt_grid(n)), dim3(256), 0, s, n, a, (double*)x);
+  else
+    hipLaunchKernelGGL(scal_kernel<float>, dim3(elt_grid(n)), dim3(256), 0, s, n, a, (float*)x);
+  return hipGetLastError();
+}
+
+hipError_t launch_convert(int src_size, const void* src, int dst_size, void* dst, int64_t n, hipStream_t s) {
+  if (n == 0) return hipSuccess;
+  if (src_size == 8 && dst_size == 4)
+    hipLaunchKernelGGL((convert_kernel<double, float>), dim3(elt_grid(n)), dim3(256), 0, s, n,
+                       (const double*)src, (float*)dst);
+  else if (src_size == 4 && dst_size == 8)
+    hipLaunchKernelGGL((convert_kernel<float, double>), dim3(elt_grid(n)), dim3(256), 0, s, n,
+                       (const float*)src, (double*)dst);
+  else if (src_size == 8)
+    hipLaunchKernelGGL((convert_kernel<double, double>), dim3(elt_grid(n)), dim3(256), 0, s, n,
+                       (const double*)src, (double*)dst);
+  else
+    hipLaunchKernelGGL((convert_kernel<float, float>), dim3(elt_grid(n)), dim3(256), 0, s, n,
+                       (const float*)src, (float*)dst);
+  return hipGetLastError();
+}
+
+}  // namespace mpsh

@@ -1,0 +1,46 @@
+// Device-side views of the planner tables and the kernel launchers.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include "planner.hpp"
+
+namespace mpsh {
+
+struct DevCopy {
+  const CopyDesc* descs = nullptr;
+  const int64_t* tile_start = nullptr;
+  int ndesc = 0;
+  int64_t ntiles = 0;
+};
+
+struct DevGemm {
+  const GemmProb* probs = nullptr;
+  const GemmSeg* segs = nullptr;
+  const GemmTile* tiles = nullptr;
+  int ntiles = 0;
+};
+
+struct DevMpo {
+  const MpoSig* sigs = nullptr;
+  const MpoTerm* terms = nullptr;
+  const MpoPair* pairs = nullptr;
+  const MpoGroup* groups = nullptr;
+  const MpoInRow* inrows = nullptr;
+  const MpoOutRow* outrows = nullptr;
+  const MpoWork* work = nullptr;
+  int nterms = 0, nwork = 0, cmax = 32;
+};
+
+hipError_t launch_copy(const DevCopy& c, int elem_size, const void* src, void* dst, hipStream_t s);
+// mode 0 = f64, 1 = f32, 2 = bf16 operands from f32 storage
+hipError_t launch_gemm(const DevGemm& g, int mode, const void* A, const void* B, void* C, hipStream_t s);
+hipError_t launch_mpo(const DevMpo& d, int elem_size, const void* WL, const void* WR, void* W, const void* T1,
+                      void* T3, hipStream_t s);
+hipError_t launch_dot(int elem_size, int64_t n, const void* x, const void* y, double* out, double* partials,
+                      int max_partials, hipStream_t s);
+hipError_t launch_axpby(int elem_size, int64_t n, double a, const void* x, double b, void* y, hipStream_t s);
+hipError_t launch_scal(int elem_size, int64_t n, double a, void* x, hipStream_t s);
+hipError_t launch_convert(int src_size, const void* src, int dst_size, void* dst, int64_t n, hipStream_t s);
+
+}  // namespace mpsh

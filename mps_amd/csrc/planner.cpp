@@ -1,0 +1,975 @@
+// Host-side integer bookkeeping (see planner.hpp).  Each routine cites the
+// reference function whose sector logic it restates.
+#include "planner.hpp"
+
+#include <algorithm>
+#include <array>
+#include <map>
+#include <numeric>
+#include <set>
+#include <tuple>
+
+namespace mpsh {
+
+static int64_t prod(const int64_t* p, int n) {
+  int64_t r = 1;
+  for (int i = 0; i < n; ++i) r *= p[i];
+  return r;
+}
+
+// ---------------------------------------------------------------------------
+// Legs (src/vspace.jl)
+// ---------------------------------------------------------------------------
+int64_t Leg::dim_of(int64_t c) const {
+  auto it = std::lower_bound(charges.begin(), charges.end(), c);
+  if (it == charges.end() || *it != c) return 0;
+  return dims[it - charges.begin()];
+}
+
+int64_t Leg::total() const { return std::accumulate(dims.begin(), dims.end(), (int64_t)0); }
+
+Leg make_leg(bool isdual, const std::vector<std::pair<int64_t, int64_t>>& pairs) {
+  std::map<int64_t, int64_t> m;
+  for (auto& p : pairs) {
+    if (m.count(p.first)) throw Error(ERR_INVALID, "duplicate charges!");
+    if (p.second <= 0) throw Error(ERR_INVALID, "space dimension can only be positive!");
+    m[p.first] = p.second;
+  }
+  Leg l;
+  l.isdual = isdual;
+  for (auto& kv : m) {
+    l.charges.push_back(kv.first);
+    l.dims.push_back(kv.second);
+  }
+  return l;
+}
+
+bool isdual_pair(const Leg& a, const Leg& b) { return a.same_map(b) && a.isdual != b.isdual; }
+
+// fuse(_dual, Vs): right-to-left recursion; the fused dims are the charge-flow
+// convolution of the constituent legs (src/vspace.jl:112-138).
+Leg fuse(bool dual, const std::vector<Leg>& Vs) {
+  const size_t N = Vs.size();
+  if (N == 0) throw Error(ERR_INVALID, "fuse of no legs");
+  if (N == 1) {
+    if (dual == Vs[0].isdual) return Vs[0];
+    std::vector<std::pair<int64_t, int64_t>> p;
+    for (size_t i = 0; i < Vs[0].charges.size(); ++i) p.push_back({-Vs[0].charges[i], Vs[0].dims[i]});
+    return make_leg(!Vs[0].isdual, p);
+  }
+  if (N > 2) {
+    std::vector<Leg> head(Vs.begin(), Vs.end() - 2);
+    head.push_back(fuse(dual, {Vs[N - 2], Vs[N - 1]}));
+    return fuse(dual, head);
+  }
+  std::map<int64_t, int64_t> acc;
+  const Leg& V1 = Vs[0];
+  const Leg& V2 = Vs[1];
+  for (size_t j = 0; j < V2.charges.size(); ++j) {
+    int64_t c2 = V2.isdual ? -V2.charges[j] : V2.charges[j];
+    for (size_t i = 0; i < V1.charges.size(); ++i) {
+      int64_t c1 = V1.isdual ? -V1.charges[i] : V1.charges[i];
+      int64_t c = dual ? -(c1 + c2) : c1 + c2;
+      acc[c] += V1.dims[i] * V2.dims[j];
+    }
+  }
+  Leg l;
+  l.isdual = dual;
+  for (auto& kv : acc) {
+    l.charges.push_back(kv.first);
+    l.dims.push_back(kv.second);
+  }
+  return l;
+}
+
+// ---------------------------------------------------------------------------
+// Sector enumeration (src/sector.jl:69-90): last leg outermost, ascending raw
+// labels, recursion on the head; output is column-major sorted by construction.
+// ---------------------------------------------------------------------------
+static void enum_rec(int64_t charge, const std::vector<Leg>& legs, int N, std::vector<int64_t>& cur,
+                     std::vector<int64_t>& curz, std::vector<std::vector<int64_t>>& outs,
+                     std::vector<std::vector<int64_t>>& outz) {
+  if (N == 1) {
+    const Leg& V = legs[0];
+    int64_t c = V.isdual ? -charge : charge;
+    int64_t d = V.dim_of(c);
+    if (d > 0) {
+      cur[0] = c;
+      curz[0] = d;
+      outs.push_back(cur);
+      outz.push_back(curz);
+    }
+    return;
+  }
+  const Leg& V = legs[N - 1];
+  for (size_t i = 0; i < V.charges.size(); ++i) {
+    int64_t c = V.charges[i];
+    int64_t cc = V.isdual ? -c : c;
+    cur[N - 1] = c;
+    curz[N - 1] = V.dims[i];
+    enum_rec(charge - cc, legs, N - 1, cur, curz, outs, outz);
+  }
+}
+
+void all_sectors(int64_t charge, const std::vector<Leg>& legs, std::vector<int64_t>& sects,
+                 std::vector<int64_t>& sizes) {
+  const int N = (int)legs.size();
+  sects.clear();
+  sizes.clear();
+  if (N == 0) return;
+  std::vector<int64_t> cur(N), curz(N);
+  std::vector<std::vector<int64_t>> outs, outz;
+  enum_rec(charge, legs, N, cur, curz, outs, outz);
+  for (size_t i = 0; i < outs.size(); ++i) {
+    sects.insert(sects.end(), outs[i].begin(), outs[i].end());
+    sizes.insert(sizes.end(), outz[i].begin(), outz[i].end());
+  }
+}
+
+int64_t Struct::block_numel(int64_t b) const { return prod(size(b), rank()); }
+
+int64_t Struct::find(const std::vector<int64_t>& s) const {
+  auto it = index.find(s);
+  return it == index.end() ? -1 : it->second;
+}
+
+Struct make_struct(int64_t charge, const std::vector<Leg>& legs) {
+  Struct S;
+  S.charge = charge;
+  S.legs = legs;
+  if (legs.empty()) throw Error(ERR_INVALID, "rank-0 structure");
+  all_sectors(charge, legs, S.sectors, S.sizes);
+  const int r = (int)legs.size();
+  S.nblocks = (int64_t)S.sectors.size() / r;
+  S.offsets.assign(S.nblocks + 1, 0);
+  for (int64_t b = 0; b < S.nblocks; ++b) {
+    S.offsets[b + 1] = S.offsets[b] + prod(&S.sizes[b * r], r);
+    S.index.emplace(std::vector<int64_t>(S.sectors.begin() + b * r, S.sectors.begin() + (b + 1) * r), b);
+  }
+  return S;
+}
+
+bool same_struct(const Struct& a, const Struct& b) { return a.charge == b.charge && a.legs == b.legs; }
+
+static std::vector<int64_t> colmajor_strides(const int64_t* dims, int n) {
+  std::vector<int64_t> s(n);
+  int64_t acc = 1;
+  for (int i = 0; i < n; ++i) {
+    s[i] = acc;
+    acc *= dims[i];
+  }
+  return s;
+}
+
+// ---------------------------------------------------------------------------
+// Copy descriptors
+// ---------------------------------------------------------------------------
+void CopyPlan::add(int ndim, const int64_t* dims, const int64_t* sst, const int64_t* dst, int64_t soff,
+                   int64_t doff) {
+  struct D {
+    int64_t n, s, d;
+  };
+  std::vector<D> v;
+  for (int i = 0; i < ndim; ++i)
+    if (dims[i] > 1) v.push_back({dims[i], sst[i], dst[i]});
+  // destination-contiguous first, then merge runs contiguous on both sides
+  std::stable_sort(v.begin(), v.end(), [](const D& a, const D& b) { return a.d < b.d; });
+  std::vector<D> m;
+  for (auto& x : v) {
+    if (!m.empty() && x.d == m.back().d * m.back().n && x.s == m.back().s * m.back().n)
+      m.back().n *= x.n;
+    else
+      m.push_back(x);
+  }
+  if (m.empty()) m.push_back({1, 1, 1});
+  int tr = 0;
+  if (m[0].s != 1) {
+    for (size_t i = 1; i < m.size(); ++i)
+      if (m[i].s == 1) {
+        D t = m[i];
+        m.erase(m.begin() + i);
+        m.insert(m.begin() + 1, t);
+        tr = 1;
+        break;
+      }
+  }
+  if ((int)m.size() > MAXD) throw Error(ERR_UNSUPPORTED, "copy with more than MAXD non-mergeable dims");
+  CopyDesc c{};
+  c.ndim = (int32_t)m.size();
+  c.transpose = tr;
+  for (size_t i = 0; i < m.size(); ++i) {
+    if (m[i].n > INT32_MAX) throw Error(ERR_UNSUPPORTED, "dimension too large");
+    c.dims[i] = (int32_t)m[i].n;
+    c.src_st[i] = m[i].s;
+    c.dst_st[i] = m[i].d;
+  }
+  c.src_off = soff;
+  c.dst_off = doff;
+  descs.push_back(c);
+  int64_t e = 1;
+  for (auto& x : m) e *= x.n;
+  elems += e;
+}
+
+void CopyPlan::finalize() {
+  tile_start.assign(descs.size() + 1, 0);
+  for (size_t i = 0; i < descs.size(); ++i) {
+    const CopyDesc& c = descs[i];
+    int64_t t;
+    int64_t d0 = c.dims[0], d1 = c.ndim > 1 ? c.dims[1] : 1;
+    int64_t z = 1;
+    for (int k = 2; k < c.ndim; ++k) z *= c.dims[k];
+    if (c.transpose)
+      t = ((d0 + 63) / 64) * ((d1 + 63) / 64) * z;
+    else
+      t = ((d0 * d1 + 4095) / 4096) * z;
+    tile_start[i + 1] = tile_start[i] + t;
+  }
+  ntiles = tile_start.back();
+}
+
+std::vector<GemmTile> GemmPlan::tiles(int BM, int BN) const {
+  std::vector<std::pair<int64_t, GemmTile>> t;
+  for (size_t p = 0; p < probs.size(); ++p) {
+    const GemmProb& q = probs[p];
+    int64_t K = 0;
+    for (int s = 0; s < q.nseg; ++s) K += segs[q.seg0 + s].K;
+    for (int m0 = 0; m0 < q.M; m0 += BM)
+      for (int n0 = 0; n0 < q.N; n0 += BN) t.push_back({K, GemmTile{(int32_t)p, m0, n0, 0}});
+  }
+  std::stable_sort(t.begin(), t.end(), [](const auto& a, const auto& b) { return a.first > b.first; });
+  std::vector<GemmTile> out;
+  out.reserve(t.size());
+  for (auto& x : t) out.push_back(x.second);
+  return out;
+}
+
+// ---------------------------------------------------------------------------
+// contract index bookkeeping (src/contract.jl:82-99)
+// ---------------------------------------------------------------------------
+IndexPerm contract_index_perm(const std::vector<int>& idx) {
+  std::vector<int> rems, cons, cons_order, tof;
+  for (size_t i = 0; i < idx.size(); ++i) {
+    if (idx[i] < 0) {
+      cons.push_back((int)i + 1);
+      cons_order.push_back(idx[i]);
+    } else {
+      rems.push_back((int)i + 1);
+      tof.push_back(idx[i]);
+    }
+  }
+  std::vector<size_t> p(tof.size()), cp(cons_order.size());
+  std::iota(p.begin(), p.end(), 0);
+  std::iota(cp.begin(), cp.end(), 0);
+  std::stable_sort(p.begin(), p.end(), [&](size_t a, size_t b) { return tof[a] < tof[b]; });
+  std::stable_sort(cp.begin(), cp.end(),
+                   [&](size_t a, size_t b) { return std::abs(cons_order[a]) < std::abs(cons_order[b]); });
+  IndexPerm r;
+  for (size_t j : p) {
+    r.rems.push_back(rems[j]);
+    r.tofinals.push_back(tof[j]);
+  }
+  for (size_t j : cp) r.cons.push_back(cons[j]);
+  return r;
+}
+
+static int64_t flow(const int64_t* ch, const std::vector<bool>& dual) {
+  int64_t r = 0;
+  for (size_t i = 0; i < dual.size(); ++i) r += dual[i] ? -ch[i] : ch[i];
+  return r;
+}
+
+static bool rev_less(const std::vector<int64_t>& a, const std::vector<int64_t>& b) {
+  for (size_t i = a.size(); i-- > 0;) {
+    if (a[i] < b[i]) return true;
+    if (a[i] > b[i]) return false;
+  }
+  return false;
+}
+
+// ---------------------------------------------------------------------------
+// SymMatrix(A, rows, cols) (src/contract.jl:112-171): blocks are consumed in
+// the order (permuted key, then fused key), both stable; inside each fused
+// block, column slabs outer and row slabs inner.
+// ---------------------------------------------------------------------------
+FusedMatrix symmatrix_plan(const Struct& A, const std::vector<int>& rows, const std::vector<int>& cols,
+                           bool transposed, int64_t dst_base) {
+  const int N = A.rank();
+  std::vector<int> idxperm(rows);
+  idxperm.insert(idxperm.end(), cols.begin(), cols.end());
+  {
+    std::vector<int> s(idxperm);
+    std::sort(s.begin(), s.end());
+    for (int i = 0; i < (int)s.size(); ++i)
+      if ((int)s.size() != N || s[i] != i + 1)
+        throw Error(ERR_INVALID, "Incorrect index set for conversion to SymMatrix!");
+  }
+  if (rows.empty()) throw Error(ERR_INVALID, "empty row for matrix not allowed!");
+  if (cols.empty()) throw Error(ERR_INVALID, "empty col for matrix not allowed!");
+  const int n = (int)rows.size();
+  std::vector<bool> rowdual, coldual;
+  std::vector<Leg> rlegs, clegs;
+  for (int i : rows) {
+    rowdual.push_back(A.legs[i - 1].isdual);
+    rlegs.push_back(A.legs[i - 1]);
+  }
+  for (int i : cols) {
+    coldual.push_back(A.legs[i - 1].isdual);
+    clegs.push_back(A.legs[i - 1]);
+  }
+  const int64_t nb = A.nblocks;
+  std::vector<std::vector<int64_t>> cs(nb);
+  for (int64_t b = 0; b < nb; ++b) {
+    cs[b].resize(N);
+    for (int t = 0; t < N; ++t) cs[b][t] = A.sector(b)[idxperm[t] - 1];
+  }
+  std::vector<int64_t> sperm1(nb);
+  std::iota(sperm1.begin(), sperm1.end(), 0);
+  std::stable_sort(sperm1.begin(), sperm1.end(), [&](int64_t a, int64_t b) { return rev_less(cs[a], cs[b]); });
+  std::vector<std::vector<int64_t>> fs(nb);
+  for (int64_t i = 0; i < nb; ++i) {
+    const auto& sct = cs[sperm1[i]];
+    fs[i] = {flow(sct.data(), rowdual), -flow(sct.data() + n, coldual)};
+  }
+  std::vector<int64_t> sperm2(nb);
+  std::iota(sperm2.begin(), sperm2.end(), 0);
+  std::stable_sort(sperm2.begin(), sperm2.end(), [&](int64_t a, int64_t b) { return rev_less(fs[a], fs[b]); });
+  std::vector<int64_t> semits(nb);
+  for (int64_t j = 0; j < nb; ++j) semits[j] = sperm1[sperm2[j]];
+
+  FusedMatrix FM;
+  FM.F = make_struct(A.charge, {fuse(false, rlegs), fuse(true, clegs)});
+  FM.k_contig_rows = transposed;
+  int64_t ptr = 0;
+  std::vector<int64_t> pd(N), sst(N), dst(N);
+  for (int64_t f = 0; f < FM.F.nblocks; ++f) {
+    const int64_t rlim = FM.F.size(f)[0], clim = FM.F.size(f)[1];
+    const int64_t foff = FM.F.offsets[f];
+    int64_t pc = 0;
+    while (pc < clim) {
+      int64_t pr = 0, fdc = 0;
+      while (pr < rlim) {
+        if (ptr >= nb) throw Error(ERR_SECTOR_MISMATCH, "SymMatrix: blocks exhausted");
+        const int64_t b = semits[ptr];
+        const int64_t* bz = A.size(b);
+        std::vector<int64_t> st = colmajor_strides(bz, N);
+        for (int t = 0; t < N; ++t) {
+          pd[t] = bz[idxperm[t] - 1];
+          sst[t] = st[idxperm[t] - 1];
+        }
+        const int64_t fdr = prod(pd.data(), n);
+        fdc = prod(pd.data() + n, N - n);
+        if (pr + fdr > rlim || pc + fdc > clim) throw Error(ERR_SECTOR_MISMATCH, "SymMatrix: slab overflow");
+        int64_t acc = 1;
+        for (int t = 0; t < n; ++t) {
+          dst[t] = transposed ? acc * clim : acc;
+          acc *= pd[t];
+        }
+        acc = 1;
+        for (int t = n; t < N; ++t) {
+          dst[t] = transposed ? acc : acc * rlim;
+          acc *= pd[t];
+        }
+        const int64_t doff = dst_base + foff + (transposed ? pr * clim + pc : pr + pc * rlim);
+        FM.gather.add(N, pd.data(), sst.data(), dst.data(), A.offsets[b], doff);
+        pr += fdr;
+        ++ptr;
+      }
+      pc += fdc;
+    }
+  }
+  if (ptr != nb) throw Error(ERR_SECTOR_MISMATCH, "SymMatrix: blocks left over");
+  FM.gather.finalize();
+  return FM;
+}
+
+static std::vector<int> invperm(const std::vector<int>& p) {
+  std::vector<int> r(p.size());
+  for (size_t i = 0; i < p.size(); ++i) r[p[i] - 1] = (int)i + 1;
+  return r;
+}
+
+// ---------------------------------------------------------------------------
+// contract (src/contract.jl:11-47 with :234-298 and :187-224)
+// ---------------------------------------------------------------------------
+ContractPlan contract_plan(const Struct& A, const std::vector<int>& idxA, const Struct& B,
+                           const std::vector<int>& idxB) {
+  if ((int)idxA.size() != A.rank() || (int)idxB.size() != B.rank())
+    throw Error(ERR_INVALID, "index set length differs from tensor rank");
+  IndexPerm pa = contract_index_perm(idxA), pb = contract_index_perm(idxB);
+  if (pa.cons.size() != pb.cons.size()) throw Error(ERR_INVALID, "contracted label counts differ");
+  if (pa.rems.empty() || pb.rems.empty())
+    throw Error(ERR_UNSUPPORTED, "rank-1 output branches of contract (src/contract.jl:24-42)");
+  ContractPlan P;
+  P.A = A;
+  P.B = B;
+  P.fa_off = 0;
+  P.FA = symmatrix_plan(A, pa.rems, pa.cons, true, P.fa_off);
+  P.fb_off = P.FA.F.numel();
+  P.FB = symmatrix_plan(B, pb.cons, pb.rems, false, P.fb_off);
+  if (!isdual_pair(P.FA.F.legs[1], P.FB.F.legs[0])) throw Error(ERR_SPACE_MISMATCH, "not contractible!");
+  const int64_t charge = A.charge + B.charge;
+  P.FC = make_struct(charge, {P.FA.F.legs[0], P.FB.F.legs[1]});
+  P.fc_off = P.fb_off + P.FB.F.numel();
+  P.ws_elems = P.fc_off + P.FC.numel();
+  const Leg& a0 = P.FA.F.legs[0];
+  const Leg& a1 = P.FA.F.legs[1];
+  const Leg& b1 = P.FB.F.legs[1];
+  for (int64_t f = 0; f < P.FC.nblocks; ++f) {
+    const int64_t c1 = P.FC.sector(f)[0], c2 = P.FC.sector(f)[1];
+    const int64_t M = P.FC.size(f)[0], Nn = P.FC.size(f)[1];
+    const int64_t _c1 = a0.isdual ? -c1 : c1;
+    const int64_t _c2 = b1.isdual ? -c2 : c2;
+    std::vector<int64_t> sA, sB;
+    if (a1.isdual) {
+      sA = {c1, -A.charge + _c1};
+      sB = {B.charge - _c2, c2};
+    } else {
+      sA = {c1, A.charge - _c1};
+      sB = {-B.charge + _c2, c2};
+    }
+    int64_t ia = P.FA.F.find(sA), ib = P.FB.F.find(sB);
+    P.gemm.add_prob(M, Nn, P.FC.offsets[f], M);
+    if (ia >= 0 && ib >= 0) {
+      const int64_t K = P.FA.F.size(ia)[1];
+      if (P.FB.F.size(ib)[0] != K) throw Error(ERR_SECTOR_MISMATCH, "fused contraction dims differ");
+      P.gemm.add_seg(K, P.FA.F.offsets[ia], K, P.FB.F.offsets[ib], K);
+    }
+  }
+  // split + final permutation in one scatter
+  std::vector<Leg> rlegs, clegs, pre;
+  for (int i : pa.rems) rlegs.push_back(A.legs[i - 1]);
+  for (int i : pb.rems) clegs.push_back(B.legs[i - 1]);
+  pre = rlegs;
+  pre.insert(pre.end(), clegs.begin(), clegs.end());
+  std::vector<int> tof(pa.tofinals);
+  tof.insert(tof.end(), pb.tofinals.begin(), pb.tofinals.end());
+  const int R = (int)pre.size();
+  {
+    std::vector<int> s(tof);
+    std::sort(s.begin(), s.end());
+    for (int i = 0; i < R; ++i)
+      if (s[i] != i + 1) throw Error(ERR_INVALID, "output labels must be 1..K");
+  }
+  std::vector<int> Pm = invperm(tof);  // final leg k = pre leg Pm[k]
+  std::vector<Leg> fin;
+  for (int k = 0; k < R; ++k) fin.push_back(pre[Pm[k] - 1]);
+  P.C = make_struct(charge, fin);
+  const int nA = (int)rlegs.size();
+  std::vector<char> covered(P.C.nblocks, 0);
+  std::vector<int64_t> rs, rz, cs_, cz;
+  std::vector<int64_t> dims(R), sst(R), dstv(R), prez(R), key(R);
+  for (int64_t f = 0; f < P.FC.nblocks; ++f) {
+    int64_t c1 = P.FC.sector(f)[0], c2 = P.FC.sector(f)[1];
+    c1 = P.FC.legs[0].isdual ? -c1 : c1;
+    c2 = P.FC.legs[1].isdual ? -c2 : c2;
+    const int64_t Mf = P.FC.size(f)[0];
+    all_sectors(c1, rlegs, rs, rz);
+    all_sectors(c2, clegs, cs_, cz);
+    const int nr = nA, nc = R - nA;
+    const int64_t npr = nr ? (int64_t)rs.size() / nr : 0, npc = nc ? (int64_t)cs_.size() / nc : 0;
+    int64_t pc = 0;
+    for (int64_t j = 0; j < npc; ++j) {
+      const int64_t csl = prod(&cz[j * nc], nc);
+      int64_t pr = 0;
+      for (int64_t i = 0; i < npr; ++i) {
+        const int64_t rsl = prod(&rz[i * nr], nr);
+        for (int t = 0; t < nr; ++t) prez[t] = rz[i * nr + t];
+        for (int t = 0; t < nc; ++t) prez[nr + t] = cz[j * nc + t];
+        std::vector<int64_t> prekey(R);
+        for (int t = 0; t < nr; ++t) prekey[t] = rs[i * nr + t];
+        for (int t = 0; t < nc; ++t) prekey[nr + t] = cs_[j * nc + t];
+        for (int k = 0; k < R; ++k) key[k] = prekey[Pm[k] - 1];
+        const int64_t cb = P.C.find(key);
+        if (cb < 0) throw Error(ERR_SECTOR_MISMATCH, "split produced a sector outside the output space");
+        if (covered[cb]) throw Error(ERR_SECTOR_MISMATCH, "split produced a sector twice");
+        covered[cb] = 1;
+        // pre-permutation strides inside the fused C block (column-major, ld Mf)
+        std::vector<int64_t> pst(R);
+        int64_t acc = 1;
+        for (int t = 0; t < nr; ++t) {
+          pst[t] = acc;
+          acc *= prez[t];
+        }
+        acc = Mf;
+        for (int t = nr; t < R; ++t) {
+          pst[t] = acc;
+          acc *= prez[t];
+        }
+        acc = 1;
+        for (int k = 0; k < R; ++k) {
+          dims[k] = prez[Pm[k] - 1];
+          sst[k] = pst[Pm[k] - 1];
+          dstv[k] = acc;
+          acc *= dims[k];
+        }
+        P.scatter.add(R, dims.data(), sst.data(), dstv.data(), P.fc_off + P.FC.offsets[f] + pr + pc * Mf,
+                      P.C.offsets[cb]);
+        pr += rsl;
+      }
+      pc += csl;
+    }
+  }
+  for (int64_t b = 0; b < P.C.nblocks; ++b)
+    if (!covered[b]) throw Error(ERR_SECTOR_MISMATCH, "output sector not produced by the split");
+  P.scatter.finalize();
+  return P;
+}
+
+// ---------------------------------------------------------------------------
+// Re-legging (src/symreleg.jl)
+// ---------------------------------------------------------------------------
+RelegPlan permutelegs_plan(const Struct& A, const std::vector<int>& perm) {
+  const int N = A.rank();
+  {
+    std::vector<int> s(perm);
+    std::sort(s.begin(), s.end());
+    if ((int)s.size() != N) throw Error(ERR_INVALID, "Not a valid permutation!");
+    for (int i = 0; i < N; ++i)
+      if (s[i] != i + 1) throw Error(ERR_INVALID, "Not a valid permutation!");
+  }
+  RelegPlan P;
+  P.in = A;
+  std::vector<Leg> legs;
+  for (int k = 0; k < N; ++k) legs.push_back(A.legs[perm[k] - 1]);
+  P.out = make_struct(A.charge, legs);
+  std::vector<int64_t> dims(N), sst(N), dst(N), key(N);
+  std::vector<char> cov(P.out.nblocks, 0);
+  for (int64_t b = 0; b < A.nblocks; ++b) {
+    std::vector<int64_t> st = colmajor_strides(A.size(b), N);
+    int64_t acc = 1;
+    for (int k = 0; k < N; ++k) {
+      key[k] = A.sector(b)[perm[k] - 1];
+      dims[k] = A.size(b)[perm[k] - 1];
+      sst[k] = st[perm[k] - 1];
+      dst[k] = acc;
+      acc *= dims[k];
+    }
+    int64_t ob = P.out.find(key);
+    if (ob < 0 || cov[ob]) throw Error(ERR_SECTOR_MISMATCH, "permutelegs: sector mismatch");
+    cov[ob] = 1;
+    P.copy.add(N, dims.data(), sst.data(), dst.data(), A.offsets[b], P.out.offsets[ob]);
+  }
+  for (char c : cov)
+    if (!c) throw Error(ERR_SECTOR_MISMATCH, "permutelegs: sector mismatch");
+  P.copy.finalize();
+  return P;
+}
+
+RelegPlan fuselegs_plan(const Struct& A, int l, int n, bool dual) {
+  const int N = A.rank();
+  if (!(0 < l && l < N + 2 - n) || n < 1) throw Error(ERR_INVALID, "fuselegs: leg range out of bounds");
+  const int lo = l - 1, hi = l - 1 + n;
+  std::vector<bool> dualinfo;
+  std::vector<Leg> fl;
+  for (int t = lo; t < hi; ++t) {
+    dualinfo.push_back(A.legs[t].isdual);
+    fl.push_back(A.legs[t]);
+  }
+  const int64_t nb = A.nblocks;
+  std::vector<std::vector<int64_t>> fsects(nb);
+  for (int64_t b = 0; b < nb; ++b) {
+    const int64_t* s = A.sector(b);
+    int64_t fc = flow(s + lo, dualinfo);
+    if (dual) fc = -fc;
+    std::vector<int64_t> k(s, s + lo);
+    k.push_back(fc);
+    k.insert(k.end(), s + hi, s + N);
+    fsects[b] = k;
+  }
+  std::vector<int64_t> sperm(nb);
+  std::iota(sperm.begin(), sperm.end(), 0);
+  std::stable_sort(sperm.begin(), sperm.end(), [&](int64_t a, int64_t b) { return rev_less(fsects[a], fsects[b]); });
+  std::vector<Leg> legs(A.legs.begin(), A.legs.begin() + lo);
+  legs.push_back(fuse(dual, fl));
+  legs.insert(legs.end(), A.legs.begin() + hi, A.legs.end());
+  RelegPlan P;
+  P.in = A;
+  P.out = make_struct(A.charge, legs);
+  const int No = P.out.rank();
+  int64_t ptr = 0;
+  std::vector<int64_t> sst(N), dst(N);
+  for (int64_t ob = 0; ob < P.out.nblocks; ++ob) {
+    std::vector<int64_t> ost = colmajor_strides(P.out.size(ob), No);
+    const int64_t sizel = P.out.size(ob)[lo];
+    int64_t p = 0;
+    while (p < sizel) {
+      if (ptr >= nb) throw Error(ERR_SECTOR_MISMATCH, "fuselegs: blocks exhausted");
+      const int64_t b = sperm[ptr];
+      const int64_t* bz = A.size(b);
+      const int64_t fd = prod(bz + lo, n);
+      std::vector<int64_t> st = colmajor_strides(bz, N);
+      for (int t = 0; t < N; ++t) {
+        sst[t] = st[t];
+        if (t < lo)
+          dst[t] = ost[t];
+        else if (t >= hi)
+          dst[t] = ost[t - n + 1];
+        else {
+          int64_t acc = ost[lo];
+          for (int u = lo; u < t; ++u) acc *= bz[u];
+          dst[t] = acc;
+        }
+      }
+      P.copy.add(N, bz, sst.data(), dst.data(), A.offsets[b], P.out.offsets[ob] + p * ost[lo]);
+      p += fd;
+      ++ptr;
+    }
+  }
+  if (ptr != nb) throw Error(ERR_SECTOR_MISMATCH, "fuselegs: blocks left over");
+  P.copy.finalize();
+  return P;
+}
+
+RelegPlan splitleg_plan(const Struct& A, int l, const std::vector<Leg>& spaces) {
+  const int N = A.rank();
+  if (!(0 < l && l <= N)) throw Error(ERR_INVALID, "splitleg: leg out of range");
+  const int M = (int)spaces.size();
+  if (M == 0) throw Error(ERR_INVALID, "splitleg: no spaces");
+  std::vector<Leg> legs(A.legs.begin(), A.legs.begin() + (l - 1));
+  legs.insert(legs.end(), spaces.begin(), spaces.end());
+  legs.insert(legs.end(), A.legs.begin() + l, A.legs.end());
+  RelegPlan P;
+  P.in = A;
+  if (M == 1 && A.legs[l - 1] != spaces[0]) throw Error(ERR_SPACE_MISMATCH, "SpaceMismatch in splitleg");
+  P.out = make_struct(A.charge, legs);
+  const int No = P.out.rank();
+  std::vector<char> cov(P.out.nblocks, 0);
+  std::map<int64_t, std::pair<std::vector<int64_t>, std::vector<int64_t>>> cache;
+  std::vector<int64_t> dims(No), sst(No), dst(No), key(No);
+  for (int64_t b = 0; b < A.nblocks; ++b) {
+    const int64_t* s = A.sector(b);
+    const int64_t* bz = A.size(b);
+    const int64_t charge = A.legs[l - 1].isdual ? -s[l - 1] : s[l - 1];
+    if (!cache.count(charge)) {
+      std::vector<int64_t> ps, pz;
+      all_sectors(charge, spaces, ps, pz);
+      cache[charge] = {ps, pz};
+    }
+    const auto& ps = cache[charge].first;
+    const auto& pz = cache[charge].second;
+    std::vector<int64_t> st = colmajor_strides(bz, N);
+    int64_t pivot = 0;
+    const int64_t np = (int64_t)ps.size() / M;
+    for (int64_t j = 0; j < np; ++j) {
+      const int64_t sl = prod(&pz[j * M], M);
+      if (pivot + sl > bz[l - 1]) throw Error(ERR_SECTOR_MISMATCH, "splitleg: patterns exceed the fused dim");
+      int k = 0;
+      for (int t = 0; t < l - 1; ++t, ++k) {
+        key[k] = s[t];
+        dims[k] = bz[t];
+        sst[k] = st[t];
+      }
+      int64_t acc = st[l - 1];
+      for (int u = 0; u < M; ++u, ++k) {
+        key[k] = ps[j * M + u];
+        dims[k] = pz[j * M + u];
+        sst[k] = acc;
+        acc *= pz[j * M + u];
+      }
+      for (int t = l; t < N; ++t, ++k) {
+        key[k] = s[t];
+        dims[k] = bz[t];
+        sst[k] = st[t];
+      }
+      int64_t ob = P.out.find(key);
+      if (ob < 0 || cov[ob]) throw Error(ERR_SECTOR_MISMATCH, "splitleg: sector mismatch");
+      cov[ob] = 1;
+      acc = 1;
+      for (int t = 0; t < No; ++t) {
+        dst[t] = acc;
+        acc *= dims[t];
+      }
+      P.copy.add(No, dims.data(), sst.data(), dst.data(), A.offsets[b] + pivot * st[l - 1], P.out.offsets[ob]);
+      pivot += sl;
+    }
+  }
+  for (char c : cov)
+    if (!c) throw Error(ERR_SECTOR_MISMATCH, "splitleg: output sector not produced");
+  P.copy.finalize();
+  return P;
+}
+
+RelegPlan symmatrix_releg(const Struct& A, const std::vector<int>& rows, const std::vector<int>& cols) {
+  RelegPlan P;
+  P.in = A;
+  FusedMatrix F = symmatrix_plan(A, rows, cols, false, 0);
+  P.out = F.F;
+  P.copy = F.gather;
+  return P;
+}
+
+// ---------------------------------------------------------------------------
+// Fused H_eff plan
+// ---------------------------------------------------------------------------
+typedef std::vector<int64_t> Key;
+
+static std::unordered_map<int64_t, std::vector<int64_t>> index_by_leg(const Struct& S, int leg) {
+  std::unordered_map<int64_t, std::vector<int64_t>> m;
+  for (int64_t b = 0; b < S.nblocks; ++b) m[S.sector(b)[leg]].push_back(b);
+  return m;
+}
+
+static void need_pair(const Leg& a, const Leg& b, const char* what) {
+  if (!isdual_pair(a, b)) throw Error(ERR_SPACE_MISMATCH, std::string("not contractible! ") + what);
+}
+
+HeffPlan heff_plan(const Struct& v, const Struct& envL, const Struct& envR, const Struct& WL, const Struct& WR) {
+  if (v.rank() != 4 || envL.rank() != 3 || envR.rank() != 3 || WL.rank() != 4 || WR.rank() != 4)
+    throw Error(ERR_INVALID, "heff: ranks must be v 4, envL 3, envR 3, WL 4, WR 4");
+  need_pair(envL.legs[0], v.legs[0], "(envL ket, v left)");
+  need_pair(envL.legs[1], WL.legs[0], "(envL mpo, WL left)");
+  need_pair(v.legs[1], WL.legs[3], "(v p1, WL p_in)");
+  need_pair(WL.legs[2], WR.legs[0], "(WL right, WR left)");
+  need_pair(v.legs[2], WR.legs[3], "(v p2, WR p_in)");
+  need_pair(WR.legs[2], envR.legs[1], "(WR right, envR mpo)");
+  need_pair(v.legs[3], envR.legs[0], "(v right, envR ket)");
+  HeffPlan H;
+  H.v = v;
+  H.envL = envL;
+  H.envR = envR;
+  H.WL = WL;
+  H.WR = WR;
+  const int64_t charge = envL.charge + v.charge + WL.charge + WR.charge + envR.charge;
+  H.out = make_struct(charge, {envL.legs[2], WL.legs[1], WR.legs[1], envR.legs[2]});
+
+  // reference-order cost from the reference's own four contracts
+  {
+    ContractPlan P1 = contract_plan(envL, {-1, 2, 1}, v, {-1, 3, 4, 5});
+    ContractPlan P2 = contract_plan(P1.C, {1, -1, -2, 4, 5}, WL, {-1, 2, 3, -2});
+    ContractPlan P3 = contract_plan(P2.C, {1, 2, -1, -2, 5}, WR, {-1, 3, 4, -2});
+    ContractPlan P4 = contract_plan(P3.C, {1, 2, 3, -1, -2}, envR, {-2, -1, 4});
+    const ContractPlan* Ps[4] = {&P1, &P2, &P3, &P4};
+    for (int s = 0; s < 4; ++s) {
+      H.step_flops[s] = Ps[s]->gemm.flops;
+      H.step_elems[s] = Ps[s]->A.numel() + Ps[s]->B.numel() + Ps[s]->C.numel();
+      H.ref_flops += H.step_flops[s];
+      H.ref_elems += H.step_elems[s];
+    }
+    if (!same_struct(P4.C, H.out) || P4.C.sectors != H.out.sectors)
+      throw Error(ERR_SECTOR_MISMATCH, "heff: output structure differs from the contraction chain");
+  }
+
+  // ---- step 1: T1 blocks = matched (envL, v) pairs, ordered by pixel group (b, r)
+  struct T1B {
+    Key key;  // (b, w, s1, s2, r)
+    int64_t e, x, off, db, Dw, d1, d2, dR, dk;
+  };
+  std::vector<T1B> t1;
+  {
+    auto vby = index_by_leg(v, 0);
+    for (int64_t e = 0; e < envL.nblocks; ++e) {
+      const int64_t* es = envL.sector(e);
+      auto it = vby.find(es[0]);
+      if (it == vby.end()) continue;
+      for (int64_t x : it->second) {
+        const int64_t* xs = v.sector(x);
+        T1B t;
+        t.key = {es[2], es[1], xs[1], xs[2], xs[3]};
+        t.e = e;
+        t.x = x;
+        t.db = envL.size(e)[2];
+        t.Dw = envL.size(e)[1];
+        t.dk = envL.size(e)[0];
+        t.d1 = v.size(x)[1];
+        t.d2 = v.size(x)[2];
+        t.dR = v.size(x)[3];
+        t1.push_back(t);
+      }
+    }
+    std::stable_sort(t1.begin(), t1.end(), [](const T1B& a, const T1B& b) {
+      auto ka = std::make_tuple(a.key[0], a.key[4], a.key[1], a.key[2], a.key[3]);
+      auto kb = std::make_tuple(b.key[0], b.key[4], b.key[1], b.key[2], b.key[3]);
+      return ka < kb;
+    });
+    int64_t off = 0;
+    for (auto& t : t1) {
+      t.off = off;
+      off += t.d1 * t.d2 * t.dR * t.Dw * t.db;
+      const int64_t M = t.d1 * t.d2 * t.dR, N = t.Dw * t.db;
+      H.g1.add_prob(M, N, t.off, M);
+      H.g1.add_seg(t.dk, v.offsets[t.x], t.dk, envL.offsets[t.e], t.dk);
+    }
+    H.t1_elems = off;
+  }
+
+  // ---- steps 2+3: pixel groups (b, r); in rows = T1 blocks, out rows = T3 blocks
+  struct T3B {
+    Key key;  // (b, s1o, s2o, x, r)
+    int64_t off, db, d1o, d2o, Dx, dR;
+  };
+  std::map<Key, T3B> t3map;  // ordered by key for deterministic allocation
+  auto wl_by = [&]() {
+    std::map<std::pair<int64_t, int64_t>, std::vector<int64_t>> m;  // (w, s1) -> WL blocks
+    for (int64_t b = 0; b < WL.nblocks; ++b) m[{WL.sector(b)[0], WL.sector(b)[3]}].push_back(b);
+    return m;
+  }();
+  auto wr_by = [&]() {
+    std::map<std::pair<int64_t, int64_t>, std::vector<int64_t>> m;  // (m, s2) -> WR blocks
+    for (int64_t b = 0; b < WR.nblocks; ++b) m[{WR.sector(b)[0], WR.sector(b)[3]}].push_back(b);
+    return m;
+  }();
+  std::map<std::vector<int64_t>, int32_t> sigindex;
+  size_t gi0 = 0;
+  int64_t w_off = 0, t3_off = 0;
+  while (gi0 < t1.size()) {
+    size_t gi1 = gi0;
+    while (gi1 < t1.size() && t1[gi1].key[0] == t1[gi0].key[0] && t1[gi1].key[4] == t1[gi0].key[4]) ++gi1;
+    const int64_t b = t1[gi0].key[0], r = t1[gi0].key[4];
+    const int64_t db = t1[gi0].db, dR = t1[gi0].dR;
+    // in types in group order (already sorted by w, s1, s2); collect out types
+    std::set<Key> outtypes;  // (s1o, s2o, x)
+    std::map<Key, std::array<int64_t, 3>> od;
+    for (size_t i = gi0; i < gi1; ++i) {
+      const Key& k = t1[i].key;
+      auto itl = wl_by.find({k[1], k[2]});
+      if (itl == wl_by.end()) continue;
+      for (int64_t wl : itl->second) {
+        const int64_t m = WL.sector(wl)[2], s1o = WL.sector(wl)[1];
+        auto itr = wr_by.find({m, k[3]});
+        if (itr == wr_by.end()) continue;
+        for (int64_t wr : itr->second) {
+          Key ot = {s1o, WR.sector(wr)[1], WR.sector(wr)[2]};
+          outtypes.insert(ot);
+          od[ot] = {WR.size(wr)[2], WL.size(wl)[1], WR.size(wr)[1]};
+        }
+      }
+    }
+    if (outtypes.empty()) {
+      gi0 = gi1;
+      continue;
+    }
+    // signature = in types + out types
+    Key sigkey;
+    for (size_t i = gi0; i < gi1; ++i) sigkey.insert(sigkey.end(), {t1[i].key[1], t1[i].key[2], t1[i].key[3]});
+    sigkey.push_back(INT64_MIN);
+    for (auto& ot : outtypes) sigkey.insert(sigkey.end(), ot.begin(), ot.end());
+    int32_t sig;
+    auto its = sigindex.find(sigkey);
+    std::vector<int64_t> col0s, row0s;
+    int64_t C = 0, R = 0;
+    for (size_t i = gi0; i < gi1; ++i) {
+      col0s.push_back(C);
+      C += t1[i].Dw * t1[i].d1 * t1[i].d2;
+    }
+    for (auto& ot : outtypes) {
+      row0s.push_back(R);
+      R += od[ot][0] * od[ot][1] * od[ot][2];
+    }
+    if (its == sigindex.end()) {
+      sig = (int32_t)H.sigs.size();
+      sigindex[sigkey] = sig;
+      H.sigs.push_back(MpoSig{(int32_t)R, (int32_t)C, w_off});
+      w_off += R * C;
+      H.maxR = std::max<int>(H.maxR, (int)R);
+      H.maxC = std::max<int>(H.maxC, (int)C);
+      int oi = 0;
+      for (auto& ot : outtypes) {
+        for (size_t i = gi0; i < gi1; ++i) {
+          const Key& k = t1[i].key;
+          MpoTerm tm{};
+          tm.sig = sig;
+          tm.row0 = (int32_t)row0s[oi];
+          tm.col0 = (int32_t)col0s[i - gi0];
+          tm.Dx = (int32_t)od[ot][0];
+          tm.d1o = (int32_t)od[ot][1];
+          tm.d2o = (int32_t)od[ot][2];
+          tm.Dw = (int32_t)t1[i].Dw;
+          tm.d1i = (int32_t)t1[i].d1;
+          tm.d2i = (int32_t)t1[i].d2;
+          tm.R = tm.Dx * tm.d1o * tm.d2o;
+          tm.C = tm.Dw * tm.d1i * tm.d2i;
+          tm.pair0 = (int32_t)H.pairs.size();
+          tm.npair = 0;
+          auto itl = wl_by.find({k[1], k[2]});
+          if (itl != wl_by.end())
+            for (int64_t wl : itl->second) {
+              if (WL.sector(wl)[1] != ot[0]) continue;
+              const int64_t m = WL.sector(wl)[2];
+              auto itr = wr_by.find({m, k[3]});
+              if (itr == wr_by.end()) continue;
+              for (int64_t wr : itr->second) {
+                if (WR.sector(wr)[1] != ot[1] || WR.sector(wr)[2] != ot[2]) continue;
+                H.pairs.push_back(MpoPair{WL.offsets[wl], WR.offsets[wr], (int32_t)WL.size(wl)[2], 0});
+                tm.npair++;
+              }
+            }
+          H.terms.push_back(tm);
+        }
+        ++oi;
+      }
+    } else {
+      sig = its->second;
+    }
+    MpoGroup g{};
+    g.sig = sig;
+    g.dR = (int32_t)dR;
+    g.db = (int32_t)db;
+    g.in0 = (int32_t)H.inrows.size();
+    g.out0 = (int32_t)H.outrows.size();
+    g.npix = db * dR;
+    for (size_t i = gi0; i < gi1; ++i) {
+      const T1B& t = t1[i];
+      // column order c = jw + Dw*(i1 + d1*i2)
+      for (int64_t c = 0; c < t.Dw * t.d1 * t.d2; ++c) {
+        const int64_t jw = c % t.Dw, i1 = (c / t.Dw) % t.d1, i2 = c / (t.Dw * t.d1);
+        H.inrows.push_back(MpoInRow{t.off + i1 + t.d1 * i2 + t.d1 * t.d2 * t.dR * jw, t.d1 * t.d2,
+                                    t.d1 * t.d2 * t.dR * t.Dw});
+      }
+    }
+    for (auto& ot : outtypes) {
+      Key tk = {b, ot[0], ot[1], ot[2], r};
+      T3B t3;
+      t3.key = tk;
+      t3.db = db;
+      t3.dR = dR;
+      t3.Dx = od[ot][0];
+      t3.d1o = od[ot][1];
+      t3.d2o = od[ot][2];
+      t3.off = t3_off;
+      t3_off += dR * t3.Dx * db * t3.d1o * t3.d2o;
+      t3map[tk] = t3;
+      for (int64_t rr = 0; rr < t3.Dx * t3.d1o * t3.d2o; ++rr) {
+        const int64_t jx = rr % t3.Dx, i1 = (rr / t3.Dx) % t3.d1o, i2 = rr / (t3.Dx * t3.d1o);
+        H.outrows.push_back(MpoOutRow{t3.off + dR * jx + dR * t3.Dx * db * (i1 + t3.d1o * i2), dR * t3.Dx});
+      }
+    }
+    const int32_t gidx = (int32_t)H.groups.size();
+    H.groups.push_back(g);
+    for (int64_t p0 = 0; p0 < g.npix; p0 += 256) H.work.push_back(MpoWork{gidx, (int32_t)p0});
+    H.mpo_flops += 2 * R * C * g.npix;
+    gi0 = gi1;
+  }
+  H.t3_elems = t3_off;
+  H.w_elems = w_off;
+
+  // ---- step 4: out blocks, K segments over envR blocks (r, x, r')
+  {
+    auto eby = index_by_leg(envR, 2);
+    for (int64_t o = 0; o < H.out.nblocks; ++o) {
+      const int64_t* os = H.out.sector(o);
+      const int64_t* oz = H.out.size(o);
+      const int64_t M = oz[0] * oz[1] * oz[2], N = oz[3];
+      H.g4.add_prob(M, N, H.out.offsets[o], M);
+      auto it = eby.find(os[3]);
+      if (it == eby.end()) continue;
+      for (int64_t f : it->second) {
+        const int64_t* fs = envR.sector(f);
+        Key tk = {os[0], os[1], os[2], fs[1], fs[0]};
+        auto t3 = t3map.find(tk);
+        if (t3 == t3map.end()) continue;
+        const int64_t K = envR.size(f)[0] * envR.size(f)[1];
+        if (t3->second.dR != envR.size(f)[0] || t3->second.Dx != envR.size(f)[1])
+          throw Error(ERR_SECTOR_MISMATCH, "heff: T3/envR dims differ");
+        H.g4.add_seg(K, t3->second.off, K, envR.offsets[f], K);
+      }
+    }
+  }
+  H.t1_off = 0;
+  H.t3_off = H.t1_elems;
+  H.w_off = H.t3_off + H.t3_elems;
+  H.ws_elems = H.w_off + H.w_elems;
+  return H;
+}
+
+}  // namespace mpsh

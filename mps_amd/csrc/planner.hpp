@@ -1,0 +1,258 @@
+// Host-side integer bookkeeping for the U(1) block-sparse contraction engine.
+//
+// Everything here is exact integer work done once per tensor structure and
+// cached by the caller: sector enumeration in column-major order, fused
+// spaces, block-pair matching, and the descriptor tables the HIP kernels
+// consume (gather/scatter copies, grouped-GEMM problems, MPO groups).
+// No device calls live in this file; it builds with plain g++.
+#pragma once
+
+#include <cstdint>
+#include <stdexcept>
+#include <string>
+#include <unordered_map>
+#include <vector>
+
+namespace mpsh {
+
+enum Status : int {
+  OK = 0,
+  ERR_INVALID = 1,
+  ERR_SPACE_MISMATCH = 2,
+  ERR_SECTOR_MISMATCH = 3,
+  ERR_NOT_SIMILAR = 4,
+  ERR_NONFINITE = 5,
+  ERR_DEVICE = 6,
+  ERR_OOM = 7,
+  ERR_UNSUPPORTED = 8,
+};
+
+struct Error : std::runtime_error {
+  int code;
+  Error(int c, const std::string& m) : std::runtime_error(m), code(c) {}
+};
+
+// One U(1) leg: sorted charge -> dim map plus the dual flag
+// (reference src/vspace.jl:5-18).
+struct Leg {
+  bool isdual = false;
+  std::vector<int64_t> charges;
+  std::vector<int64_t> dims;
+  int64_t dim_of(int64_t c) const;  // 0 when absent
+  int64_t total() const;
+  bool same_map(const Leg& o) const { return charges == o.charges && dims == o.dims; }
+  bool operator==(const Leg& o) const { return isdual == o.isdual && same_map(o); }
+  bool operator!=(const Leg& o) const { return !(*this == o); }
+  Leg dual() const {
+    Leg l = *this;
+    l.isdual = !isdual;
+    return l;
+  }
+};
+
+Leg make_leg(bool isdual, const std::vector<std::pair<int64_t, int64_t>>& pairs);
+bool isdual_pair(const Leg& a, const Leg& b);  // reference src/vspace.jl:24-25
+Leg fuse(bool dual, const std::vector<Leg>& Vs);  // reference src/vspace.jl:112-138
+
+struct KeyHash {
+  size_t operator()(const std::vector<int64_t>& k) const {
+    uint64_t h = 1469598103934665603ull;
+    for (int64_t x : k) {
+      h ^= (uint64_t)x + 0x9e3779b97f4a7c15ull;
+      h *= 1099511628211ull;
+    }
+    return (size_t)h;
+  }
+};
+
+// A tensor structure: charge + legs + every allowed sector in column-major
+// order with its block size and packed element offset (reference
+// src/symtensor.jl:12-30, src/sector.jl:69-90).
+struct Struct {
+  int64_t charge = 0;
+  std::vector<Leg> legs;
+  int64_t nblocks = 0;
+  std::vector<int64_t> sectors;  // nblocks * rank
+  std::vector<int64_t> sizes;    // nblocks * rank
+  std::vector<int64_t> offsets;  // nblocks + 1
+  std::unordered_map<std::vector<int64_t>, int64_t, KeyHash> index;
+
+  int rank() const { return (int)legs.size(); }
+  int64_t numel() const { return offsets.empty() ? 0 : offsets.back(); }
+  const int64_t* sector(int64_t b) const { return &sectors[(size_t)b * rank()]; }
+  const int64_t* size(int64_t b) const { return &sizes[(size_t)b * rank()]; }
+  int64_t block_numel(int64_t b) const;
+  int64_t find(const std::vector<int64_t>& s) const;
+};
+
+void all_sectors(int64_t charge, const std::vector<Leg>& legs, std::vector<int64_t>& sects,
+                 std::vector<int64_t>& sizes);
+Struct make_struct(int64_t charge, const std::vector<Leg>& legs);
+bool same_struct(const Struct& a, const Struct& b);  // same charge and legs
+
+// ---------------------------------------------------------------------------
+// Device descriptor records (plain data, copied verbatim to the GPU)
+// ---------------------------------------------------------------------------
+constexpr int MAXD = 10;
+
+// Strided N-d copy: dst[dst_off + sum i_k*dst_st[k]] = src[src_off + sum i_k*src_st[k]].
+// dims[0] is the destination-contiguous dimension where one exists and
+// dims[1] the source-contiguous one when it differs ("transpose").
+struct CopyDesc {
+  int32_t ndim;
+  int32_t transpose;
+  int32_t dims[MAXD];
+  int64_t src_st[MAXD];
+  int64_t dst_st[MAXD];
+  int64_t src_off;
+  int64_t dst_off;
+};
+
+struct CopyPlan {
+  std::vector<CopyDesc> descs;
+  std::vector<int64_t> tile_start;  // ndesc + 1, prefix of 64x64 tiles
+  int64_t ntiles = 0;
+  int64_t elems = 0;
+  void add(int ndim, const int64_t* dims, const int64_t* sst, const int64_t* dst, int64_t soff,
+           int64_t doff);
+  void finalize();
+};
+
+// One grouped-GEMM problem: C[M x N] (column-major, ldc) = sum over its K
+// segments of A_s^T * B_s with both operands K-contiguous:
+//   A_s(k, m) at a_off + k + m*lda,  B_s(k, n) at b_off + k + n*ldb.
+// nseg == 0 means "write zeros" (reference src/contract.jl:292-294).
+struct GemmProb {
+  int32_t M, N, seg0, nseg;
+  int64_t c_off, ldc;
+};
+struct GemmSeg {
+  int64_t K, a_off, lda, b_off, ldb;
+};
+struct GemmTile {
+  int32_t prob, m0, n0, pad;
+};
+
+struct GemmPlan {
+  std::vector<GemmProb> probs;
+  std::vector<GemmSeg> segs;
+  int64_t flops = 0;  // 2*M*N*K summed
+  void add_prob(int64_t M, int64_t N, int64_t c_off, int64_t ldc) {
+    probs.push_back(GemmProb{(int32_t)M, (int32_t)N, (int32_t)segs.size(), 0, c_off, ldc});
+  }
+  void add_seg(int64_t K, int64_t a_off, int64_t lda, int64_t b_off, int64_t ldb) {
+    GemmProb& p = probs.back();
+    segs.push_back(GemmSeg{K, a_off, lda, b_off, ldb});
+    p.nseg++;
+    flops += 2 * (int64_t)p.M * p.N * K;
+  }
+  // Tile list for a BMxBN tiling, longest-K first (static LPT order).
+  std::vector<GemmTile> tiles(int BM, int BN) const;
+};
+
+// ---------------------------------------------------------------------------
+// contract(A, idxA, B, idxB): the reference's fuse -> GEMM -> split -> permute
+// (src/contract.jl:11-47, :112-171, :187-224, :234-298).
+// ---------------------------------------------------------------------------
+struct IndexPerm {
+  std::vector<int> rems, cons, tofinals;  // 1-based, as in the reference
+};
+IndexPerm contract_index_perm(const std::vector<int>& idx);  // src/contract.jl:82-99
+
+struct FusedMatrix {
+  Struct F;          // the fused rank-2 structure
+  CopyPlan gather;   // A blocks -> fused blocks
+  bool k_contig_rows;  // storage: true = rows K-contiguous (A^T), false = column-major
+};
+
+// SymMatrix(A, rows, cols) (src/contract.jl:112-171).  `transposed` stores
+// each fused block row-major (element (r,c) at r*ncols + c) instead of
+// column-major, which is the K-contiguous layout of a left GEMM operand.
+FusedMatrix symmatrix_plan(const Struct& A, const std::vector<int>& rows, const std::vector<int>& cols,
+                           bool transposed, int64_t dst_base);
+
+struct ContractPlan {
+  Struct A, B, C;
+  FusedMatrix FA, FB;
+  Struct FC;
+  GemmPlan gemm;      // A base = ws, B base = ws + FA.numel, C base = ws + FA+FB
+  CopyPlan scatter;   // fused C (ws) -> C
+  int64_t ws_elems = 0;
+  int64_t fa_off = 0, fb_off = 0, fc_off = 0;
+};
+ContractPlan contract_plan(const Struct& A, const std::vector<int>& idxA, const Struct& B,
+                           const std::vector<int>& idxB);
+
+// Re-legging plans: pure data movement (src/symreleg.jl:34-214).
+struct RelegPlan {
+  Struct in, out;
+  CopyPlan copy;
+};
+RelegPlan permutelegs_plan(const Struct& A, const std::vector<int>& perm);
+RelegPlan fuselegs_plan(const Struct& A, int l, int n, bool dual);
+RelegPlan splitleg_plan(const Struct& A, int l, const std::vector<Leg>& spaces);
+RelegPlan symmatrix_releg(const Struct& A, const std::vector<int>& rows, const std::vector<int>& cols);
+
+// ---------------------------------------------------------------------------
+// Fused two-site H_eff (src/apply.jl:93-105) computed as
+//   step 1: T1 = v^T . envL per matched block pair (grouped TN GEMM),
+//   step 2+3: T3 = W_group . T1 per (bra_l, ket_r) pixel group, where
+//             W_group = WL (x) WR combined over the middle MPO bond,
+//   step 4: out = T3^T . envR, K segments over (mpo_r, ket_r).
+// ---------------------------------------------------------------------------
+struct MpoSig {
+  int32_t R, C;
+  int64_t w_off;  // row-major R x C matrix in the W workspace
+};
+struct MpoTerm {   // one (out type, in type) sub-matrix of a signature's W
+  int32_t sig, row0, col0, R, C;  // R = Dx*d1o*d2o rows, C = Dw*d1*d2 cols
+  int32_t Dw, d1i, d2i, Dx, d1o, d2o;
+  int32_t pair0, npair;
+};
+struct MpoPair {   // WL block (w, s1o, m, s1) x WR block (m, s2o, x, s2)
+  int64_t wl_off, wr_off;
+  int32_t Dm, pad;
+};
+struct MpoGroup {
+  int32_t sig, dR, db, pad;
+  int32_t in0, out0;  // first row in the in/out row tables
+  int64_t npix;
+};
+struct MpoInRow {
+  int64_t base, nstride, mstride;
+};
+struct MpoOutRow {
+  int64_t base, mstride;
+};
+struct MpoWork {
+  int32_t group, pix0;
+};
+
+struct HeffPlan {
+  Struct v, envL, envR, WL, WR, out;
+  GemmPlan g1;  // A = v, B = envL, C = T1 (ws)
+  int64_t t1_elems = 0;
+  std::vector<MpoSig> sigs;
+  std::vector<MpoTerm> terms;
+  std::vector<MpoPair> pairs;
+  std::vector<MpoGroup> groups;
+  std::vector<MpoInRow> inrows;
+  std::vector<MpoOutRow> outrows;
+  std::vector<MpoWork> work;
+  int64_t w_elems = 0;
+  int64_t t3_elems = 0;
+  int maxR = 0, maxC = 0;
+  GemmPlan g4;  // A = T3 (ws), B = envR, C = out
+  // workspace layout (elements): [T1 | T3 | W]
+  int64_t t1_off = 0, t3_off = 0, w_off = 0, ws_elems = 0;
+  // reference-order algorithmic cost (src/apply.jl:99-103 via src/contract.jl:290-291)
+  int64_t ref_flops = 0;
+  int64_t ref_elems = 0;  // sum over the 4 steps of numel(A)+numel(B)+numel(C)
+  int64_t step_flops[4] = {0, 0, 0, 0};
+  int64_t step_elems[4] = {0, 0, 0, 0};
+  int64_t mpo_flops = 0;  // dense W_group application flops of this engine
+};
+HeffPlan heff_plan(const Struct& v, const Struct& envL, const Struct& envR, const Struct& WL,
+                   const Struct& WR);
+
+}  // namespace mpsh

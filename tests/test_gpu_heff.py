@@ -1,0 +1,119 @@
+"""Device parity of the two-site H_eff (fused and plain paths) and the environment updates.
+
+Small and mid-size workloads are compared with the oracle's reference-order
+chain (FP64: 1e-12 relative, norm-wise per block).  The full-size workloads
+(chi=1000, 2048) are checked through properties: fused == plain (two
+different device algorithms), linearity, and FP32/BF16 vs FP64 agreement.
+"""
+import numpy as np
+import pytest
+import torch
+
+from oracle import tensornetalgs as ota
+from tests.helpers import global_rel_err, o_from_flat, o_from_product, o_space, rel_block_err
+
+pytestmark = pytest.mark.gpu
+
+if not torch.cuda.is_available():
+    pytest.skip("needs a GPU", allow_module_level=True)
+
+import mps_amd as M  # noqa: E402
+from mps_amd.tensornetalgs import heff_workload  # noqa: E402
+from mps_amd.workloads import gaussian  # noqa: E402
+
+DEV = torch.device("cuda", 0)
+
+
+def oracle_of(P):
+    return o_from_product(P)
+
+
+def _workload(name, dtype=torch.float64, seed=1911):
+    return heff_workload(name, seed=seed, dtype=dtype, device=DEV)
+
+
+def _check_structure(P, O):
+    assert P.charge == O.charge
+    assert tuple(o_space(V) for V in P.space) == O.space
+    assert P.sectors() == list(O.blocks.keys())
+
+
+@pytest.mark.parametrize("name", ["xxz_L32_chi64", "xxz_L64_chi256", "xxz_L100_chi512"])
+def test_heff_fused_and_plain_f64_vs_oracle(name):
+    ts = _workload(name)
+    O = [oracle_of(t) for t in ts]
+    want = ota.applymps2site(*O)
+    for path in ("fused", "plain"):
+        got = M._applymps2site(*ts, path=path)
+        _check_structure(got, want)
+        err = rel_block_err(oracle_of(got), want)
+        print(f"{name} {path}: max block rel err {err:.2e}", flush=True)
+        assert err <= 1e-12
+
+
+@pytest.mark.parametrize("name", ["xxz_L32_chi64", "xxz_L100_chi512"])
+def test_heff_fused_f32_bf16_vs_oracle(name):
+    ts64 = _workload(name)
+    want = ota.applymps2site(*[oracle_of(t) for t in ts64])
+    ts32 = [t.to(torch.float32) for t in ts64]
+    got32 = M._applymps2site(*ts32, prec="f32")
+    _check_structure(got32, want)
+    e32 = global_rel_err(oracle_of(got32), want)
+    got16 = M._applymps2site(*ts32, prec="bf16")
+    e16 = global_rel_err(oracle_of(got16), want)
+    print(f"{name}: f32 rel err {e32:.2e}, bf16 rel err {e16:.2e}", flush=True)
+    assert e32 < 1e-5
+    assert e16 < 3e-2
+
+
+@pytest.mark.parametrize("name", ["refbench_chi1000", "cyl6x24_chi2048"])
+def test_heff_full_size_properties(name):
+    ts = _workload(name)
+    fused = M._applymps2site(*ts, path="fused")
+    plain = M._applymps2site(*ts, path="plain")
+    assert fused.sectors() == plain.sectors()
+    d = (fused.data - plain.data).norm().item() / plain.data.norm().item()
+    print(f"{name}: fused vs plain f64 rel diff {d:.2e}", flush=True)
+    assert d < 1e-12
+    # linearity: H(a x + b y) = a Hx + b Hy
+    v, L, R, WL, WR = ts
+    y = M.randn(v.space, v.charge, seed=7, stream=11, device=DEV)
+    z = v.copy()
+    M.axpby_(0.75, y, -1.25, z)  # z = 0.75 y - 1.25 v
+    Hz = M._applymps2site(z, L, R, WL, WR)
+    Hy = M._applymps2site(y, L, R, WL, WR)
+    comb = Hy.copy()
+    M.axpby_(-1.25, fused, 0.75, comb)
+    lin = (Hz.data - comb.data).norm().item() / Hz.data.norm().item()
+    assert lin < 1e-12
+    # reduced precision against FP64
+    t32 = [t.to(torch.float32) for t in ts]
+    for prec, tol in (("f32", 1e-5), ("bf16", 3e-2)):
+        got = M._applymps2site(*t32, prec=prec)
+        e = (got.data.double() - fused.data).norm().item() / fused.data.norm().item()
+        print(f"{name}: {prec} vs f64 rel err {e:.2e}", flush=True)
+        assert e < tol
+
+
+def test_heff_repeated_calls_are_deterministic():
+    ts = _workload("xxz_L100_chi512")
+    a = M._applymps2site(*ts)
+    b = M._applymps2site(*ts)
+    assert torch.equal(a.data, b.data)
+
+
+@pytest.mark.parametrize("name", ["xxz_L32_chi64", "xxz_L100_chi512"])
+def test_env_updates_vs_oracle(name):
+    v, L, R, WL, WR = _workload(name)
+    # site tensors A(l, p, r̄) living on the left and on the right bond of v
+    Vl, Vd, Vr = v.space[0], v.space[1], v.space[3].dual()
+    Al = M.randn((Vl, Vd, Vl.dual()), 0, seed=3, stream=22, device=DEV)
+    Ar = M.randn((Vr, Vd, Vr.dual()), 0, seed=3, stream=23, device=DEV)
+    gotL = M._mpsupdateleft(L, Al, WL)
+    wantL = ota.mpsupdateleft(oracle_of(L), oracle_of(Al), oracle_of(WL))
+    _check_structure(gotL, wantL)
+    assert rel_block_err(oracle_of(gotL), wantL) <= 1e-12
+    gotR = M._mpsupdateright(R, Ar, WR)
+    wantR = ota.mpsupdateright(oracle_of(R), oracle_of(Ar), oracle_of(WR))
+    _check_structure(gotR, wantR)
+    assert rel_block_err(oracle_of(gotR), wantR) <= 1e-12
