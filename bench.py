@@ -1,0 +1,224 @@
+#!/usr/bin/env python3
+"""Benchmark: two-site H_eff matvecs/s at chi=2048 (6x24 Heisenberg cylinder bond, D=20).
+
+One step = one H_eff application v -> H_eff v with all inputs resident in
+HBM (fused path: step-1 grouped GEMM, combined-MPO pass, step-4 grouped
+GEMM).  Prints ONE JSON line.  ``--gpus N`` with N > 1 is not supported:
+this engine runs on one GPU.
+
+Roofline: the dominant kernel is the grouped MFMA GEMM (two launches per
+matvec: step 1 and step 4); its algorithmic FLOPs per launch are the
+reference-order 2*M*N*K of those two contracts, its duration comes from HIP
+events recorded on the launch stream around each stage.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+if ROOT not in sys.path:
+    sys.path.insert(0, ROOT)
+
+PEAK = {"f32": 157.3e12, "f64": 78.6e12, "bf16": 2.5e15}
+HBM_PEAK = 8.0e12
+METRIC = "H_eff matvecs/sec + DMRG sweep time at χ=2048; |ΔE| vs FP64 CPU oracle"
+
+
+def parse():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=200)
+    ap.add_argument("--warmup", type=int, default=20)
+    ap.add_argument("--workload", default="cyl6x24_chi2048")
+    ap.add_argument("--prec", default="f32", choices=["f32", "f64", "bf16"])
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-extra-modes", action="store_true")
+    ap.add_argument("--cpu-budget", type=float, default=20.0, help="seconds of oracle work for the CPU baseline")
+    return ap.parse_args()
+
+
+def time_fused(plan, prec, ts, steps, warmup, stage_events=True):
+    """Returns (ms_per_step, per-stage ms list)."""
+    import torch
+    v, L, R, WL, WR = ts
+    out = torch.empty(plan.out.numel, dtype=v.dtype, device=v.device)
+    ws = plan.workspace(prec, v.device)
+    for _ in range(warmup):
+        plan.apply(prec, v.data, L.data, R.data, WL.data, WR.data, out, ws)
+    torch.cuda.synchronize()
+    n_ev = 4 if stage_events else 0
+    evs = [[torch.cuda.Event(enable_timing=True) for _ in range(n_ev)] for _ in range(steps)]
+    t_start = torch.cuda.Event(enable_timing=True)
+    t_end = torch.cuda.Event(enable_timing=True)
+    torch.cuda.synchronize()
+    wall0 = time.perf_counter()
+    t_start.record()
+    for i in range(steps):
+        if stage_events:
+            e = evs[i]
+            e[0].record()
+            plan.apply_stages(prec, 1, v.data, L.data, R.data, WL.data, WR.data, out, ws)
+            e[1].record()
+            plan.apply_stages(prec, 2, v.data, L.data, R.data, WL.data, WR.data, out, ws)
+            e[2].record()
+            plan.apply_stages(prec, 4, v.data, L.data, R.data, WL.data, WR.data, out, ws)
+            e[3].record()
+        else:
+            plan.apply(prec, v.data, L.data, R.data, WL.data, WR.data, out, ws)
+    t_end.record()
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - wall0
+    total_ms = t_start.elapsed_time(t_end)
+    stages = [0.0, 0.0, 0.0]
+    if stage_events:
+        for e in evs:
+            for k in range(3):
+                stages[k] += e[k].elapsed_time(e[k + 1])
+        stages = [s / steps for s in stages]
+    return max(total_ms, wall * 1e3) / steps, stages, out
+
+
+def time_plain(ts, steps, warmup):
+    import torch
+    import mps_amd as M
+    for _ in range(warmup):
+        M._applymps2site(*ts, path="plain")
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        M._applymps2site(*ts, path="plain")
+    torch.cuda.synchronize()
+    return (time.perf_counter() - t0) * 1e3 / steps
+
+
+def cpu_baseline(name, budget):
+    """Oracle (numpy restatement of the reference algorithm) on the host cores."""
+    import numpy as np
+    from oracle import symtensors as ost
+    from oracle import tensornetalgs as ota
+    from mps_amd.workloads import gaussian, heff_spaces
+    left, right, mpo = heff_spaces(name)
+    Vl, Vr, Vw = ost.Space(left), ost.Space(right), ost.Space(mpo)
+    Vd = ost.U1Space((0, 1), (1, 1))
+    spaces = [(Vl, Vd, Vd, Vr.dual()), (Vl.dual(), Vw.dual(), Vl), (Vr, Vw, Vr.dual()),
+              (Vw, Vd, Vw.dual(), Vd.dual()), (Vw, Vd, Vw.dual(), Vd.dual())]
+    ts = []
+    for stream, sp in enumerate(spaces, start=1):
+        sects, sizes = ost.allsectorsandsizes(0, sp)
+        n = sum(int(np.prod(z)) for z in sizes)
+        flat = gaussian(1911, stream, n)
+        blocks, p = {}, 0
+        for s, z in zip(sects, sizes):
+            k = int(np.prod(z))
+            blocks[s] = flat[p:p + k].reshape(z, order="F")
+            p += k
+        ts.append(ost.SymTensor(0, sp, blocks))
+    calls, t0 = 0, time.perf_counter()
+    out = None
+    while True:
+        out = ota.applymps2site(*ts)
+        calls += 1
+        if time.perf_counter() - t0 > budget or calls >= 50:
+            break
+    dt = time.perf_counter() - t0
+    threads = 1
+    try:
+        from threadpoolctl import threadpool_info
+        threads = max([d.get("num_threads", 1) for d in threadpool_info() if d.get("user_api") == "blas"] or [1])
+    except Exception:
+        pass
+    flat_out = np.concatenate([b.reshape(-1, order="F") for b in out.blocks.values()])
+    flat_v = np.concatenate([b.reshape(-1, order="F") for b in ts[0].blocks.values()])
+    return calls / dt, calls, dt, threads, flat_out, flat_v
+
+
+def main():
+    args = parse()
+    if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
+        print(f"bench.py: this engine runs on one GPU; --gpus {args.gpus} is not supported "
+              "(multi-GPU sharding is design-only, see DESIGN.md)", file=sys.stderr)
+        sys.exit(2)
+    import numpy as np
+    import torch
+    import mps_amd as M
+    from mps_amd.tensornetalgs import heff_plan, heff_workload
+
+    dev = torch.device("cuda", 0)
+    torch.cuda.set_device(dev)
+    prec = args.prec
+    dt = torch.float64 if prec == "f64" else torch.float32
+    ts64 = heff_workload(args.workload, dtype=torch.float64, device=dev)
+    ts = [t.to(dt) for t in ts64] if dt != torch.float64 else ts64
+    plan = heff_plan(*ts)
+    ms, stages, out = time_fused(plan, prec, ts, args.steps, args.warmup)
+    value = 1e3 / ms
+    F = plan.ref_flops
+    gemm_flops = plan.step_flops[0] + plan.step_flops[3]
+    gemm_ms = stages[0] + stages[2]
+    achieved = gemm_flops / (gemm_ms * 1e-3) if gemm_ms > 0 else 0.0
+    result = {
+        "metric": METRIC,
+        "value": round(value, 3),
+        "unit": "matvecs/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(ms, 5),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": prec,
+        "data": "synthetic: seeded Gaussian blocks (SplitMix64/Box-Muller), random env and MPO tensors",
+        "config": {"workload": args.workload, "chi": sum(V for V in ts[0].space[0].dims),
+                   "sectors_per_bond": len(ts[0].space[0].charges), "mpo_D": ts[1].space[1].dim(),
+                   "path": "fused", "precision": {"f32": "FP32 MFMA", "f64": "FP64 MFMA",
+                                                  "bf16": "BF16 MFMA operands, FP32 accumulate"}[prec]},
+        "gflops_per_matvec": round(F / 1e9, 4),
+        "achieved_tflops": round(F / (ms * 1e-3) / 1e12, 3),
+        "per_kernel_ms": {"step1_gemm": round(stages[0], 5), "mpo_pass": round(stages[1], 5),
+                          "step4_gemm": round(stages[2], 5)},
+        "roofline": {"bound": "mfma", "kernel": f"gemm_grouped<{ {'f64': 0, 'f32': 1, 'bf16': 2}[prec] }>",
+                     "achieved": round(achieved / 1e12, 3), "peak": PEAK[prec] / 1e12, "unit": "TFLOP/s",
+                     "frac": round(achieved / PEAK[prec], 4), "traffic": None},
+    }
+    # E = <v|H v>/<v|v> of this matvec in the benchmark precision, FP64 device path as reference
+    v64 = ts64[0]
+    Hv64 = M._applymps2site(*ts64, prec="f64")
+    torch.cuda.synchronize()
+    vv = float(torch.dot(v64.data, v64.data))
+    e64 = float(torch.dot(v64.data, Hv64.data)) / vv
+    e_prec = float(torch.dot(ts[0].data.double(), out.double())) / vv
+    result["rayleigh_rel_err_vs_f64"] = abs(e_prec - e64) / abs(e64)
+    if not args.no_extra_modes:
+        extra = {}
+        for p2 in ("f64", "f32", "bf16"):
+            if p2 == prec:
+                continue
+            t2 = ts64 if p2 == "f64" else [t.to(torch.float32) for t in ts64]
+            pl2 = heff_plan(*t2)
+            ms2, st2, _ = time_fused(pl2, p2, t2, max(10, args.steps // 4), 5)
+            extra[f"fused_{p2}_matvecs_per_s"] = round(1e3 / ms2, 3)
+        ms_plain = time_plain(ts64, max(3, args.steps // 20), 2)
+        extra["plain_f64_matvecs_per_s"] = round(1e3 / ms_plain, 3)
+        extra["speedup_vs_plain_f64"] = round(ms_plain / ms, 3)
+        result["other_paths"] = extra
+    if not args.no_cpu_baseline:
+        rate, calls, secs, threads, oflat, vflat = cpu_baseline(args.workload, args.cpu_budget)
+        e_or = float(np.dot(vflat, oflat) / np.dot(vflat, vflat))
+        result["cpu_baseline"] = {"value": round(rate, 5), "unit": "matvecs/s", "cores": threads,
+                                  "kind": "port",
+                                  "sample": f"{calls} H_eff call(s) of the full {args.workload} workload in "
+                                            f"{secs:.1f} s (oracle: numpy restatement of the reference "
+                                            f"fuse-GEMM-split-permute chain, FP64, OpenBLAS threads={threads}, "
+                                            f"host nproc={os.cpu_count()})"}
+        result["energy_rel_err_vs_cpu_oracle"] = abs(e_prec - e_or) / abs(e_or)
+        result["f64_energy_rel_err_vs_cpu_oracle"] = abs(e64 - e_or) / abs(e_or)
+    print(json.dumps(result), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -119,6 +119,12 @@ size_t mpsh_heff_workspace(mpsh_heff plan, mpsh_dtype dt); /* bytes */
 mpsh_status mpsh_heff_stats(mpsh_heff plan, int64_t* stats);
 mpsh_status mpsh_heff_apply(mpsh_heff plan, mpsh_dtype dt, const void* v, const void* envL, const void* envR,
                             const void* WL, const void* WR, void* out, void* workspace, void* stream);
+/* Same computation split by stage for per-kernel timing: stages is a bit mask,
+ * 1 = step-1 GEMM, 2 = combined-MPO pass, 4 = step-4 GEMM; run in that order.
+ * mask 7 equals mpsh_heff_apply.  No reference counterpart (instrumentation). */
+mpsh_status mpsh_heff_apply_stages(mpsh_heff plan, mpsh_dtype dt, int32_t stages, const void* v, const void* envL,
+                                   const void* envR, const void* WL, const void* WR, void* out, void* workspace,
+                                   void* stream);
 void mpsh_heff_destroy(mpsh_heff plan);
 
 /* ------------------------------------------------------------------------ */

@@ -335,7 +335,9 @@ mpsh_status mpsh_contract_execute(mpsh_contract plan, mpsh_dtype dt, const void*
                                   void* workspace, void* stream) {
   API_TRY
   check_dtype(dt);
-  if (!plan || !A || !B || !C || (!workspace && plan->p.ws_elems)) throw Error(ERR_INVALID, "null argument");
+  if (!plan || (!A && plan->p.A.numel()) || (!B && plan->p.B.numel()) || (!C && plan->p.C.numel()) ||
+      (!workspace && plan->p.ws_elems))
+    throw Error(ERR_INVALID, "null argument");
   const ContractDev& d = plan->get();
   hipStream_t s = (hipStream_t)stream;
   const int es = elem_size(dt);
@@ -393,9 +395,19 @@ mpsh_status mpsh_heff_stats(mpsh_heff plan, int64_t* stats) {
 
 mpsh_status mpsh_heff_apply(mpsh_heff plan, mpsh_dtype dt, const void* v, const void* envL, const void* envR,
                             const void* WL, const void* WR, void* out, void* workspace, void* stream) {
+  return mpsh_heff_apply_stages(plan, dt, 7, v, envL, envR, WL, WR, out, workspace, stream);
+}
+
+mpsh_status mpsh_heff_apply_stages(mpsh_heff plan, mpsh_dtype dt, int32_t stages, const void* v, const void* envL,
+                                   const void* envR, const void* WL, const void* WR, void* out, void* workspace,
+                                   void* stream) {
   API_TRY
   check_dtype(dt);
-  if (!plan || !v || !envL || !envR || !WL || !WR || !out || !workspace) throw Error(ERR_INVALID, "null argument");
+  if (!plan) throw Error(ERR_INVALID, "null plan");
+  const HeffPlan& hp = plan->p;
+  if ((!v && hp.v.numel()) || (!envL && hp.envL.numel()) || (!envR && hp.envR.numel()) || (!WL && hp.WL.numel()) ||
+      (!WR && hp.WR.numel()) || (!out && hp.out.numel()) || (!workspace && hp.ws_elems))
+    throw Error(ERR_INVALID, "null argument");
   const HeffDev& d = plan->get();
   const HeffPlan& p = plan->p;
   hipStream_t s = (hipStream_t)stream;
@@ -404,9 +416,9 @@ mpsh_status mpsh_heff_apply(mpsh_heff plan, mpsh_dtype dt, const void* v, const 
   void* T1 = ws + p.t1_off * es;
   void* T3 = ws + p.t3_off * es;
   void* W = ws + p.w_off * es;
-  check(launch_gemm(d.g1, gemm_mode(dt), v, envL, T1, s), "step-1 gemm");
-  check(launch_mpo(d.mpo, es, WL, WR, W, T1, T3, s), "mpo");
-  check(launch_gemm(d.g4, gemm_mode(dt), T3, envR, out, s), "step-4 gemm");
+  if (stages & 1) check(launch_gemm(d.g1, gemm_mode(dt), v, envL, T1, s), "step-1 gemm");
+  if (stages & 2) check(launch_mpo(d.mpo, es, WL, WR, W, T1, T3, s), "mpo");
+  if (stages & 4) check(launch_gemm(d.g4, gemm_mode(dt), T3, envR, out, s), "step-4 gemm");
   return MPSH_OK;
   API_CATCH
 }

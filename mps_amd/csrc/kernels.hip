@@ -252,7 +252,8 @@ __global__ __launch_bounds__(256) void gemm_grouped(const GemmProb* __restrict__
       }
       buf ^= 1;
     }
-    // epilogue: lane holds D[4*g + r][l16] of each 16x16 sub-tile
+    // epilogue: column l16 of each 16x16 sub-tile; accumulator register r holds
+    // row g + 4r for the FP64 16x16x4 form and row 4g + r for the FP32/BF16 forms.
 #pragma unroll
     for (int i = 0; i < 2; ++i)
 #pragma unroll
@@ -260,7 +261,7 @@ __global__ __launch_bounds__(256) void gemm_grouped(const GemmProb* __restrict__
         const int n = tile.n0 + wn * 32 + j * 16 + l16;
 #pragma unroll
         for (int r = 0; r < 4; ++r) {
-          const int m = tile.m0 + wm * 32 + i * 16 + 4 * g + r;
+          const int m = tile.m0 + wm * 32 + i * 16 + (MODE == 0 ? g + 4 * r : 4 * g + r);
           if (m < p.M && n < p.N) C[p.c_off + m + (int64_t)n * p.ldc] = (T)acc[i][j][r];
         }
       }

@@ -58,6 +58,13 @@ class HeffPlan:
                                     _ptr(out), _ptr(ws), stream if stream is not None else _stream()))
         return out
 
+    def apply_stages(self, prec: str, stages: int, v, envL, envR, WL, WR, out, ws, stream=None):
+        """Run a subset of the three kernel stages (1 step-1 GEMM, 2 MPO pass, 4 step-4 GEMM)."""
+        check(lib().mpsh_heff_apply_stages(self.plan, _PREC[prec], int(stages), _ptr(v), _ptr(envL), _ptr(envR),
+                                           _ptr(WL), _ptr(WR), _ptr(out), _ptr(ws),
+                                           stream if stream is not None else _stream()))
+        return out
+
     def __del__(self):
         try:
             lib().mpsh_heff_destroy(self.plan)
