@@ -1,0 +1,435 @@
+"""MPS / MPO containers, model MPOs and the SVD truncation step on the device.
+
+Mirrors ``src/symmps.jl``, ``src/symmpo.jl`` and ``src/mpsutils.jl`` of
+amiragha/mps for the parts the two-site sweep needs:
+
+* ``MPState`` / ``MPOperator`` containers (symmps.jl:1-4, symmpo.jl:1-3);
+* ``fsm_u1mpo``: U(1) MPO of a spin-1/2 Hamiltonian sum_{i<j} Jz SzSz +
+  Jxy/2 (S+S- + S-S+) as a finite-state machine; for nearest-neighbour
+  chains it reproduces ``xxz_u1mpo`` (symmpo.jl:45-97) block for block;
+  ``heisenberg_cylinder_mpo`` builds the width-W cylinder in snake order;
+* ``svd_`` / ``truncatedsizes`` / ``svdtrunc`` (symtools.jl:1-35,
+  mpsutils.jl:89-179) with FP64 per-sector SVDs on the device (rocSOLVER
+  through torch.linalg.svd) and the reference's greedy global truncation;
+* ``random_mps``, ``center_at``, ``normalize``, ``measure``.
+"""
+from __future__ import annotations
+
+import math
+from typing import Dict, List, Sequence, Tuple
+
+import numpy as np
+import torch
+
+from .symtensors import (SymTensor, U1Space, contract, default_device, dual, fuselegs, get_struct, splitleg)
+from .workloads import binom_capped_profile, gaussian
+
+
+class MPState:
+    """``MPState`` (symmps.jl:1-4): site tensors (l, p, r̄) and the orthogonality center."""
+
+    def __init__(self, As: List[SymTensor], center: int = 0):
+        self.As = list(As)
+        self.center = center
+
+    def __len__(self):
+        return len(self.As)
+
+    def leftspace(self):
+        return self.As[0].space[0]
+
+    def rightspace(self):
+        return self.As[-1].space[2].dual()
+
+    def bonddims(self):
+        return [A.space[2].dim() for A in self.As[:-1]]
+
+
+class MPOperator:
+    """``MPOperator`` (symmpo.jl:1-3): site tensors (wl, p_out, w̄r, p̄_in)."""
+
+    def __init__(self, Ws: List[SymTensor]):
+        self.Ws = list(Ws)
+
+    def __len__(self):
+        return len(self.Ws)
+
+    def leftspace(self):
+        return self.Ws[0].space[0]
+
+    def rightspace(self):
+        return self.Ws[-1].space[2].dual()
+
+
+# ---------------------------------------------------------------------------
+# finite-state-machine U(1) MPO for spin-1/2 two-site terms
+# ---------------------------------------------------------------------------
+_SZ = {(0, 0): -0.5, (1, 1): 0.5}
+_SP = {(1, 0): 1.0}
+_SM = {(0, 1): 1.0}
+_ID = {(0, 0): 1.0, (1, 1): 1.0}
+
+
+def fsm_u1mpo(L: int, terms: Sequence[Tuple[int, int, float, float]], dtype=torch.float64, device=None) -> MPOperator:
+    """MPO of H = sum over terms (i, j, Jz, Jxy), 1 <= i < j <= L, of Jz Sz_i Sz_j + Jxy/2 (S+_i S-_j + h.c.).
+
+    Bond states per charge: 0 -> [after, Sz channels by source site, before],
+    +1 -> S+ channels, -1 -> S- channels; the two boundary bonds keep only
+    "before" (left) and "after" (right).  Physical charge 1 is spin up.
+    """
+    partners: Dict[int, Dict[int, Tuple[float, float]]] = {}
+    for i, j, jz, jxy in terms:
+        if not (1 <= i < j <= L):
+            raise ValueError(f"bad term sites ({i}, {j})")
+        a, b = partners.setdefault(i, {}).get(j, (0.0, 0.0))
+        partners[i][j] = (a + jz, b + jxy)
+    maxp = {i: max(ps) for i, ps in partners.items()}
+    has_z = {i: any(v[0] != 0.0 for v in ps.values()) for i, ps in partners.items()}
+    has_xy = {i: any(v[1] != 0.0 for v in ps.values()) for i, ps in partners.items()}
+
+    def bond_states(k):
+        """ordered list of (charge, label) for bond k (between sites k and k+1)."""
+        if k == 0:
+            return [(0, "before")]
+        if k == L:
+            return [(0, "after")]
+        srcs = sorted(i for i in partners if i <= k < maxp[i])
+        st = [(0, "after")] + [(0, ("z", i)) for i in srcs if has_z[i]] + [(0, "before")]
+        st += [(1, ("p", i)) for i in srcs if has_xy[i]]
+        st += [(-1, ("m", i)) for i in srcs if has_xy[i]]
+        return st
+
+    bonds = [bond_states(k) for k in range(L + 1)]
+
+    def space_of(states):
+        cnt: Dict[int, int] = {}
+        for c, _ in states:
+            cnt[c] = cnt.get(c, 0) + 1
+        return U1Space(*cnt.items())
+
+    def index_of(states):
+        pos, cnt = {}, {}
+        for c, lab in states:
+            pos[lab] = (c, cnt.get(c, 0))
+            cnt[c] = cnt.get(c, 0) + 1
+        return pos
+
+    Vd = U1Space((0, 1), (1, 1))
+    Ws = []
+    for s in range(1, L + 1):
+        left, right = bonds[s - 1], bonds[s]
+        Vl, Vr = space_of(left), space_of(right)
+        li, ri = index_of(left), index_of(right)
+        # the last tensor keeps a non-dual charge-0 boundary leg, like Wend (symmpo.jl:55)
+        space = (Vl, Vd, Vr if s == L else Vr.dual(), Vd.dual())
+        st = get_struct(0, space)
+        sec, siz, off = st.tables()
+        flat = np.zeros(st.numel)
+        bidx = {tuple(int(x) for x in r): b for b, r in enumerate(sec)}
+
+        def put(lab_l, lab_r, op, coef):
+            if lab_l not in li or lab_r not in ri or coef == 0.0:
+                return
+            (cl, il), (cr, ir) = li[lab_l], ri[lab_r]
+            for (so, si), val in op.items():
+                b = bidx[(cl, so, cr, si)]
+                dl, dr = int(siz[b][0]), int(siz[b][2])
+                flat[off[b] + il + dl * (0 + 1 * (ir + dr * 0))] += coef * val
+
+        put("before", "before", _ID, 1.0)
+        put("after", "after", _ID, 1.0)
+        if s in partners:
+            if has_xy[s]:
+                put("before", ("p", s), _SP, 1.0)
+                put("before", ("m", s), _SM, 1.0)
+            if has_z[s]:
+                put("before", ("z", s), _SZ, 1.0)
+        for i in partners:
+            if i < s < maxp[i]:
+                for t in ("p", "m", "z"):
+                    put((t, i), (t, i), _ID, 1.0)
+            if i < s and s in partners[i]:
+                jz, jxy = partners[i][s]
+                put(("p", i), "after", _SM, 0.5 * jxy)
+                put(("m", i), "after", _SP, 0.5 * jxy)
+                put(("z", i), "after", _SZ, jz)
+        Ws.append(SymTensor.from_flat(0, space, flat, dtype, device))
+    return MPOperator(Ws)
+
+
+def xxz_u1mpo(lx: int, delta: float = 1.0, dtype=torch.float64, device=None) -> MPOperator:
+    """``xxz_u1mpo(T, lx, d, delta)`` (symmpo.jl:45-97): open XXZ chain, D = 5."""
+    return fsm_u1mpo(lx, [(i, i + 1, delta, 1.0) for i in range(1, lx)], dtype, device)
+
+
+def cylinder_sites(W: int, Lx: int, ordering: str = "snake"):
+    """Site numbering (1-based) of a W x Lx cylinder: x major; 'snake' reverses odd columns."""
+    idx = {}
+    for x in range(Lx):
+        for y in range(W):
+            yy = y if (ordering != "snake" or x % 2 == 0) else W - 1 - y
+            idx[(x, y)] = x * W + yy + 1
+    return idx
+
+
+def heisenberg_cylinder_bonds(W: int, Lx: int, ordering: str = "snake"):
+    """Nearest-neighbour bonds (i < j) of a cylinder periodic in y (width W), open in x."""
+    idx = cylinder_sites(W, Lx, ordering)
+    bonds = set()
+    for x in range(Lx):
+        for y in range(W):
+            a = idx[(x, y)]
+            if W > 2 or y + 1 < W:
+                b = idx[(x, (y + 1) % W)]
+                if a != b:
+                    bonds.add((min(a, b), max(a, b)))
+            if x + 1 < Lx:
+                b = idx[(x + 1, y)]
+                bonds.add((min(a, b), max(a, b)))
+    return sorted(bonds)
+
+
+def heisenberg_cylinder_mpo(W: int, Lx: int, J: float = 1.0, ordering: str = "snake", dtype=torch.float64,
+                            device=None) -> MPOperator:
+    """Heisenberg J1 model on a W x Lx cylinder as a U(1) MPO (channels per open source site)."""
+    return fsm_u1mpo(W * Lx, [(i, j, J, J) for i, j in heisenberg_cylinder_bonds(W, Lx, ordering)], dtype, device)
+
+
+# ---------------------------------------------------------------------------
+# SVD step   (symtools.jl:1-35, mpsutils.jl:89-179)
+# ---------------------------------------------------------------------------
+def _blocks_view(M: SymTensor):
+    """[(sector, column-major matrix view)] of a rank-2 SymTensor's device data."""
+    sec, siz, off = M.struct.tables()
+    out = []
+    for b in range(M.struct.nblocks):
+        m, n = int(siz[b][0]), int(siz[b][1])
+        out.append(((int(sec[b][0]), int(sec[b][1])), M.data[off[b]:off[b] + m * n].view(n, m).t()))
+    return out
+
+
+def _block_svd(blk: torch.Tensor):
+    """FP64 thin SVD; on failure retries on the host with LAPACK gesvd (symtools.jl:14-19)."""
+    try:
+        U, S, Vh = torch.linalg.svd(blk, full_matrices=False)
+        if not torch.isfinite(S).all():
+            raise RuntimeError("non-finite singular values")
+        return U, S, Vh
+    except RuntimeError:
+        import scipy.linalg
+        u, s, vh = scipy.linalg.svd(blk.double().cpu().numpy(), full_matrices=False, lapack_driver="gesvd")
+        d = blk.device
+        return (torch.from_numpy(u).to(d, blk.dtype), torch.from_numpy(s).to(d, blk.dtype),
+                torch.from_numpy(vh).to(d, blk.dtype))
+
+
+def truncatedsizes(S: List[np.ndarray], maxdim: int = 200, tol: float = 1e-14) -> List[int]:
+    """``truncatedsizes`` (mpsutils.jl:89-112): greedy largest-first, absolute ``tol``."""
+    pointers = [1] * len(S)
+    c = [float(s[0]) for s in S]
+    total = sum(len(s) for s in S)
+    for n in range(1, min(maxdim, total) + 1):
+        index = int(np.argmax(c))
+        m = c[index]
+        if m < tol:
+            if n == 1:
+                raise ValueError(f"The largest singular value is smaller than tol! {m} < {tol}")
+            break
+        if pointers[index] >= len(S[index]):
+            pointers[index] += 1
+            c[index] = 0.0
+        else:
+            pointers[index] += 1
+            c[index] = float(S[index][pointers[index] - 1])
+    return [p - 1 for p in pointers]
+
+
+def _assemble(charge, space, parts: Dict[Tuple[int, int], torch.Tensor], dtype, device) -> SymTensor:
+    """Pack column-major blocks (dict sector -> matrix) into a SymTensor of (charge, space)."""
+    st = get_struct(charge, space)
+    sec, siz, off = st.tables()
+    data = torch.empty(st.numel, dtype=dtype, device=device)
+    for b in range(st.nblocks):
+        key = (int(sec[b][0]), int(sec[b][1]))
+        blk = parts[key]
+        data[off[b]:off[b + 1]] = blk.t().reshape(-1)
+    return SymTensor(charge, space, data, st)
+
+
+class SVDResult:
+    """u, s, v of ``svdtrunc`` kept as device blocks; products are formed on request."""
+
+    def __init__(self, M: SymTensor, keys, U, S, Vh, Vl: U1Space, Vr: U1Space):
+        self.M, self.keys, self.U, self.S, self.Vh, self.Vl, self.Vr = M, keys, U, S, Vh, Vl, Vr
+
+    def singular_values(self) -> np.ndarray:
+        if not self.S:
+            return np.zeros(0)
+        return np.sort(torch.cat(self.S).double().cpu().numpy())[::-1]
+
+    def norm_s(self) -> float:
+        return math.sqrt(sum(float((s.double() ** 2).sum()) for s in self.S))
+
+    def normalize_s(self):
+        """``normalize!(s)`` (symtensor.jl:325)."""
+        nrm = self.norm_s()
+        self.S = [s / nrm for s in self.S]
+
+    def u(self) -> SymTensor:
+        return _assemble(0, (self.M.space[0], self.Vl.dual()),
+                         {(k[0], k[0]): U for k, U in zip(self.keys, self.U)}, self.M.dtype, self.M.device)
+
+    def v(self) -> SymTensor:
+        return _assemble(0, (self.Vr.dual(), self.M.space[1]),
+                         {(k[1], k[1]): V for k, V in zip(self.keys, self.Vh)}, self.M.dtype, self.M.device)
+
+    def us(self) -> SymTensor:
+        """u*s (charge of s, legs (space1, Vr))."""
+        return _assemble(self.M.charge, (self.M.space[0], self.Vr),
+                         {(k[0], k[1]): U * s for k, U, s in zip(self.keys, self.U, self.S)}, self.M.dtype,
+                         self.M.device)
+
+    def sv(self) -> SymTensor:
+        """s*v (legs (Vl, space2))."""
+        return _assemble(self.M.charge, (self.Vl, self.M.space[1]),
+                         {(k[0], k[1]): s[:, None] * V for k, V, s in zip(self.keys, self.Vh, self.S)},
+                         self.M.dtype, self.M.device)
+
+    def usv(self) -> SymTensor:
+        """u*s*v: same structure as the factorised matrix (all its sectors; dropped ones are zero)."""
+        kept = {k: (U * s) @ V for k, U, s, V in zip(self.keys, self.U, self.S, self.Vh)}
+        sec, siz, off = self.M.struct.tables()
+        data = torch.zeros(self.M.struct.numel, dtype=self.M.dtype, device=self.M.device)
+        for b in range(self.M.struct.nblocks):
+            key = (int(sec[b][0]), int(sec[b][1]))
+            if key in kept:
+                data[off[b]:off[b + 1]] = kept[key].t().reshape(-1)
+        return SymTensor(self.M.charge, self.M.space, data, self.M.struct)
+
+
+def svdtrunc(M: SymTensor, maxdim: int = None, tol: float = 1e-12) -> SVDResult:
+    """``svdtrunc(A; maxdim, tol)`` (mpsutils.jl:116-179) on a rank-2 SymTensor."""
+    if maxdim is None:
+        maxdim = max(V.dim() for V in M.space)
+    items = _blocks_view(M)
+    facts = [_block_svd(blk) for _, blk in items]
+    svals = [f[1].double().cpu().numpy() for f in facts]
+    ns = truncatedsizes(svals, maxdim=maxdim, tol=tol)
+    keys, Us, Ss, Vs, midl, midr = [], [], [], [], [], []
+    for (key, _), (U, S, Vh), k in zip(items, facts, ns):
+        if k <= 0:
+            continue
+        keys.append(key)
+        Us.append(U[:, :k])
+        Ss.append(S[:k])
+        Vs.append(Vh[:k, :])
+        midl.append((key[0], k))
+        midr.append((key[1], k))
+    Vl = U1Space(*midl, isdual=M.space[0].isdual)
+    Vr = U1Space(*midr, isdual=M.space[1].isdual)
+    return SVDResult(M, keys, Us, Ss, Vs, Vl, Vr)
+
+
+def svd_(M: SymTensor) -> SVDResult:
+    """``_svd_(A)`` (symtools.jl:1-35): untruncated per-sector SVD."""
+    items = _blocks_view(M)
+    keys, Us, Ss, Vs, midl, midr = [], [], [], [], [], []
+    for key, blk in items:
+        U, S, Vh = _block_svd(blk)
+        keys.append(key)
+        Us.append(U)
+        Ss.append(S)
+        Vs.append(Vh)
+        midl.append((key[0], S.numel()))
+        midr.append((key[1], S.numel()))
+    Vl = U1Space(*midl, isdual=M.space[0].isdual)
+    Vr = U1Space(*midr, isdual=M.space[1].isdual)
+    return SVDResult(M, keys, Us, Ss, Vs, Vl, Vr)
+
+
+# ---------------------------------------------------------------------------
+# MPS construction and gauge   (symmps.jl:44-70, 150-200, 354-371)
+# ---------------------------------------------------------------------------
+def bond_spaces(L: int, N: int, chi: int, nsec: int = None, dmax: int = None) -> List[U1Space]:
+    """U(1) bond legs 0..L at total up-count N with Gaussian sector profiles capped at chi."""
+    out = [U1Space((0, 1))]
+    for l in range(1, L):
+        out.append(U1Space(*binom_capped_profile(l, L, N, chi, dmax=dmax, nsec=nsec)))
+    out.append(U1Space((N, 1)))
+    return out
+
+
+def random_mps(L: int, N: int, chi: int, seed: int = 1911, nsec: int = None, dmax: int = None,
+               dtype=torch.float64, device=None) -> MPState:
+    """Seeded Gaussian U(1) MPS (legs (l, p, r̄), charge 0 per site), right-canonical at 1, normalised."""
+    device = device or default_device()
+    Vd = U1Space((0, 1), (1, 1))
+    bs = bond_spaces(L, N, chi, nsec, dmax)
+    As = []
+    for l in range(1, L + 1):
+        space = (bs[l - 1], Vd, bs[l].dual())
+        st = get_struct(0, space)
+        if st.numel == 0:
+            raise ValueError(f"empty site tensor at {l}: widen the sector window")
+        As.append(SymTensor.from_flat(0, space, gaussian(seed, 1000 + l, st.numel), dtype, device))
+    mps = MPState(As, 0)
+    center_at(mps, 1)
+    normalize(mps)
+    return mps
+
+
+def _pushleft(mps: MPState, l: int):
+    """``_pushleft!`` (symmps.jl:167-174)."""
+    A = mps.As[l - 1]
+    r = svd_(fuselegs(A, 2, 2))
+    mps.As[l - 1] = splitleg(r.v(), 2, A.space[1:3])
+    mps.As[l - 2] = contract(mps.As[l - 2], (1, 2, -1), r.us(), (-1, 3))
+
+
+def _pushright(mps: MPState, l: int):
+    """``_pushright!`` (symmps.jl:176-183)."""
+    A = mps.As[l - 1]
+    r = svd_(fuselegs(A, 1, 2))
+    mps.As[l - 1] = splitleg(r.u(), 1, A.space[0:2])
+    mps.As[l] = contract(r.sv(), (1, -1), mps.As[l], (-1, 2, 3))
+
+
+def center_at(mps: MPState, center: int) -> MPState:
+    """``center_at!`` (symmps.jl:185-200)."""
+    x1, xL = 1, len(mps)
+    if mps.center > 0:
+        x1 = xL = mps.center
+    for p in range(x1, center):
+        _pushright(mps, p)
+    for p in range(xL, center, -1):
+        _pushleft(mps, p)
+    mps.center = center
+    return mps
+
+
+def normalize(mps: MPState) -> MPState:
+    """``normalize!(mps)`` (symmps.jl:150-165)."""
+    if mps.center < 1:
+        center_at(mps, 1)
+    A = mps.As[mps.center - 1]
+    if mps.center > len(mps) // 2:
+        r = svd_(fuselegs(A, 1, 2, False))
+        r.normalize_s()
+        mps.As[mps.center - 1] = splitleg(r.usv(), 1, A.space[0:2])
+    else:
+        r = svd_(fuselegs(A, 2, 2, True))
+        r.normalize_s()
+        mps.As[mps.center - 1] = splitleg(r.usv(), 2, A.space[1:3])
+    return mps
+
+
+def measure(mps: MPState, mpo: MPOperator) -> float:
+    """``measure(mps, mpo)`` (symmps.jl:354-371): <psi|H|psi> by a left sweep of device contracts."""
+    from .symtensors import fill
+    dummy = U1Space((0, 1))
+    Lt = fill(1.0, (dummy, dummy, dummy), dtype=mps.As[0].dtype, device=mps.As[0].device)
+    for A, W in zip(mps.As, mpo.Ws):
+        Lt = contract(contract(contract(Lt, (-1, 3, 4), A, (-1, 2, 1)), (1, -1, -2, 4), W, (-2, 3, 2, -1)),
+                      (1, 2, -1, -2), dual(A), (-2, -1, 3))
+    return float(contract(Lt, (-1, -2, -3), fill(1.0, (dummy, dummy, dummy), dtype=Lt.dtype, device=Lt.device),
+                          (-1, -2, -3)))

@@ -491,18 +491,21 @@ def contract(A: SymTensor, idxA: Sequence[int], B: SymTensor, idxB: Sequence[int
     if len(idxA) != A.rank or len(idxB) != B.rank:
         raise ValueError("index set length differs from tensor rank")
     if all(x < 0 for x in idxA) and all(x < 0 for x in idxB):
-        # SymVector(A) * SymVector(B): align B's legs to A's label order, then dot
-        order = {lab: i + 1 for i, lab in enumerate(idxB)}
-        Bp = permutelegs(B, [order[lab] for lab in idxA])
-        if not all(Bp.space[i] == A.space[i].dual() or
-                   (Bp.space[i].charges == A.space[i].charges and Bp.space[i].dims == A.space[i].dims)
-                   for i in range(A.rank)):
-            raise ValueError("not contractible!")
-        if A.dtype != Bp.dtype:
-            Bp = Bp.to(A.dtype)
+        # SymVector(A, consA, false) * SymVector(B, consB, true) (contract.jl:24-28, :101-110, :226-232)
+        consA = [i + 1 for i in sorted(range(len(idxA)), key=lambda i: abs(idxA[i]))]
+        consB = [i + 1 for i in sorted(range(len(idxB)), key=lambda i: abs(idxB[i]))]
+        Av = fuselegs(permutelegs(A, consA), 1, A.rank, False)
+        Bv = fuselegs(permutelegs(B, consB), 1, B.rank, True)
+        Va, Vb = Av.space[0], Bv.space[0]
+        if not (Va.charges == Vb.charges and Va.dims == Vb.dims and Va.isdual != Vb.isdual):
+            raise ValueError(f"not contractible! {Va} and {Vb}")
+        if Av.data.numel() == 0:
+            return 0.0
+        if A.dtype != B.dtype:
+            Bv = Bv.to(A.dtype)
         sc = _scratch.get(A.device)
         res = torch.empty(1, dtype=torch.float64, device=A.device)
-        check(lib().mpsh_dot(_DT[A.dtype], A.data.numel(), _ptr(A.data), _ptr(Bp.data), _ptr(res), _ptr(sc),
+        check(lib().mpsh_dot(_DT[A.dtype], Av.data.numel(), _ptr(Av.data), _ptr(Bv.data), _ptr(res), _ptr(sc),
                              _stream()))
         return float(res.item())
     if A.dtype != B.dtype:

@@ -143,3 +143,187 @@ def heff_workload(name: str, seed: int = 1911, dtype=torch.float64, device=None)
     WL = randn((Vw, Vd, Vw.dual(), Vd.dual()), 0, seed, 4, dtype, device)
     WR = randn((Vw, Vd, Vw.dual(), Vd.dual()), 0, seed, 5, dtype, device)
     return v, envL, envR, WL, WR
+
+
+# ---------------------------------------------------------------------------
+# Lanczos on device vectors (stands in for KrylovKit.eigsolve, dmrg.jl:77-82)
+# ---------------------------------------------------------------------------
+class EigInfo:
+    def __init__(self, normres, numiter, numops, converged):
+        self.normres, self.numiter, self.numops, self.converged = normres, numiter, numops, converged
+
+
+def lanczos_lowest(apply, x0: SymTensor, tol: float = 1e-7, krylovdim: int = 5, maxiter: int = 8):
+    """Restarted Lanczos for the lowest eigenpair of a Hermitian map on FP64 device vectors.
+
+    Each cycle builds up to ``krylovdim`` fully re-orthogonalised vectors
+    from the current Ritz vector (FP64 inner products on the device, FP64
+    tridiagonal eigenproblem); converged when |beta_k y_k| < tol.
+    """
+    import numpy as np
+    from .symtensors import axpby_, dot, norm, rmul_
+    x = x0.copy()
+    rmul_(x, 1.0 / norm(x))
+    numops, theta, res, it = 0, None, float("inf"), 0
+    for it in range(1, maxiter + 1):
+        V = [x]
+        alphas, betas = [], []
+        w = apply(x)
+        numops += 1
+        y = np.array([1.0])
+        for j in range(krylovdim):
+            a = dot(V[j], w)
+            alphas.append(a)
+            axpby_(-a, V[j], 1.0, w)
+            if j > 0:
+                axpby_(-betas[j - 1], V[j - 1], 1.0, w)
+            for i in range(j + 1):
+                axpby_(-dot(V[i], w), V[i], 1.0, w)
+            b = norm(w)
+            T = np.diag(alphas) + np.diag(betas, 1) + np.diag(betas, -1)
+            evals, evecs = np.linalg.eigh(T)
+            theta, y = float(evals[0]), evecs[:, 0]
+            res = abs(b * y[-1])
+            if res < tol or j == krylovdim - 1 or b < 1e-14:
+                break
+            betas.append(b)
+            rmul_(w, 1.0 / b)
+            V.append(w)
+            w = apply(w)
+            numops += 1
+        xn = V[0].copy()
+        rmul_(xn, float(y[0]))
+        for i in range(1, len(y)):
+            axpby_(float(y[i]), V[i], 1.0, xn)
+        rmul_(xn, 1.0 / norm(xn))
+        x = xn
+        if res < tol:
+            break
+    return theta, x, EigInfo(res, it, numops, res < tol)
+
+
+class HeffOperator:
+    """H_eff of one bond as a map on FP64 device vectors; matvec in ``prec`` (f64/f32/bf16)."""
+
+    def __init__(self, envL: SymTensor, envR: SymTensor, WL: SymTensor, WR: SymTensor, prec: str = "f64"):
+        self.prec = prec
+        self.ops64 = (envL, envR, WL, WR)
+        if prec == "f64":
+            self.ops = self.ops64
+        else:
+            self.ops = tuple(t.to(torch.float32) for t in self.ops64)
+        self.matvecs = 0
+
+    def __call__(self, x: SymTensor) -> SymTensor:
+        self.matvecs += 1
+        if self.prec == "f64":
+            return _applymps2site(x, *self.ops)
+        y = _applymps2site(x.to(torch.float32), *self.ops, prec=self.prec)
+        return y.to(torch.float64)
+
+    def exact(self, x: SymTensor) -> SymTensor:
+        self.matvecs += 1
+        return _applymps2site(x, *self.ops64)
+
+
+def eigsolve(H: HeffOperator, x0: SymTensor, tol=1e-7, krylovdim=5, maxiter=8, refine_tol=None):
+    """Lowest eigenpair; reduced-precision cycles then (optionally) FP64 refinement cycles."""
+    theta, x, info = lanczos_lowest(H, x0, tol, krylovdim, maxiter)
+    if H.prec != "f64" and refine_tol is not None:
+        theta, x, info2 = lanczos_lowest(H.exact, x, refine_tol, krylovdim, max(1, maxiter // 2))
+        info = EigInfo(info2.normres, info.numiter + info2.numiter, info.numops + info2.numops, info2.converged)
+    return theta, x, info
+
+
+# ---------------------------------------------------------------------------
+# environments and the sweep   (src/dmrg.jl)
+# ---------------------------------------------------------------------------
+def initialenv(mps, mpo, at: int = 1):
+    """``initialenv(mps, mpo; at)`` (dmrg.jl:9-36); returns env[1..L+2] as a 0-based list."""
+    from .symtensors import fill
+    if mps.center != at:
+        raise ValueError(f"MPS center has to be {at}")
+    lx = len(mps)
+    env = [None] * (lx + 2)
+    lVa, rVa = mps.leftspace(), mps.rightspace()
+    lVw, rVw = mpo.leftspace(), mpo.rightspace()
+    dt, dev = mps.As[0].dtype, mps.As[0].device
+    env[0] = fill(1.0, (lVa.dual(), lVw.dual(), lVa), dtype=dt, device=dev)
+    env[lx + 1] = fill(1.0, (rVa, rVw, rVa.dual()), dtype=dt, device=dev)
+    for l in range(lx, at - 1, -1):
+        env[l] = _mpsupdateright(env[l + 1], mps.As[l - 1], mpo.Ws[l - 1])
+    for l in range(1, at):
+        env[l] = _mpsupdateleft(env[l - 1], mps.As[l - 1], mpo.Ws[l - 1])
+    return env
+
+
+def dmrg2sitesweep(mps, mpo, env, maxdim: int = 200, tol: float = 1e-9, lanczostol: float = 1e-7,
+                   krylovdim: int = 5, krylovmaxiter: int = 8, verbose: bool = False, prec: str = "f64",
+                   refine_tol: float = None, log: list = None):
+    """``dmrg2sitesweep!(mps, mpo, env; ...)`` (dmrg.jl:58-179) with every contraction on the device.
+
+    Keeps the reference's energy list including quirk Q6 (each R->L energy
+    pushed twice, dmrg.jl:132,156).  ``prec`` is the H_eff arithmetic of the
+    Lanczos matvecs; with ``refine_tol`` the eigenpair of a reduced-precision
+    solve is refined by FP64 Lanczos cycles.  Energies after truncation are
+    always FP64 matvecs.
+    """
+    from .mpstools import svdtrunc
+    from .symtensors import dot, splitleg, symmatrix
+    lx = len(mps)
+    if mps.center != 1:
+        raise ValueError("The center of MPS has to be 1 for dmrg L->R->L")
+    Ws = mpo.Ws
+    energies = []
+
+    def solve(AA, l):
+        H = HeffOperator(env[l - 1], env[l + 2], Ws[l - 1], Ws[l], prec)
+        theta, v, info = eigsolve(H, AA, lanczostol, krylovdim, krylovmaxiter, refine_tol)
+        if log is not None:
+            log.append({"bond": l, "lanczos": theta, "normres": info.normres, "matvecs": info.numops})
+        if verbose:
+            print(f"bond {l},{l + 1}: energy {theta} normres {info.normres} #applications {info.numops}")
+        return v
+
+    def trunc(v, AA):
+        r = svdtrunc(symmatrix(v, [1, 2], [3, 4]), maxdim=maxdim, tol=tol)
+        if log is not None:
+            s = r.singular_values()
+            log.append({"trunc_1_minus_norm_s": 1 - r.norm_s(), "discarded_weight": max(0.0, 1 - float((s ** 2).sum()))})
+        r.normalize_s()
+        AA2 = splitleg(splitleg(r.usv(), 2, AA.space[2:4]), 1, AA.space[0:2])
+        return r, AA2
+
+    def energy(AA2, l):
+        return dot(AA2, _applymps2site(AA2, env[l - 1], env[l + 2], Ws[l - 1], Ws[l]))
+
+    A = mps.As[0]
+    for l in range(1, lx - 1):
+        AA = contract(A, (1, 2, -1), mps.As[l], (-1, 3, 4))
+        v = solve(AA, l)
+        r, AA2 = trunc(v, AA)
+        energies.append(energy(AA2, l))
+        mps.As[l - 1] = splitleg(r.u(), 1, A.space[0:2])
+        env[l] = _mpsupdateleft(env[l - 1], mps.As[l - 1], Ws[l - 1])
+        A = splitleg(r.sv(), 2, AA2.space[2:4])
+    l = lx - 1
+    AA = contract(A, (1, 2, -1), mps.As[l], (-1, 3, 4))
+    v = solve(AA, l)
+    e = None
+    for l in range(lx - 1, 1, -1):
+        r, AA2 = trunc(v, AA)
+        e = energy(AA2, l)
+        energies.append(e)
+        mps.As[l] = splitleg(r.v(), 2, AA2.space[2:4])
+        env[l + 1] = _mpsupdateright(env[l + 2], mps.As[l], Ws[l])
+        A = splitleg(r.us(), 1, AA2.space[0:2])
+        AA = contract(mps.As[l - 2], (1, 2, -1), A, (-1, 3, 4))
+        v = solve(AA, l - 1)
+        energies.append(e)
+    l = 1
+    r, AA2 = trunc(v, AA)
+    energies.append(energy(AA2, l))
+    mps.As[l] = splitleg(r.v(), 2, AA2.space[2:4])
+    env[l + 1] = _mpsupdateright(env[l + 2], mps.As[l], Ws[l])
+    mps.As[l - 1] = splitleg(r.us(), 1, AA2.space[0:2])
+    return energies

@@ -54,7 +54,8 @@ def mpsupdateright(renv: SymTensor, A: SymTensor, W: SymTensor) -> SymTensor:
 
 def dense_applymps2site(v, envL, envR, WL, WR):
     """Dense ``_applymps2site`` (apply.jl:80-91) as one einsum."""
-    return np.einsum("amb,apc,mqnp,nrkc,dke->bqre", envL, v, WL, WR, envR, optimize=True)
+    # envL[a,m,b] v[a,p,s,c] WL[m,q,n,p] WR[n,r,k,s] envR[c,k,e] -> out[b,q,r,e]
+    return np.einsum("amb,apsc,mqnp,nrks,cke->bqre", envL, v, WL, WR, envR, optimize=True)
 
 
 def dense_mpsupdateleft(lenv, A, W):
@@ -374,3 +375,57 @@ def dmrg2sitesweep(mps: MPState, mpo: MPOperator, env: List[SymTensor], maxdim=2
     env[l + 1] = mpsupdateright(env[l + 2], mps.As[l], Ws[l])
     mps.As[l - 1] = st.splitleg(st.matmul(u, s), 1, AA2.space[0:2])
     return energies
+
+
+# ---------------------------------------------------------------------------
+# Exact diagonalisation in a fixed-magnetisation block   (src/xxz.jl:56-113)
+# ---------------------------------------------------------------------------
+def u1_block_hamiltonian(L: int, bonds, M: int, jz=1.0, jxy=1.0):
+    """Sparse H on the states with M up spins (bit i = site i+1 up), for two-site terms on ``bonds``.
+
+    Restates ``xxz_u1sym`` (xxz.jl:56-113) for an arbitrary bond list: each
+    parallel pair adds +jz/4, each antiparallel pair adds -jz/4 and a
+    jxy/2 spin-flip matrix element.  ``bonds`` are 1-based (i, j) pairs.
+    """
+    import itertools
+    import scipy.sparse as sp
+    states = [sum(1 << b for b in c) for c in itertools.combinations(range(L), M)]
+    states.sort()
+    index = {s: k for k, s in enumerate(states)}
+    I, J, V = [], [], []
+    for k, s in enumerate(states):
+        for (i, j) in bonds:
+            a, b = i - 1, j - 1
+            if ((s >> a) & 1) == ((s >> b) & 1):
+                I.append(k); J.append(k); V.append(0.25 * jz)
+            else:
+                I.append(k); J.append(k); V.append(-0.25 * jz)
+                t = s ^ ((1 << a) | (1 << b))
+                I.append(k); J.append(index[t]); V.append(0.5 * jxy)
+    n = len(states)
+    return sp.csr_matrix((V, (I, J)), shape=(n, n))
+
+
+def u1_ground_energy(L: int, bonds, M: int, jz=1.0, jxy=1.0) -> float:
+    H = u1_block_hamiltonian(L, bonds, M, jz, jxy)
+    if H.shape[0] <= 2000:
+        return float(np.linalg.eigvalsh(H.toarray())[0])
+    import scipy.sparse.linalg as sla
+    return float(sla.eigsh(H, k=1, which="SA", tol=1e-13)[0][0])
+
+
+def random_u1mps_oracle(L, N, bond_pairs, flats):
+    """Oracle MPS from given bond charge->dim lists and flat Gaussian buffers (for parity tests)."""
+    Vd = st.U1Space((0, 1), (1, 1))
+    As = []
+    for l in range(1, L + 1):
+        Vl, Vr = st.Space(bond_pairs[l - 1]), st.Space(bond_pairs[l])
+        space = (Vl, Vd, Vr.dual())
+        sects, sizes = st.allsectorsandsizes(0, space)
+        blocks, p = {}, 0
+        for s, z in zip(sects, sizes):
+            n = int(np.prod(z))
+            blocks[s] = np.asarray(flats[l - 1][p:p + n]).reshape(z, order="F")
+            p += n
+        As.append(SymTensor(0, space, blocks, check=True))
+    return MPState(As, 0)
