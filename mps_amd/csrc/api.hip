@@ -67,13 +67,26 @@ struct DevTables {
     d.ntiles = c.ntiles;
     return d;
   }
-  DevGemm gemm(const GemmPlan& g) {
+  DevGemm gemm(const GemmPlan& g0, int BM = 0, int BN = 0) {
+    GemmPlan g = g0;
+    if (BM) g.BM = BM;
+    if (BN) g.BN = BN;
     DevGemm d;
-    std::vector<GemmTile> t = g.tiles(64, 64);
+    std::vector<GemmTile> t = g.tiles();
+    std::vector<RedDesc> r = g.reductions();
     d.probs = up(g.probs);
     d.segs = up(g.segs);
+    d.atab = up(g.atab);
+    d.btab = up(g.btab);
+    d.crow = up(g.crow);
+    d.cld = up(g.cld);
     d.tiles = up(t);
     d.ntiles = (int)t.size();
+    d.nred = (int)r.size() - 1;
+    d.red_groups = r.back().start;
+    if (d.nred > 0) d.reds = up(r);
+    d.BM = g.BM;
+    d.BN = g.BN;
     return d;
   }
 };
@@ -163,7 +176,7 @@ struct mpsh_contract_s {
 
 struct HeffDev {
   std::unique_ptr<DevTables> t;
-  DevGemm g1, g4;
+  DevGemm g1, g1_f64, g4;  // FP64 step 1 uses 64x64 tiles (register budget of the f64 MFMA tile)
   DevMpo mpo;
 };
 
@@ -179,6 +192,7 @@ struct mpsh_heff_s {
       HeffDev h;
       h.t = std::make_unique<DevTables>();
       h.g1 = h.t->gemm(p.g1);
+      h.g1_f64 = h.t->gemm(p.g1, 64, 64);
       h.g4 = h.t->gemm(p.g4);
       h.mpo.sigs = h.t->up(p.sigs);
       h.mpo.terms = h.t->up(p.terms);
@@ -345,7 +359,7 @@ mpsh_status mpsh_contract_execute(mpsh_contract plan, mpsh_dtype dt, const void*
   check(launch_copy(d.ga, es, A, ws, s), "gather A");
   check(launch_copy(d.gb, es, B, ws, s), "gather B");
   check(launch_gemm(d.gm, gemm_mode(dt), ws + plan->p.fa_off * es, ws + plan->p.fb_off * es,
-                    ws + plan->p.fc_off * es, s),
+                    ws + plan->p.fc_off * es, nullptr, s),
         "grouped gemm");
   check(launch_copy(d.sc, es, ws, C, s), "scatter C");
   return MPSH_OK;
@@ -416,9 +430,11 @@ mpsh_status mpsh_heff_apply_stages(mpsh_heff plan, mpsh_dtype dt, int32_t stages
   void* T1 = ws + p.t1_off * es;
   void* T3 = ws + p.t3_off * es;
   void* W = ws + p.w_off * es;
-  if (stages & 1) check(launch_gemm(d.g1, gemm_mode(dt), v, envL, T1, s), "step-1 gemm");
+  void* Pt = ws + p.part_off * es;
+  if (stages & 1)
+    check(launch_gemm(dt == MPSH_F64 ? d.g1_f64 : d.g1, gemm_mode(dt), v, envL, T1, Pt, s), "step-1 gemm");
   if (stages & 2) check(launch_mpo(d.mpo, es, WL, WR, W, T1, T3, s), "mpo");
-  if (stages & 4) check(launch_gemm(d.g4, gemm_mode(dt), T3, envR, out, s), "step-4 gemm");
+  if (stages & 4) check(launch_gemm(d.g4, gemm_mode(dt), T3, envR, out, Pt, s), "step-4 gemm");
   return MPSH_OK;
   API_CATCH
 }

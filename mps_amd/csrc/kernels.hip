@@ -133,8 +133,6 @@ struct GT<2> {  // BF16 operands, FP32 accumulate/storage
   static constexpr int BK = 32, PAD = 8;
 };
 
-constexpr int GBM = 64, GBN = 64;
-
 template <int MODE>
 __device__ __forceinline__ void mfma_chunk(typename GT<MODE>::Acc& c, const typename GT<MODE>::L* a,
                                            const typename GT<MODE>::L* b);
@@ -162,22 +160,26 @@ __device__ __forceinline__ void mfma_chunk<2>(floatx4& c, const __bf16* a, const
   c = __builtin_amdgcn_mfma_f32_16x16x32_bf16(av, bv, c, 0, 0, 0);
 }
 
-template <int MODE>
-__global__ __launch_bounds__(256) void gemm_grouped(const GemmProb* __restrict__ probs,
-                                                    const GemmSeg* __restrict__ segs,
-                                                    const GemmTile* __restrict__ tiles, int ntiles,
-                                                    const typename GT<MODE>::T* __restrict__ A,
-                                                    const typename GT<MODE>::T* __restrict__ B,
-                                                    typename GT<MODE>::T* __restrict__ C) {
+// Tile BM x BN, 4 waves in 2x2, each wave (BM/2) x (BN/2) = SM x SN MFMA 16x16
+// sub-tiles.  Operand rows come from base-address tables; the K loop walks
+// the problem's segments chunk by chunk (split-K: a contiguous chunk range).
+template <int MODE, int BM, int BN>
+__global__ __launch_bounds__(256) void gemm_grouped(
+    const GemmProb* __restrict__ probs, const GemmSeg* __restrict__ segs, const int64_t* __restrict__ atab,
+    const int64_t* __restrict__ btab, const int64_t* __restrict__ crow, const int64_t* __restrict__ cld,
+    const GemmTile* __restrict__ tiles, int ntiles, const typename GT<MODE>::T* __restrict__ A,
+    const typename GT<MODE>::T* __restrict__ B, typename GT<MODE>::T* __restrict__ C,
+    typename GT<MODE>::T* __restrict__ P) {
   typedef typename GT<MODE>::T T;
   typedef typename GT<MODE>::L L;
   typedef typename GT<MODE>::Acc Acc;
   constexpr int BK = GT<MODE>::BK, LD = GT<MODE>::BK + GT<MODE>::PAD;
-  constexpr int PER = GBM * BK / 256;   // elements per thread per operand
-  constexpr int RSTEP = 256 / BK;       // rows covered per pass
-  constexpr int KL = (MODE == 2) ? 8 : 4;  // K run read per lane per chunk step
-  __shared__ __attribute__((aligned(16))) L As[2][GBM][LD];
-  __shared__ __attribute__((aligned(16))) L Bs[2][GBN][LD];
+  constexpr int PA = BM * BK / 256, PB = BN * BK / 256;
+  constexpr int RSTEP = 256 / BK;
+  constexpr int KL = (MODE == 2) ? 8 : 4;
+  constexpr int SM = BM / 32, SN = BN / 32;
+  __shared__ __attribute__((aligned(16))) L As[2][BM][LD];
+  __shared__ __attribute__((aligned(16))) L Bs[2][BN][LD];
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   const int wm = wave >> 1, wn = wave & 1;
   const int kk = tid % BK, mm = tid / BK;
@@ -186,86 +188,147 @@ __global__ __launch_bounds__(256) void gemm_grouped(const GemmProb* __restrict__
   for (int t = blockIdx.x; t < ntiles; t += gridDim.x) {
     const GemmTile tile = tiles[t];
     const GemmProb p = probs[tile.prob];
-    Acc acc[2][2];
+    Acc acc[SM][SN];
 #pragma unroll
-    for (int i = 0; i < 2; ++i)
+    for (int i = 0; i < SM; ++i)
 #pragma unroll
-      for (int j = 0; j < 2; ++j)
+      for (int j = 0; j < SN; ++j)
 #pragma unroll
         for (int r = 0; r < 4; ++r) acc[i][j][r] = 0;
 
-    // chunk iterator over (segment, k0)
+    // chunk range [cb, ce) of this split over the flattened segment chunks
+    int64_t ctot = 0;
+    for (int s = 0; s < p.nseg; ++s) ctot += (segs[p.seg0 + s].K + BK - 1) / BK;
+    const int64_t cb = ctot * tile.split / p.nsplit, ce = ctot * (tile.split + 1) / p.nsplit;
     int seg = 0;
-    int64_t k0 = 0;
-    int64_t K = 0, a_off = 0, lda = 0, b_off = 0, ldb = 0;
-    bool have = p.nseg > 0;
-    if (have) {
-      const GemmSeg s = segs[p.seg0];
-      K = s.K; a_off = s.a_off; lda = s.lda; b_off = s.b_off; ldb = s.ldb;
+    int64_t k0 = 0, K = 0, cpos = 0;
+    for (; seg < p.nseg; ++seg) {
+      const int64_t nc = (segs[p.seg0 + seg].K + BK - 1) / BK;
+      if (cpos + nc > cb) {
+        k0 = (cb - cpos) * BK;
+        break;
+      }
+      cpos += nc;
     }
-    T ra[PER], rb[PER];
-    auto load = [&]() {
+    int64_t ab[PA], bb[PB];
+    auto bases = [&]() {
+      const GemmSeg sg = segs[p.seg0 + seg];
+      K = sg.K;
 #pragma unroll
-      for (int i = 0; i < PER; ++i) {
-        const int row = mm + i * RSTEP;
-        const int64_t k = k0 + kk;
-        const int m = tile.m0 + row, n = tile.n0 + row;
-        ra[i] = (k < K && m < p.M) ? A[a_off + k + (int64_t)m * lda] : T(0);
-        rb[i] = (k < K && n < p.N) ? B[b_off + k + (int64_t)n * ldb] : T(0);
+      for (int i = 0; i < PA; ++i) {
+        const int m = tile.m0 + mm + i * RSTEP;
+        ab[i] = m < p.M ? atab[sg.atab0 + m] : -1;
+      }
+#pragma unroll
+      for (int i = 0; i < PB; ++i) {
+        const int n = tile.n0 + mm + i * RSTEP;
+        bb[i] = n < p.N ? btab[sg.btab0 + n] : -1;
       }
     };
+    int64_t chunk = cb;
+    bool have = chunk < ce;
+    if (have) bases();
+    T ra[PA], rb[PB];
+    auto load = [&]() {
+      const int64_t k = k0 + kk;
+      const bool kin = k < K;
+#pragma unroll
+      for (int i = 0; i < PA; ++i) ra[i] = (kin && ab[i] >= 0) ? A[ab[i] + k] : T(0);
+#pragma unroll
+      for (int i = 0; i < PB; ++i) rb[i] = (kin && bb[i] >= 0) ? B[bb[i] + k] : T(0);
+    };
     auto advance = [&]() {
+      ++chunk;
       k0 += BK;
+      if (chunk >= ce) {
+        have = false;
+        return;
+      }
       if (k0 >= K) {
         ++seg;
         k0 = 0;
-        if (seg < p.nseg) {
-          const GemmSeg s = segs[p.seg0 + seg];
-          K = s.K; a_off = s.a_off; lda = s.lda; b_off = s.b_off; ldb = s.ldb;
-        } else {
-          have = false;
-        }
+        bases();
       }
     };
     if (have) load();
     int buf = 0;
     while (have) {
 #pragma unroll
-      for (int i = 0; i < PER; ++i) {
-        As[buf][mm + i * RSTEP][kk] = (L)ra[i];
-        Bs[buf][mm + i * RSTEP][kk] = (L)rb[i];
-      }
+      for (int i = 0; i < PA; ++i) As[buf][mm + i * RSTEP][kk] = (L)ra[i];
+#pragma unroll
+      for (int i = 0; i < PB; ++i) Bs[buf][mm + i * RSTEP][kk] = (L)rb[i];
       __syncthreads();
       advance();
       if (have) load();
 #pragma unroll
       for (int kc = 0; kc < BK; kc += 4 * KL) {
+        const L* bp[SN];
 #pragma unroll
-        for (int i = 0; i < 2; ++i) {
-          const L* ap = &As[buf][wm * 32 + i * 16 + l16][kc + KL * g];
+        for (int j = 0; j < SN; ++j) bp[j] = &Bs[buf][wn * (BN / 2) + j * 16 + l16][kc + KL * g];
 #pragma unroll
-          for (int j = 0; j < 2; ++j) {
-            const L* bp = &Bs[buf][wn * 32 + j * 16 + l16][kc + KL * g];
-            mfma_chunk<MODE>(acc[i][j], ap, bp);
-          }
+        for (int i = 0; i < SM; ++i) {
+          const L* ap = &As[buf][wm * (BM / 2) + i * 16 + l16][kc + KL * g];
+#pragma unroll
+          for (int j = 0; j < SN; ++j) mfma_chunk<MODE>(acc[i][j], ap, bp[j]);
         }
       }
       buf ^= 1;
     }
     // epilogue: column l16 of each 16x16 sub-tile; accumulator register r holds
     // row g + 4r for the FP64 16x16x4 form and row 4g + r for the FP32/BF16 forms.
+    const bool split = p.nsplit > 1;
+    T* Pp = P + (split ? p.part_off + (int64_t)tile.split * p.M * p.N : 0);
 #pragma unroll
-    for (int i = 0; i < 2; ++i)
+    for (int i = 0; i < SM; ++i)
 #pragma unroll
-      for (int j = 0; j < 2; ++j) {
-        const int n = tile.n0 + wn * 32 + j * 16 + l16;
+      for (int r = 0; r < 4; ++r) {
+        const int m = tile.m0 + wm * (BM / 2) + i * 16 + (MODE == 0 ? g + 4 * r : 4 * g + r);
+        if (m >= p.M) continue;
+        int64_t cb_ = 0, cl_ = 0;
+        if (!split) {
+          cb_ = crow[p.crow0 + m];
+          cl_ = cld[p.crow0 + m];
+        }
 #pragma unroll
-        for (int r = 0; r < 4; ++r) {
-          const int m = tile.m0 + wm * 32 + i * 16 + (MODE == 0 ? g + 4 * r : 4 * g + r);
-          if (m < p.M && n < p.N) C[p.c_off + m + (int64_t)n * p.ldc] = (T)acc[i][j][r];
+        for (int j = 0; j < SN; ++j) {
+          const int n = tile.n0 + wn * (BN / 2) + j * 16 + l16;
+          if (n >= p.N) continue;
+          if (split)
+            Pp[m + (int64_t)p.M * n] = (T)acc[i][j][r];
+          else
+            C[cb_ + (int64_t)n * cl_] = (T)acc[i][j][r];
         }
       }
     __syncthreads();  // LDS reuse by the next tile
+  }
+}
+
+// C(m, n) = sum_s P[part_off + s*M*N + m + M*n] for every split problem
+template <typename T>
+__global__ __launch_bounds__(256) void splitk_reduce(const RedDesc* __restrict__ reds, int nred,
+                                                     const GemmProb* __restrict__ probs,
+                                                     const int64_t* __restrict__ crow,
+                                                     const int64_t* __restrict__ cld, const T* __restrict__ P,
+                                                     T* __restrict__ C) {
+  const int64_t bid = blockIdx.x;
+  int lo = 0, hi = nred - 1;
+  while (lo < hi) {
+    const int mid = (lo + hi + 1) >> 1;
+    if (reds[mid].start <= bid)
+      lo = mid;
+    else
+      hi = mid - 1;
+  }
+  const GemmProb p = probs[reds[lo].prob];
+  const int64_t MN = (int64_t)p.M * p.N;
+  const int64_t e0 = (bid - reds[lo].start) * 1024;
+  for (int i = threadIdx.x; i < 1024; i += 256) {
+    const int64_t e = e0 + i;
+    if (e >= MN) break;
+    const int64_t n = e / p.M, m = e - n * p.M;
+    T acc = 0;
+    for (int s = 0; s < p.nsplit; ++s) acc += P[p.part_off + s * MN + e];
+    C[crow[p.crow0 + m] + n * cld[p.crow0 + m]] = acc;
   }
 }
 
@@ -282,19 +345,48 @@ static int num_cu() {
   return g_num_cu;
 }
 
-hipError_t launch_gemm(const DevGemm& gm, int mode, const void* A, const void* B, void* C, hipStream_t s) {
-  if (gm.ntiles == 0) return hipSuccess;
+template <int MODE, int BM, int BN>
+static void launch_variant(const DevGemm& gm, const void* A, const void* B, void* C, void* P, hipStream_t s) {
+  typedef typename GT<MODE>::T T;
   const int maxgrid = num_cu() * 8;
   dim3 grid((unsigned)(gm.ntiles < maxgrid ? gm.ntiles : maxgrid)), block(256);
-  if (mode == 0)
-    hipLaunchKernelGGL(gemm_grouped<0>, grid, block, 0, s, gm.probs, gm.segs, gm.tiles, gm.ntiles,
-                       (const double*)A, (const double*)B, (double*)C);
-  else if (mode == 1)
-    hipLaunchKernelGGL(gemm_grouped<1>, grid, block, 0, s, gm.probs, gm.segs, gm.tiles, gm.ntiles,
-                       (const float*)A, (const float*)B, (float*)C);
+  hipLaunchKernelGGL((gemm_grouped<MODE, BM, BN>), grid, block, 0, s, gm.probs, gm.segs, gm.atab, gm.btab,
+                     gm.crow, gm.cld, gm.tiles, gm.ntiles, (const T*)A, (const T*)B, (T*)C, (T*)P);
+}
+
+template <int MODE>
+static hipError_t launch_mode(const DevGemm& gm, const void* A, const void* B, void* C, void* P, hipStream_t s) {
+  if (gm.BM == 64 && gm.BN == 64)
+    launch_variant<MODE, 64, 64>(gm, A, B, C, P, s);
+  else if (gm.BM == 128 && gm.BN == 64)
+    launch_variant<MODE, 128, 64>(gm, A, B, C, P, s);
+  else if (gm.BM == 64 && gm.BN == 128)
+    launch_variant<MODE, 64, 128>(gm, A, B, C, P, s);
+  else if (gm.BM == 128 && gm.BN == 128)
+    launch_variant<MODE, 128, 128>(gm, A, B, C, P, s);
   else
-    hipLaunchKernelGGL(gemm_grouped<2>, grid, block, 0, s, gm.probs, gm.segs, gm.tiles, gm.ntiles,
-                       (const float*)A, (const float*)B, (float*)C);
+    return hipErrorInvalidValue;
+  return hipGetLastError();
+}
+
+hipError_t launch_gemm(const DevGemm& gm, int mode, const void* A, const void* B, void* C, void* P,
+                       hipStream_t s) {
+  if (gm.ntiles == 0) return hipSuccess;
+  if (gm.nred > 0 && P == nullptr) return hipErrorInvalidValue;
+  hipError_t e;
+  if (mode == 0)
+    e = launch_mode<0>(gm, A, B, C, P, s);
+  else if (mode == 1)
+    e = launch_mode<1>(gm, A, B, C, P, s);
+  else
+    e = launch_mode<2>(gm, A, B, C, P, s);
+  if (e != hipSuccess || gm.nred == 0) return e;
+  if (mode == 0)
+    hipLaunchKernelGGL(splitk_reduce<double>, dim3((unsigned)gm.red_groups), dim3(256), 0, s, gm.reds, gm.nred,
+                       gm.probs, gm.crow, gm.cld, (const double*)P, (double*)C);
+  else
+    hipLaunchKernelGGL(splitk_reduce<float>, dim3((unsigned)gm.red_groups), dim3(256), 0, s, gm.reds, gm.nred,
+                       gm.probs, gm.crow, gm.cld, (const float*)P, (float*)C);
   return hipGetLastError();
 }
 

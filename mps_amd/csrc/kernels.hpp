@@ -17,8 +17,12 @@ struct DevCopy {
 struct DevGemm {
   const GemmProb* probs = nullptr;
   const GemmSeg* segs = nullptr;
+  const int64_t *atab = nullptr, *btab = nullptr, *crow = nullptr, *cld = nullptr;
   const GemmTile* tiles = nullptr;
-  int ntiles = 0;
+  const RedDesc* reds = nullptr;
+  int ntiles = 0, nred = 0;
+  int64_t red_groups = 0;
+  int BM = 64, BN = 64;
 };
 
 struct DevMpo {
@@ -34,7 +38,8 @@ struct DevMpo {
 
 hipError_t launch_copy(const DevCopy& c, int elem_size, const void* src, void* dst, hipStream_t s);
 // mode 0 = f64, 1 = f32, 2 = bf16 operands from f32 storage
-hipError_t launch_gemm(const DevGemm& g, int mode, const void* A, const void* B, void* C, hipStream_t s);
+// P: split-K partial buffer (unused when no problem is split)
+hipError_t launch_gemm(const DevGemm& g, int mode, const void* A, const void* B, void* C, void* P, hipStream_t s);
 hipError_t launch_mpo(const DevMpo& d, int elem_size, const void* WL, const void* WR, void* W, const void* T1,
                       void* T3, hipStream_t s);
 hipError_t launch_dot(int elem_size, int64_t n, const void* x, const void* y, double* out, double* partials,

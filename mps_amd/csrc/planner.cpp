@@ -228,20 +228,81 @@ void CopyPlan::finalize() {
   ntiles = tile_start.back();
 }
 
-std::vector<GemmTile> GemmPlan::tiles(int BM, int BN) const {
+void GemmPlan::add_seg_tab(int64_t K, const std::vector<int64_t>& arows, const std::vector<int64_t>& brows) {
+  GemmProb& p = probs.back();
+  if ((int64_t)arows.size() != p.M || (int64_t)brows.size() != p.N)
+    throw Error(ERR_INVALID, "gemm segment row tables do not match the problem");
+  segs.push_back(GemmSeg{K, (int64_t)atab.size(), (int64_t)btab.size()});
+  atab.insert(atab.end(), arows.begin(), arows.end());
+  btab.insert(btab.end(), brows.begin(), brows.end());
+  p.nseg++;
+  flops += 2 * (int64_t)p.M * p.N * K;
+}
+
+void GemmPlan::add_seg(int64_t K, int64_t a_off, int64_t lda, int64_t b_off, int64_t ldb) {
+  const GemmProb& p = probs.back();
+  std::vector<int64_t> a(p.M), b(p.N);
+  for (int64_t m = 0; m < p.M; ++m) a[m] = a_off + m * lda;
+  for (int64_t n = 0; n < p.N; ++n) b[n] = b_off + n * ldb;
+  add_seg_tab(K, a, b);
+}
+
+static int64_t prob_chunks(const GemmPlan& g, const GemmProb& p, int BK) {
+  int64_t c = 0;
+  for (int s = 0; s < p.nseg; ++s) c += (g.segs[p.seg0 + s].K + BK - 1) / BK;
+  return c;
+}
+
+void GemmPlan::choose_splits(int64_t target, int BK, int64_t min_chunks_per_split) {
+  int64_t tiles = 0;
+  for (auto& p : probs) tiles += ((p.M + BM - 1) / BM) * ((p.N + BN - 1) / BN);
+  part_elems = 0;
+  for (auto& p : probs) {
+    p.nsplit = 1;
+    p.part_off = 0;
+  }
+  if (tiles == 0 || tiles >= target) return;
+  const int64_t want = (target + tiles - 1) / tiles;
+  for (auto& p : probs) {
+    const int64_t ch = prob_chunks(*this, p, BK);
+    int64_t s = std::min<int64_t>(want, ch / std::max<int64_t>(1, min_chunks_per_split));
+    if (s > 1) {
+      p.nsplit = (int32_t)s;
+      p.part_off = part_elems;
+      part_elems += s * (int64_t)p.M * p.N;
+    }
+  }
+}
+
+std::vector<GemmTile> GemmPlan::tiles() const {
   std::vector<std::pair<int64_t, GemmTile>> t;
-  for (size_t p = 0; p < probs.size(); ++p) {
-    const GemmProb& q = probs[p];
+  for (size_t pi = 0; pi < probs.size(); ++pi) {
+    const GemmProb& q = probs[pi];
     int64_t K = 0;
     for (int s = 0; s < q.nseg; ++s) K += segs[q.seg0 + s].K;
+    const int64_t Ks = K / std::max(1, q.nsplit);
     for (int m0 = 0; m0 < q.M; m0 += BM)
-      for (int n0 = 0; n0 < q.N; n0 += BN) t.push_back({K, GemmTile{(int32_t)p, m0, n0, 0}});
+      for (int n0 = 0; n0 < q.N; n0 += BN)
+        for (int sp = 0; sp < q.nsplit; ++sp) t.push_back({Ks, GemmTile{(int32_t)pi, m0, n0, sp}});
   }
   std::stable_sort(t.begin(), t.end(), [](const auto& a, const auto& b) { return a.first > b.first; });
   std::vector<GemmTile> out;
   out.reserve(t.size());
   for (auto& x : t) out.push_back(x.second);
   return out;
+}
+
+std::vector<RedDesc> GemmPlan::reductions() const {
+  std::vector<RedDesc> r;
+  int64_t start = 0;
+  for (size_t pi = 0; pi < probs.size(); ++pi) {
+    const GemmProb& q = probs[pi];
+    if (q.nsplit <= 1) continue;
+    r.push_back(RedDesc{(int32_t)pi, 0, start});
+    start += ((int64_t)q.M * q.N + 1023) / 1024;
+  }
+  r.push_back(RedDesc{-1, 0, start});  // sentinel: total work groups
+  return r;
 }
 
 // ---------------------------------------------------------------------------
@@ -429,7 +490,8 @@ ContractPlan contract_plan(const Struct& A, const std::vector<int>& idxA, const 
       sB = {-B.charge + _c2, c2};
     }
     int64_t ia = P.FA.F.find(sA), ib = P.FB.F.find(sB);
-    P.gemm.add_prob(M, Nn, P.FC.offsets[f], M);
+    P.gemm.add_prob(M, Nn);
+    P.gemm.add_crows_affine(P.FC.offsets[f], M, M);
     if (ia >= 0 && ib >= 0) {
       const int64_t K = P.FA.F.size(ia)[1];
       if (P.FB.F.size(ib)[0] != K) throw Error(ERR_SECTOR_MISMATCH, "fused contraction dims differ");
@@ -704,12 +766,6 @@ RelegPlan symmatrix_releg(const Struct& A, const std::vector<int>& rows, const s
 // ---------------------------------------------------------------------------
 typedef std::vector<int64_t> Key;
 
-static std::unordered_map<int64_t, std::vector<int64_t>> index_by_leg(const Struct& S, int leg) {
-  std::unordered_map<int64_t, std::vector<int64_t>> m;
-  for (int64_t b = 0; b < S.nblocks; ++b) m[S.sector(b)[leg]].push_back(b);
-  return m;
-}
-
 static void need_pair(const Leg& a, const Leg& b, const char* what) {
   if (!isdual_pair(a, b)) throw Error(ERR_SPACE_MISMATCH, std::string("not contractible! ") + what);
 }
@@ -750,55 +806,120 @@ HeffPlan heff_plan(const Struct& v, const Struct& envL, const Struct& envR, cons
       throw Error(ERR_SECTOR_MISMATCH, "heff: output structure differs from the contraction chain");
   }
 
-  // ---- step 1: T1 blocks = matched (envL, v) pairs, ordered by pixel group (b, r)
+  // ---- step 1: one fused GEMM per ket charge k (the reference's fused sizes),
+  // T1_k = [v blocks (k,.)]^T x [envL blocks (k,.)], stored column-major (ld M_k).
   struct T1B {
     Key key;  // (b, w, s1, s2, r)
-    int64_t e, x, off, db, Dw, d1, d2, dR, dk;
+    int64_t base, rowoff, coloff, Mf, db, Dw, d1, d2, dR;
   };
   std::vector<T1B> t1;
+  H.g1.BM = 128;
+  H.g1.BN = 128;
   {
-    auto vby = index_by_leg(v, 0);
-    for (int64_t e = 0; e < envL.nblocks; ++e) {
-      const int64_t* es = envL.sector(e);
-      auto it = vby.find(es[0]);
-      if (it == vby.end()) continue;
-      for (int64_t x : it->second) {
-        const int64_t* xs = v.sector(x);
-        T1B t;
-        t.key = {es[2], es[1], xs[1], xs[2], xs[3]};
-        t.e = e;
-        t.x = x;
-        t.db = envL.size(e)[2];
-        t.Dw = envL.size(e)[1];
-        t.dk = envL.size(e)[0];
-        t.d1 = v.size(x)[1];
-        t.d2 = v.size(x)[2];
-        t.dR = v.size(x)[3];
-        t1.push_back(t);
-      }
-    }
-    std::stable_sort(t1.begin(), t1.end(), [](const T1B& a, const T1B& b) {
-      auto ka = std::make_tuple(a.key[0], a.key[4], a.key[1], a.key[2], a.key[3]);
-      auto kb = std::make_tuple(b.key[0], b.key[4], b.key[1], b.key[2], b.key[3]);
-      return ka < kb;
-    });
+    std::map<int64_t, std::vector<int64_t>> vby, eby;
+    for (int64_t x = 0; x < v.nblocks; ++x) vby[v.sector(x)[0]].push_back(x);
+    for (int64_t e = 0; e < envL.nblocks; ++e) eby[envL.sector(e)[0]].push_back(e);
     int64_t off = 0;
-    for (auto& t : t1) {
-      t.off = off;
-      off += t.d1 * t.d2 * t.dR * t.Dw * t.db;
-      const int64_t M = t.d1 * t.d2 * t.dR, N = t.Dw * t.db;
-      H.g1.add_prob(M, N, t.off, M);
-      H.g1.add_seg(t.dk, v.offsets[t.x], t.dk, envL.offsets[t.e], t.dk);
+    for (auto& kv : eby) {
+      auto itv = vby.find(kv.first);
+      if (itv == vby.end()) continue;
+      const std::vector<int64_t>& X = itv->second;
+      const std::vector<int64_t>& E = kv.second;
+      const int64_t dk = envL.size(E[0])[0];
+      std::vector<int64_t> arows, brows, rowoff(X.size()), coloff(E.size());
+      for (size_t i = 0; i < X.size(); ++i) {
+        const int64_t* z = v.size(X[i]);
+        rowoff[i] = (int64_t)arows.size();
+        for (int64_t c = 0; c < z[1] * z[2] * z[3]; ++c) arows.push_back(v.offsets[X[i]] + dk * c);
+      }
+      for (size_t j = 0; j < E.size(); ++j) {
+        const int64_t* z = envL.size(E[j]);
+        coloff[j] = (int64_t)brows.size();
+        for (int64_t c = 0; c < z[1] * z[2]; ++c) brows.push_back(envL.offsets[E[j]] + dk * c);
+      }
+      const int64_t Mf = (int64_t)arows.size(), Nf = (int64_t)brows.size();
+      H.g1.add_prob(Mf, Nf);
+      H.g1.add_crows_affine(off, Mf, Mf);
+      H.g1.add_seg_tab(dk, arows, brows);
+      for (size_t i = 0; i < X.size(); ++i)
+        for (size_t j = 0; j < E.size(); ++j) {
+          const int64_t* xs = v.sector(X[i]);
+          const int64_t* es = envL.sector(E[j]);
+          T1B t;
+          t.key = {es[2], es[1], xs[1], xs[2], xs[3]};
+          t.base = off;
+          t.rowoff = rowoff[i];
+          t.coloff = coloff[j];
+          t.Mf = Mf;
+          t.db = envL.size(E[j])[2];
+          t.Dw = envL.size(E[j])[1];
+          t.d1 = v.size(X[i])[1];
+          t.d2 = v.size(X[i])[2];
+          t.dR = v.size(X[i])[3];
+          t1.push_back(t);
+        }
+      off += Mf * Nf;
     }
     H.t1_elems = off;
+    std::stable_sort(t1.begin(), t1.end(), [](const T1B& a, const T1B& b) {
+      return std::make_tuple(a.key[0], a.key[4], a.key[1], a.key[2], a.key[3]) <
+             std::make_tuple(b.key[0], b.key[4], b.key[1], b.key[2], b.key[3]);
+    });
   }
 
-  // ---- steps 2+3: pixel groups (b, r); in rows = T1 blocks, out rows = T3 blocks
-  struct T3B {
-    Key key;  // (b, s1o, s2o, x, r)
-    int64_t off, db, d1o, d2o, Dx, dR;
+  // ---- step 4 layout: one fused GEMM per output ket charge r'; rows = the out
+  // blocks (b, s1o, s2o, r'), K segments = envR blocks (r, x, r').  T3 region of
+  // (r', f) is K_f x M_f, K-contiguous; cell (o, f) is T3 block (b, s1o, s2o, x, r).
+  struct Cell {
+    int64_t base, rowoff, Kf, db, d1o, d2o, Dx, dR;
   };
-  std::map<Key, T3B> t3map;  // ordered by key for deterministic allocation
+  std::map<Key, Cell> cells;  // key (b, s1o, s2o, x, r)
+  H.g4.BM = 64;
+  H.g4.BN = 64;
+  {
+    std::map<int64_t, std::vector<int64_t>> oby, fby;
+    for (int64_t o = 0; o < H.out.nblocks; ++o) oby[H.out.sector(o)[3]].push_back(o);
+    for (int64_t f = 0; f < envR.nblocks; ++f) fby[envR.sector(f)[2]].push_back(f);
+    int64_t off = 0;
+    for (auto& kv : oby) {
+      const std::vector<int64_t>& O = kv.second;
+      std::vector<int64_t> rowoff(O.size());
+      int64_t Mf = 0;
+      for (size_t i = 0; i < O.size(); ++i) {
+        rowoff[i] = Mf;
+        const int64_t* z = H.out.size(O[i]);
+        Mf += z[0] * z[1] * z[2];
+      }
+      const int64_t N = H.out.size(O[0])[3];
+      H.g4.add_prob(Mf, N);
+      for (size_t i = 0; i < O.size(); ++i) {
+        const int64_t* z = H.out.size(O[i]);
+        const int64_t Mo = z[0] * z[1] * z[2];
+        for (int64_t m = 0; m < Mo; ++m) H.g4.add_crow(H.out.offsets[O[i]] + m, Mo);
+      }
+      auto itf = fby.find(kv.first);
+      if (itf == fby.end()) continue;
+      for (int64_t f : itf->second) {
+        const int64_t* fs = envR.sector(f);
+        const int64_t dR = envR.size(f)[0], Dx = envR.size(f)[1];
+        const int64_t Kf = dR * Dx;
+        std::vector<int64_t> arows(Mf), brows(N);
+        for (int64_t m = 0; m < Mf; ++m) arows[m] = off + Kf * m;
+        for (int64_t n = 0; n < N; ++n) brows[n] = envR.offsets[f] + Kf * n;
+        H.g4.add_seg_tab(Kf, arows, brows);
+        for (size_t i = 0; i < O.size(); ++i) {
+          const int64_t* os = H.out.sector(O[i]);
+          const int64_t* z = H.out.size(O[i]);
+          cells[{os[0], os[1], os[2], fs[1], fs[0]}] = Cell{off, rowoff[i], Kf, z[0], z[1], z[2], Dx, dR};
+        }
+        off += Kf * Mf;
+      }
+    }
+    H.t3_elems = off;
+    H.g4.choose_splits(2048, 16, 16);
+  }
+
+  // ---- steps 2+3: pixel groups (b, r); out types = the T3 cells of (b, r)
   auto wl_by = [&]() {
     std::map<std::pair<int64_t, int64_t>, std::vector<int64_t>> m;  // (w, s1) -> WL blocks
     for (int64_t b = 0; b < WL.nblocks; ++b) m[{WL.sector(b)[0], WL.sector(b)[3]}].push_back(b);
@@ -809,43 +930,32 @@ HeffPlan heff_plan(const Struct& v, const Struct& envL, const Struct& envR, cons
     for (int64_t b = 0; b < WR.nblocks; ++b) m[{WR.sector(b)[0], WR.sector(b)[3]}].push_back(b);
     return m;
   }();
+  std::map<std::pair<int64_t, int64_t>, std::vector<Key>> cells_by_group;  // (b, r) -> out types
+  for (auto& c : cells) cells_by_group[{c.first[0], c.first[4]}].push_back({c.first[1], c.first[2], c.first[3]});
+  std::map<std::pair<int64_t, int64_t>, std::pair<size_t, size_t>> t1_range;
+  for (size_t i = 0; i < t1.size();) {
+    size_t j = i;
+    while (j < t1.size() && t1[j].key[0] == t1[i].key[0] && t1[j].key[4] == t1[i].key[4]) ++j;
+    t1_range[{t1[i].key[0], t1[i].key[4]}] = {i, j};
+    i = j;
+  }
   std::map<std::vector<int64_t>, int32_t> sigindex;
-  size_t gi0 = 0;
-  int64_t w_off = 0, t3_off = 0;
-  while (gi0 < t1.size()) {
-    size_t gi1 = gi0;
-    while (gi1 < t1.size() && t1[gi1].key[0] == t1[gi0].key[0] && t1[gi1].key[4] == t1[gi0].key[4]) ++gi1;
-    const int64_t b = t1[gi0].key[0], r = t1[gi0].key[4];
-    const int64_t db = t1[gi0].db, dR = t1[gi0].dR;
-    // in types in group order (already sorted by w, s1, s2); collect out types
-    std::set<Key> outtypes;  // (s1o, s2o, x)
-    std::map<Key, std::array<int64_t, 3>> od;
-    for (size_t i = gi0; i < gi1; ++i) {
-      const Key& k = t1[i].key;
-      auto itl = wl_by.find({k[1], k[2]});
-      if (itl == wl_by.end()) continue;
-      for (int64_t wl : itl->second) {
-        const int64_t m = WL.sector(wl)[2], s1o = WL.sector(wl)[1];
-        auto itr = wr_by.find({m, k[3]});
-        if (itr == wr_by.end()) continue;
-        for (int64_t wr : itr->second) {
-          Key ot = {s1o, WR.sector(wr)[1], WR.sector(wr)[2]};
-          outtypes.insert(ot);
-          od[ot] = {WR.size(wr)[2], WL.size(wl)[1], WR.size(wr)[1]};
-        }
-      }
+  int64_t w_off = 0;
+  for (auto& gc : cells_by_group) {
+    const int64_t b = gc.first.first, r = gc.first.second;
+    const std::vector<Key>& outtypes = gc.second;  // sorted (map order)
+    size_t gi0 = 0, gi1 = 0;
+    auto itr = t1_range.find(gc.first);
+    if (itr != t1_range.end()) {
+      gi0 = itr->second.first;
+      gi1 = itr->second.second;
     }
-    if (outtypes.empty()) {
-      gi0 = gi1;
-      continue;
-    }
-    // signature = in types + out types
+    const Cell& c0 = cells[{b, outtypes[0][0], outtypes[0][1], outtypes[0][2], r}];
+    const int64_t db = c0.db, dR = c0.dR;
     Key sigkey;
     for (size_t i = gi0; i < gi1; ++i) sigkey.insert(sigkey.end(), {t1[i].key[1], t1[i].key[2], t1[i].key[3]});
     sigkey.push_back(INT64_MIN);
     for (auto& ot : outtypes) sigkey.insert(sigkey.end(), ot.begin(), ot.end());
-    int32_t sig;
-    auto its = sigindex.find(sigkey);
     std::vector<int64_t> col0s, row0s;
     int64_t C = 0, R = 0;
     for (size_t i = gi0; i < gi1; ++i) {
@@ -854,8 +964,11 @@ HeffPlan heff_plan(const Struct& v, const Struct& envL, const Struct& envR, cons
     }
     for (auto& ot : outtypes) {
       row0s.push_back(R);
-      R += od[ot][0] * od[ot][1] * od[ot][2];
+      const Cell& cc = cells[{b, ot[0], ot[1], ot[2], r}];
+      R += cc.Dx * cc.d1o * cc.d2o;
     }
+    int32_t sig;
+    auto its = sigindex.find(sigkey);
     if (its == sigindex.end()) {
       sig = (int32_t)H.sigs.size();
       sigindex[sigkey] = sig;
@@ -863,17 +976,18 @@ HeffPlan heff_plan(const Struct& v, const Struct& envL, const Struct& envR, cons
       w_off += R * C;
       H.maxR = std::max<int>(H.maxR, (int)R);
       H.maxC = std::max<int>(H.maxC, (int)C);
-      int oi = 0;
-      for (auto& ot : outtypes) {
+      for (size_t oi = 0; oi < outtypes.size(); ++oi) {
+        const Key& ot = outtypes[oi];
+        const Cell& cc = cells[{b, ot[0], ot[1], ot[2], r}];
         for (size_t i = gi0; i < gi1; ++i) {
           const Key& k = t1[i].key;
           MpoTerm tm{};
           tm.sig = sig;
           tm.row0 = (int32_t)row0s[oi];
           tm.col0 = (int32_t)col0s[i - gi0];
-          tm.Dx = (int32_t)od[ot][0];
-          tm.d1o = (int32_t)od[ot][1];
-          tm.d2o = (int32_t)od[ot][2];
+          tm.Dx = (int32_t)cc.Dx;
+          tm.d1o = (int32_t)cc.d1o;
+          tm.d2o = (int32_t)cc.d2o;
           tm.Dw = (int32_t)t1[i].Dw;
           tm.d1i = (int32_t)t1[i].d1;
           tm.d2i = (int32_t)t1[i].d2;
@@ -886,9 +1000,9 @@ HeffPlan heff_plan(const Struct& v, const Struct& envL, const Struct& envR, cons
             for (int64_t wl : itl->second) {
               if (WL.sector(wl)[1] != ot[0]) continue;
               const int64_t m = WL.sector(wl)[2];
-              auto itr = wr_by.find({m, k[3]});
-              if (itr == wr_by.end()) continue;
-              for (int64_t wr : itr->second) {
+              auto itw = wr_by.find({m, k[3]});
+              if (itw == wr_by.end()) continue;
+              for (int64_t wr : itw->second) {
                 if (WR.sector(wr)[1] != ot[1] || WR.sector(wr)[2] != ot[2]) continue;
                 H.pairs.push_back(MpoPair{WL.offsets[wl], WR.offsets[wr], (int32_t)WL.size(wl)[2], 0});
                 tm.npair++;
@@ -896,7 +1010,6 @@ HeffPlan heff_plan(const Struct& v, const Struct& envL, const Struct& envR, cons
             }
           H.terms.push_back(tm);
         }
-        ++oi;
       }
     } else {
       sig = its->second;
@@ -910,65 +1023,35 @@ HeffPlan heff_plan(const Struct& v, const Struct& envL, const Struct& envR, cons
     g.npix = db * dR;
     for (size_t i = gi0; i < gi1; ++i) {
       const T1B& t = t1[i];
-      // column order c = jw + Dw*(i1 + d1*i2)
+      if (t.db != db || t.dR != dR) throw Error(ERR_SECTOR_MISMATCH, "heff: pixel group dims differ");
+      // column order c = jw + Dw*(i1 + d1*i2); T1 element (i1 + d1 i2 + d1 d2 n, jw + Dw m)
       for (int64_t c = 0; c < t.Dw * t.d1 * t.d2; ++c) {
         const int64_t jw = c % t.Dw, i1 = (c / t.Dw) % t.d1, i2 = c / (t.Dw * t.d1);
-        H.inrows.push_back(MpoInRow{t.off + i1 + t.d1 * i2 + t.d1 * t.d2 * t.dR * jw, t.d1 * t.d2,
-                                    t.d1 * t.d2 * t.dR * t.Dw});
+        H.inrows.push_back(MpoInRow{t.base + t.rowoff + i1 + t.d1 * i2 + t.Mf * (t.coloff + jw), t.d1 * t.d2,
+                                    t.Mf * t.Dw});
       }
     }
     for (auto& ot : outtypes) {
-      Key tk = {b, ot[0], ot[1], ot[2], r};
-      T3B t3;
-      t3.key = tk;
-      t3.db = db;
-      t3.dR = dR;
-      t3.Dx = od[ot][0];
-      t3.d1o = od[ot][1];
-      t3.d2o = od[ot][2];
-      t3.off = t3_off;
-      t3_off += dR * t3.Dx * db * t3.d1o * t3.d2o;
-      t3map[tk] = t3;
-      for (int64_t rr = 0; rr < t3.Dx * t3.d1o * t3.d2o; ++rr) {
-        const int64_t jx = rr % t3.Dx, i1 = (rr / t3.Dx) % t3.d1o, i2 = rr / (t3.Dx * t3.d1o);
-        H.outrows.push_back(MpoOutRow{t3.off + dR * jx + dR * t3.Dx * db * (i1 + t3.d1o * i2), dR * t3.Dx});
+      const Cell& cc = cells[{b, ot[0], ot[1], ot[2], r}];
+      // row order r = jx + Dx*(i1o + d1o*i2o); T3 element (n + dR jx, rowoff + mb + db (i1o + d1o i2o))
+      for (int64_t rr = 0; rr < cc.Dx * cc.d1o * cc.d2o; ++rr) {
+        const int64_t jx = rr % cc.Dx, i1 = (rr / cc.Dx) % cc.d1o, i2 = rr / (cc.Dx * cc.d1o);
+        H.outrows.push_back(
+            MpoOutRow{cc.base + dR * jx + cc.Kf * (cc.rowoff + db * (i1 + cc.d1o * i2)), cc.Kf});
       }
     }
     const int32_t gidx = (int32_t)H.groups.size();
     H.groups.push_back(g);
     for (int64_t p0 = 0; p0 < g.npix; p0 += 256) H.work.push_back(MpoWork{gidx, (int32_t)p0});
     H.mpo_flops += 2 * R * C * g.npix;
-    gi0 = gi1;
   }
-  H.t3_elems = t3_off;
   H.w_elems = w_off;
-
-  // ---- step 4: out blocks, K segments over envR blocks (r, x, r')
-  {
-    auto eby = index_by_leg(envR, 2);
-    for (int64_t o = 0; o < H.out.nblocks; ++o) {
-      const int64_t* os = H.out.sector(o);
-      const int64_t* oz = H.out.size(o);
-      const int64_t M = oz[0] * oz[1] * oz[2], N = oz[3];
-      H.g4.add_prob(M, N, H.out.offsets[o], M);
-      auto it = eby.find(os[3]);
-      if (it == eby.end()) continue;
-      for (int64_t f : it->second) {
-        const int64_t* fs = envR.sector(f);
-        Key tk = {os[0], os[1], os[2], fs[1], fs[0]};
-        auto t3 = t3map.find(tk);
-        if (t3 == t3map.end()) continue;
-        const int64_t K = envR.size(f)[0] * envR.size(f)[1];
-        if (t3->second.dR != envR.size(f)[0] || t3->second.Dx != envR.size(f)[1])
-          throw Error(ERR_SECTOR_MISMATCH, "heff: T3/envR dims differ");
-        H.g4.add_seg(K, t3->second.off, K, envR.offsets[f], K);
-      }
-    }
-  }
   H.t1_off = 0;
   H.t3_off = H.t1_elems;
   H.w_off = H.t3_off + H.t3_elems;
-  H.ws_elems = H.w_off + H.w_elems;
+  H.part_off = H.w_off + H.w_elems;
+  H.ws_elems = H.part_off + H.g4.part_elems + H.g1.part_elems;
+  H.g1.part_elems = 0;
   return H;
 }
 

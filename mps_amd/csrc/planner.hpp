@@ -118,36 +118,58 @@ struct CopyPlan {
   void finalize();
 };
 
-// One grouped-GEMM problem: C[M x N] (column-major, ldc) = sum over its K
-// segments of A_s^T * B_s with both operands K-contiguous:
-//   A_s(k, m) at a_off + k + m*lda,  B_s(k, n) at b_off + k + n*ldb.
-// nseg == 0 means "write zeros" (reference src/contract.jl:292-294).
+// One grouped-GEMM problem: C[M x N] = sum over its K segments of A_s^T B_s.
+// Operand rows are gathered through base-address tables, so a problem may
+// span several stored blocks without copies:
+//   A_s(k, m) = A[atab[s.atab0 + m] + k],   B_s(k, n) = B[btab[s.btab0 + n] + k],
+//   C(m, n)   = C[crow[p.crow0 + m] + n * cld[p.crow0 + m]].
+// nseg == 0 means "write zeros" (reference src/contract.jl:292-294).  With
+// nsplit > 1 the K chunks are split over work items that write dense
+// partials at part_off (+ split * M * N, column-major), summed by a reduction.
 struct GemmProb {
   int32_t M, N, seg0, nseg;
-  int64_t c_off, ldc;
+  int64_t crow0;
+  int32_t nsplit, pad;
+  int64_t part_off;
 };
 struct GemmSeg {
-  int64_t K, a_off, lda, b_off, ldb;
+  int64_t K, atab0, btab0;
 };
 struct GemmTile {
-  int32_t prob, m0, n0, pad;
+  int32_t prob, m0, n0, split;
+};
+struct RedDesc {  // one split problem of the reduction pass
+  int32_t prob, pad;
+  int64_t start;  // prefix over ceil(M*N/1024) work groups
 };
 
 struct GemmPlan {
   std::vector<GemmProb> probs;
   std::vector<GemmSeg> segs;
-  int64_t flops = 0;  // 2*M*N*K summed
-  void add_prob(int64_t M, int64_t N, int64_t c_off, int64_t ldc) {
-    probs.push_back(GemmProb{(int32_t)M, (int32_t)N, (int32_t)segs.size(), 0, c_off, ldc});
+  std::vector<int64_t> atab, btab, crow, cld;
+  int64_t flops = 0;  // 2*M*N*K summed (algorithmic)
+  int BM = 64, BN = 64;
+  int64_t part_elems = 0;
+  // open a problem; its C rows follow with add_crow()
+  void add_prob(int64_t M, int64_t N) {
+    probs.push_back(GemmProb{(int32_t)M, (int32_t)N, (int32_t)segs.size(), 0, (int64_t)crow.size(), 1, 0, 0});
   }
-  void add_seg(int64_t K, int64_t a_off, int64_t lda, int64_t b_off, int64_t ldb) {
-    GemmProb& p = probs.back();
-    segs.push_back(GemmSeg{K, a_off, lda, b_off, ldb});
-    p.nseg++;
-    flops += 2 * (int64_t)p.M * p.N * K;
+  void add_crow(int64_t base, int64_t ld) {
+    crow.push_back(base);
+    cld.push_back(ld);
   }
-  // Tile list for a BMxBN tiling, longest-K first (static LPT order).
-  std::vector<GemmTile> tiles(int BM, int BN) const;
+  // affine C rows: base + m, ld
+  void add_crows_affine(int64_t base, int64_t ld, int64_t M) {
+    for (int64_t m = 0; m < M; ++m) add_crow(base + m, ld);
+  }
+  // a K segment with explicit row tables (M entries for A, N for B)
+  void add_seg_tab(int64_t K, const std::vector<int64_t>& arows, const std::vector<int64_t>& brows);
+  // a K segment with affine rows: A rows a_off + m*lda, B rows b_off + n*ldb
+  void add_seg(int64_t K, int64_t a_off, int64_t lda, int64_t b_off, int64_t ldb);
+  // split long-K problems so that the number of work items reaches `target`
+  void choose_splits(int64_t target, int BK, int64_t min_chunks_per_split);
+  std::vector<GemmTile> tiles() const;  // longest-K first (static LPT order)
+  std::vector<RedDesc> reductions() const;
 };
 
 // ---------------------------------------------------------------------------
@@ -221,7 +243,7 @@ struct MpoGroup {
 struct MpoInRow {
   int64_t base, nstride, mstride;
 };
-struct MpoOutRow {
+struct MpoOutRow {  // T3 address = base + n + m*mstride
   int64_t base, mstride;
 };
 struct MpoWork {
@@ -243,8 +265,8 @@ struct HeffPlan {
   int64_t t3_elems = 0;
   int maxR = 0, maxC = 0;
   GemmPlan g4;  // A = T3 (ws), B = envR, C = out
-  // workspace layout (elements): [T1 | T3 | W]
-  int64_t t1_off = 0, t3_off = 0, w_off = 0, ws_elems = 0;
+  // workspace layout (elements): [T1 | T3 | W | split-K partials of step 4]
+  int64_t t1_off = 0, t3_off = 0, w_off = 0, part_off = 0, ws_elems = 0;
   // reference-order algorithmic cost (src/apply.jl:99-103 via src/contract.jl:290-291)
   int64_t ref_flops = 0;
   int64_t ref_elems = 0;  // sum over the 4 steps of numel(A)+numel(B)+numel(C)
