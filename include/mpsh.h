@@ -113,18 +113,33 @@ typedef struct mpsh_heff_s* mpsh_heff;
 
 mpsh_status mpsh_heff_plan(mpsh_struct v, mpsh_struct envL, mpsh_struct envR, mpsh_struct WL, mpsh_struct WR,
                            mpsh_heff* plan, mpsh_struct* out);
-size_t mpsh_heff_workspace(mpsh_heff plan, mpsh_dtype dt); /* bytes */
 /* stats[0..9]: reference-order FLOPs, reference-order elements moved, step FLOPs x4,
  * step elements x4 (the reference's own four contracts, src/apply.jl:99-103). */
 mpsh_status mpsh_heff_stats(mpsh_heff plan, int64_t* stats);
+
+/* One-shot application, the drop-in for _applymps2site: workspace of
+ * mpsh_heff_workspace() bytes, 16-byte aligned. */
+size_t mpsh_heff_workspace(mpsh_heff plan, mpsh_dtype dt);
 mpsh_status mpsh_heff_apply(mpsh_heff plan, mpsh_dtype dt, const void* v, const void* envL, const void* envR,
                             const void* WL, const void* WR, void* out, void* workspace, void* stream);
-/* Same computation split by stage for per-kernel timing: stages is a bit mask,
- * 1 = step-1 GEMM, 2 = combined-MPO pass, 4 = step-4 GEMM; run in that order.
- * mask 7 equals mpsh_heff_apply.  No reference counterpart (instrumentation). */
-mpsh_status mpsh_heff_apply_stages(mpsh_heff plan, mpsh_dtype dt, int32_t stages, const void* v, const void* envL,
-                                   const void* envR, const void* WL, const void* WR, void* out, void* workspace,
-                                   void* stream);
+
+/* Prepared operator for repeated application inside one eigensolve (the
+ * environments and MPO of a bond are fixed while Lanczos runs, dmrg.jl:77-82):
+ * envL/envR are repacked into 16-byte aligned K-contiguous rows and the
+ * combined two-site MPO is formed once; each matvec then packs v and runs
+ * the three kernel stages.  Buffers are 16-byte aligned device memory. */
+size_t mpsh_heff_prepared_size(mpsh_heff plan, mpsh_dtype dt);
+size_t mpsh_heff_apply_workspace(mpsh_heff plan, mpsh_dtype dt);
+mpsh_status mpsh_heff_prepare(mpsh_heff plan, mpsh_dtype dt, const void* envL, const void* envR, const void* WL,
+                              const void* WR, void* prepared, void* stream);
+mpsh_status mpsh_heff_apply_prepared(mpsh_heff plan, mpsh_dtype dt, const void* prepared, const void* v,
+                                     void* out, void* workspace, void* stream);
+/* The same split by stage for per-kernel timing: stages is a bit mask,
+ * 1 = pack v, 2 = step-1 GEMM, 4 = combined-MPO pass, 8 = step-4 GEMM; run in
+ * that order; mask 15 equals mpsh_heff_apply_prepared.  Instrumentation,
+ * no reference counterpart. */
+mpsh_status mpsh_heff_apply_prepared_stages(mpsh_heff plan, mpsh_dtype dt, int32_t stages, const void* prepared,
+                                            const void* v, void* out, void* workspace, void* stream);
 void mpsh_heff_destroy(mpsh_heff plan);
 
 /* ------------------------------------------------------------------------ */

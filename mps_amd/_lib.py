@@ -72,7 +72,11 @@ _SIGS = {
     "mpsh_heff_workspace": (_sz, [_p, C.c_int]),
     "mpsh_heff_stats": (C.c_int, [_p, _pi64]),
     "mpsh_heff_apply": (C.c_int, [_p, C.c_int, _p, _p, _p, _p, _p, _p, _p, _p]),
-    "mpsh_heff_apply_stages": (C.c_int, [_p, C.c_int, _i32, _p, _p, _p, _p, _p, _p, _p, _p]),
+    "mpsh_heff_prepared_size": (_sz, [_p, C.c_int]),
+    "mpsh_heff_apply_workspace": (_sz, [_p, C.c_int]),
+    "mpsh_heff_prepare": (C.c_int, [_p, C.c_int, _p, _p, _p, _p, _p, _p]),
+    "mpsh_heff_apply_prepared": (C.c_int, [_p, C.c_int, _p, _p, _p, _p, _p]),
+    "mpsh_heff_apply_prepared_stages": (C.c_int, [_p, C.c_int, _i32, _p, _p, _p, _p, _p]),
     "mpsh_heff_destroy": (None, [_p]),
     "mpsh_dot": (C.c_int, [C.c_int, _i64, _p, _p, _p, _p, _p]),
     "mpsh_axpby": (C.c_int, [C.c_int, _i64, _dbl, _p, _dbl, _p, _p]),

@@ -4,6 +4,8 @@
 // device memory except the one-time table upload of a plan.
 #include <hip/hip_runtime.h>
 
+#include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <map>
 #include <memory>
@@ -67,10 +69,7 @@ struct DevTables {
     d.ntiles = c.ntiles;
     return d;
   }
-  DevGemm gemm(const GemmPlan& g0, int BM = 0, int BN = 0) {
-    GemmPlan g = g0;
-    if (BM) g.BM = BM;
-    if (BN) g.BN = BN;
+  DevGemm gemm(const GemmPlan& g) {
     DevGemm d;
     std::vector<GemmTile> t = g.tiles();
     std::vector<RedDesc> r = g.reductions();
@@ -87,6 +86,7 @@ struct DevTables {
     if (d.nred > 0) d.reds = up(r);
     d.BM = g.BM;
     d.BN = g.BN;
+    d.KS = g.KS;
     return d;
   }
 };
@@ -98,6 +98,41 @@ int current_device() {
 }
 
 int elem_size(mpsh_dtype dt) { return dt == MPSH_F64 ? 8 : 4; }
+
+void need_aligned(const void* p, const char* what) {
+  if (((uintptr_t)p) & 15) throw Error(ERR_INVALID, std::string(what) + " must be 16-byte aligned");
+}
+
+// Optional override of the H_eff kernel variants, read at plan creation:
+// MPSH_HEFF_TILES="s1=128x64x1,s4=64x64x1,s1d=64x64x1,s4d=64x64x1,split=1024"
+HeffTiles tiles_from_env() {
+  HeffTiles t;
+  const char* e = std::getenv("MPSH_HEFF_TILES");
+  if (!e) return t;
+  std::string str(e);
+  size_t pos = 0;
+  while (pos < str.size()) {
+    size_t comma = str.find(',', pos);
+    std::string item = str.substr(pos, comma == std::string::npos ? std::string::npos : comma - pos);
+    pos = comma == std::string::npos ? str.size() : comma + 1;
+    size_t eq = item.find('=');
+    if (eq == std::string::npos) continue;
+    std::string k = item.substr(0, eq), v = item.substr(eq + 1);
+    if (k == "split") {
+      t.split_target = std::atoll(v.c_str());
+      continue;
+    }
+    int bm = 0, bn = 0, ks = 1;
+    if (std::sscanf(v.c_str(), "%dx%dx%d", &bm, &bn, &ks) < 2) throw Error(ERR_INVALID, "bad MPSH_HEFF_TILES");
+    TileCfg c{bm, bn, ks};
+    if (k == "s1") t.s1 = c;
+    else if (k == "s4") t.s4 = c;
+    else if (k == "s1d") t.s1d = c;
+    else if (k == "s4d") t.s4d = c;
+    else throw Error(ERR_INVALID, "bad MPSH_HEFF_TILES key");
+  }
+  return t;
+}
 int gemm_mode(mpsh_dtype dt) { return dt == MPSH_F64 ? 0 : (dt == MPSH_F32 ? 1 : 2); }
 
 void check_dtype(int dt) {
@@ -176,8 +211,9 @@ struct mpsh_contract_s {
 
 struct HeffDev {
   std::unique_ptr<DevTables> t;
-  DevGemm g1, g1_f64, g4;  // FP64 step 1 uses 64x64 tiles (register budget of the f64 MFMA tile)
+  DevGemm g1, g4, g1d, g4d;  // FP32/BF16 and FP64 kernel variants
   DevMpo mpo;
+  DevCopy pack_v, pack_envL, pack_envR;
 };
 
 struct mpsh_heff_s {
@@ -192,8 +228,9 @@ struct mpsh_heff_s {
       HeffDev h;
       h.t = std::make_unique<DevTables>();
       h.g1 = h.t->gemm(p.g1);
-      h.g1_f64 = h.t->gemm(p.g1, 64, 64);
       h.g4 = h.t->gemm(p.g4);
+      h.g1d = h.t->gemm(p.g1d);
+      h.g4d = h.t->gemm(p.g4d);
       h.mpo.sigs = h.t->up(p.sigs);
       h.mpo.terms = h.t->up(p.terms);
       h.mpo.pairs = h.t->up(p.pairs);
@@ -203,7 +240,10 @@ struct mpsh_heff_s {
       h.mpo.work = h.t->up(p.work);
       h.mpo.nterms = (int)p.terms.size();
       h.mpo.nwork = (int)p.work.size();
-      h.mpo.cmax = p.maxC <= 32 ? 32 : 64;
+      h.mpo.cmax = (int)p.maxC;
+      h.pack_v = h.t->copy(p.pack_v);
+      h.pack_envL = h.t->copy(p.pack_envL);
+      h.pack_envR = h.t->copy(p.pack_envR);
       it = dev.emplace(d, std::move(h)).first;
     }
     return it->second;
@@ -352,6 +392,7 @@ mpsh_status mpsh_contract_execute(mpsh_contract plan, mpsh_dtype dt, const void*
   if (!plan || (!A && plan->p.A.numel()) || (!B && plan->p.B.numel()) || (!C && plan->p.C.numel()) ||
       (!workspace && plan->p.ws_elems))
     throw Error(ERR_INVALID, "null argument");
+  need_aligned(workspace, "workspace");
   const ContractDev& d = plan->get();
   hipStream_t s = (hipStream_t)stream;
   const int es = elem_size(dt);
@@ -377,7 +418,7 @@ mpsh_status mpsh_heff_plan(mpsh_struct v, mpsh_struct envL, mpsh_struct envR, mp
   if (!v || !envL || !envR || !WL || !WR || !plan || !out) throw Error(ERR_INVALID, "null argument");
   mpsh_heff h = new mpsh_heff_s;
   try {
-    h->p = heff_plan(v->s, envL->s, envR->s, WL->s, WR->s);
+    h->p = heff_plan(v->s, envL->s, envR->s, WL->s, WR->s, tiles_from_env());
     if (h->p.maxR > 64 || h->p.maxC > 64)
       throw Error(ERR_UNSUPPORTED, "combined MPO group wider than 64 channels");
   } catch (...) {
@@ -390,9 +431,20 @@ mpsh_status mpsh_heff_plan(mpsh_struct v, mpsh_struct envL, mpsh_struct envR, mp
   API_CATCH
 }
 
-size_t mpsh_heff_workspace(mpsh_heff plan, mpsh_dtype dt) {
+static size_t align16(size_t b) { return (b + 15) / 16 * 16; }
+
+size_t mpsh_heff_prepared_size(mpsh_heff plan, mpsh_dtype dt) {
   if (!plan) return 0;
-  return (size_t)plan->p.ws_elems * (size_t)elem_size(dt);
+  return align16((size_t)plan->p.prep_elems * (size_t)elem_size(dt));
+}
+
+size_t mpsh_heff_apply_workspace(mpsh_heff plan, mpsh_dtype dt) {
+  if (!plan) return 0;
+  return align16((size_t)std::max<int64_t>(plan->p.ws_elems, 1) * (size_t)elem_size(dt));
+}
+
+size_t mpsh_heff_workspace(mpsh_heff plan, mpsh_dtype dt) {
+  return mpsh_heff_apply_workspace(plan, dt) + mpsh_heff_prepared_size(plan, dt);
 }
 
 mpsh_status mpsh_heff_stats(mpsh_heff plan, int64_t* stats) {
@@ -407,36 +459,70 @@ mpsh_status mpsh_heff_stats(mpsh_heff plan, int64_t* stats) {
   return MPSH_OK;
 }
 
-mpsh_status mpsh_heff_apply(mpsh_heff plan, mpsh_dtype dt, const void* v, const void* envL, const void* envR,
-                            const void* WL, const void* WR, void* out, void* workspace, void* stream) {
-  return mpsh_heff_apply_stages(plan, dt, 7, v, envL, envR, WL, WR, out, workspace, stream);
-}
-
-mpsh_status mpsh_heff_apply_stages(mpsh_heff plan, mpsh_dtype dt, int32_t stages, const void* v, const void* envL,
-                                   const void* envR, const void* WL, const void* WR, void* out, void* workspace,
-                                   void* stream) {
+mpsh_status mpsh_heff_prepare(mpsh_heff plan, mpsh_dtype dt, const void* envL, const void* envR, const void* WL,
+                              const void* WR, void* prepared, void* stream) {
   API_TRY
   check_dtype(dt);
   if (!plan) throw Error(ERR_INVALID, "null plan");
-  const HeffPlan& hp = plan->p;
-  if ((!v && hp.v.numel()) || (!envL && hp.envL.numel()) || (!envR && hp.envR.numel()) || (!WL && hp.WL.numel()) ||
-      (!WR && hp.WR.numel()) || (!out && hp.out.numel()) || (!workspace && hp.ws_elems))
-    throw Error(ERR_INVALID, "null argument");
-  const HeffDev& d = plan->get();
   const HeffPlan& p = plan->p;
+  if ((!envL && p.envL.numel()) || (!envR && p.envR.numel()) || (!WL && p.WL.numel()) || (!WR && p.WR.numel()) ||
+      (!prepared && p.prep_elems))
+    throw Error(ERR_INVALID, "null argument");
+  need_aligned(prepared, "prepared");
+  const HeffDev& d = plan->get();
   hipStream_t s = (hipStream_t)stream;
   const int es = elem_size(dt);
-  char* ws = (char*)workspace;
-  void* T1 = ws + p.t1_off * es;
-  void* T3 = ws + p.t3_off * es;
-  void* W = ws + p.w_off * es;
-  void* Pt = ws + p.part_off * es;
-  if (stages & 1)
-    check(launch_gemm(dt == MPSH_F64 ? d.g1_f64 : d.g1, gemm_mode(dt), v, envL, T1, Pt, s), "step-1 gemm");
-  if (stages & 2) check(launch_mpo(d.mpo, es, WL, WR, W, T1, T3, s), "mpo");
-  if (stages & 4) check(launch_gemm(d.g4, gemm_mode(dt), T3, envR, out, Pt, s), "step-4 gemm");
+  char* pr = (char*)prepared;
+  check(launch_copy(d.pack_envL, es, envL, pr + p.el_off * es, s), "pack envL");
+  check(launch_copy(d.pack_envR, es, envR, pr + p.er_off * es, s), "pack envR");
+  check(launch_mpo_combine(d.mpo, es, WL, WR, pr + p.pw_off * es, s), "mpo combine");
   return MPSH_OK;
   API_CATCH
+}
+
+mpsh_status mpsh_heff_apply_prepared_stages(mpsh_heff plan, mpsh_dtype dt, int32_t stages, const void* prepared,
+                                            const void* v, void* out, void* workspace, void* stream) {
+  API_TRY
+  check_dtype(dt);
+  if (!plan) throw Error(ERR_INVALID, "null plan");
+  const HeffPlan& p = plan->p;
+  if ((!v && p.v.numel()) || (!out && p.out.numel()) || (!prepared && p.prep_elems) || !workspace)
+    throw Error(ERR_INVALID, "null argument");
+  need_aligned(prepared, "prepared");
+  need_aligned(workspace, "workspace");
+  const HeffDev& d = plan->get();
+  hipStream_t s = (hipStream_t)stream;
+  const int es = elem_size(dt);
+  const char* pr = (const char*)prepared;
+  char* ws = (char*)workspace;
+  void* vp = ws + p.vp_off * es;
+  void* T1 = ws + p.t1_off * es;
+  void* T3 = ws + p.t3_off * es;
+  void* Pt = ws + p.part_off * es;
+  const bool f64 = dt == MPSH_F64;
+  if (stages & 1) check(launch_copy(d.pack_v, es, v, vp, s), "pack v");
+  if (stages & 2)
+    check(launch_gemm(f64 ? d.g1d : d.g1, gemm_mode(dt), vp, pr + p.el_off * es, T1, Pt, s), "step-1 gemm");
+  if (stages & 4) check(launch_mpo_apply(d.mpo, es, pr + p.pw_off * es, T1, T3, s), "mpo");
+  if (stages & 8)
+    check(launch_gemm(f64 ? d.g4d : d.g4, gemm_mode(dt), T3, pr + p.er_off * es, out, Pt, s), "step-4 gemm");
+  return MPSH_OK;
+  API_CATCH
+}
+
+mpsh_status mpsh_heff_apply_prepared(mpsh_heff plan, mpsh_dtype dt, const void* prepared, const void* v,
+                                     void* out, void* workspace, void* stream) {
+  return mpsh_heff_apply_prepared_stages(plan, dt, 15, prepared, v, out, workspace, stream);
+}
+
+mpsh_status mpsh_heff_apply(mpsh_heff plan, mpsh_dtype dt, const void* v, const void* envL, const void* envR,
+                            const void* WL, const void* WR, void* out, void* workspace, void* stream) {
+  if (!plan || !workspace) return fail(ERR_INVALID, "null argument");
+  char* ws = (char*)workspace;
+  void* prepared = ws + mpsh_heff_apply_workspace(plan, dt);
+  mpsh_status st = mpsh_heff_prepare(plan, dt, envL, envR, WL, WR, prepared, stream);
+  if (st != MPSH_OK) return st;
+  return mpsh_heff_apply_prepared(plan, dt, prepared, v, out, workspace, stream);
 }
 
 void mpsh_heff_destroy(mpsh_heff plan) { delete plan; }

@@ -12,6 +12,7 @@
 #include <hip/hip_runtime.h>
 
 #include <cstdint>
+#include <type_traits>
 
 #include "kernels.hpp"
 
@@ -111,27 +112,72 @@ hipError_t launch_copy(const DevCopy& c, int elem_size, const void* src, void* d
 // ---------------------------------------------------------------------------
 template <int MODE>
 struct GT;
+// T: storage type, L: LDS/MFMA operand type, V: 16-byte global load vector (VEC elements)
 template <>
 struct GT<0> {  // FP64
   typedef double T;
   typedef double L;
   typedef doublex4 Acc;
-  static constexpr int BK = 16, PAD = 2;
+  typedef double2 V;
+  static constexpr int BK = 16, PAD = 2, VEC = 2;
 };
 template <>
 struct GT<1> {  // FP32
   typedef float T;
   typedef float L;
   typedef floatx4 Acc;
-  static constexpr int BK = 16, PAD = 4;
+  typedef floatx4 V;
+  static constexpr int BK = 16, PAD = 4, VEC = 4;
 };
 template <>
 struct GT<2> {  // BF16 operands, FP32 accumulate/storage
   typedef float T;
   typedef __bf16 L;
   typedef floatx4 Acc;
-  static constexpr int BK = 32, PAD = 8;
+  typedef floatx4 V;
+  static constexpr int BK = 32, PAD = 8, VEC = 4;
 };
+
+typedef __bf16 bf16x4 __attribute__((ext_vector_type(4)));
+
+// keep the first `rem` lanes of a loaded vector (rem <= 0: all zero)
+template <int MODE>
+__device__ __forceinline__ typename GT<MODE>::V vmask(typename GT<MODE>::V x, int64_t rem) {
+  if constexpr (MODE == 0) {
+    if (rem < 2) x.y = 0;
+    if (rem < 1) x.x = 0;
+  } else {
+#pragma unroll
+    for (int e = 0; e < 4; ++e)
+      if (e >= rem) x[e] = 0;
+  }
+  return x;
+}
+
+template <int MODE>
+__device__ __forceinline__ typename GT<MODE>::V vzero() {
+  typename GT<MODE>::V x;
+  if constexpr (MODE == 0) {
+    x.x = 0;
+    x.y = 0;
+  } else {
+#pragma unroll
+    for (int e = 0; e < 4; ++e) x[e] = 0;
+  }
+  return x;
+}
+
+template <int MODE>
+__device__ __forceinline__ void vstore(typename GT<MODE>::L* dst, const typename GT<MODE>::V& x) {
+  if constexpr (MODE == 2) {
+    bf16x4 y;
+#pragma unroll
+    for (int e = 0; e < 4; ++e) y[e] = (__bf16)x[e];
+    *(bf16x4*)dst = y;
+  } else {
+    *(typename GT<MODE>::V*)dst = x;
+  }
+}
 
 template <int MODE>
 __device__ __forceinline__ void mfma_chunk(typename GT<MODE>::Acc& c, const typename GT<MODE>::L* a,
@@ -161,9 +207,12 @@ __device__ __forceinline__ void mfma_chunk<2>(floatx4& c, const __bf16* a, const
 }
 
 // Tile BM x BN, 4 waves in 2x2, each wave (BM/2) x (BN/2) = SM x SN MFMA 16x16
-// sub-tiles.  Operand rows come from base-address tables; the K loop walks
-// the problem's segments chunk by chunk (split-K: a contiguous chunk range).
-template <int MODE, int BM, int BN>
+// sub-tiles; K chunk = KS k-slices of one MFMA group each.  Operand rows come
+// from base-address tables and are 16-byte aligned and K-contiguous, so each
+// thread moves 16-byte vectors (masked past K); the K loop walks the problem's
+// segments chunk by chunk (split-K: a contiguous chunk range).  Global loads
+// run DEPTH chunks ahead in registers; LDS is double buffered (one barrier per chunk).
+template <int MODE, int BM, int BN, int KS>
 __global__ __launch_bounds__(256) void gemm_grouped(
     const GemmProb* __restrict__ probs, const GemmSeg* __restrict__ segs, const int64_t* __restrict__ atab,
     const int64_t* __restrict__ btab, const int64_t* __restrict__ crow, const int64_t* __restrict__ cld,
@@ -173,16 +222,21 @@ __global__ __launch_bounds__(256) void gemm_grouped(
   typedef typename GT<MODE>::T T;
   typedef typename GT<MODE>::L L;
   typedef typename GT<MODE>::Acc Acc;
-  constexpr int BK = GT<MODE>::BK, LD = GT<MODE>::BK + GT<MODE>::PAD;
-  constexpr int PA = BM * BK / 256, PB = BN * BK / 256;
-  constexpr int RSTEP = 256 / BK;
+  typedef typename GT<MODE>::V V;
   constexpr int KL = (MODE == 2) ? 8 : 4;
+  constexpr int BK = 4 * KL * KS, LD = BK + GT<MODE>::PAD;
+  constexpr int VEC = GT<MODE>::VEC;
+  constexpr int TPR = BK / VEC;   // threads per operand row
+  constexpr int RP = 256 / TPR;   // rows per pass
+  constexpr int NA = BM / RP, NB = BN / RP;
+  static_assert(NA >= 1 && NB >= 1, "tile too small for the vector loader");
   constexpr int SM = BM / 32, SN = BN / 32;
+  constexpr int DEPTH = (NA + NB) <= 4 ? 2 : 1;
   __shared__ __attribute__((aligned(16))) L As[2][BM][LD];
   __shared__ __attribute__((aligned(16))) L Bs[2][BN][LD];
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   const int wm = wave >> 1, wn = wave & 1;
-  const int kk = tid % BK, mm = tid / BK;
+  const int q = tid % TPR, row = tid / TPR;
   const int g = lane >> 4, l16 = lane & 15;
 
   for (int t = blockIdx.x; t < ntiles; t += gridDim.x) {
@@ -200,6 +254,8 @@ __global__ __launch_bounds__(256) void gemm_grouped(
     int64_t ctot = 0;
     for (int s = 0; s < p.nseg; ++s) ctot += (segs[p.seg0 + s].K + BK - 1) / BK;
     const int64_t cb = ctot * tile.split / p.nsplit, ce = ctot * (tile.split + 1) / p.nsplit;
+    const int64_t nch = ce - cb;
+    // loader state: segment, k offset, row bases
     int seg = 0;
     int64_t k0 = 0, K = 0, cpos = 0;
     for (; seg < p.nseg; ++seg) {
@@ -210,69 +266,76 @@ __global__ __launch_bounds__(256) void gemm_grouped(
       }
       cpos += nc;
     }
-    int64_t ab[PA], bb[PB];
+    // Row bases for the current segment.  Out-of-range rows read row 0 (a valid
+    // address) and are zeroed at the LDS store: loads are never predicated, so a
+    // load in flight is only waited for where its data is consumed.
+    int64_t ab[NA], bb[NB];
+    bool av[NA], bv[NB];
+#pragma unroll
+    for (int i = 0; i < NA; ++i) av[i] = tile.m0 + row + i * RP < p.M;
+#pragma unroll
+    for (int i = 0; i < NB; ++i) bv[i] = tile.n0 + row + i * RP < p.N;
     auto bases = [&]() {
       const GemmSeg sg = segs[p.seg0 + seg];
       K = sg.K;
 #pragma unroll
-      for (int i = 0; i < PA; ++i) {
-        const int m = tile.m0 + mm + i * RSTEP;
-        ab[i] = m < p.M ? atab[sg.atab0 + m] : -1;
-      }
+      for (int i = 0; i < NA; ++i) ab[i] = atab[sg.atab0 + (av[i] ? tile.m0 + row + i * RP : 0)];
 #pragma unroll
-      for (int i = 0; i < PB; ++i) {
-        const int n = tile.n0 + mm + i * RSTEP;
-        bb[i] = n < p.N ? btab[sg.btab0 + n] : -1;
-      }
+      for (int i = 0; i < NB; ++i) bb[i] = btab[sg.btab0 + (bv[i] ? tile.n0 + row + i * RP : 0)];
     };
-    int64_t chunk = cb;
-    bool have = chunk < ce;
-    if (have) bases();
-    T ra[PA], rb[PB];
-    auto load = [&]() {
-      const int64_t k = k0 + kk;
-      const bool kin = k < K;
+    if (nch > 0) bases();
+    V ra[DEPTH][NA], rb[DEPTH][NB];
+    int64_t krem[DEPTH];
+    auto load = [&](auto D) {
+      constexpr int d = decltype(D)::value;
+      const int64_t k = k0 + VEC * q;
+      const int64_t rem = K - k;
+      const int64_t kk_ = rem > 0 ? k : 0;
+      krem[d] = rem;
 #pragma unroll
-      for (int i = 0; i < PA; ++i) ra[i] = (kin && ab[i] >= 0) ? A[ab[i] + k] : T(0);
+      for (int i = 0; i < NA; ++i) ra[d][i] = *(const V*)(A + ab[i] + kk_);
 #pragma unroll
-      for (int i = 0; i < PB; ++i) rb[i] = (kin && bb[i] >= 0) ? B[bb[i] + k] : T(0);
-    };
-    auto advance = [&]() {
-      ++chunk;
+      for (int i = 0; i < NB; ++i) rb[d][i] = *(const V*)(B + bb[i] + kk_);
+      // advance the loader by one chunk
       k0 += BK;
-      if (chunk >= ce) {
-        have = false;
-        return;
-      }
-      if (k0 >= K) {
+      if (k0 >= K && seg + 1 < p.nseg) {
         ++seg;
         k0 = 0;
         bases();
       }
     };
-    if (have) load();
-    int buf = 0;
-    while (have) {
+    auto step = [&](auto D, int64_t it) {
+      constexpr int d = decltype(D)::value;
+      const int lb = (int)(it & 1);
+      const int64_t rem = krem[d];
 #pragma unroll
-      for (int i = 0; i < PA; ++i) As[buf][mm + i * RSTEP][kk] = (L)ra[i];
+      for (int i = 0; i < NA; ++i)
+        vstore<MODE>(&As[lb][row + i * RP][VEC * q], vmask<MODE>(ra[d][i], av[i] ? rem : 0));
 #pragma unroll
-      for (int i = 0; i < PB; ++i) Bs[buf][mm + i * RSTEP][kk] = (L)rb[i];
+      for (int i = 0; i < NB; ++i)
+        vstore<MODE>(&Bs[lb][row + i * RP][VEC * q], vmask<MODE>(rb[d][i], bv[i] ? rem : 0));
       __syncthreads();
-      advance();
-      if (have) load();
+      if (it + DEPTH < nch) load(D);
 #pragma unroll
       for (int kc = 0; kc < BK; kc += 4 * KL) {
         const L* bp[SN];
 #pragma unroll
-        for (int j = 0; j < SN; ++j) bp[j] = &Bs[buf][wn * (BN / 2) + j * 16 + l16][kc + KL * g];
+        for (int j = 0; j < SN; ++j) bp[j] = &Bs[lb][wn * (BN / 2) + j * 16 + l16][kc + KL * g];
 #pragma unroll
         for (int i = 0; i < SM; ++i) {
-          const L* ap = &As[buf][wm * (BM / 2) + i * 16 + l16][kc + KL * g];
+          const L* ap = &As[lb][wm * (BM / 2) + i * 16 + l16][kc + KL * g];
 #pragma unroll
           for (int j = 0; j < SN; ++j) mfma_chunk<MODE>(acc[i][j], ap, bp[j]);
         }
       }
-      buf ^= 1;
+    };
+    typedef std::integral_constant<int, 0> D0;
+    typedef std::integral_constant<int, DEPTH - 1> D1;
+    if (nch > 0) load(D0{});
+    if (DEPTH > 1 && nch > 1) load(D1{});
+    for (int64_t it = 0; it < nch; it += DEPTH) {
+      step(D0{}, it);
+      if (DEPTH > 1 && it + 1 < nch) step(D1{}, it + 1);
     }
     // epilogue: column l16 of each 16x16 sub-tile; accumulator register r holds
     // row g + 4r for the FP64 16x16x4 form and row 4g + r for the FP32/BF16 forms.
@@ -345,27 +408,47 @@ static int num_cu() {
   return g_num_cu;
 }
 
-template <int MODE, int BM, int BN>
+template <int MODE, int BM, int BN, int KS>
 static void launch_variant(const DevGemm& gm, const void* A, const void* B, void* C, void* P, hipStream_t s) {
   typedef typename GT<MODE>::T T;
   const int maxgrid = num_cu() * 8;
   dim3 grid((unsigned)(gm.ntiles < maxgrid ? gm.ntiles : maxgrid)), block(256);
-  hipLaunchKernelGGL((gemm_grouped<MODE, BM, BN>), grid, block, 0, s, gm.probs, gm.segs, gm.atab, gm.btab,
+  hipLaunchKernelGGL((gemm_grouped<MODE, BM, BN, KS>), grid, block, 0, s, gm.probs, gm.segs, gm.atab, gm.btab,
                      gm.crow, gm.cld, gm.tiles, gm.ntiles, (const T*)A, (const T*)B, (T*)C, (T*)P);
 }
 
+// Compiled tile variants: (BM, BN) in {64x64, 128x64, 64x128, 128x128} with K
+// chunk multiplier KS in {1, 2} (FP64: KS = 1 only).
 template <int MODE>
 static hipError_t launch_mode(const DevGemm& gm, const void* A, const void* B, void* C, void* P, hipStream_t s) {
-  if (gm.BM == 64 && gm.BN == 64)
-    launch_variant<MODE, 64, 64>(gm, A, B, C, P, s);
-  else if (gm.BM == 128 && gm.BN == 64)
-    launch_variant<MODE, 128, 64>(gm, A, B, C, P, s);
-  else if (gm.BM == 64 && gm.BN == 128)
-    launch_variant<MODE, 64, 128>(gm, A, B, C, P, s);
-  else if (gm.BM == 128 && gm.BN == 128)
-    launch_variant<MODE, 128, 128>(gm, A, B, C, P, s);
-  else
-    return hipErrorInvalidValue;
+  const int key = gm.BM * 10000 + gm.BN * 10 + gm.KS;
+#define MPSH_V(bm, bn, ks)                                  \
+  case bm * 10000 + bn * 10 + ks:                          \
+    launch_variant<MODE, bm, bn, ks>(gm, A, B, C, P, s); \
+    break;
+  if constexpr (MODE == 0) {
+    switch (key) {
+      MPSH_V(64, 64, 1)
+      MPSH_V(128, 64, 1)
+      MPSH_V(64, 128, 1)
+      default:
+        return hipErrorInvalidValue;
+    }
+  } else {
+    switch (key) {
+      MPSH_V(64, 64, 1)
+      MPSH_V(128, 64, 1)
+      MPSH_V(64, 128, 1)
+      MPSH_V(128, 128, 1)
+      MPSH_V(64, 64, 2)
+      MPSH_V(128, 64, 2)
+      MPSH_V(64, 128, 2)
+      MPSH_V(128, 128, 2)
+      default:
+        return hipErrorInvalidValue;
+    }
+  }
+#undef MPSH_V
   return hipGetLastError();
 }
 
@@ -458,12 +541,24 @@ __global__ __launch_bounds__(256) void mpo_apply(const MpoSig* __restrict__ sigs
     om[i] = q.mstride;
   }
   __syncthreads();
-  const int64_t p = (int64_t)wk.pix0 + threadIdx.x;
-  if (p >= g.npix) return;
+  const int64_t p0 = (int64_t)wk.pix0 + threadIdx.x;
+  const bool live = p0 < g.npix;
+  const int64_t p = live ? p0 : 0;
   const int64_t m = p / g.dR, n = p - m * g.dR;
   T v[CMAX];
+  if (C > 0) {
+    // every row index is clamped to a real input row and W is zero past C, so
+    // the loads are unconditional (no wait is forced before the FMAs)
 #pragma unroll
-  for (int c = 0; c < CMAX; ++c) v[c] = c < C ? T1[ib[c] + n * ins[c] + m * im[c]] : T(0);
+    for (int c = 0; c < CMAX; ++c) {
+      const int cc = c < C ? c : C - 1;
+      v[c] = T1[ib[cc] + n * ins[cc] + m * im[cc]];
+    }
+  } else {
+#pragma unroll
+    for (int c = 0; c < CMAX; ++c) v[c] = 0;
+  }
+  if (!live) return;
   for (int r = 0; r < R; ++r) {
     const T* wr = Ws + r * CMAX;
     T acc = 0;
@@ -473,35 +568,34 @@ __global__ __launch_bounds__(256) void mpo_apply(const MpoSig* __restrict__ sigs
   }
 }
 
-hipError_t launch_mpo(const DevMpo& d, int elem_size, const void* WL, const void* WR, void* W, const void* T1,
-                      void* T3, hipStream_t s) {
-  if (d.nterms > 0) {
-    if (elem_size == 8)
-      hipLaunchKernelGGL(mpo_combine<double>, dim3(d.nterms), dim3(256), 0, s, d.sigs, d.terms, d.pairs,
-                         (const double*)WL, (const double*)WR, (double*)W);
-    else
-      hipLaunchKernelGGL(mpo_combine<float>, dim3(d.nterms), dim3(256), 0, s, d.sigs, d.terms, d.pairs,
-                         (const float*)WL, (const float*)WR, (float*)W);
-    hipError_t e = hipGetLastError();
-    if (e != hipSuccess) return e;
-  }
+hipError_t launch_mpo_combine(const DevMpo& d, int elem_size, const void* WL, const void* WR, void* W,
+                              hipStream_t s) {
+  if (d.nterms == 0) return hipSuccess;
+  if (elem_size == 8)
+    hipLaunchKernelGGL(mpo_combine<double>, dim3(d.nterms), dim3(256), 0, s, d.sigs, d.terms, d.pairs,
+                       (const double*)WL, (const double*)WR, (double*)W);
+  else
+    hipLaunchKernelGGL(mpo_combine<float>, dim3(d.nterms), dim3(256), 0, s, d.sigs, d.terms, d.pairs,
+                       (const float*)WL, (const float*)WR, (float*)W);
+  return hipGetLastError();
+}
+
+hipError_t launch_mpo_apply(const DevMpo& d, int elem_size, const void* W, const void* T1, void* T3, hipStream_t s) {
   if (d.nwork == 0) return hipSuccess;
   dim3 grid(d.nwork), block(256);
+#define MPSH_MPO(T, CM)                                                                                  \
+  hipLaunchKernelGGL((mpo_apply<T, CM>), grid, block, 0, s, d.sigs, d.groups, d.inrows, d.outrows, d.work, \
+                     (const T*)W, (const T*)T1, (T*)T3)
   if (elem_size == 8) {
-    if (d.cmax <= 32)
-      hipLaunchKernelGGL((mpo_apply<double, 32>), grid, block, 0, s, d.sigs, d.groups, d.inrows, d.outrows,
-                         d.work, (const double*)W, (const double*)T1, (double*)T3);
-    else
-      hipLaunchKernelGGL((mpo_apply<double, 64>), grid, block, 0, s, d.sigs, d.groups, d.inrows, d.outrows,
-                         d.work, (const double*)W, (const double*)T1, (double*)T3);
+    if (d.cmax <= 16) MPSH_MPO(double, 16);
+    else if (d.cmax <= 32) MPSH_MPO(double, 32);
+    else MPSH_MPO(double, 64);
   } else {
-    if (d.cmax <= 32)
-      hipLaunchKernelGGL((mpo_apply<float, 32>), grid, block, 0, s, d.sigs, d.groups, d.inrows, d.outrows,
-                         d.work, (const float*)W, (const float*)T1, (float*)T3);
-    else
-      hipLaunchKernelGGL((mpo_apply<float, 64>), grid, block, 0, s, d.sigs, d.groups, d.inrows, d.outrows,
-                         d.work, (const float*)W, (const float*)T1, (float*)T3);
+    if (d.cmax <= 16) MPSH_MPO(float, 16);
+    else if (d.cmax <= 32) MPSH_MPO(float, 32);
+    else MPSH_MPO(float, 64);
   }
+#undef MPSH_MPO
   return hipGetLastError();
 }
 

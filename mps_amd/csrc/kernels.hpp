@@ -22,7 +22,7 @@ struct DevGemm {
   const RedDesc* reds = nullptr;
   int ntiles = 0, nred = 0;
   int64_t red_groups = 0;
-  int BM = 64, BN = 64;
+  int BM = 64, BN = 64, KS = 1;  // tile and K-chunk multiplier
 };
 
 struct DevMpo {
@@ -40,8 +40,9 @@ hipError_t launch_copy(const DevCopy& c, int elem_size, const void* src, void* d
 // mode 0 = f64, 1 = f32, 2 = bf16 operands from f32 storage
 // P: split-K partial buffer (unused when no problem is split)
 hipError_t launch_gemm(const DevGemm& g, int mode, const void* A, const void* B, void* C, void* P, hipStream_t s);
-hipError_t launch_mpo(const DevMpo& d, int elem_size, const void* WL, const void* WR, void* W, const void* T1,
-                      void* T3, hipStream_t s);
+hipError_t launch_mpo_combine(const DevMpo& d, int elem_size, const void* WL, const void* WR, void* W,
+                              hipStream_t s);
+hipError_t launch_mpo_apply(const DevMpo& d, int elem_size, const void* W, const void* T1, void* T3, hipStream_t s);
 hipError_t launch_dot(int elem_size, int64_t n, const void* x, const void* y, double* out, double* partials,
                       int max_partials, hipStream_t s);
 hipError_t launch_axpby(int elem_size, int64_t n, double a, const void* x, double b, void* y, hipStream_t s);

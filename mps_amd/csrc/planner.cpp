@@ -354,7 +354,7 @@ static bool rev_less(const std::vector<int64_t>& a, const std::vector<int64_t>& 
 // block, column slabs outer and row slabs inner.
 // ---------------------------------------------------------------------------
 FusedMatrix symmatrix_plan(const Struct& A, const std::vector<int>& rows, const std::vector<int>& cols,
-                           bool transposed, int64_t dst_base) {
+                           bool transposed, int64_t dst_base, int kpad) {
   const int N = A.rank();
   std::vector<int> idxperm(rows);
   idxperm.insert(idxperm.end(), cols.begin(), cols.end());
@@ -401,11 +401,19 @@ FusedMatrix symmatrix_plan(const Struct& A, const std::vector<int>& rows, const 
   FusedMatrix FM;
   FM.F = make_struct(A.charge, {fuse(false, rlegs), fuse(true, clegs)});
   FM.k_contig_rows = transposed;
+  FM.ld.resize(FM.F.nblocks);
+  FM.poff.assign(FM.F.nblocks + 1, 0);
+  for (int64_t f = 0; f < FM.F.nblocks; ++f) {
+    const int64_t r = FM.F.size(f)[0], c = FM.F.size(f)[1];
+    // transposed: rows of length c (K) stored with stride ld; else column-major, ld >= r (K)
+    FM.ld[f] = transposed ? (c + kpad - 1) / kpad * kpad : (r + kpad - 1) / kpad * kpad;
+    FM.poff[f + 1] = FM.poff[f] + (transposed ? r : c) * FM.ld[f];
+  }
   int64_t ptr = 0;
   std::vector<int64_t> pd(N), sst(N), dst(N);
   for (int64_t f = 0; f < FM.F.nblocks; ++f) {
     const int64_t rlim = FM.F.size(f)[0], clim = FM.F.size(f)[1];
-    const int64_t foff = FM.F.offsets[f];
+    const int64_t foff = FM.poff[f], ldf = FM.ld[f];
     int64_t pc = 0;
     while (pc < clim) {
       int64_t pr = 0, fdc = 0;
@@ -423,15 +431,15 @@ FusedMatrix symmatrix_plan(const Struct& A, const std::vector<int>& rows, const 
         if (pr + fdr > rlim || pc + fdc > clim) throw Error(ERR_SECTOR_MISMATCH, "SymMatrix: slab overflow");
         int64_t acc = 1;
         for (int t = 0; t < n; ++t) {
-          dst[t] = transposed ? acc * clim : acc;
+          dst[t] = transposed ? acc * ldf : acc;
           acc *= pd[t];
         }
         acc = 1;
         for (int t = n; t < N; ++t) {
-          dst[t] = transposed ? acc : acc * rlim;
+          dst[t] = transposed ? acc : acc * ldf;
           acc *= pd[t];
         }
-        const int64_t doff = dst_base + foff + (transposed ? pr * clim + pc : pr + pc * rlim);
+        const int64_t doff = dst_base + foff + (transposed ? pr * ldf + pc : pr + pc * ldf);
         FM.gather.add(N, pd.data(), sst.data(), dst.data(), A.offsets[b], doff);
         pr += fdr;
         ++ptr;
@@ -465,13 +473,14 @@ ContractPlan contract_plan(const Struct& A, const std::vector<int>& idxA, const 
   P.A = A;
   P.B = B;
   P.fa_off = 0;
-  P.FA = symmatrix_plan(A, pa.rems, pa.cons, true, P.fa_off);
-  P.fb_off = P.FA.F.numel();
-  P.FB = symmatrix_plan(B, pb.cons, pb.rems, false, P.fb_off);
+  P.FA = symmatrix_plan(A, pa.rems, pa.cons, true, P.fa_off, 4);
+  P.fb_off = P.FA.numel_padded();
+  P.FB = symmatrix_plan(B, pb.cons, pb.rems, false, P.fb_off, 4);
   if (!isdual_pair(P.FA.F.legs[1], P.FB.F.legs[0])) throw Error(ERR_SPACE_MISMATCH, "not contractible!");
   const int64_t charge = A.charge + B.charge;
   P.FC = make_struct(charge, {P.FA.F.legs[0], P.FB.F.legs[1]});
-  P.fc_off = P.fb_off + P.FB.F.numel();
+  P.fc_off = P.fb_off + P.FB.numel_padded();
+  P.fc_off = (P.fc_off + 3) / 4 * 4;
   P.ws_elems = P.fc_off + P.FC.numel();
   const Leg& a0 = P.FA.F.legs[0];
   const Leg& a1 = P.FA.F.legs[1];
@@ -495,7 +504,7 @@ ContractPlan contract_plan(const Struct& A, const std::vector<int>& idxA, const 
     if (ia >= 0 && ib >= 0) {
       const int64_t K = P.FA.F.size(ia)[1];
       if (P.FB.F.size(ib)[0] != K) throw Error(ERR_SECTOR_MISMATCH, "fused contraction dims differ");
-      P.gemm.add_seg(K, P.FA.F.offsets[ia], K, P.FB.F.offsets[ib], K);
+      P.gemm.add_seg(K, P.FA.poff[ia], P.FA.ld[ia], P.FB.poff[ib], P.FB.ld[ib]);
     }
   }
   // split + final permutation in one scatter
@@ -770,7 +779,8 @@ static void need_pair(const Leg& a, const Leg& b, const char* what) {
   if (!isdual_pair(a, b)) throw Error(ERR_SPACE_MISMATCH, std::string("not contractible! ") + what);
 }
 
-HeffPlan heff_plan(const Struct& v, const Struct& envL, const Struct& envR, const Struct& WL, const Struct& WR) {
+HeffPlan heff_plan(const Struct& v, const Struct& envL, const Struct& envR, const Struct& WL, const Struct& WR,
+                   const HeffTiles& tiles) {
   if (v.rank() != 4 || envL.rank() != 3 || envR.rank() != 3 || WL.rank() != 4 || WR.rank() != 4)
     throw Error(ERR_INVALID, "heff: ranks must be v 4, envL 3, envR 3, WL 4, WR 4");
   need_pair(envL.legs[0], v.legs[0], "(envL ket, v left)");
@@ -808,13 +818,15 @@ HeffPlan heff_plan(const Struct& v, const Struct& envL, const Struct& envR, cons
 
   // ---- step 1: one fused GEMM per ket charge k (the reference's fused sizes),
   // T1_k = [v blocks (k,.)]^T x [envL blocks (k,.)], stored column-major (ld M_k).
+  // Operand rows (v block columns, envL block columns) are packed into
+  // K-contiguous rows of stride Kp = roundup(dk, 4): v per matvec, envL per bond.
   struct T1B {
     Key key;  // (b, w, s1, s2, r)
     int64_t base, rowoff, coloff, Mf, db, Dw, d1, d2, dR;
   };
+  auto up4 = [](int64_t x) { return (x + 3) / 4 * 4; };
   std::vector<T1B> t1;
-  H.g1.BM = 128;
-  H.g1.BN = 128;
+  int64_t vp_elems = 0, el_elems = 0;
   {
     std::map<int64_t, std::vector<int64_t>> vby, eby;
     for (int64_t x = 0; x < v.nblocks; ++x) vby[v.sector(x)[0]].push_back(x);
@@ -825,21 +837,37 @@ HeffPlan heff_plan(const Struct& v, const Struct& envL, const Struct& envR, cons
       if (itv == vby.end()) continue;
       const std::vector<int64_t>& X = itv->second;
       const std::vector<int64_t>& E = kv.second;
-      const int64_t dk = envL.size(E[0])[0];
-      std::vector<int64_t> arows, brows, rowoff(X.size()), coloff(E.size());
+      const int64_t dk = envL.size(E[0])[0], Kp = up4(dk);
+      std::vector<int64_t> rowoff(X.size()), coloff(E.size());
+      int64_t Mf = 0, Nf = 0;
       for (size_t i = 0; i < X.size(); ++i) {
         const int64_t* z = v.size(X[i]);
-        rowoff[i] = (int64_t)arows.size();
-        for (int64_t c = 0; c < z[1] * z[2] * z[3]; ++c) arows.push_back(v.offsets[X[i]] + dk * c);
+        rowoff[i] = Mf;
+        Mf += z[1] * z[2] * z[3];
       }
       for (size_t j = 0; j < E.size(); ++j) {
         const int64_t* z = envL.size(E[j]);
-        coloff[j] = (int64_t)brows.size();
-        for (int64_t c = 0; c < z[1] * z[2]; ++c) brows.push_back(envL.offsets[E[j]] + dk * c);
+        coloff[j] = Nf;
+        Nf += z[1] * z[2];
       }
-      const int64_t Mf = (int64_t)arows.size(), Nf = (int64_t)brows.size();
+      const int64_t vbase = vp_elems, ebase = el_elems;
+      vp_elems += Mf * Kp;
+      el_elems += Nf * Kp;
+      for (size_t i = 0; i < X.size(); ++i) {
+        const int64_t* z = v.size(X[i]);
+        const int64_t dims[2] = {dk, z[1] * z[2] * z[3]}, sst[2] = {1, dk}, dst[2] = {1, Kp};
+        H.pack_v.add(2, dims, sst, dst, v.offsets[X[i]], vbase + rowoff[i] * Kp);
+      }
+      for (size_t j = 0; j < E.size(); ++j) {
+        const int64_t* z = envL.size(E[j]);
+        const int64_t dims[2] = {dk, z[1] * z[2]}, sst[2] = {1, dk}, dst[2] = {1, Kp};
+        H.pack_envL.add(2, dims, sst, dst, envL.offsets[E[j]], ebase + coloff[j] * Kp);
+      }
       H.g1.add_prob(Mf, Nf);
       H.g1.add_crows_affine(off, Mf, Mf);
+      std::vector<int64_t> arows(Mf), brows(Nf);
+      for (int64_t m = 0; m < Mf; ++m) arows[m] = vbase + m * Kp;
+      for (int64_t n = 0; n < Nf; ++n) brows[n] = ebase + n * Kp;
       H.g1.add_seg_tab(dk, arows, brows);
       for (size_t i = 0; i < X.size(); ++i)
         for (size_t j = 0; j < E.size(); ++j) {
@@ -871,11 +899,10 @@ HeffPlan heff_plan(const Struct& v, const Struct& envL, const Struct& envR, cons
   // blocks (b, s1o, s2o, r'), K segments = envR blocks (r, x, r').  T3 region of
   // (r', f) is K_f x M_f, K-contiguous; cell (o, f) is T3 block (b, s1o, s2o, x, r).
   struct Cell {
-    int64_t base, rowoff, Kf, db, d1o, d2o, Dx, dR;
+    int64_t base, rowoff, Kf, Kfp, db, d1o, d2o, Dx, dR;  // T3 rows of stride Kfp = roundup(Kf, 4)
   };
+  int64_t er_elems = 0;
   std::map<Key, Cell> cells;  // key (b, s1o, s2o, x, r)
-  H.g4.BM = 64;
-  H.g4.BN = 64;
   {
     std::map<int64_t, std::vector<int64_t>> oby, fby;
     for (int64_t o = 0; o < H.out.nblocks; ++o) oby[H.out.sector(o)[3]].push_back(o);
@@ -902,21 +929,26 @@ HeffPlan heff_plan(const Struct& v, const Struct& envL, const Struct& envR, cons
       for (int64_t f : itf->second) {
         const int64_t* fs = envR.sector(f);
         const int64_t dR = envR.size(f)[0], Dx = envR.size(f)[1];
-        const int64_t Kf = dR * Dx;
+        const int64_t Kf = dR * Dx, Kfp = up4(Kf);
+        const int64_t rbase = er_elems;
+        er_elems += Kfp * N;
+        {
+          const int64_t dims[2] = {Kf, N}, sst[2] = {1, Kf}, dst[2] = {1, Kfp};
+          H.pack_envR.add(2, dims, sst, dst, envR.offsets[f], rbase);
+        }
         std::vector<int64_t> arows(Mf), brows(N);
-        for (int64_t m = 0; m < Mf; ++m) arows[m] = off + Kf * m;
-        for (int64_t n = 0; n < N; ++n) brows[n] = envR.offsets[f] + Kf * n;
+        for (int64_t m = 0; m < Mf; ++m) arows[m] = off + Kfp * m;
+        for (int64_t n = 0; n < N; ++n) brows[n] = rbase + Kfp * n;
         H.g4.add_seg_tab(Kf, arows, brows);
         for (size_t i = 0; i < O.size(); ++i) {
           const int64_t* os = H.out.sector(O[i]);
           const int64_t* z = H.out.size(O[i]);
-          cells[{os[0], os[1], os[2], fs[1], fs[0]}] = Cell{off, rowoff[i], Kf, z[0], z[1], z[2], Dx, dR};
+          cells[{os[0], os[1], os[2], fs[1], fs[0]}] = Cell{off, rowoff[i], Kf, Kfp, z[0], z[1], z[2], Dx, dR};
         }
-        off += Kf * Mf;
+        off += Kfp * Mf;
       }
     }
     H.t3_elems = off;
-    H.g4.choose_splits(2048, 16, 16);
   }
 
   // ---- steps 2+3: pixel groups (b, r); out types = the T3 cells of (b, r)
@@ -1037,7 +1069,7 @@ HeffPlan heff_plan(const Struct& v, const Struct& envL, const Struct& envR, cons
       for (int64_t rr = 0; rr < cc.Dx * cc.d1o * cc.d2o; ++rr) {
         const int64_t jx = rr % cc.Dx, i1 = (rr / cc.Dx) % cc.d1o, i2 = rr / (cc.Dx * cc.d1o);
         H.outrows.push_back(
-            MpoOutRow{cc.base + dR * jx + cc.Kf * (cc.rowoff + db * (i1 + cc.d1o * i2)), cc.Kf});
+            MpoOutRow{cc.base + dR * jx + cc.Kfp * (cc.rowoff + db * (i1 + cc.d1o * i2)), cc.Kfp});
       }
     }
     const int32_t gidx = (int32_t)H.groups.size();
@@ -1046,12 +1078,34 @@ HeffPlan heff_plan(const Struct& v, const Struct& envL, const Struct& envR, cons
     H.mpo_flops += 2 * R * C * g.npix;
   }
   H.w_elems = w_off;
-  H.t1_off = 0;
-  H.t3_off = H.t1_elems;
-  H.w_off = H.t3_off + H.t3_elems;
-  H.part_off = H.w_off + H.w_elems;
-  H.ws_elems = H.part_off + H.g4.part_elems + H.g1.part_elems;
-  H.g1.part_elems = 0;
+  // kernel variants and split-K choices (step 1 is never split: K = one bond charge)
+  auto cfg = [](GemmPlan& g, const TileCfg& c) {
+    g.BM = c.BM;
+    g.BN = c.BN;
+    g.KS = c.KS;
+  };
+  H.g1d = H.g1;
+  H.g4d = H.g4;
+  cfg(H.g1, tiles.s1);
+  cfg(H.g4, tiles.s4);
+  cfg(H.g1d, tiles.s1d);
+  cfg(H.g4d, tiles.s4d);
+  H.g4.choose_splits(tiles.split_target, 16 * tiles.s4.KS, 32);
+  H.g4d.choose_splits(tiles.split_target, 16 * tiles.s4d.KS, 32);
+  H.pack_v.finalize();
+  H.pack_envL.finalize();
+  H.pack_envR.finalize();
+  // prepared operator: [envL_p | envR_p | W]
+  H.el_off = 0;
+  H.er_off = up4(el_elems);
+  H.pw_off = up4(H.er_off + er_elems);
+  H.prep_elems = H.pw_off + H.w_elems;
+  // per-matvec workspace: [v_p | T1 | T3 | partials]
+  H.vp_off = 0;
+  H.t1_off = up4(vp_elems);
+  H.t3_off = up4(H.t1_off + H.t1_elems);
+  H.part_off = up4(H.t3_off + H.t3_elems);
+  H.ws_elems = H.part_off + std::max(H.g4.part_elems, H.g4d.part_elems);
   return H;
 }
 

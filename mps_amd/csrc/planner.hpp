@@ -148,7 +148,7 @@ struct GemmPlan {
   std::vector<GemmSeg> segs;
   std::vector<int64_t> atab, btab, crow, cld;
   int64_t flops = 0;  // 2*M*N*K summed (algorithmic)
-  int BM = 64, BN = 64;
+  int BM = 64, BN = 64, KS = 1;  // tile shape and K-chunk multiplier of the kernel variant
   int64_t part_elems = 0;
   // open a problem; its C rows follow with add_crow()
   void add_prob(int64_t M, int64_t N) {
@@ -185,13 +185,17 @@ struct FusedMatrix {
   Struct F;          // the fused rank-2 structure
   CopyPlan gather;   // A blocks -> fused blocks
   bool k_contig_rows;  // storage: true = rows K-contiguous (A^T), false = column-major
+  // storage of each fused block with its K-contiguous dimension padded to a
+  // multiple of kpad (16-byte aligned rows for vector loads): leading dim and offsets
+  std::vector<int64_t> ld, poff;  // nblocks, nblocks + 1
+  int64_t numel_padded() const { return poff.empty() ? 0 : poff.back(); }
 };
 
 // SymMatrix(A, rows, cols) (src/contract.jl:112-171).  `transposed` stores
 // each fused block row-major (element (r,c) at r*ncols + c) instead of
 // column-major, which is the K-contiguous layout of a left GEMM operand.
 FusedMatrix symmatrix_plan(const Struct& A, const std::vector<int>& rows, const std::vector<int>& cols,
-                           bool transposed, int64_t dst_base);
+                           bool transposed, int64_t dst_base, int kpad = 1);
 
 struct ContractPlan {
   Struct A, B, C;
@@ -250,9 +254,20 @@ struct MpoWork {
   int32_t group, pix0;
 };
 
+// Kernel variant of one grouped GEMM: tile BM x BN, K chunk multiplier KS.
+struct TileCfg {
+  int BM = 64, BN = 64, KS = 1;
+};
+struct HeffTiles {
+  TileCfg s1{128, 64, 1}, s4{64, 64, 1};     // FP32 / BF16 storage
+  TileCfg s1d{64, 64, 1}, s4d{64, 64, 1};    // FP64
+  int64_t split_target = 1024;               // minimum work items for the step-4 GEMM
+};
+
 struct HeffPlan {
   Struct v, envL, envR, WL, WR, out;
-  GemmPlan g1;  // A = v, B = envL, C = T1 (ws)
+  GemmPlan g1;  // A = v, B = envL, C = T1 (ws); FP32/BF16 tile variant
+  GemmPlan g1d, g4d;  // the same problems with the FP64 tile variants
   int64_t t1_elems = 0;
   std::vector<MpoSig> sigs;
   std::vector<MpoTerm> terms;
@@ -264,9 +279,14 @@ struct HeffPlan {
   int64_t w_elems = 0;
   int64_t t3_elems = 0;
   int maxR = 0, maxC = 0;
-  GemmPlan g4;  // A = T3 (ws), B = envR, C = out
-  // workspace layout (elements): [T1 | T3 | W | split-K partials of step 4]
-  int64_t t1_off = 0, t3_off = 0, w_off = 0, part_off = 0, ws_elems = 0;
+  GemmPlan g4;  // A = T3 (ws), B = envR_p (prepared), C = out
+  // Prepared operator (per bond): envL and envR repacked into 16-byte aligned
+  // K-contiguous rows (K padded to a multiple of 4) and the combined-MPO matrices.
+  CopyPlan pack_envL, pack_envR;
+  int64_t el_off = 0, er_off = 0, pw_off = 0, prep_elems = 0;
+  // Per-matvec workspace: [v_p | T1 | T3 | split-K partials]; v is packed the same way.
+  CopyPlan pack_v;
+  int64_t vp_off = 0, t1_off = 0, t3_off = 0, part_off = 0, ws_elems = 0;
   // reference-order algorithmic cost (src/apply.jl:99-103 via src/contract.jl:290-291)
   int64_t ref_flops = 0;
   int64_t ref_elems = 0;  // sum over the 4 steps of numel(A)+numel(B)+numel(C)
@@ -275,6 +295,6 @@ struct HeffPlan {
   int64_t mpo_flops = 0;  // dense W_group application flops of this engine
 };
 HeffPlan heff_plan(const Struct& v, const Struct& envL, const Struct& envR, const Struct& WL,
-                   const Struct& WR);
+                   const Struct& WR, const HeffTiles& tiles = HeffTiles());
 
 }  // namespace mpsh

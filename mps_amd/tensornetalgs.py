@@ -40,29 +40,53 @@ class HeffPlan:
         self._ws: Dict = {}
 
     def workspace_bytes(self, prec: str) -> int:
+        """Workspace of the one-shot apply (prepared operator included)."""
         return int(lib().mpsh_heff_workspace(self.plan, _PREC[prec]))
 
-    def workspace(self, prec: str, device) -> torch.Tensor:
-        key = (prec, device)
+    def apply_workspace_bytes(self, prec: str) -> int:
+        return int(lib().mpsh_heff_apply_workspace(self.plan, _PREC[prec]))
+
+    def prepared_bytes(self, prec: str) -> int:
+        return int(lib().mpsh_heff_prepared_size(self.plan, _PREC[prec]))
+
+    def workspace(self, prec: str, device, kind: str = "oneshot") -> torch.Tensor:
+        key = (prec, device, kind)
         ws = self._ws.get(key)
         if ws is None:
-            ws = torch.empty(max(1, self.workspace_bytes(prec)), dtype=torch.uint8, device=device)
+            n = self.workspace_bytes(prec) if kind == "oneshot" else self.apply_workspace_bytes(prec)
+            ws = torch.empty(max(16, n), dtype=torch.uint8, device=device)
             self._ws[key] = ws
         return ws
 
     def apply(self, prec: str, v: torch.Tensor, envL: torch.Tensor, envR: torch.Tensor, WL: torch.Tensor,
               WR: torch.Tensor, out: torch.Tensor, ws: torch.Tensor = None, stream=None):
-        """Raw apply on packed device buffers (all of the precision's storage dtype)."""
+        """One-shot apply on packed device buffers (all of the precision's storage dtype)."""
         ws = ws if ws is not None else self.workspace(prec, v.device)
         check(lib().mpsh_heff_apply(self.plan, _PREC[prec], _ptr(v), _ptr(envL), _ptr(envR), _ptr(WL), _ptr(WR),
                                     _ptr(out), _ptr(ws), stream if stream is not None else _stream()))
         return out
 
-    def apply_stages(self, prec: str, stages: int, v, envL, envR, WL, WR, out, ws, stream=None):
-        """Run a subset of the three kernel stages (1 step-1 GEMM, 2 MPO pass, 4 step-4 GEMM)."""
-        check(lib().mpsh_heff_apply_stages(self.plan, _PREC[prec], int(stages), _ptr(v), _ptr(envL), _ptr(envR),
-                                           _ptr(WL), _ptr(WR), _ptr(out), _ptr(ws),
-                                           stream if stream is not None else _stream()))
+    def prepare(self, prec: str, envL: torch.Tensor, envR: torch.Tensor, WL: torch.Tensor, WR: torch.Tensor,
+                prepared: torch.Tensor = None, stream=None) -> torch.Tensor:
+        """Repack the bond's environments and form the combined MPO (once per bond)."""
+        if prepared is None:
+            prepared = torch.empty(max(16, self.prepared_bytes(prec)), dtype=torch.uint8, device=envL.device)
+        check(lib().mpsh_heff_prepare(self.plan, _PREC[prec], _ptr(envL), _ptr(envR), _ptr(WL), _ptr(WR),
+                                      _ptr(prepared), stream if stream is not None else _stream()))
+        return prepared
+
+    def apply_prepared(self, prec: str, prepared: torch.Tensor, v: torch.Tensor, out: torch.Tensor,
+                       ws: torch.Tensor = None, stream=None):
+        ws = ws if ws is not None else self.workspace(prec, v.device, "apply")
+        check(lib().mpsh_heff_apply_prepared(self.plan, _PREC[prec], _ptr(prepared), _ptr(v), _ptr(out), _ptr(ws),
+                                             stream if stream is not None else _stream()))
+        return out
+
+    def apply_stages(self, prec: str, stages: int, prepared, v, out, ws, stream=None):
+        """Run a subset of the kernel stages (1 pack v, 2 step-1 GEMM, 4 MPO pass, 8 step-4 GEMM)."""
+        check(lib().mpsh_heff_apply_prepared_stages(self.plan, _PREC[prec], int(stages), _ptr(prepared), _ptr(v),
+                                                    _ptr(out), _ptr(ws),
+                                                    stream if stream is not None else _stream()))
         return out
 
     def __del__(self):
@@ -203,27 +227,37 @@ def lanczos_lowest(apply, x0: SymTensor, tol: float = 1e-7, krylovdim: int = 5, 
 
 
 class HeffOperator:
-    """H_eff of one bond as a map on FP64 device vectors; matvec in ``prec`` (f64/f32/bf16)."""
+    """H_eff of one bond as a map on FP64 device vectors; matvec in ``prec`` (f64/f32/bf16).
+
+    The environments and MPO are prepared once (repacked rows, combined MPO);
+    every matvec then runs the three kernel stages on the prepared operator.
+    """
 
     def __init__(self, envL: SymTensor, envR: SymTensor, WL: SymTensor, WR: SymTensor, prec: str = "f64"):
         self.prec = prec
         self.ops64 = (envL, envR, WL, WR)
-        if prec == "f64":
-            self.ops = self.ops64
-        else:
-            self.ops = tuple(t.to(torch.float32) for t in self.ops64)
+        self.ops = self.ops64 if prec == "f64" else tuple(t.to(torch.float32) for t in self.ops64)
+        self._prep = {}
         self.matvecs = 0
+
+    def _apply(self, x: SymTensor, prec: str, ops) -> SymTensor:
+        plan = heff_plan(x, *ops)
+        key = (prec, x.struct.key)
+        if key not in self._prep:
+            self._prep[key] = plan.prepare(prec, *(t.data for t in ops))
+        out = torch.empty(plan.out.numel, dtype=x.dtype, device=x.device)
+        plan.apply_prepared(prec, self._prep[key], x.data, out)
+        return SymTensor(plan.out.charge, plan.out.spaces, out, plan.out)
 
     def __call__(self, x: SymTensor) -> SymTensor:
         self.matvecs += 1
         if self.prec == "f64":
-            return _applymps2site(x, *self.ops)
-        y = _applymps2site(x.to(torch.float32), *self.ops, prec=self.prec)
-        return y.to(torch.float64)
+            return self._apply(x, "f64", self.ops)
+        return self._apply(x.to(torch.float32), self.prec, self.ops).to(torch.float64)
 
     def exact(self, x: SymTensor) -> SymTensor:
         self.matvecs += 1
-        return _applymps2site(x, *self.ops64)
+        return self._apply(x, "f64", self.ops64)
 
 
 def eigsolve(H: HeffOperator, x0: SymTensor, tol=1e-7, krylovdim=5, maxiter=8, refine_tol=None):
