@@ -71,7 +71,9 @@ struct DevTables {
   }
   DevGemm gemm(const GemmPlan& g) {
     DevGemm d;
-    std::vector<GemmTile> t = g.tiles();
+    std::vector<int32_t> qoff;
+    std::vector<GemmTile> t = g.tiles(&qoff);
+    for (int x = 0; x <= NXCD; ++x) d.q.off[x] = qoff[x];
     std::vector<RedDesc> r = g.reductions();
     d.probs = up(g.probs);
     d.segs = up(g.segs);

@@ -11,6 +11,7 @@
 //  * dot/axpby/scal/convert -- block-wise vector ops on packed buffers.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cstdint>
 #include <type_traits>
 
@@ -179,32 +180,42 @@ __device__ __forceinline__ void vstore(typename GT<MODE>::L* dst, const typename
   }
 }
 
+// One lane's K-run of an operand row in LDS and the MFMAs that consume it:
+// FP64 16x16x4 (4 k per lane = 4 MFMAs), FP32 16x16x4 (4 k = 4 MFMAs),
+// BF16 16x16x32 (8 k = 1 MFMA).  mfma(c, b, a, kk) issues k-slice kk.
 template <int MODE>
-__device__ __forceinline__ void mfma_chunk(typename GT<MODE>::Acc& c, const typename GT<MODE>::L* a,
-                                           const typename GT<MODE>::L* b);
-
+struct Frag;
 template <>
-__device__ __forceinline__ void mfma_chunk<0>(doublex4& c, const double* a, const double* b) {
-  const double2 a01 = *(const double2*)a, a23 = *(const double2*)(a + 2);
-  const double2 b01 = *(const double2*)b, b23 = *(const double2*)(b + 2);
-  c = __builtin_amdgcn_mfma_f64_16x16x4f64(a01.x, b01.x, c, 0, 0, 0);
-  c = __builtin_amdgcn_mfma_f64_16x16x4f64(a01.y, b01.y, c, 0, 0, 0);
-  c = __builtin_amdgcn_mfma_f64_16x16x4f64(a23.x, b23.x, c, 0, 0, 0);
-  c = __builtin_amdgcn_mfma_f64_16x16x4f64(a23.y, b23.y, c, 0, 0, 0);
-}
+struct Frag<0> {
+  struct F {
+    double2 lo, hi;
+  };
+  static constexpr int NK = 4;
+  static __device__ __forceinline__ F load(const double* p) { return F{*(const double2*)p, *(const double2*)(p + 2)}; }
+  static __device__ __forceinline__ void mfma(doublex4& c, const F& b, const F& a, int kk) {
+    const double bv = kk == 0 ? b.lo.x : kk == 1 ? b.lo.y : kk == 2 ? b.hi.x : b.hi.y;
+    const double av = kk == 0 ? a.lo.x : kk == 1 ? a.lo.y : kk == 2 ? a.hi.x : a.hi.y;
+    c = __builtin_amdgcn_mfma_f64_16x16x4f64(bv, av, c, 0, 0, 0);
+  }
+};
 template <>
-__device__ __forceinline__ void mfma_chunk<1>(floatx4& c, const float* a, const float* b) {
-  const floatx4 av = *(const floatx4*)a, bv = *(const floatx4*)b;
-  c = __builtin_amdgcn_mfma_f32_16x16x4f32(av[0], bv[0], c, 0, 0, 0);
-  c = __builtin_amdgcn_mfma_f32_16x16x4f32(av[1], bv[1], c, 0, 0, 0);
-  c = __builtin_amdgcn_mfma_f32_16x16x4f32(av[2], bv[2], c, 0, 0, 0);
-  c = __builtin_amdgcn_mfma_f32_16x16x4f32(av[3], bv[3], c, 0, 0, 0);
-}
+struct Frag<1> {
+  typedef floatx4 F;
+  static constexpr int NK = 4;
+  static __device__ __forceinline__ F load(const float* p) { return *(const floatx4*)p; }
+  static __device__ __forceinline__ void mfma(floatx4& c, const F& b, const F& a, int kk) {
+    c = __builtin_amdgcn_mfma_f32_16x16x4f32(b[kk], a[kk], c, 0, 0, 0);
+  }
+};
 template <>
-__device__ __forceinline__ void mfma_chunk<2>(floatx4& c, const __bf16* a, const __bf16* b) {
-  const bf16x8 av = *(const bf16x8*)a, bv = *(const bf16x8*)b;
-  c = __builtin_amdgcn_mfma_f32_16x16x32_bf16(av, bv, c, 0, 0, 0);
-}
+struct Frag<2> {
+  typedef bf16x8 F;
+  static constexpr int NK = 1;
+  static __device__ __forceinline__ F load(const __bf16* p) { return *(const bf16x8*)p; }
+  static __device__ __forceinline__ void mfma(floatx4& c, const F& b, const F& a, int) {
+    c = __builtin_amdgcn_mfma_f32_16x16x32_bf16(b, a, c, 0, 0, 0);
+  }
+};
 
 // Tile BM x BN, 4 waves in 2x2, each wave (BM/2) x (BN/2) = SM x SN MFMA 16x16
 // sub-tiles; K chunk = KS k-slices of one MFMA group each.  Operand rows come
@@ -216,7 +227,7 @@ template <int MODE, int BM, int BN, int KS>
 __global__ __launch_bounds__(256) void gemm_grouped(
     const GemmProb* __restrict__ probs, const GemmSeg* __restrict__ segs, const int64_t* __restrict__ atab,
     const int64_t* __restrict__ btab, const int64_t* __restrict__ crow, const int64_t* __restrict__ cld,
-    const GemmTile* __restrict__ tiles, int ntiles, const typename GT<MODE>::T* __restrict__ A,
+    const GemmTile* __restrict__ tiles, XcdQueues xq, const typename GT<MODE>::T* __restrict__ A,
     const typename GT<MODE>::T* __restrict__ B, typename GT<MODE>::T* __restrict__ C,
     typename GT<MODE>::T* __restrict__ P) {
   typedef typename GT<MODE>::T T;
@@ -239,7 +250,11 @@ __global__ __launch_bounds__(256) void gemm_grouped(
   const int q = tid % TPR, row = tid / TPR;
   const int g = lane >> 4, l16 = lane & 15;
 
-  for (int t = blockIdx.x; t < ntiles; t += gridDim.x) {
+  // work group b runs on XCD b % NXCD (round-robin dispatch) and walks that
+  // XCD's tile queue; gridDim.x is a multiple of NXCD
+  const int xcd = blockIdx.x % NXCD, gq = gridDim.x / NXCD;
+  const int tend = xq.off[xcd + 1];
+  for (int t = xq.off[xcd] + blockIdx.x / NXCD; t < tend; t += gq) {
     const GemmTile tile = tiles[t];
     const GemmProb p = probs[tile.prob];
     Acc acc[SM][SN];
@@ -318,15 +333,19 @@ __global__ __launch_bounds__(256) void gemm_grouped(
       if (it + DEPTH < nch) load(D);
 #pragma unroll
       for (int kc = 0; kc < BK; kc += 4 * KL) {
-        const L* bp[SN];
+        // operands of all sub-tiles first, then the MFMAs k-slice-outer so that
+        // consecutive MFMAs accumulate into different sub-tiles (no dependent chain)
+        typename Frag<MODE>::F fa[SM], fb[SN];
 #pragma unroll
-        for (int j = 0; j < SN; ++j) bp[j] = &Bs[lb][wn * (BN / 2) + j * 16 + l16][kc + KL * g];
+        for (int j = 0; j < SN; ++j) fb[j] = Frag<MODE>::load(&Bs[lb][wn * (BN / 2) + j * 16 + l16][kc + KL * g]);
 #pragma unroll
-        for (int i = 0; i < SM; ++i) {
-          const L* ap = &As[lb][wm * (BM / 2) + i * 16 + l16][kc + KL * g];
+        for (int i = 0; i < SM; ++i) fa[i] = Frag<MODE>::load(&As[lb][wm * (BM / 2) + i * 16 + l16][kc + KL * g]);
 #pragma unroll
-          for (int j = 0; j < SN; ++j) mfma_chunk<MODE>(acc[i][j], ap, bp[j]);
-        }
+        for (int kk = 0; kk < Frag<MODE>::NK; ++kk)
+#pragma unroll
+          for (int i = 0; i < SM; ++i)
+#pragma unroll
+            for (int j = 0; j < SN; ++j) Frag<MODE>::mfma(acc[i][j], fb[j], fa[i], kk);
       }
     };
     typedef std::integral_constant<int, 0> D0;
@@ -337,31 +356,30 @@ __global__ __launch_bounds__(256) void gemm_grouped(
       step(D0{}, it);
       if (DEPTH > 1 && it + 1 < nch) step(D1{}, it + 1);
     }
-    // epilogue: column l16 of each 16x16 sub-tile; accumulator register r holds
-    // row g + 4r for the FP64 16x16x4 form and row 4g + r for the FP32/BF16 forms.
+    // epilogue: the MFMA is fed (B, A), so sub-tile (i, j) holds C^T: lane l16
+    // indexes the row m (contiguous in C) and accumulator register r the column
+    // n = g + 4r (FP64 16x16x4 form) or 4g + r (FP32/BF16 forms).  Each store
+    // instruction writes 16 consecutive rows of 4 columns.
     const bool split = p.nsplit > 1;
     T* Pp = P + (split ? p.part_off + (int64_t)tile.split * p.M * p.N : 0);
 #pragma unroll
-    for (int i = 0; i < SM; ++i)
-#pragma unroll
-      for (int r = 0; r < 4; ++r) {
-        const int m = tile.m0 + wm * (BM / 2) + i * 16 + (MODE == 0 ? g + 4 * r : 4 * g + r);
-        if (m >= p.M) continue;
-        int64_t cb_ = 0, cl_ = 0;
-        if (!split) {
-          cb_ = crow[p.crow0 + m];
-          cl_ = cld[p.crow0 + m];
-        }
-#pragma unroll
-        for (int j = 0; j < SN; ++j) {
-          const int n = tile.n0 + wn * (BN / 2) + j * 16 + l16;
-          if (n >= p.N) continue;
-          if (split)
-            Pp[m + (int64_t)p.M * n] = (T)acc[i][j][r];
-          else
-            C[cb_ + (int64_t)n * cl_] = (T)acc[i][j][r];
-        }
+    for (int i = 0; i < SM; ++i) {
+      const int m = tile.m0 + wm * (BM / 2) + i * 16 + l16;
+      if (m >= p.M) continue;
+      int64_t cb_ = m, cl_ = p.M;
+      if (!split) {
+        cb_ = crow[p.crow0 + m];
+        cl_ = cld[p.crow0 + m];
       }
+      T* Cb = split ? Pp : C;
+#pragma unroll
+      for (int j = 0; j < SN; ++j)
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+          const int n = tile.n0 + wn * (BN / 2) + j * 16 + (MODE == 0 ? g + 4 * r : 4 * g + r);
+          if (n < p.N) Cb[cb_ + (int64_t)n * cl_] = (T)acc[i][j][r];
+        }
+    }
     __syncthreads();  // LDS reuse by the next tile
   }
 }
@@ -411,10 +429,13 @@ static int num_cu() {
 template <int MODE, int BM, int BN, int KS>
 static void launch_variant(const DevGemm& gm, const void* A, const void* B, void* C, void* P, hipStream_t s) {
   typedef typename GT<MODE>::T T;
-  const int maxgrid = num_cu() * 8;
-  dim3 grid((unsigned)(gm.ntiles < maxgrid ? gm.ntiles : maxgrid)), block(256);
+  // per XCD: up to num_cu work groups (8 per CU of the XCD), no more than its queue holds
+  int qmax = 0;
+  for (int x = 0; x < NXCD; ++x) qmax = std::max(qmax, gm.q.off[x + 1] - gm.q.off[x]);
+  const int per = std::min(num_cu(), qmax);
+  dim3 grid((unsigned)(per * NXCD)), block(256);
   hipLaunchKernelGGL((gemm_grouped<MODE, BM, BN, KS>), grid, block, 0, s, gm.probs, gm.segs, gm.atab, gm.btab,
-                     gm.crow, gm.cld, gm.tiles, gm.ntiles, (const T*)A, (const T*)B, (T*)C, (T*)P);
+                     gm.crow, gm.cld, gm.tiles, gm.q, (const T*)A, (const T*)B, (T*)C, (T*)P);
 }
 
 // Compiled tile variants: (BM, BN) in {64x64, 128x64, 64x128, 128x128} with K

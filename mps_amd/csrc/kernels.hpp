@@ -14,6 +14,10 @@ struct DevCopy {
   int64_t ntiles = 0;
 };
 
+struct XcdQueues {
+  int32_t off[NXCD + 1];
+};
+
 struct DevGemm {
   const GemmProb* probs = nullptr;
   const GemmSeg* segs = nullptr;
@@ -23,6 +27,7 @@ struct DevGemm {
   int ntiles = 0, nred = 0;
   int64_t red_groups = 0;
   int BM = 64, BN = 64, KS = 1;  // tile and K-chunk multiplier
+  XcdQueues q{};                 // tiles [q.off[x], q.off[x+1]) run on XCD x
 };
 
 struct DevMpo {

@@ -274,21 +274,55 @@ void GemmPlan::choose_splits(int64_t target, int BK, int64_t min_chunks_per_spli
   }
 }
 
-std::vector<GemmTile> GemmPlan::tiles() const {
-  std::vector<std::pair<int64_t, GemmTile>> t;
+std::vector<GemmTile> GemmPlan::tiles(std::vector<int32_t>* qoff) const {
+  // Panel pieces: the tiles of one (problem, split) over a contiguous range of
+  // m0 rows (all n0).  Tiles of a piece share A rows (same m0) and B rows (same
+  // n0), so a piece is placed whole on one XCD; large (problem, split) units are
+  // cut by m0 rows into pieces of at most a quarter of one XCD's share.
+  struct Piece {
+    int64_t work;
+    int32_t prob, split, mr0, mr1;
+  };
+  std::vector<std::pair<int32_t, int64_t>> units;  // (prob, chunk K of one split)
+  int64_t total = 0;
   for (size_t pi = 0; pi < probs.size(); ++pi) {
     const GemmProb& q = probs[pi];
     int64_t K = 0;
     for (int s = 0; s < q.nseg; ++s) K += segs[q.seg0 + s].K;
-    const int64_t Ks = K / std::max(1, q.nsplit);
-    for (int m0 = 0; m0 < q.M; m0 += BM)
-      for (int n0 = 0; n0 < q.N; n0 += BN)
-        for (int sp = 0; sp < q.nsplit; ++sp) t.push_back({Ks, GemmTile{(int32_t)pi, m0, n0, sp}});
+    const int64_t Ks = std::max<int64_t>(1, K / std::max(1, q.nsplit));
+    const int64_t nm = (q.M + BM - 1) / BM, nn = (q.N + BN - 1) / BN;
+    total += nm * nn * q.nsplit * Ks;
+    units.push_back({(int32_t)pi, Ks});
   }
-  std::stable_sort(t.begin(), t.end(), [](const auto& a, const auto& b) { return a.first > b.first; });
+  const int64_t cap = std::max<int64_t>(1, total / (NXCD * 4));
+  std::vector<Piece> pieces;
+  for (auto& u : units) {
+    const GemmProb& q = probs[u.first];
+    const int64_t nm = (q.M + BM - 1) / BM, nn = (q.N + BN - 1) / BN;
+    const int64_t row_work = nn * u.second;
+    const int64_t rows_per = std::max<int64_t>(1, cap / std::max<int64_t>(1, row_work));
+    for (int sp = 0; sp < q.nsplit; ++sp)
+      for (int64_t r0 = 0; r0 < nm; r0 += rows_per) {
+        const int64_t r1 = std::min(nm, r0 + rows_per);
+        pieces.push_back(Piece{(r1 - r0) * row_work, u.first, sp, (int32_t)r0, (int32_t)r1});
+      }
+  }
+  std::stable_sort(pieces.begin(), pieces.end(), [](const Piece& a, const Piece& b) { return a.work > b.work; });
+  std::vector<std::vector<GemmTile>> queue(NXCD);
+  std::vector<int64_t> load(NXCD, 0);
+  for (const Piece& pc : pieces) {
+    const int x = (int)(std::min_element(load.begin(), load.end()) - load.begin());
+    load[x] += pc.work;
+    const GemmProb& q = probs[pc.prob];
+    for (int32_t r = pc.mr0; r < pc.mr1; ++r)
+      for (int n0 = 0; n0 < q.N; n0 += BN) queue[x].push_back(GemmTile{pc.prob, r * BM, n0, pc.split});
+  }
   std::vector<GemmTile> out;
-  out.reserve(t.size());
-  for (auto& x : t) out.push_back(x.second);
+  if (qoff) qoff->assign(NXCD + 1, 0);
+  for (int x = 0; x < NXCD; ++x) {
+    out.insert(out.end(), queue[x].begin(), queue[x].end());
+    if (qoff) (*qoff)[x + 1] = (int32_t)out.size();
+  }
   return out;
 }
 
@@ -817,7 +851,8 @@ HeffPlan heff_plan(const Struct& v, const Struct& envL, const Struct& envR, cons
   }
 
   // ---- step 1: one fused GEMM per ket charge k (the reference's fused sizes),
-  // T1_k = [v blocks (k,.)]^T x [envL blocks (k,.)], stored column-major (ld M_k).
+  // T1_k = [v blocks (k,.)]^T x [envL blocks (k,.)], stored column-major (ld M_k
+  // rounded up to 32).
   // Operand rows (v block columns, envL block columns) are packed into
   // K-contiguous rows of stride Kp = roundup(dk, 4): v per matvec, envL per bond.
   struct T1B {
@@ -863,8 +898,10 @@ HeffPlan heff_plan(const Struct& v, const Struct& envL, const Struct& envR, cons
         const int64_t dims[2] = {dk, z[1] * z[2]}, sst[2] = {1, dk}, dst[2] = {1, Kp};
         H.pack_envL.add(2, dims, sst, dst, envL.offsets[E[j]], ebase + coloff[j] * Kp);
       }
+      // columns of T1_k start on 128-byte boundaries (ld = Mf rounded up to 32)
+      const int64_t Mld = (Mf + 31) / 32 * 32;
       H.g1.add_prob(Mf, Nf);
-      H.g1.add_crows_affine(off, Mf, Mf);
+      H.g1.add_crows_affine(off, Mld, Mf);
       std::vector<int64_t> arows(Mf), brows(Nf);
       for (int64_t m = 0; m < Mf; ++m) arows[m] = vbase + m * Kp;
       for (int64_t n = 0; n < Nf; ++n) brows[n] = ebase + n * Kp;
@@ -878,7 +915,7 @@ HeffPlan heff_plan(const Struct& v, const Struct& envL, const Struct& envR, cons
           t.base = off;
           t.rowoff = rowoff[i];
           t.coloff = coloff[j];
-          t.Mf = Mf;
+          t.Mf = Mld;
           t.db = envL.size(E[j])[2];
           t.Dw = envL.size(E[j])[1];
           t.d1 = v.size(X[i])[1];
@@ -886,7 +923,7 @@ HeffPlan heff_plan(const Struct& v, const Struct& envL, const Struct& envR, cons
           t.dR = v.size(X[i])[3];
           t1.push_back(t);
         }
-      off += Mf * Nf;
+      off += Mld * Nf;
     }
     H.t1_elems = off;
     std::stable_sort(t1.begin(), t1.end(), [](const T1B& a, const T1B& b) {
@@ -1102,9 +1139,10 @@ HeffPlan heff_plan(const Struct& v, const Struct& envL, const Struct& envR, cons
   H.prep_elems = H.pw_off + H.w_elems;
   // per-matvec workspace: [v_p | T1 | T3 | partials]
   H.vp_off = 0;
-  H.t1_off = up4(vp_elems);
-  H.t3_off = up4(H.t1_off + H.t1_elems);
-  H.part_off = up4(H.t3_off + H.t3_elems);
+  auto up64 = [](int64_t x) { return (x + 63) / 64 * 64; };  // 256-byte (FP32) region starts
+  H.t1_off = up64(vp_elems);
+  H.t3_off = up64(H.t1_off + H.t1_elems);
+  H.part_off = up64(H.t3_off + H.t3_elems);
   H.ws_elems = H.part_off + std::max(H.g4.part_elems, H.g4d.part_elems);
   return H;
 }

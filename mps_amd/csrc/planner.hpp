@@ -135,6 +135,8 @@ struct GemmProb {
 struct GemmSeg {
   int64_t K, atab0, btab0;
 };
+constexpr int NXCD = 8;  // MI355X: 8 accelerator complex dies, one L2 each
+
 struct GemmTile {
   int32_t prob, m0, n0, split;
 };
@@ -168,7 +170,11 @@ struct GemmPlan {
   void add_seg(int64_t K, int64_t a_off, int64_t lda, int64_t b_off, int64_t ldb);
   // split long-K problems so that the number of work items reaches `target`
   void choose_splits(int64_t target, int BK, int64_t min_chunks_per_split);
-  std::vector<GemmTile> tiles() const;  // longest-K first (static LPT order)
+  // Tiles in per-XCD queues: queue x = tiles [qoff[x], qoff[x+1]) runs on the
+  // work groups with blockIdx % NXCD == x.  Work is balanced over the queues
+  // (LPT over panel groups) and the tiles that share operand panels stay in
+  // one queue, so each XCD's L2 fetches its panels once.
+  std::vector<GemmTile> tiles(std::vector<int32_t>* qoff = nullptr) const;
   std::vector<RedDesc> reductions() const;
 };
 
@@ -259,7 +265,7 @@ struct TileCfg {
   int BM = 64, BN = 64, KS = 1;
 };
 struct HeffTiles {
-  TileCfg s1{128, 64, 1}, s4{64, 64, 1};     // FP32 / BF16 storage
+  TileCfg s1{128, 64, 1}, s4{64, 64, 2};     // FP32 / BF16 storage
   TileCfg s1d{64, 64, 1}, s4d{64, 64, 1};    // FP64
   int64_t split_target = 1024;               // minimum work items for the step-4 GEMM
 };
