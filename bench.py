@@ -26,6 +26,9 @@ if ROOT not in sys.path:
 PEAK = {"f32": 157.3e12, "f64": 78.6e12, "bf16": 2.5e15}
 HBM_PEAK = 8.0e12
 METRIC = "H_eff matvecs/sec + DMRG sweep time at χ=2048; |ΔE| vs FP64 CPU oracle"
+# HBM bytes per step-1 GEMM launch: rocprofv3 FETCH_SIZE x 2.000 + WRITE_SIZE x 1.000
+# (scales from a 256 MiB device copy), profiles/heff_cyl6x24_chi2048_f32_hbm_traffic.txt
+TRAFFIC = {("cyl6x24_chi2048", "f32"): 228.4e6}
 
 
 def parse():
@@ -213,8 +216,9 @@ def main():
         "prepare_ms_per_bond": round(prep_ms, 5),
         "roofline": {"bound": "mfma", "kernel": f"gemm_grouped<{ {'f64': 0, 'f32': 1, 'bf16': 2}[prec] },...> step-1 launch (v^T envL)",
                      "achieved": round(achieved / 1e12, 3), "peak": PEAK[prec] / 1e12, "unit": "TFLOP/s",
-                     "frac": round(achieved / PEAK[prec], 4), "traffic": None,
-                     "flops_per_launch": s1_flops, "avg_launch_ms": round(stages[1], 5)},
+                     "frac": round(achieved / PEAK[prec], 4), "traffic": TRAFFIC.get((args.workload, prec)),
+                     "flops_per_launch": s1_flops, "avg_launch_ms": round(stages[1], 5),
+                     "algorithmic_bytes_per_launch": plan.step_elems[0] * (8 if prec == "f64" else 4)},
         "step4_gemm_tflops": round(s4_tflops, 3),
     }
     # E = <v|H v>/<v|v> of this matvec in the benchmark precision, FP64 device path as reference
