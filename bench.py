@@ -41,6 +41,8 @@ def parse():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-extra-modes", action="store_true")
     ap.add_argument("--cpu-budget", type=float, default=20.0, help="seconds of oracle work for the CPU baseline")
+    ap.add_argument("--dmrg-bonds", type=int, default=6,
+                    help="timed two-site updates at chi=2048 in the bulk of the 6x24 cylinder (0: skip)")
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
                     help="only run a plain device copy of MIB MiB (FP32) for HBM-counter calibration")
     return ap.parse_args()
@@ -122,6 +124,59 @@ def time_plain(ts, steps, warmup):
     return (time.perf_counter() - t0) * 1e3 / steps
 
 
+DMRG_CFG = dict(W=6, Lx=24, chi=2048, nsec=60, dmax=200, seed=1911, maxdim=2048, tol=1e-10, lanczostol=1e-6,
+                krylovdim=8, krylovmaxiter=2, prec="f32", refine_tol=1e-9)
+
+
+def dmrg_window(nbonds, dev):
+    """Two-site DMRG updates at chi=2048 on the 6x24 Heisenberg cylinder (snake order, D=20):
+    a random U(1) MPS (N = L/2) canonical at the middle site, initial environments, one
+    warm-up update, then ``nbonds`` timed left-to-right updates (eigsolve with FP32
+    matvecs + FP64 refinement, batched FP64 SVD truncation, energy, env update)."""
+    import torch
+    from mps_amd import mpstools as mt
+    from mps_amd import tensornetalgs as ta
+    from mps_amd.symtensors import contract
+    c = DMRG_CFG
+    L, N = c["W"] * c["Lx"], c["W"] * c["Lx"] // 2
+    t0 = time.perf_counter()
+    mpo = mt.heisenberg_cylinder_mpo(c["W"], c["Lx"], device=dev)
+    mps = mt.random_mps(L, N, c["chi"], seed=c["seed"], nsec=c["nsec"], dmax=c["dmax"], device=dev)
+    print(f"dmrg: random chi={c['chi']} MPS ready ({time.perf_counter() - t0:.1f} s)", file=sys.stderr, flush=True)
+    mt.center_at(mps, L // 2)
+    env = ta.initialenv(mps, mpo, at=L // 2)
+    torch.cuda.synchronize()
+    setup_s = time.perf_counter() - t0
+    print(f"dmrg: center {L // 2}, environments ready ({setup_s:.1f} s)", file=sys.stderr, flush=True)
+    kw = dict(maxdim=c["maxdim"], tol=c["tol"], lanczostol=c["lanczostol"], krylovdim=c["krylovdim"],
+              krylovmaxiter=c["krylovmaxiter"], prec=c["prec"], refine_tol=c["refine_tol"])
+    ta.dmrg2site_window(mps, mpo, env, 1, **kw)
+    timing = []
+    torch.cuda.synchronize()
+    t1 = time.perf_counter()
+    energies = ta.dmrg2site_window(mps, mpo, env, nbonds, timing=timing, **kw)
+    torch.cuda.synchronize()
+    print(f"dmrg: {nbonds} timed bond updates done", file=sys.stderr, flush=True)
+    wall = time.perf_counter() - t1
+    ms_bond = 1e3 * wall / nbonds
+    stages = {k: round(sum(t[k] for t in timing) / nbonds, 3)
+              for k in ("ms_theta", "ms_eigsolve", "ms_svdtrunc", "ms_energy", "ms_envupdate")}
+    cen = mps.center
+    AA = contract(mps.As[cen - 1], (1, 2, -1), mps.As[cen], (-1, 3, 4))
+    state = (AA, env[cen - 1], env[cen + 2], mpo.Ws[cen - 1], mpo.Ws[cen])
+    out = {"model": "Heisenberg 6x24 cylinder (snake), L=144, N=72, MPO D=20", "chi": c["chi"],
+           "bonds_timed": nbonds, "first_bond": L // 2 + 1, "ms_per_bond_update": round(ms_bond, 3),
+           "stage_ms_per_bond": stages,
+           "matvecs_per_bond": round(sum(t["matvecs"] for t in timing) / nbonds, 2),
+           "chi_bulk": [timing[0]["chi_left"], timing[0]["chi_right"]],
+           "sweep_s_extrapolated": round(2 * (L - 1) * ms_bond / 1e3, 2),
+           "sweep_note": "2(L-1) bulk two-site updates x measured ms per bulk update (edge bonds are cheaper)",
+           "setup_s": round(setup_s, 2), "energies": [round(e, 10) for e in energies],
+           "settings": {k: c[k] for k in ("maxdim", "tol", "lanczostol", "krylovdim", "krylovmaxiter", "prec",
+                                           "refine_tol")}}
+    return out, state
+
+
 def cpu_baseline(name, budget):
     """Oracle (numpy restatement of the reference algorithm) on the host cores."""
     import numpy as np
@@ -161,6 +216,30 @@ def cpu_baseline(name, budget):
     flat_out = np.concatenate([b.reshape(-1, order="F") for b in out.blocks.values()])
     flat_v = np.concatenate([b.reshape(-1, order="F") for b in ts[0].blocks.values()])
     return calls / dt, calls, dt, threads, flat_out, flat_v
+
+
+def dmrg_energy_check(state):
+    """<AA|H_eff AA>/<AA|AA> of the DMRG state after the timed window: device FP64
+    fused path vs the oracle (numpy FP64 restatement) on the host."""
+    import numpy as np
+    import torch
+    import mps_amd as M
+    from oracle import symtensors as ost
+    from oracle import tensornetalgs as ota
+    AA = state[0]
+    Hd = M._applymps2site(*state, prec="f64")
+    nn = M.dot(AA, AA)
+    e_dev = M.dot(AA, Hd) / nn
+
+    def host(t):
+        sp = tuple(ost.Space(list(V.items()), V.isdual) for V in t.space)
+        return ost.SymTensor(t.charge, sp, {k: np.asarray(b) for k, b in t.blocks().items()})
+    ho = [host(t) for t in state]
+    t0 = time.perf_counter()
+    Ho = ota.applymps2site(*ho)
+    e_or = ost.dot(ho[0], Ho) / ost.dot(ho[0], ho[0])
+    return {"energy_device_f64": e_dev, "energy_cpu_oracle": e_or, "abs_diff": abs(e_dev - e_or),
+            "rel_diff": abs(e_dev - e_or) / abs(e_or), "oracle_s": round(time.perf_counter() - t0, 2)}
 
 
 def main():
@@ -242,6 +321,9 @@ def main():
         extra["plain_f64_matvecs_per_s"] = round(1e3 / ms_plain, 3)
         extra["speedup_vs_plain_f64"] = round(ms_plain / ms, 3)
         result["other_paths"] = extra
+    dmrg_state = None
+    if args.dmrg_bonds > 0:
+        result["dmrg"], dmrg_state = dmrg_window(args.dmrg_bonds, dev)
     if not args.no_cpu_baseline:
         rate, calls, secs, threads, oflat, vflat = cpu_baseline(args.workload, args.cpu_budget)
         e_or = float(np.dot(vflat, oflat) / np.dot(vflat, vflat))
@@ -253,6 +335,8 @@ def main():
                                             f"host nproc={os.cpu_count()})"}
         result["energy_rel_err_vs_cpu_oracle"] = abs(e_prec - e_or) / abs(e_or)
         result["f64_energy_rel_err_vs_cpu_oracle"] = abs(e64 - e_or) / abs(e_or)
+        if dmrg_state is not None:
+            result["dmrg"]["energy_vs_cpu_oracle"] = dmrg_energy_check(dmrg_state)
     print(json.dumps(result), flush=True)
 
 

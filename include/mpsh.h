@@ -143,6 +143,27 @@ mpsh_status mpsh_heff_apply_prepared_stages(mpsh_heff plan, mpsh_dtype dt, int32
 void mpsh_heff_destroy(mpsh_heff plan);
 
 /* ------------------------------------------------------------------------ */
+/* Batched per-block FP64 SVD: the per-fused-sector SVD of svdtrunc / svd     */
+/* (src/mpsutils.jl:116-179; LAPACK gesdd, retried with gesvd on failure,     */
+/* src/symtools.jl:14-19).  Block b is the m[b] x n[b] column-major matrix at */
+/* A + a_off[b].  Outputs are packed block after block, k = min(m, n):        */
+/* S (k values, descending), U (m x k), Vt (k x n), offsets from              */
+/* mpsh_svd_sizes (totals = {S, U, Vt} element counts).  Device buffers;     */
+/* workspace of mpsh_svd_workspace() bytes, 16-byte aligned.  Synchronizes   */
+/* the stream once (convergence flags).                                       */
+/* ------------------------------------------------------------------------ */
+typedef struct mpsh_svd_s* mpsh_svd;
+
+mpsh_status mpsh_svd_plan(int64_t nblocks, const int64_t* m, const int64_t* n, const int64_t* a_off,
+                          mpsh_svd* plan);
+mpsh_status mpsh_svd_sizes(mpsh_svd plan, int64_t* k, int64_t* s_off, int64_t* u_off, int64_t* v_off,
+                           int64_t* totals);
+size_t mpsh_svd_workspace(mpsh_svd plan);
+mpsh_status mpsh_svd_execute(mpsh_svd plan, const double* A, double* S, double* U, double* Vt, void* workspace,
+                             void* stream);
+void mpsh_svd_destroy(mpsh_svd plan);
+
+/* ------------------------------------------------------------------------ */
 /* Block-wise vector operations on packed buffers (KrylovKit's interface on   */
 /* SymTensors, src/symtensor.jl:266-323).  n = number of elements.            */
 /* ------------------------------------------------------------------------ */

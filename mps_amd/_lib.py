@@ -78,6 +78,11 @@ _SIGS = {
     "mpsh_heff_apply_prepared": (C.c_int, [_p, C.c_int, _p, _p, _p, _p, _p]),
     "mpsh_heff_apply_prepared_stages": (C.c_int, [_p, C.c_int, _i32, _p, _p, _p, _p, _p]),
     "mpsh_heff_destroy": (None, [_p]),
+    "mpsh_svd_plan": (C.c_int, [_i64, _pi64, _pi64, _pi64, C.POINTER(_p)]),
+    "mpsh_svd_sizes": (C.c_int, [_p, _pi64, _pi64, _pi64, _pi64, _pi64]),
+    "mpsh_svd_workspace": (C.c_size_t, [_p]),
+    "mpsh_svd_execute": (C.c_int, [_p, _p, _p, _p, _p, _p, _p]),
+    "mpsh_svd_destroy": (None, [_p]),
     "mpsh_dot": (C.c_int, [C.c_int, _i64, _p, _p, _p, _p, _p]),
     "mpsh_axpby": (C.c_int, [C.c_int, _i64, _dbl, _p, _dbl, _p, _p]),
     "mpsh_scal": (C.c_int, [C.c_int, _i64, _dbl, _p, _p]),

@@ -33,12 +33,13 @@ def build(verbose: bool = True) -> str:
     o = os.path.join(BUILD, "planner.o")
     _run(["g++", "-std=c++17", "-O2", "-fPIC", "-Wall", "-c", os.path.join(CSRC, "planner.cpp"), "-o", o])
     objs.append(o)
-    for src in ("kernels.hip", "api.hip"):
+    for src in ("kernels.hip", "api.hip", "svd.hip"):
         o = os.path.join(BUILD, src.replace(".hip", ".o"))
         _run([HIPCC, *HIP_FLAGS, "-c", os.path.join(CSRC, src), "-o", o])
         objs.append(o)
     tmp = LIB + ".tmp"
-    _run([HIPCC, f"--offload-arch={ARCH}", "-shared", "-fPIC", *objs, "-o", tmp])
+    _run([HIPCC, f"--offload-arch={ARCH}", "-shared", "-fPIC", *objs, "-o", tmp, "-L/opt/rocm/lib",
+          "-lrocsolver", "-lrocblas", "-Wl,-rpath,/opt/rocm/lib"])
     os.replace(tmp, LIB)
     return LIB
 

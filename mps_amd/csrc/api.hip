@@ -20,6 +20,10 @@ using namespace mpsh;
 
 static thread_local std::string g_err;
 
+namespace mpsh {
+void set_last_error(const std::string& m) { g_err = m; }
+}  // namespace mpsh
+
 static mpsh_status fail(int code, const std::string& m) {
   g_err = m;
   return (mpsh_status)code;

@@ -294,9 +294,15 @@ __global__ __launch_bounds__(256) void gemm_grouped(
       const GemmSeg sg = segs[p.seg0 + seg];
       K = sg.K;
 #pragma unroll
-      for (int i = 0; i < NA; ++i) ab[i] = atab[sg.atab0 + (av[i] ? tile.m0 + row + i * RP : 0)];
+      for (int i = 0; i < NA; ++i) {
+        const int64_t r = av[i] ? tile.m0 + row + i * RP : 0;
+        ab[i] = sg.atab0 >= 0 ? atab[sg.atab0 + r] : sg.a_off + r * sg.lda;
+      }
 #pragma unroll
-      for (int i = 0; i < NB; ++i) bb[i] = btab[sg.btab0 + (bv[i] ? tile.n0 + row + i * RP : 0)];
+      for (int i = 0; i < NB; ++i) {
+        const int64_t r = bv[i] ? tile.n0 + row + i * RP : 0;
+        bb[i] = sg.btab0 >= 0 ? btab[sg.btab0 + r] : sg.b_off + r * sg.ldb;
+      }
     };
     if (nch > 0) bases();
     V ra[DEPTH][NA], rb[DEPTH][NB];
@@ -368,8 +374,13 @@ __global__ __launch_bounds__(256) void gemm_grouped(
       if (m >= p.M) continue;
       int64_t cb_ = m, cl_ = p.M;
       if (!split) {
-        cb_ = crow[p.crow0 + m];
-        cl_ = cld[p.crow0 + m];
+        if (p.crow0 >= 0) {
+          cb_ = crow[p.crow0 + m];
+          cl_ = cld[p.crow0 + m];
+        } else {
+          cb_ = p.c_off + m;
+          cl_ = p.ldc;
+        }
       }
       T* Cb = split ? Pp : C;
 #pragma unroll
@@ -409,7 +420,10 @@ __global__ __launch_bounds__(256) void splitk_reduce(const RedDesc* __restrict__
     const int64_t n = e / p.M, m = e - n * p.M;
     T acc = 0;
     for (int s = 0; s < p.nsplit; ++s) acc += P[p.part_off + s * MN + e];
-    C[crow[p.crow0 + m] + n * cld[p.crow0 + m]] = acc;
+    if (p.crow0 >= 0)
+      C[crow[p.crow0 + m] + n * cld[p.crow0 + m]] = acc;
+    else
+      C[p.c_off + m + n * p.ldc] = acc;
   }
 }
 

@@ -3,6 +3,8 @@
 
 #include <hip/hip_runtime.h>
 
+#include <string>
+
 #include "planner.hpp"
 
 namespace mpsh {
@@ -13,6 +15,9 @@ struct DevCopy {
   int ndesc = 0;
   int64_t ntiles = 0;
 };
+
+// thread-local message returned by mpsh_last_error (defined in api.hip)
+void set_last_error(const std::string& m);
 
 struct XcdQueues {
   int32_t off[NXCD + 1];

@@ -232,7 +232,7 @@ void GemmPlan::add_seg_tab(int64_t K, const std::vector<int64_t>& arows, const s
   GemmProb& p = probs.back();
   if ((int64_t)arows.size() != p.M || (int64_t)brows.size() != p.N)
     throw Error(ERR_INVALID, "gemm segment row tables do not match the problem");
-  segs.push_back(GemmSeg{K, (int64_t)atab.size(), (int64_t)btab.size()});
+  segs.push_back(GemmSeg{K, (int64_t)atab.size(), (int64_t)btab.size(), 0, 0, 0, 0});
   atab.insert(atab.end(), arows.begin(), arows.end());
   btab.insert(btab.end(), brows.begin(), brows.end());
   p.nseg++;
@@ -240,11 +240,10 @@ void GemmPlan::add_seg_tab(int64_t K, const std::vector<int64_t>& arows, const s
 }
 
 void GemmPlan::add_seg(int64_t K, int64_t a_off, int64_t lda, int64_t b_off, int64_t ldb) {
-  const GemmProb& p = probs.back();
-  std::vector<int64_t> a(p.M), b(p.N);
-  for (int64_t m = 0; m < p.M; ++m) a[m] = a_off + m * lda;
-  for (int64_t n = 0; n < p.N; ++n) b[n] = b_off + n * ldb;
-  add_seg_tab(K, a, b);
+  GemmProb& p = probs.back();
+  segs.push_back(GemmSeg{K, -1, -1, a_off, lda, b_off, ldb});
+  p.nseg++;
+  flops += 2 * (int64_t)p.M * p.N * K;
 }
 
 static int64_t prob_chunks(const GemmPlan& g, const GemmProb& p, int BK) {
@@ -621,6 +620,48 @@ ContractPlan contract_plan(const Struct& A, const std::vector<int>& idxA, const 
   return P;
 }
 
+ContractCost contract_cost(const Struct& A, const std::vector<int>& idxA, const Struct& B,
+                           const std::vector<int>& idxB) {
+  if ((int)idxA.size() != A.rank() || (int)idxB.size() != B.rank())
+    throw Error(ERR_INVALID, "index set length differs from tensor rank");
+  IndexPerm pa = contract_index_perm(idxA), pb = contract_index_perm(idxB);
+  if (pa.cons.size() != pb.cons.size()) throw Error(ERR_INVALID, "contracted label counts differ");
+  if (pa.rems.empty() || pb.rems.empty())
+    throw Error(ERR_UNSUPPORTED, "rank-1 output branches of contract (src/contract.jl:24-42)");
+  std::vector<Leg> ar, ac, bc, br;
+  for (int i : pa.rems) ar.push_back(A.legs[i - 1]);
+  for (int i : pa.cons) ac.push_back(A.legs[i - 1]);
+  for (int i : pb.cons) bc.push_back(B.legs[i - 1]);
+  for (int i : pb.rems) br.push_back(B.legs[i - 1]);
+  // the fused legs of SymMatrix(A, rems, cons) and SymMatrix(B, cons, rems)
+  const Leg a0 = fuse(false, ar), a1 = fuse(true, ac), b0 = fuse(false, bc), b1 = fuse(true, br);
+  if (!isdual_pair(a1, b0)) throw Error(ERR_SPACE_MISMATCH, "not contractible!");
+  const int64_t charge = A.charge + B.charge;
+  ContractCost R;
+  const Struct FC = make_struct(charge, {a0, b1});
+  for (int64_t f = 0; f < FC.nblocks; ++f) {
+    const int64_t c1 = FC.sector(f)[0], c2 = FC.sector(f)[1];
+    const int64_t _c1 = a0.isdual ? -c1 : c1;
+    const int64_t _c2 = b1.isdual ? -c2 : c2;
+    const int64_t ka = a1.isdual ? -A.charge + _c1 : A.charge - _c1;
+    const int64_t kb = a1.isdual ? B.charge - _c2 : -B.charge + _c2;
+    const int64_t K = a1.dim_of(ka);
+    if (K > 0 && b0.dim_of(kb) > 0) {
+      if (b0.dim_of(kb) != K) throw Error(ERR_SECTOR_MISMATCH, "fused contraction dims differ");
+      R.flops += 2 * FC.size(f)[0] * FC.size(f)[1] * K;
+    }
+  }
+  std::vector<Leg> pre = ar;
+  pre.insert(pre.end(), br.begin(), br.end());
+  std::vector<int> tof(pa.tofinals);
+  tof.insert(tof.end(), pb.tofinals.begin(), pb.tofinals.end());
+  std::vector<int> Pm = invperm(tof);
+  std::vector<Leg> fin;
+  for (size_t k = 0; k < pre.size(); ++k) fin.push_back(pre[Pm[k] - 1]);
+  R.C = make_struct(charge, fin);
+  return R;
+}
+
 // ---------------------------------------------------------------------------
 // Re-legging (src/symreleg.jl)
 // ---------------------------------------------------------------------------
@@ -835,14 +876,15 @@ HeffPlan heff_plan(const Struct& v, const Struct& envL, const Struct& envR, cons
 
   // reference-order cost from the reference's own four contracts
   {
-    ContractPlan P1 = contract_plan(envL, {-1, 2, 1}, v, {-1, 3, 4, 5});
-    ContractPlan P2 = contract_plan(P1.C, {1, -1, -2, 4, 5}, WL, {-1, 2, 3, -2});
-    ContractPlan P3 = contract_plan(P2.C, {1, 2, -1, -2, 5}, WR, {-1, 3, 4, -2});
-    ContractPlan P4 = contract_plan(P3.C, {1, 2, 3, -1, -2}, envR, {-2, -1, 4});
-    const ContractPlan* Ps[4] = {&P1, &P2, &P3, &P4};
+    ContractCost P1 = contract_cost(envL, {-1, 2, 1}, v, {-1, 3, 4, 5});
+    ContractCost P2 = contract_cost(P1.C, {1, -1, -2, 4, 5}, WL, {-1, 2, 3, -2});
+    ContractCost P3 = contract_cost(P2.C, {1, 2, -1, -2, 5}, WR, {-1, 3, 4, -2});
+    ContractCost P4 = contract_cost(P3.C, {1, 2, 3, -1, -2}, envR, {-2, -1, 4});
+    const ContractCost* Ps[4] = {&P1, &P2, &P3, &P4};
+    const Struct* ins[4][2] = {{&envL, &v}, {&P1.C, &WL}, {&P2.C, &WR}, {&P3.C, &envR}};
     for (int s = 0; s < 4; ++s) {
-      H.step_flops[s] = Ps[s]->gemm.flops;
-      H.step_elems[s] = Ps[s]->A.numel() + Ps[s]->B.numel() + Ps[s]->C.numel();
+      H.step_flops[s] = Ps[s]->flops;
+      H.step_elems[s] = ins[s][0]->numel() + ins[s][1]->numel() + Ps[s]->C.numel();
       H.ref_flops += H.step_flops[s];
       H.ref_elems += H.step_elems[s];
     }
@@ -902,10 +944,7 @@ HeffPlan heff_plan(const Struct& v, const Struct& envL, const Struct& envR, cons
       const int64_t Mld = (Mf + 31) / 32 * 32;
       H.g1.add_prob(Mf, Nf);
       H.g1.add_crows_affine(off, Mld, Mf);
-      std::vector<int64_t> arows(Mf), brows(Nf);
-      for (int64_t m = 0; m < Mf; ++m) arows[m] = vbase + m * Kp;
-      for (int64_t n = 0; n < Nf; ++n) brows[n] = ebase + n * Kp;
-      H.g1.add_seg_tab(dk, arows, brows);
+      H.g1.add_seg(dk, vbase, Kp, ebase, Kp);
       for (size_t i = 0; i < X.size(); ++i)
         for (size_t j = 0; j < E.size(); ++j) {
           const int64_t* xs = v.sector(X[i]);
@@ -973,10 +1012,7 @@ HeffPlan heff_plan(const Struct& v, const Struct& envL, const Struct& envR, cons
           const int64_t dims[2] = {Kf, N}, sst[2] = {1, Kf}, dst[2] = {1, Kfp};
           H.pack_envR.add(2, dims, sst, dst, envR.offsets[f], rbase);
         }
-        std::vector<int64_t> arows(Mf), brows(N);
-        for (int64_t m = 0; m < Mf; ++m) arows[m] = off + Kfp * m;
-        for (int64_t n = 0; n < N; ++n) brows[n] = rbase + Kfp * n;
-        H.g4.add_seg_tab(Kf, arows, brows);
+        H.g4.add_seg(Kf, off, Kfp, rbase, Kfp);
         for (size_t i = 0; i < O.size(); ++i) {
           const int64_t* os = H.out.sector(O[i]);
           const int64_t* z = H.out.size(O[i]);

@@ -123,6 +123,8 @@ struct CopyPlan {
 // span several stored blocks without copies:
 //   A_s(k, m) = A[atab[s.atab0 + m] + k],   B_s(k, n) = B[btab[s.btab0 + n] + k],
 //   C(m, n)   = C[crow[p.crow0 + m] + n * cld[p.crow0 + m]].
+// Affine rows need no table: atab0 < 0 means A row m at a_off + m * lda (the
+// same for B), crow0 < 0 means C(m, n) = C[c_off + m + n * ldc].
 // nseg == 0 means "write zeros" (reference src/contract.jl:292-294).  With
 // nsplit > 1 the K chunks are split over work items that write dense
 // partials at part_off (+ split * M * N, column-major), summed by a reduction.
@@ -131,9 +133,11 @@ struct GemmProb {
   int64_t crow0;
   int32_t nsplit, pad;
   int64_t part_off;
+  int64_t c_off, ldc;
 };
 struct GemmSeg {
   int64_t K, atab0, btab0;
+  int64_t a_off, lda, b_off, ldb;
 };
 constexpr int NXCD = 8;  // MI355X: 8 accelerator complex dies, one L2 each
 
@@ -154,15 +158,19 @@ struct GemmPlan {
   int64_t part_elems = 0;
   // open a problem; its C rows follow with add_crow()
   void add_prob(int64_t M, int64_t N) {
-    probs.push_back(GemmProb{(int32_t)M, (int32_t)N, (int32_t)segs.size(), 0, (int64_t)crow.size(), 1, 0, 0});
+    probs.push_back(GemmProb{(int32_t)M, (int32_t)N, (int32_t)segs.size(), 0, (int64_t)crow.size(), 1, 0, 0, 0, 0});
   }
   void add_crow(int64_t base, int64_t ld) {
     crow.push_back(base);
     cld.push_back(ld);
   }
-  // affine C rows: base + m, ld
+  // affine C rows of the open problem: C(m, n) = C[base + m + n * ld]
   void add_crows_affine(int64_t base, int64_t ld, int64_t M) {
-    for (int64_t m = 0; m < M; ++m) add_crow(base + m, ld);
+    GemmProb& p = probs.back();
+    if (M != p.M || (int64_t)crow.size() != p.crow0) throw Error(ERR_INVALID, "affine C rows: bad problem");
+    p.crow0 = -1;
+    p.c_off = base;
+    p.ldc = ld;
   }
   // a K segment with explicit row tables (M entries for A, N for B)
   void add_seg_tab(int64_t K, const std::vector<int64_t>& arows, const std::vector<int64_t>& brows);
@@ -213,6 +221,16 @@ struct ContractPlan {
   int64_t fa_off = 0, fb_off = 0, fc_off = 0;
 };
 ContractPlan contract_plan(const Struct& A, const std::vector<int>& idxA, const Struct& B,
+                           const std::vector<int>& idxB);
+
+// Cost of contract(A, idxA, B, idxB) without building its copy/GEMM tables:
+// the reference's fused-charge GEMM FLOPs (sum 2*M*N*K over matched fused
+// charges) and the output structure C.
+struct ContractCost {
+  int64_t flops = 0;
+  Struct C;
+};
+ContractCost contract_cost(const Struct& A, const std::vector<int>& idxA, const Struct& B,
                            const std::vector<int>& idxB);
 
 // Re-legging plans: pure data movement (src/symreleg.jl:34-214).

@@ -9,8 +9,9 @@ amiragha/mps for the parts the two-site sweep needs:
   chains it reproduces ``xxz_u1mpo`` (symmpo.jl:45-97) block for block;
   ``heisenberg_cylinder_mpo`` builds the width-W cylinder in snake order;
 * ``svd_`` / ``truncatedsizes`` / ``svdtrunc`` (symtools.jl:1-35,
-  mpsutils.jl:89-179) with FP64 per-sector SVDs on the device (rocSOLVER
-  through torch.linalg.svd) and the reference's greedy global truncation;
+  mpsutils.jl:89-179) with all FP64 per-sector SVDs of a matrix in one
+  batched device call (``mpsh_svd_*``: rocSOLVER gesdd over padded shape
+  buckets) and the reference's greedy global truncation;
 * ``random_mps``, ``center_at``, ``normalize``, ``measure``.
 """
 from __future__ import annotations
@@ -21,7 +22,10 @@ from typing import Dict, List, Sequence, Tuple
 import numpy as np
 import torch
 
-from .symtensors import (SymTensor, U1Space, contract, default_device, dual, fuselegs, get_struct, splitleg)
+from . import _lib as L
+from ._lib import check, lib
+from .symtensors import (SymTensor, U1Space, _ptr, _stream, contract, default_device, dual, fuselegs, get_struct,
+                         splitleg)
 from .workloads import binom_capped_profile, gaussian
 
 
@@ -198,50 +202,92 @@ def heisenberg_cylinder_mpo(W: int, Lx: int, J: float = 1.0, ordering: str = "sn
 # ---------------------------------------------------------------------------
 # SVD step   (symtools.jl:1-35, mpsutils.jl:89-179)
 # ---------------------------------------------------------------------------
-def _blocks_view(M: SymTensor):
-    """[(sector, column-major matrix view)] of a rank-2 SymTensor's device data."""
-    sec, siz, off = M.struct.tables()
+class _SvdPlan:
+    """Cached batched-SVD plan of one rank-2 structure (``mpsh_svd_*``)."""
+
+    def __init__(self, st):
+        sec, siz, off = st.tables()
+        nb = st.nblocks
+        self.keys = [(int(sec[b][0]), int(sec[b][1])) for b in range(nb)]
+        self.m = [int(siz[b][0]) for b in range(nb)]
+        self.n = [int(siz[b][1]) for b in range(nb)]
+        self.handle = L._p()
+        check(lib().mpsh_svd_plan(nb, L.i64arr(self.m), L.i64arr(self.n), L.i64arr([int(x) for x in off[:nb]]),
+                                  self.handle))
+        k, so, uo, vo, tot = (L._i64 * max(1, nb))(), (L._i64 * max(1, nb))(), (L._i64 * max(1, nb))(), \
+            (L._i64 * max(1, nb))(), (L._i64 * 3)()
+        check(lib().mpsh_svd_sizes(self.handle, k, so, uo, vo, tot))
+        self.k = [int(k[b]) for b in range(nb)]
+        self.s_off, self.u_off, self.v_off = [list(map(int, x[:nb])) for x in (so, uo, vo)]
+        self.totals = [int(t) for t in tot]
+        self.ws_bytes = int(lib().mpsh_svd_workspace(self.handle))
+
+    def __del__(self):
+        try:
+            lib().mpsh_svd_destroy(self.handle)
+        except Exception:
+            pass
+
+
+_SVD_PLANS: Dict[object, _SvdPlan] = {}
+
+
+def _svd_blocks(M: SymTensor):
+    """Every per-sector FP64 SVD of a rank-2 SymTensor in one batched device call
+    (symtools.jl:1-35): [(sector, U (m x k), S (k), Vh (k x n))], views of packed buffers."""
+    st = M.struct
+    P = _SVD_PLANS.get(st)
+    if P is None:
+        P = _SVD_PLANS[st] = _SvdPlan(st)
+    x = M.data if M.dtype == torch.float64 else M.data.double()
+    dev = x.device
+    S = torch.empty(max(1, P.totals[0]), dtype=torch.float64, device=dev)
+    U = torch.empty(max(1, P.totals[1]), dtype=torch.float64, device=dev)
+    V = torch.empty(max(1, P.totals[2]), dtype=torch.float64, device=dev)
+    ws = torch.empty(max(16, P.ws_bytes), dtype=torch.uint8, device=dev)
+    check(lib().mpsh_svd_execute(P.handle, _ptr(x), _ptr(S), _ptr(U), _ptr(V), _ptr(ws), _stream()))
     out = []
-    for b in range(M.struct.nblocks):
-        m, n = int(siz[b][0]), int(siz[b][1])
-        out.append(((int(sec[b][0]), int(sec[b][1])), M.data[off[b]:off[b] + m * n].view(n, m).t()))
+    for b, key in enumerate(P.keys):
+        m, n, k = P.m[b], P.n[b], P.k[b]
+        out.append((key, U[P.u_off[b]:P.u_off[b] + m * k].view(k, m).t(), S[P.s_off[b]:P.s_off[b] + k],
+                    V[P.v_off[b]:P.v_off[b] + k * n].view(n, k).t()))
     return out
 
 
-def _block_svd(blk: torch.Tensor):
-    """FP64 thin SVD; on failure retries on the host with LAPACK gesvd (symtools.jl:14-19)."""
-    try:
-        U, S, Vh = torch.linalg.svd(blk, full_matrices=False)
-        if not torch.isfinite(S).all():
-            raise RuntimeError("non-finite singular values")
-        return U, S, Vh
-    except RuntimeError:
-        import scipy.linalg
-        u, s, vh = scipy.linalg.svd(blk.double().cpu().numpy(), full_matrices=False, lapack_driver="gesvd")
-        d = blk.device
-        return (torch.from_numpy(u).to(d, blk.dtype), torch.from_numpy(s).to(d, blk.dtype),
-                torch.from_numpy(vh).to(d, blk.dtype))
-
-
 def truncatedsizes(S: List[np.ndarray], maxdim: int = 200, tol: float = 1e-14) -> List[int]:
-    """``truncatedsizes`` (mpsutils.jl:89-112): greedy largest-first, absolute ``tol``."""
-    pointers = [1] * len(S)
-    c = [float(s[0]) for s in S]
+    """``truncatedsizes`` (mpsutils.jl:89-112): greedy largest-first, absolute ``tol``.
+
+    The greedy loop takes, one at a time, the largest remaining head of the
+    per-sector descending lists (first sector on ties) until ``maxdim`` values
+    are taken or the head drops below ``tol``.  That is the prefix of all
+    values sorted by (value descending, sector, position), which is how it is
+    computed here for ``tol > 0``; ``tol <= 0`` keeps the literal loop (an
+    exhausted sector then keeps being chosen at value 0, as in the reference).
+    """
+    if not S:
+        return []
     total = sum(len(s) for s in S)
-    for n in range(1, min(maxdim, total) + 1):
-        index = int(np.argmax(c))
-        m = c[index]
-        if m < tol:
-            if n == 1:
-                raise ValueError(f"The largest singular value is smaller than tol! {m} < {tol}")
-            break
-        if pointers[index] >= len(S[index]):
+    if tol <= 0:
+        pointers = [1] * len(S)
+        c = [float(s[0]) for s in S]
+        for n in range(1, min(maxdim, total) + 1):
+            index = int(np.argmax(c))
+            if c[index] < tol:
+                if n == 1:
+                    raise ValueError(f"The largest singular value is smaller than tol! {c[index]} < {tol}")
+                break
             pointers[index] += 1
-            c[index] = 0.0
-        else:
-            pointers[index] += 1
-            c[index] = float(S[index][pointers[index] - 1])
-    return [p - 1 for p in pointers]
+            c[index] = float(S[index][pointers[index] - 1]) if pointers[index] - 1 < len(S[index]) else 0.0
+        return [p - 1 for p in pointers]
+    vals = np.concatenate([np.asarray(s, dtype=np.float64) for s in S])
+    blk = np.concatenate([np.full(len(s), i) for i, s in enumerate(S)])
+    pos = np.concatenate([np.arange(len(s)) for s in S])
+    order = np.lexsort((pos, blk, -vals))[:min(maxdim, total)]
+    below = np.nonzero(vals[order] < tol)[0]
+    if len(below) and below[0] == 0:
+        raise ValueError(f"The largest singular value is smaller than tol! {vals[order[0]]} < {tol}")
+    take = order[:below[0]] if len(below) else order
+    return [int(x) for x in np.bincount(blk[take], minlength=len(S))]
 
 
 def _assemble(charge, space, parts: Dict[Tuple[int, int], torch.Tensor], dtype, device) -> SymTensor:
@@ -311,18 +357,19 @@ def svdtrunc(M: SymTensor, maxdim: int = None, tol: float = 1e-12) -> SVDResult:
     """``svdtrunc(A; maxdim, tol)`` (mpsutils.jl:116-179) on a rank-2 SymTensor."""
     if maxdim is None:
         maxdim = max(V.dim() for V in M.space)
-    items = _blocks_view(M)
-    facts = [_block_svd(blk) for _, blk in items]
-    svals = [f[1].double().cpu().numpy() for f in facts]
-    ns = truncatedsizes(svals, maxdim=maxdim, tol=tol)
+    facts = _svd_blocks(M)
+    svals = np.split(torch.cat([f[2] for f in facts]).cpu().numpy(), np.cumsum([len(f[2]) for f in facts])[:-1]) \
+        if facts else []
+    ns = truncatedsizes(svals, maxdim=maxdim, tol=tol) if facts else []
     keys, Us, Ss, Vs, midl, midr = [], [], [], [], [], []
-    for (key, _), (U, S, Vh), k in zip(items, facts, ns):
+    dt = M.dtype
+    for (key, U, S, Vh), k in zip(facts, ns):
         if k <= 0:
             continue
         keys.append(key)
-        Us.append(U[:, :k])
-        Ss.append(S[:k])
-        Vs.append(Vh[:k, :])
+        Us.append(U[:, :k].to(dt))
+        Ss.append(S[:k].to(dt))
+        Vs.append(Vh[:k, :].to(dt))
         midl.append((key[0], k))
         midr.append((key[1], k))
     Vl = U1Space(*midl, isdual=M.space[0].isdual)
@@ -332,14 +379,13 @@ def svdtrunc(M: SymTensor, maxdim: int = None, tol: float = 1e-12) -> SVDResult:
 
 def svd_(M: SymTensor) -> SVDResult:
     """``_svd_(A)`` (symtools.jl:1-35): untruncated per-sector SVD."""
-    items = _blocks_view(M)
     keys, Us, Ss, Vs, midl, midr = [], [], [], [], [], []
-    for key, blk in items:
-        U, S, Vh = _block_svd(blk)
+    dt = M.dtype
+    for key, U, S, Vh in _svd_blocks(M):
         keys.append(key)
-        Us.append(U)
-        Ss.append(S)
-        Vs.append(Vh)
+        Us.append(U.to(dt))
+        Ss.append(S.to(dt))
+        Vs.append(Vh.to(dt))
         midl.append((key[0], S.numel()))
         midr.append((key[1], S.numel()))
     Vl = U1Space(*midl, isdual=M.space[0].isdual)
@@ -372,8 +418,16 @@ def random_mps(L: int, N: int, chi: int, seed: int = 1911, nsec: int = None, dma
         if st.numel == 0:
             raise ValueError(f"empty site tensor at {l}: widen the sector window")
         As.append(SymTensor.from_flat(0, space, gaussian(seed, 1000 + l, st.numel), dtype, device))
+    # right-canonicalise (center_at(mps, 1) of symmps.jl:185-200), rescaling the
+    # moving center to unit norm after every step: the product of L random
+    # Gaussian tensors otherwise overflows FP64 at large chi.  Only the overall
+    # scale changes, and normalize() fixes it below.
+    from .symtensors import norm, rmul_
     mps = MPState(As, 0)
-    center_at(mps, 1)
+    for p in range(L, 1, -1):
+        _pushleft(mps, p)
+        rmul_(mps.As[p - 2], 1.0 / norm(mps.As[p - 2]))
+    mps.center = 1
     normalize(mps)
     return mps
 

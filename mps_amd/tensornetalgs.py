@@ -291,6 +291,59 @@ def initialenv(mps, mpo, at: int = 1):
     return env
 
 
+def dmrg2site_window(mps, mpo, env, nbonds: int, maxdim: int = 200, tol: float = 1e-9, lanczostol: float = 1e-7,
+                     krylovdim: int = 5, krylovmaxiter: int = 8, prec: str = "f64", refine_tol: float = None,
+                     timing: list = None):
+    """``nbonds`` consecutive left-to-right two-site updates starting at the MPS
+    center c, exactly the L->R loop body of ``dmrg2sitesweep!`` (dmrg.jl:70-106):
+    eigsolve on bonds (c, c+1) ... (c+nbonds-1, c+nbonds), truncate, update the
+    left environments; the center ends at c + nbonds.  ``env`` must hold the
+    left environments up to c-1 and the right ones from c+2 (``initialenv(at=c)``).
+    With ``timing`` a list, one dict of per-stage wall milliseconds per bond is
+    appended (the device is synchronised at stage boundaries)."""
+    import time
+    from .mpstools import svdtrunc
+    from .symtensors import dot, splitleg, symmatrix
+    c = mps.center
+    if c < 1 or c + nbonds > len(mps):
+        raise ValueError("window leaves the chain")
+    Ws = mpo.Ws
+    energies = []
+    sync = torch.cuda.synchronize if mps.As[0].data.is_cuda else (lambda: None)
+    A = mps.As[c - 1]
+    for l in range(c, c + nbonds):
+        sync()
+        t0 = time.perf_counter()
+        AA = contract(A, (1, 2, -1), mps.As[l], (-1, 3, 4))
+        sync()
+        t1 = time.perf_counter()
+        H = HeffOperator(env[l - 1], env[l + 2], Ws[l - 1], Ws[l], prec)
+        theta, v, info = eigsolve(H, AA, lanczostol, krylovdim, krylovmaxiter, refine_tol)
+        sync()
+        t2 = time.perf_counter()
+        r = svdtrunc(symmatrix(v, [1, 2], [3, 4]), maxdim=maxdim, tol=tol)
+        r.normalize_s()
+        AA2 = splitleg(splitleg(r.usv(), 2, AA.space[2:4]), 1, AA.space[0:2])
+        sync()
+        t3 = time.perf_counter()
+        energies.append(dot(AA2, _applymps2site(AA2, env[l - 1], env[l + 2], Ws[l - 1], Ws[l])))
+        sync()
+        t4 = time.perf_counter()
+        mps.As[l - 1] = splitleg(r.u(), 1, A.space[0:2])
+        env[l] = _mpsupdateleft(env[l - 1], mps.As[l - 1], Ws[l - 1])
+        A = splitleg(r.sv(), 2, AA2.space[2:4])
+        sync()
+        t5 = time.perf_counter()
+        if timing is not None:
+            timing.append({"bond": l, "chi_left": AA.space[0].dim(), "chi_right": AA.space[3].dim(),
+                           "kept": r.Vl.dim(), "lanczos_energy": theta, "matvecs": info.numops,
+                           "ms_theta": 1e3 * (t1 - t0), "ms_eigsolve": 1e3 * (t2 - t1), "ms_svdtrunc": 1e3 * (t3 - t2),
+                           "ms_energy": 1e3 * (t4 - t3), "ms_envupdate": 1e3 * (t5 - t4), "ms_total": 1e3 * (t5 - t0)})
+    mps.As[c + nbonds - 1] = A
+    mps.center = c + nbonds
+    return energies
+
+
 def dmrg2sitesweep(mps, mpo, env, maxdim: int = 200, tol: float = 1e-9, lanczostol: float = 1e-7,
                    krylovdim: int = 5, krylovmaxiter: int = 8, verbose: bool = False, prec: str = "f64",
                    refine_tol: float = None, log: list = None):

@@ -141,3 +141,20 @@ def test_dmrg_cylinder_reaches_ed():
                                krylovmaxiter=10)
     print(f"cylinder {W}x{Lx}: E = {es[-1]:.12f} (ED {e_ed:.12f})", flush=True)
     assert abs(es[-1] - e_ed) < 1e-8 * abs(e_ed)
+
+
+def test_dmrg_window_equals_sweep_left_to_right():
+    """dmrg2site_window over bonds 1..L-2 from center 1 = the L->R half of dmrg2sitesweep."""
+    L, N = 10, 5
+    mps = mt.random_mps(L, N, 8, seed=12, device=DEV)
+    mpo = mt.xxz_u1mpo(L, 1.0, device=DEV)
+    env = ta.initialenv(mps, mpo)
+    mps2 = mt.MPState(list(mps.As), mps.center)
+    env2 = list(env)
+    kw = dict(maxdim=16, tol=1e-10, lanczostol=1e-9, krylovdim=6, krylovmaxiter=6)
+    es = ta.dmrg2sitesweep(mps, mpo, env, **kw)
+    timing = []
+    ew = ta.dmrg2site_window(mps2, mpo, env2, L - 2, timing=timing, **kw)
+    np.testing.assert_allclose(ew, es[:L - 2], rtol=1e-12, atol=1e-12)
+    assert mps2.center == L - 1 and len(timing) == L - 2
+    assert all(t["ms_total"] > 0 for t in timing)

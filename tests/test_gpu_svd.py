@@ -1,0 +1,91 @@
+"""Device batched per-sector SVD (``mpsh_svd_*``) and svdtrunc vs the oracle.
+
+Pinned by mathematics, not by the oracle's formula: U diag(S) Vh reconstructs
+each block, U and Vh have orthonormal columns / rows, S is descending and
+equals LAPACK's singular values; svdtrunc keeps exactly the oracle's counts
+(mpsutils.jl:89-112 greedy, absolute tol) and its u*s*v equals the oracle's
+truncated product (gauge-invariant).  FP64 bar: 1e-12 relative.
+"""
+import numpy as np
+import pytest
+import torch
+
+from oracle import symtensors as ost
+from tests.helpers import o_from_product
+
+pytestmark = pytest.mark.gpu
+
+if not torch.cuda.is_available():
+    pytest.skip("needs a GPU", allow_module_level=True)
+
+import mps_amd as M  # noqa: E402
+from mps_amd import mpstools as mt  # noqa: E402
+
+DEV = torch.device("cuda", 0)
+
+
+def _matrix(rows, cols, charge=0, seed=5, scale=None):
+    """Rank-2 SymTensor with legs (rows, cols dual): block (c, c - charge)."""
+    V1 = M.U1Space(*rows)
+    V2 = M.U1Space(*cols, isdual=True)
+    A = M.randn((V1, V2), charge, seed=seed, stream=3, device=DEV)
+    if scale is not None:  # graded spectrum so truncation has something to cut
+        blocks = A.blocks()
+        for k, b in blocks.items():
+            m, n = b.shape
+            u, _, vt = np.linalg.svd(b, full_matrices=False)
+            s = scale ** np.arange(min(m, n)) * (1.0 + 0.1 * abs(k[0]))
+            blocks[k] = (u * s) @ vt
+        A = M.SymTensor.from_blocks(A.charge, A.space, blocks, dtype=torch.float64, device=DEV)
+    return A
+
+
+SHAPES = [
+    ([(0, 1), (1, 3), (2, 2)], [(0, 2), (1, 1), (2, 5)]),                 # tiny, m < n and m > n
+    ([(q, 17 + 3 * q) for q in range(6)], [(q, 40 - 2 * q) for q in range(6)]),
+    ([(0, 130), (1, 250), (2, 300)], [(0, 129), (1, 520), (2, 301)]),    # crosses padding granularities
+    ([(q, 64) for q in range(9)], [(q, 64) for q in range(9)]),          # one bucket, nine blocks
+]
+
+
+@pytest.mark.parametrize("shape", range(len(SHAPES)))
+def test_svd_blocks_reconstruct_and_orthonormal(shape):
+    rows, cols = SHAPES[shape]
+    A = _matrix(rows, cols, seed=11 + shape)
+    blocks = A.blocks()
+    facts = mt._svd_blocks(A)
+    assert [f[0] for f in facts] == list(blocks.keys())
+    for key, U, S, Vh in facts:
+        b = blocks[key]
+        m, n = b.shape
+        k = min(m, n)
+        U, S, Vh = U.cpu().numpy(), S.cpu().numpy(), Vh.cpu().numpy()
+        assert U.shape == (m, k) and S.shape == (k,) and Vh.shape == (k, n)
+        assert np.all(np.diff(S) <= 1e-12 * S[0])
+        np.testing.assert_allclose(S, np.linalg.svd(b, compute_uv=False), rtol=1e-12, atol=1e-12 * S[0])
+        assert np.linalg.norm((U * S) @ Vh - b) <= 1e-12 * np.linalg.norm(b)
+        np.testing.assert_allclose(U.T @ U, np.eye(k), atol=1e-12)
+        np.testing.assert_allclose(Vh @ Vh.T, np.eye(k), atol=1e-12)
+
+
+def test_svd_of_charged_matrix():
+    A = _matrix([(q, 5 + q) for q in range(5)], [(q, 7) for q in range(5)], charge=1, seed=3)
+    r = mt.svd_(A)
+    assert np.linalg.norm(M.array(r.usv()) - M.array(A)) <= 1e-12 * np.linalg.norm(M.array(A))
+
+
+@pytest.mark.parametrize("maxdim,tol", [(40, 1e-12), (12, 1e-12), (1000, 1e-3)])
+def test_svdtrunc_matches_oracle(maxdim, tol):
+    A = _matrix([(q, 9 + 2 * q) for q in range(5)], [(q, 14 - q) for q in range(5)], seed=21, scale=0.5)
+    O = o_from_product(A)
+    r = mt.svdtrunc(A, maxdim=maxdim, tol=tol)
+    u, s, v = ost.svdtrunc(O, maxdim=maxdim, tol=tol)
+    kept_dev = {c: d for c, d in r.Vl.items()}
+    kept_or = dict(s.space[0].items())
+    assert kept_dev == kept_or
+    sv_or = np.sort(np.concatenate([np.diag(b) for b in s.blocks.values()]))[::-1]
+    np.testing.assert_allclose(r.singular_values(), sv_or, rtol=1e-12, atol=1e-14)
+    usv_or = ost.contract(ost.contract(u, (1, -1), s, (-1, 2)), (1, -1), v, (-1, 2))
+    got = M.array(r.usv())
+    want = ost.array(usv_or)
+    assert np.linalg.norm(got - want) <= 1e-12 * np.linalg.norm(want)

@@ -58,3 +58,29 @@ def test_cylinder_6x24_mpo_dimension():
     dims = [W.space[2].dim() for W in mpo.Ws[:-1]]
     assert max(dims) in (20, 23)
     assert len(mt.heisenberg_cylinder_bonds(6, 24)) == 6 * 24 + 6 * 23
+
+
+def test_truncatedsizes_matches_oracle_greedy():
+    """The sorted-prefix form of truncatedsizes equals the oracle's literal greedy loop
+    (mpsutils.jl:89-112), ties between sectors included."""
+    import numpy as np
+    from mps_amd.mpstools import truncatedsizes
+    from oracle.symtensors import truncatedsizes as o_trunc
+    rng = np.random.default_rng(4)
+    for trial in range(200):
+        nb = int(rng.integers(1, 8))
+        S = []
+        for _ in range(nb):
+            k = int(rng.integers(1, 12))
+            v = np.sort(rng.choice([1.0, 0.5, 0.25, 0.125, 1e-3, 1e-9], size=k) if trial % 2 else rng.random(k))[::-1]
+            S.append(v)
+        maxdim = int(rng.integers(1, 40))
+        tol = [1e-12, 1e-2, 0.3][trial % 3]
+        try:
+            want = o_trunc([s.copy() for s in S], maxdim=maxdim, tol=tol)
+        except ValueError:
+            import pytest
+            with pytest.raises(ValueError):
+                truncatedsizes(S, maxdim=maxdim, tol=tol)
+            continue
+        assert truncatedsizes(S, maxdim=maxdim, tol=tol) == list(want)
