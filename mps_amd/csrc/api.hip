@@ -110,7 +110,7 @@ void need_aligned(const void* p, const char* what) {
 }
 
 // Optional override of the H_eff kernel variants, read at plan creation:
-// MPSH_HEFF_TILES="s1=128x64x1,s4=64x64x1,s1d=64x64x1,s4d=64x64x1,split=1024"
+// MPSH_HEFF_TILES="s1=64x128x1,s4=64x64x2,s1h=128x64x1,s4h=64x64x2,s1d=64x64x1,s4d=64x64x1,split=1024"
 HeffTiles tiles_from_env() {
   HeffTiles t;
   const char* e = std::getenv("MPSH_HEFF_TILES");
@@ -133,6 +133,8 @@ HeffTiles tiles_from_env() {
     TileCfg c{bm, bn, ks};
     if (k == "s1") t.s1 = c;
     else if (k == "s4") t.s4 = c;
+    else if (k == "s1h") t.s1h = c;
+    else if (k == "s4h") t.s4h = c;
     else if (k == "s1d") t.s1d = c;
     else if (k == "s4d") t.s4d = c;
     else throw Error(ERR_INVALID, "bad MPSH_HEFF_TILES key");
@@ -217,7 +219,7 @@ struct mpsh_contract_s {
 
 struct HeffDev {
   std::unique_ptr<DevTables> t;
-  DevGemm g1, g4, g1d, g4d;  // FP32/BF16 and FP64 kernel variants
+  DevGemm g1, g4, g1h, g4h, g1d, g4d;  // FP32, BF16 and FP64 kernel variants
   DevMpo mpo;
   DevCopy pack_v, pack_envL, pack_envR;
 };
@@ -235,6 +237,8 @@ struct mpsh_heff_s {
       h.t = std::make_unique<DevTables>();
       h.g1 = h.t->gemm(p.g1);
       h.g4 = h.t->gemm(p.g4);
+      h.g1h = h.t->gemm(p.g1h);
+      h.g4h = h.t->gemm(p.g4h);
       h.g1d = h.t->gemm(p.g1d);
       h.g4d = h.t->gemm(p.g4d);
       h.mpo.sigs = h.t->up(p.sigs);
@@ -507,11 +511,11 @@ mpsh_status mpsh_heff_apply_prepared_stages(mpsh_heff plan, mpsh_dtype dt, int32
   void* Pt = ws + p.part_off * es;
   const bool f64 = dt == MPSH_F64;
   if (stages & 1) check(launch_copy(d.pack_v, es, v, vp, s), "pack v");
-  if (stages & 2)
-    check(launch_gemm(f64 ? d.g1d : d.g1, gemm_mode(dt), vp, pr + p.el_off * es, T1, Pt, s), "step-1 gemm");
+  const DevGemm& g1 = f64 ? d.g1d : (dt == MPSH_BF16 ? d.g1h : d.g1);
+  const DevGemm& g4 = f64 ? d.g4d : (dt == MPSH_BF16 ? d.g4h : d.g4);
+  if (stages & 2) check(launch_gemm(g1, gemm_mode(dt), vp, pr + p.el_off * es, T1, Pt, s), "step-1 gemm");
   if (stages & 4) check(launch_mpo_apply(d.mpo, es, pr + p.pw_off * es, T1, T3, s), "mpo");
-  if (stages & 8)
-    check(launch_gemm(f64 ? d.g4d : d.g4, gemm_mode(dt), T3, pr + p.er_off * es, out, Pt, s), "step-4 gemm");
+  if (stages & 8) check(launch_gemm(g4, gemm_mode(dt), T3, pr + p.er_off * es, out, Pt, s), "step-4 gemm");
   return MPSH_OK;
   API_CATCH
 }

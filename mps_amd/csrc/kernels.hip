@@ -307,12 +307,15 @@ __global__ __launch_bounds__(256) void gemm_grouped(
     if (nch > 0) bases();
     V ra[DEPTH][NA], rb[DEPTH][NB];
     int64_t krem[DEPTH];
+    bool full[DEPTH];  // chunk needs no K or row masking (uniform over the work group)
+    const bool rows_full = tile.m0 + BM <= p.M && tile.n0 + BN <= p.N;
     auto load = [&](auto D) {
       constexpr int d = decltype(D)::value;
       const int64_t k = k0 + VEC * q;
       const int64_t rem = K - k;
       const int64_t kk_ = rem > 0 ? k : 0;
       krem[d] = rem;
+      full[d] = rows_full && k0 + BK <= K;
 #pragma unroll
       for (int i = 0; i < NA; ++i) ra[d][i] = *(const V*)(A + ab[i] + kk_);
 #pragma unroll
@@ -327,14 +330,23 @@ __global__ __launch_bounds__(256) void gemm_grouped(
     };
     auto step = [&](auto D, int64_t it) {
       constexpr int d = decltype(D)::value;
-      const int lb = (int)(it & 1);
-      const int64_t rem = krem[d];
+      // with two register stages chunk `it` always sits in stage it & 1, so the
+      // LDS buffer index is a compile-time constant (immediate LDS offsets)
+      const int lb = DEPTH == 2 ? d : (int)(it & 1);
+      if (full[d]) {
 #pragma unroll
-      for (int i = 0; i < NA; ++i)
-        vstore<MODE>(&As[lb][row + i * RP][VEC * q], vmask<MODE>(ra[d][i], av[i] ? rem : 0));
+        for (int i = 0; i < NA; ++i) vstore<MODE>(&As[lb][row + i * RP][VEC * q], ra[d][i]);
 #pragma unroll
-      for (int i = 0; i < NB; ++i)
-        vstore<MODE>(&Bs[lb][row + i * RP][VEC * q], vmask<MODE>(rb[d][i], bv[i] ? rem : 0));
+        for (int i = 0; i < NB; ++i) vstore<MODE>(&Bs[lb][row + i * RP][VEC * q], rb[d][i]);
+      } else {
+        const int64_t rem = krem[d];
+#pragma unroll
+        for (int i = 0; i < NA; ++i)
+          vstore<MODE>(&As[lb][row + i * RP][VEC * q], vmask<MODE>(ra[d][i], av[i] ? rem : 0));
+#pragma unroll
+        for (int i = 0; i < NB; ++i)
+          vstore<MODE>(&Bs[lb][row + i * RP][VEC * q], vmask<MODE>(rb[d][i], bv[i] ? rem : 0));
+      }
       __syncthreads();
       if (it + DEPTH < nch) load(D);
 #pragma unroll
@@ -443,10 +455,19 @@ static int num_cu() {
 template <int MODE, int BM, int BN, int KS>
 static void launch_variant(const DevGemm& gm, const void* A, const void* B, void* C, void* P, hipStream_t s) {
   typedef typename GT<MODE>::T T;
-  // per XCD: up to num_cu work groups (8 per CU of the XCD), no more than its queue holds
+  // persistent grid: per XCD as many work groups as its CUs hold at once
+  // (occupancy of this variant), no more than its queue holds
+  static int occ = 0;
+  if (occ == 0) {
+    int nb = 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, gemm_grouped<MODE, BM, BN, KS>, 256, 0) != hipSuccess ||
+        nb <= 0)
+      nb = 2;
+    occ = nb;
+  }
   int qmax = 0;
   for (int x = 0; x < NXCD; ++x) qmax = std::max(qmax, gm.q.off[x + 1] - gm.q.off[x]);
-  const int per = std::min(num_cu(), qmax);
+  const int per = std::min(std::max(1, num_cu() / NXCD) * occ, qmax);
   dim3 grid((unsigned)(per * NXCD)), block(256);
   hipLaunchKernelGGL((gemm_grouped<MODE, BM, BN, KS>), grid, block, 0, s, gm.probs, gm.segs, gm.atab, gm.btab,
                      gm.crow, gm.cld, gm.tiles, gm.q, (const T*)A, (const T*)B, (T*)C, (T*)P);

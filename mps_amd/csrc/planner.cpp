@@ -1157,13 +1157,17 @@ HeffPlan heff_plan(const Struct& v, const Struct& envL, const Struct& envR, cons
     g.BN = c.BN;
     g.KS = c.KS;
   };
-  H.g1d = H.g1;
-  H.g4d = H.g4;
+  H.g1d = H.g1h = H.g1;
+  H.g4d = H.g4h = H.g4;
   cfg(H.g1, tiles.s1);
   cfg(H.g4, tiles.s4);
+  cfg(H.g1h, tiles.s1h);
+  cfg(H.g4h, tiles.s4h);
   cfg(H.g1d, tiles.s1d);
   cfg(H.g4d, tiles.s4d);
+  // K chunk: 16 * KS for FP32 / FP64, 32 * KS for BF16
   H.g4.choose_splits(tiles.split_target, 16 * tiles.s4.KS, 32);
+  H.g4h.choose_splits(tiles.split_target, 32 * tiles.s4h.KS, 32);
   H.g4d.choose_splits(tiles.split_target, 16 * tiles.s4d.KS, 32);
   H.pack_v.finalize();
   H.pack_envL.finalize();
@@ -1179,7 +1183,7 @@ HeffPlan heff_plan(const Struct& v, const Struct& envL, const Struct& envR, cons
   H.t1_off = up64(vp_elems);
   H.t3_off = up64(H.t1_off + H.t1_elems);
   H.part_off = up64(H.t3_off + H.t3_elems);
-  H.ws_elems = H.part_off + std::max(H.g4.part_elems, H.g4d.part_elems);
+  H.ws_elems = H.part_off + std::max({H.g4.part_elems, H.g4h.part_elems, H.g4d.part_elems});
   return H;
 }
 

@@ -283,7 +283,8 @@ struct TileCfg {
   int BM = 64, BN = 64, KS = 1;
 };
 struct HeffTiles {
-  TileCfg s1{128, 64, 1}, s4{64, 64, 2};     // FP32 / BF16 storage
+  TileCfg s1{64, 128, 1}, s4{64, 64, 2};     // FP32
+  TileCfg s1h{128, 64, 1}, s4h{64, 64, 2};   // BF16 operands
   TileCfg s1d{64, 64, 1}, s4d{64, 64, 1};    // FP64
   int64_t split_target = 1024;               // minimum work items for the step-4 GEMM
 };
@@ -291,6 +292,7 @@ struct HeffTiles {
 struct HeffPlan {
   Struct v, envL, envR, WL, WR, out;
   GemmPlan g1;  // A = v, B = envL, C = T1 (ws); FP32/BF16 tile variant
+  GemmPlan g1h, g4h;  // the same problems with the BF16 tile variants
   GemmPlan g1d, g4d;  // the same problems with the FP64 tile variants
   int64_t t1_elems = 0;
   std::vector<MpoSig> sigs;
