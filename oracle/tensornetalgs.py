@@ -54,18 +54,26 @@ def mpsupdateright(renv: SymTensor, A: SymTensor, W: SymTensor) -> SymTensor:
 
 def dense_applymps2site(v, envL, envR, WL, WR):
     """Dense ``_applymps2site`` (apply.jl:80-91) as one einsum."""
-    # envL[a,m,b] v[a,p,s,c] WL[m,q,n,p] WR[n,r,k,s] envR[c,k,e] -> out[b,q,r,e]
-    return np.einsum("amb,apsc,mqnp,nrks,cke->bqre", envL, v, WL, WR, envR, optimize=True)
+    # envL[a,m,b] v[a,p,s,c] WL[m,q,n,p] WR[n,r,k,s] envR[c,k,e] -> out[b,q,r,e],
+    # pairwise in the @tensor order (((envL v) WL) WR) envR
+    t = np.einsum("amb,apsc->mbpsc", envL, v)
+    t = np.einsum("mbpsc,mqnp->bscqn", t, WL)
+    t = np.einsum("bscqn,nrks->bcqrk", t, WR)
+    return np.einsum("bcqrk,cke->bqre", t, envR)
 
 
 def dense_mpsupdateleft(lenv, A, W):
     """Dense ``_mpsupdateleft`` (apply.jl:41-47); real data, so conj is a no-op."""
-    return np.einsum("pmq,pou,mzno,qzd->und", lenv, A, W, np.conj(A), optimize=True)
+    t = np.einsum("pmq,pou->mqou", lenv, A)
+    t = np.einsum("mqou,mzno->quzn", t, W)
+    return np.einsum("quzn,qzd->und", t, np.conj(A))
 
 
 def dense_mpsupdateright(renv, A, W):
     """Dense ``_mpsupdateright`` (apply.jl:9-15)."""
-    return np.einsum("pmq,uop,nzmo,dzq->und", renv, A, W, np.conj(A), optimize=True)
+    t = np.einsum("pmq,uop->mquo", renv, A)
+    t = np.einsum("mquo,nzmo->quzn", t, W)
+    return np.einsum("quzn,dzq->und", t, np.conj(A))
 
 
 # ---------------------------------------------------------------------------

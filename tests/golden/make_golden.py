@@ -1,0 +1,118 @@
+"""Write the golden fixtures of this directory from the oracle (numpy FP64
+restatement of the reference path) on seeded inputs.
+
+Run from the repository root:  python tests/golden/make_golden.py
+Fixtures (npz, no pickled objects; structures as a JSON string):
+
+* heff_xxz_L32_chi64.npz   -- _applymps2site (src/apply.jl:93-105) inputs and output
+* envupdate_xxz_L32_chi64.npz -- _mpsupdateleft / _mpsupdateright (src/apply.jl:17-58)
+* sweep_xxz_L8.npz        -- one dmrg2sitesweep! (src/dmrg.jl:58-179) energy trace from a
+                              canonical random U(1) MPS, plus the U(1)-sector ED energy
+
+Inputs come from the SplitMix64 / Box-Muller generator (mps_amd.workloads.gaussian).
+"""
+import json
+import os
+import sys
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.dirname(os.path.dirname(os.path.abspath(__file__))))
+sys.path.insert(0, ROOT)
+
+from mps_amd.workloads import binom_capped_profile, gaussian, heff_spaces  # noqa: E402
+from oracle import symtensors as ost  # noqa: E402
+from oracle import tensornetalgs as ota  # noqa: E402
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+VD = ost.U1Space((0, 1), (1, 1))
+
+
+def space_json(t):
+    return {"charge": int(t.charge), "legs": [[int(V.isdual), [[int(c), int(d)] for c, d in V.items()]]
+                                              for V in t.space]}
+
+
+def flat(t):
+    return np.concatenate([np.asarray(b, np.float64).reshape(-1, order="F") for b in t.blocks.values()]) \
+        if t.blocks else np.zeros(0)
+
+
+def from_flat(charge, spaces, x):
+    sects, sizes = ost.allsectorsandsizes(charge, spaces)
+    blocks, p = {}, 0
+    for s, z in zip(sects, sizes):
+        n = int(np.prod(z))
+        blocks[s] = x[p:p + n].reshape(z, order="F")
+        p += n
+    return ost.SymTensor(charge, spaces, blocks)
+
+
+def save(name, tensors):
+    meta = {k: space_json(t) for k, t in tensors.items()}
+    arrays = {f"flat_{k}": flat(t) for k, t in tensors.items()}
+    np.savez_compressed(os.path.join(HERE, name), meta=np.array(json.dumps(meta)), **arrays)
+    print("wrote", name, {k: v.size for k, v in arrays.items()})
+
+
+def heff_inputs(name, seed=1911):
+    left, right, mpo = heff_spaces(name)
+    Vl, Vr, Vw = ost.Space(left), ost.Space(right), ost.Space(mpo)
+    spaces = [(Vl, VD, VD, Vr.dual()), (Vl.dual(), Vw.dual(), Vl), (Vr, Vw, Vr.dual()),
+              (Vw, VD, Vw.dual(), VD.dual()), (Vw, VD, Vw.dual(), VD.dual())]
+    out = []
+    for stream, sp in enumerate(spaces, start=1):
+        n = sum(int(np.prod(z)) for z in ost.allsectorsandsizes(0, sp)[1])
+        out.append(from_flat(0, sp, gaussian(seed, stream, n)))
+    return out
+
+
+def main():
+    v, L, R, WL, WR = heff_inputs("xxz_L32_chi64")
+    save("heff_xxz_L32_chi64.npz", {"v": v, "envL": L, "envR": R, "WL": WL, "WR": WR,
+                                    "out": ota.applymps2site(v, L, R, WL, WR)})
+    # environment updates: A = (Vl, Vd, Vr†) site tensor, W the bulk XXZ MPO tensor
+    left, right, _ = heff_spaces("xxz_L32_chi64")
+    Vl, Vr = ost.Space(left), ost.Space(right)
+    Vw = WL.space[0]
+    sA = (Vl, VD, Vr.dual())
+    nA = sum(int(np.prod(z)) for z in ost.allsectorsandsizes(0, sA)[1])
+    A = from_flat(0, sA, gaussian(1911, 11, nA))
+    W = ota.xxz_u1mpo(8, 0.7).Ws[3]
+    Vw = W.space[0]
+    sLenv = (Vl.dual(), Vw.dual(), Vl)
+    sRenv = (Vr, W.space[2].dual(), Vr.dual())
+    nL = sum(int(np.prod(z)) for z in ost.allsectorsandsizes(0, sLenv)[1])
+    nR = sum(int(np.prod(z)) for z in ost.allsectorsandsizes(0, sRenv)[1])
+    Lenv = from_flat(0, sLenv, gaussian(1911, 12, nL))
+    Renv = from_flat(0, sRenv, gaussian(1911, 13, nR))
+    save("envupdate_xxz_L32_chi64.npz", {"A": A, "W": W, "Lenv": Lenv, "Renv": Renv,
+                                         "left": ota.mpsupdateleft(Lenv, A, W),
+                                         "right": ota.mpsupdateright(Renv, A, W)})
+    # one sweep on a canonical random MPS, XXZ L=8, N=4, chi <= 8
+    Lx, N, chi = 8, 4, 8
+    bonds = [[(0, 1)]] + [binom_capped_profile(l, Lx, N, chi) for l in range(1, Lx)] + [[(N, 1)]]
+    flats = []
+    for l in range(1, Lx + 1):
+        sp = (ost.Space(bonds[l - 1]), VD, ost.Space(bonds[l]).dual())
+        n = sum(int(np.prod(z)) for z in ost.allsectorsandsizes(0, sp)[1])
+        flats.append(gaussian(1911, 1000 + l, n))
+    mps = ota.random_u1mps_oracle(Lx, N, bonds, flats)
+    ota.center_at(mps, 1)
+    ota.normalize_mps(mps)
+    init = {f"A{l + 1}": A for l, A in enumerate(mps.As)}
+    mpo = ota.xxz_u1mpo(Lx, 1.0)
+    env = ota.initialenv(mps, mpo)
+    kw = dict(maxdim=8, tol=1e-10, lanczostol=1e-10, krylovdim=6, krylovmaxiter=6)
+    energies = ota.dmrg2sitesweep(mps, mpo, env, **kw)
+    e_ed = ota.u1_ground_energy(Lx, [(i, i + 1) for i in range(1, Lx)], N)
+    meta = {k: space_json(t) for k, t in init.items()}
+    meta["settings"] = kw
+    np.savez_compressed(os.path.join(HERE, "sweep_xxz_L8.npz"), meta=np.array(json.dumps(meta)),
+                        energies=np.asarray(energies), e_ed=np.asarray(e_ed),
+                        **{f"flat_{k}": flat(t) for k, t in init.items()})
+    print("wrote sweep_xxz_L8.npz", len(energies), "energies, last", energies[-1], "ED", e_ed)
+
+
+if __name__ == "__main__":
+    main()

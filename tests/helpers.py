@@ -1,4 +1,6 @@
 """Shared test helpers: conversions between the product and the oracle."""
+import os
+
 import numpy as np
 
 from oracle import symtensors as ost
@@ -59,3 +61,23 @@ def rel_block_err(A, B):
 def global_rel_err(A, B):
     fa, fb = o_flat(A), o_flat(B)
     return float(np.linalg.norm(fa - fb) / max(np.linalg.norm(fb), 1e-300))
+
+
+GOLDEN = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
+
+
+def load_fixture(name):
+    """Golden fixture -> ({tensor name: oracle SymTensor}, extra arrays, meta dict).
+
+    npz written by tests/golden/make_golden.py; loaded without pickle."""
+    import json
+    z = np.load(os.path.join(GOLDEN, name), allow_pickle=False)
+    meta = json.loads(str(z["meta"]))
+    tensors = {}
+    for k, m in meta.items():
+        if not isinstance(m, dict) or "legs" not in m:
+            continue
+        spaces = tuple(ost.Space([tuple(p) for p in pairs], bool(d)) for d, pairs in m["legs"])
+        tensors[k] = o_from_flat(m["charge"], spaces, z[f"flat_{k}"])
+    extra = {k: z[k] for k in z.files if k != "meta" and not k.startswith("flat_")}
+    return tensors, extra, meta
