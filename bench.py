@@ -41,6 +41,8 @@ def parse():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-extra-modes", action="store_true")
     ap.add_argument("--cpu-budget", type=float, default=20.0, help="seconds of oracle work for the CPU baseline")
+    ap.add_argument("--lanczos-graphs", type=int, default=0, choices=[0, 1],
+                    help="capture each Lanczos iteration of the DMRG window in a hipGraph")
     ap.add_argument("--dmrg-bonds", type=int, default=6,
                     help="timed two-site updates at chi=2048 in the bulk of the 6x24 cylinder (0: skip)")
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
@@ -323,7 +325,10 @@ def main():
         result["other_paths"] = extra
     dmrg_state = None
     if args.dmrg_bonds > 0:
+        from mps_amd import tensornetalgs as ta
+        ta.LANCZOS_GRAPHS = bool(args.lanczos_graphs)
         result["dmrg"], dmrg_state = dmrg_window(args.dmrg_bonds, dev)
+        result["dmrg"]["lanczos_hipgraph"] = ta.LANCZOS_GRAPHS
     if not args.no_cpu_baseline:
         rate, calls, secs, threads, oflat, vflat = cpu_baseline(args.workload, args.cpu_budget)
         e_or = float(np.dot(vflat, oflat) / np.dot(vflat, vflat))

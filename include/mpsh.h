@@ -179,6 +179,24 @@ mpsh_status mpsh_scal(mpsh_dtype dt, int64_t n, double a, void* x, void* stream)
 mpsh_status mpsh_convert(mpsh_dtype src_dt, const void* src, mpsh_dtype dst_dt, void* dst, int64_t n,
                          void* stream);
 
+/* ------------------------------------------------------------------------ */
+/* Lanczos basis operations for the eigensolve on the device (the role of    */
+/* KrylovKit.eigsolve's orthogonalisation, src/dmrg.jl:77-82).  FP64; basis  */
+/* vector i is V + i*ldv (n elements); scalars stay in device memory so one  */
+/* Lanczos iteration needs no host round trip and can be graph-captured.     */
+/* partials: device scratch of k * MPSH_LZ_PARTS doubles.                    */
+/* ------------------------------------------------------------------------ */
+#define MPSH_LZ_PARTS 256
+/* c[i] = <V_i, w>, i < k */
+mpsh_status mpsh_lanczos_dots(int32_t k, int64_t n, const double* V, int64_t ldv, const double* w, double* c,
+                              double* partials, void* stream);
+/* w += sign * sum_{i<k} c[i] V_i   (c in device memory) */
+mpsh_status mpsh_lanczos_update(int32_t k, int64_t n, const double* V, int64_t ldv, const double* c, double sign,
+                                double* w, void* stream);
+/* *beta = ||w||, out = w / *beta (out may alias w) */
+mpsh_status mpsh_lanczos_normalize(int64_t n, const double* w, double* out, double* beta, double* partials,
+                                   void* stream);
+
 #ifdef __cplusplus
 }
 #endif

@@ -13,6 +13,7 @@ LIB_PATH = os.path.join(_HERE, "libmpsh.so")
 
 MPSH_F64, MPSH_F32, MPSH_BF16 = 0, 1, 2
 DOT_PARTIALS = 1024
+LZ_PARTS = 256  # MPSH_LZ_PARTS
 
 STATUS_NAMES = {
     0: "OK", 1: "INVALID", 2: "SPACE_MISMATCH", 3: "SECTOR_MISMATCH", 4: "NOT_SIMILAR",
@@ -84,6 +85,9 @@ _SIGS = {
     "mpsh_svd_execute": (C.c_int, [_p, _p, _p, _p, _p, _p, _p]),
     "mpsh_svd_destroy": (None, [_p]),
     "mpsh_dot": (C.c_int, [C.c_int, _i64, _p, _p, _p, _p, _p]),
+    "mpsh_lanczos_dots": (C.c_int, [_i32, _i64, _p, _i64, _p, _p, _p, _p]),
+    "mpsh_lanczos_update": (C.c_int, [_i32, _i64, _p, _i64, _p, _dbl, _p, _p]),
+    "mpsh_lanczos_normalize": (C.c_int, [_i64, _p, _p, _p, _p, _p]),
     "mpsh_axpby": (C.c_int, [C.c_int, _i64, _dbl, _p, _dbl, _p, _p]),
     "mpsh_scal": (C.c_int, [C.c_int, _i64, _dbl, _p, _p]),
     "mpsh_convert": (C.c_int, [C.c_int, _p, C.c_int, _p, _i64, _p]),

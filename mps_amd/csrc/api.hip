@@ -568,6 +568,38 @@ mpsh_status mpsh_scal(mpsh_dtype dt, int64_t n, double a, void* x, void* stream)
   API_CATCH
 }
 
+mpsh_status mpsh_lanczos_dots(int32_t k, int64_t n, const double* V, int64_t ldv, const double* w, double* c,
+                              double* partials, void* stream) {
+  API_TRY
+  if (k < 0 || n < 0 || k > 4096) throw Error(ERR_INVALID, "bad Lanczos dimensions");
+  if (k > 0 && n > 0 && (!V || !w || !c || !partials)) throw Error(ERR_INVALID, "null argument");
+  if (k > 1 && ldv < n) throw Error(ERR_INVALID, "ldv < n");
+  check(launch_lz_dots(k, n, V, ldv, w, c, partials, (hipStream_t)stream), "lanczos dots");
+  return MPSH_OK;
+  API_CATCH
+}
+
+mpsh_status mpsh_lanczos_update(int32_t k, int64_t n, const double* V, int64_t ldv, const double* c, double sign,
+                                double* w, void* stream) {
+  API_TRY
+  if (k < 0 || n < 0) throw Error(ERR_INVALID, "bad Lanczos dimensions");
+  if (k > 0 && n > 0 && (!V || !w || !c)) throw Error(ERR_INVALID, "null argument");
+  if (k > 1 && ldv < n) throw Error(ERR_INVALID, "ldv < n");
+  check(launch_lz_update(k, n, V, ldv, c, sign, w, (hipStream_t)stream), "lanczos update");
+  return MPSH_OK;
+  API_CATCH
+}
+
+mpsh_status mpsh_lanczos_normalize(int64_t n, const double* w, double* out, double* beta, double* partials,
+                                   void* stream) {
+  API_TRY
+  if (n < 0) throw Error(ERR_INVALID, "bad Lanczos dimensions");
+  if (!w || !out || !beta || !partials) throw Error(ERR_INVALID, "null argument");
+  check(launch_lz_normalize(n, w, out, beta, partials, (hipStream_t)stream), "lanczos normalize");
+  return MPSH_OK;
+  API_CATCH
+}
+
 mpsh_status mpsh_convert(mpsh_dtype src_dt, const void* src, mpsh_dtype dst_dt, void* dst, int64_t n,
                          void* stream) {
   API_TRY

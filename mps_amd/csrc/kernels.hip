@@ -703,6 +703,102 @@ hipError_t launch_dot(int elem_size, int64_t n, const void* x, const void* y, do
   return hipGetLastError();
 }
 
+// ---------------------------------------------------------------------------
+// Lanczos basis operations (FP64, scalars stay on the device)
+//   dots:      c[i] = <V_i, w>, i < k       (block partials, fixed-order reduction)
+//   update:    w += sign * sum_i c[i] V_i   (c read from device memory)
+//   normalize: beta = ||w||, out = w / beta
+// V_i = V + i * ldv.  Partials: k * LZ_PARTS doubles.
+// ---------------------------------------------------------------------------
+constexpr int LZ_PARTS = 256;
+
+__global__ __launch_bounds__(256) void lz_dots_partial(int k, int64_t n, const double* __restrict__ V, int64_t ldv,
+                                                       const double* __restrict__ w, double* __restrict__ part) {
+  __shared__ double sh[8][256];
+  // block b handles a contiguous slice of n for up to 8 basis vectors at a time
+  const int64_t per = (n + LZ_PARTS - 1) / LZ_PARTS;
+  const int64_t e0 = (int64_t)blockIdx.x * per, e1 = e0 + per < n ? e0 + per : n;
+  for (int i0 = 0; i0 < k; i0 += 8) {
+    const int ni = k - i0 < 8 ? k - i0 : 8;
+    double acc[8];
+#pragma unroll
+    for (int t = 0; t < 8; ++t) acc[t] = 0.0;
+    for (int64_t e = e0 + threadIdx.x; e < e1; e += 256) {
+      const double we = w[e];
+#pragma unroll
+      for (int t = 0; t < 8; ++t)
+        if (t < ni) acc[t] += V[(int64_t)(i0 + t) * ldv + e] * we;
+    }
+#pragma unroll
+    for (int t = 0; t < 8; ++t) sh[t][threadIdx.x] = acc[t];
+    __syncthreads();
+    for (int h = 128; h > 0; h >>= 1) {
+      if (threadIdx.x < h)
+#pragma unroll
+        for (int t = 0; t < 8; ++t) sh[t][threadIdx.x] += sh[t][threadIdx.x + h];
+      __syncthreads();
+    }
+    if (threadIdx.x < ni) part[(int64_t)(i0 + threadIdx.x) * LZ_PARTS + blockIdx.x] = sh[threadIdx.x][0];
+    __syncthreads();
+  }
+}
+
+// one block per basis vector: c[i] = sum of its LZ_PARTS partials (fixed order)
+__global__ __launch_bounds__(256) void lz_dots_final(const double* __restrict__ part, double* __restrict__ c,
+                                                     int sqrt_out) {
+  __shared__ double sh[256];
+  sh[threadIdx.x] = part[(int64_t)blockIdx.x * LZ_PARTS + threadIdx.x];
+  __syncthreads();
+  for (int h = 128; h > 0; h >>= 1) {
+    if (threadIdx.x < h) sh[threadIdx.x] += sh[threadIdx.x + h];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) c[blockIdx.x] = sqrt_out ? sqrt(sh[0]) : sh[0];
+}
+
+__global__ __launch_bounds__(256) void lz_update(int k, int64_t n, const double* __restrict__ V, int64_t ldv,
+                                                 const double* __restrict__ c, double sign, double* __restrict__ w) {
+  for (int64_t e = (int64_t)blockIdx.x * 256 + threadIdx.x; e < n; e += (int64_t)gridDim.x * 256) {
+    double acc = 0.0;
+    for (int i = 0; i < k; ++i) acc += c[i] * V[(int64_t)i * ldv + e];
+    w[e] += sign * acc;
+  }
+}
+
+__global__ __launch_bounds__(256) void lz_scale(int64_t n, const double* __restrict__ w, const double* __restrict__ beta,
+                                                double* __restrict__ out) {
+  const double b = beta[0];
+  const double r = b > 0.0 ? 1.0 / b : 0.0;
+  for (int64_t e = (int64_t)blockIdx.x * 256 + threadIdx.x; e < n; e += (int64_t)gridDim.x * 256) out[e] = w[e] * r;
+}
+
+hipError_t launch_lz_dots(int k, int64_t n, const double* V, int64_t ldv, const double* w, double* c, double* part,
+                          hipStream_t s) {
+  if (k <= 0) return hipSuccess;
+  hipLaunchKernelGGL(lz_dots_partial, dim3(LZ_PARTS), dim3(256), 0, s, k, n, V, ldv, w, part);
+  hipLaunchKernelGGL(lz_dots_final, dim3(k), dim3(256), 0, s, part, c, 0);
+  return hipGetLastError();
+}
+
+hipError_t launch_lz_update(int k, int64_t n, const double* V, int64_t ldv, const double* c, double sign, double* w,
+                            hipStream_t s) {
+  if (k <= 0 || n <= 0) return hipSuccess;
+  int64_t nb = (n + 255) / 256;
+  if (nb > 4096) nb = 4096;
+  hipLaunchKernelGGL(lz_update, dim3((unsigned)nb), dim3(256), 0, s, k, n, V, ldv, c, sign, w);
+  return hipGetLastError();
+}
+
+hipError_t launch_lz_normalize(int64_t n, const double* w, double* out, double* beta, double* part, hipStream_t s) {
+  hipLaunchKernelGGL(lz_dots_partial, dim3(LZ_PARTS), dim3(256), 0, s, 1, n, w, (int64_t)0, w, part);
+  hipLaunchKernelGGL(lz_dots_final, dim3(1), dim3(256), 0, s, part, beta, 1);
+  int64_t nb = (n + 255) / 256;
+  if (nb > 4096) nb = 4096;
+  if (nb < 1) nb = 1;
+  hipLaunchKernelGGL(lz_scale, dim3((unsigned)nb), dim3(256), 0, s, n, w, beta, out);
+  return hipGetLastError();
+}
+
 template <typename T>
 __global__ __launch_bounds__(256) void axpby_kernel(int64_t n, double a, const T* __restrict__ x, double b,
                                                     T* __restrict__ y) {

@@ -13,6 +13,7 @@ from __future__ import annotations
 
 from typing import Dict, Optional
 
+import numpy as np
 import torch
 
 from . import _lib as L
@@ -150,6 +151,47 @@ def _mpsupdateright(renv: SymTensor, mat: SymTensor, hmpo: SymTensor) -> SymTens
                     (1, -1, 2, -2), dual(mat), (3, -1, -2))
 
 
+def rayleigh_gradient(v: SymTensor, envL: SymTensor, envR: SymTensor, WL: SymTensor, WR: SymTensor,
+                      prec: str = None):
+    """Gradient of E = <v|H_eff v>/<v|v> (H_eff of ``_applymps2site``, apply.jl:93-105)
+    with respect to the two-site tensor and both environments.
+
+    Returns (E, dE/dv, dE/denvL, dE/denvR), each gradient with the structure of
+    its variable.  E is linear in each environment, so dE/denvL is the adjoint
+    contraction of (bra v, WL, WR, envR, ket v) with the envL legs left open,
+    divided by <v|v> (likewise for envR); both are chains of block-sparse device
+    contracts (the adjoints of the reference's four contracts).  dE/dv =
+    2 (H v - E v)/<v|v> holds for the Hermitian H_eff of DMRG (environments of
+    one MPS and a Hermitian MPO).  ``prec`` selects the H_eff arithmetic of the
+    v-gradient; the environment adjoints run in the tensors' dtype.
+    """
+    from .symtensors import axpby_, dot, rmul_
+    nn = dot(v, v)
+    Hv = _applymps2site(v, envL, envR, WL, WR, prec=prec) if prec else _applymps2site(v, envL, envR, WL, WR)
+    E = dot(v, Hv) / nn
+    gv = Hv.copy()
+    axpby_(-2.0 * E / nn, v, 2.0 / nn, gv)
+    bra = dual(v)
+    # dE/denvL[a,m,b] = sum v[a,p,s,c] WL[m,q,n,p] WR[n,r,k,s] envR[c,k,e] v[b,q,r,e] / <v|v>
+    t = contract(v, (1, 2, 3, -1), envR, (-1, 4, 5))
+    t = contract(t, (1, 2, -1, -2, 3), WR, (4, 5, -2, -1))
+    t = contract(t, (1, -1, 2, -2, 3), WL, (4, 5, -2, -1))
+    X = contract(t, (1, -1, -2, 2, -3), bra, (3, -3, -2, -1))
+    # dE/denvR[c,k,e] = sum envL[a,m,b] v[a,p,s,c] WL[m,q,n,p] WR[n,r,k,s] v[b,q,r,e] / <v|v>
+    t = contract(envL, (-1, 2, 1), v, (-1, 3, 4, 5))
+    t = contract(t, (1, -1, -2, 4, 5), WL, (-1, 2, 3, -2))
+    t = contract(t, (1, 2, -1, -2, 5), WR, (-1, 3, 4, -2))
+    Y = contract(t, (-1, -2, -3, 2, 1), bra, (-1, -2, -3, 3))
+
+    def as_grad(G, like):
+        # the adjoint's legs are the duals of the variable's: same sectors, same block order
+        if G.struct.sectors() != like.struct.sectors() or G.struct.numel != like.struct.numel:
+            raise ValueError("adjoint structure does not match the environment")
+        return rmul_(SymTensor(like.charge, like.space, G.data.to(like.dtype).clone(), like.struct), 1.0 / nn)
+
+    return E, gv, as_grad(X, envL), as_grad(Y, envR)
+
+
 def heff_workload(name: str, seed: int = 1911, dtype=torch.float64, device=None):
     """Random (v, envL, envR, WL, WR) with the leg layout of a two-site DMRG bond.
 
@@ -177,53 +219,111 @@ class EigInfo:
         self.normres, self.numiter, self.numops, self.converged = normres, numiter, numops, converged
 
 
-def lanczos_lowest(apply, x0: SymTensor, tol: float = 1e-7, krylovdim: int = 5, maxiter: int = 8):
+LANCZOS_GRAPHS = False  # capture each Lanczos iteration in a hipGraph (eigsolve default)
+
+
+class _LanczosBasis:
+    """Device-resident Lanczos basis: V[0..kdim] (FP64 rows), the device scalars of
+    one iteration and the partial-sum scratch of the ``mpsh_lanczos_*`` kernels."""
+
+    def __init__(self, x: SymTensor, kdim: int):
+        self.like = x
+        self.n = x.struct.numel
+        self.kdim = kdim
+        dev = x.device
+        self.ld = max(1, self.n)
+        self.V = torch.empty((kdim + 1, self.ld), dtype=torch.float64, device=dev)
+        self.sc = torch.zeros(2 * (kdim + 1) + 2, dtype=torch.float64, device=dev)  # c1 | c2 | beta
+        self.part = torch.empty((kdim + 1) * L.LZ_PARTS, dtype=torch.float64, device=dev)
+        self.graphs = {}
+        self.outs = {}
+
+    def vec(self, j) -> SymTensor:
+        return SymTensor(self.like.charge, self.like.space, self.V[j, :self.n], self.like.struct)
+
+    def iterate(self, apply, j):
+        """w = apply(V_j); two classical Gram-Schmidt passes against V_0..V_j (scalars
+        c1, c2 on the device); V_{j+1} = w / ||w||.  No host synchronisation."""
+        w = apply(self.vec(j))
+        lb, st, k, kd = lib(), _stream(), j + 1, self.kdim + 1
+        V, wp, sc = _ptr(self.V), _ptr(w.data), self.sc.data_ptr()
+        check(lb.mpsh_lanczos_dots(k, self.n, V, self.ld, wp, sc, _ptr(self.part), st))
+        check(lb.mpsh_lanczos_update(k, self.n, V, self.ld, sc, -1.0, wp, st))
+        check(lb.mpsh_lanczos_dots(k, self.n, V, self.ld, wp, sc + 8 * kd, _ptr(self.part), st))
+        check(lb.mpsh_lanczos_update(k, self.n, V, self.ld, sc + 8 * kd, -1.0, wp, st))
+        check(lb.mpsh_lanczos_normalize(self.n, wp, self.V[j + 1].data_ptr(), sc + 16 * kd, _ptr(self.part), st))
+        return w
+
+    def step(self, apply, j, graph: bool):
+        if not graph:
+            self.outs[j] = self.iterate(apply, j)
+            return
+        g = self.graphs.get(j)
+        if g is None:
+            g = torch.cuda.CUDAGraph()
+            with torch.cuda.graph(g):
+                self.outs[j] = self.iterate(apply, j)
+            self.graphs[j] = g
+        g.replay()
+
+    def scalars(self, j):
+        """(alpha_j, beta_j) after iteration j (one device-to-host copy)."""
+        s = self.sc.cpu().numpy()
+        kd = self.kdim + 1
+        return float(s[j] + s[kd + j]), float(s[2 * kd])
+
+    def combine(self, y) -> None:
+        """V_0 <- normalised sum_i y_i V_i (the Ritz vector)."""
+        k = len(y)
+        c = torch.tensor(np.asarray(y, dtype=np.float64), device=self.V.device)
+        x = torch.zeros(self.ld, dtype=torch.float64, device=self.V.device)
+        lb, st = lib(), _stream()
+        check(lb.mpsh_lanczos_update(k, self.n, _ptr(self.V), self.ld, _ptr(c), 1.0, _ptr(x), st))
+        check(lb.mpsh_lanczos_normalize(self.n, _ptr(x), self.V[0].data_ptr(), self.sc.data_ptr() + 16 * (self.kdim + 1),
+                                        _ptr(self.part), st))
+
+
+def lanczos_lowest(apply, x0: SymTensor, tol: float = 1e-7, krylovdim: int = 5, maxiter: int = 8,
+                   graphs: bool = False):
     """Restarted Lanczos for the lowest eigenpair of a Hermitian map on FP64 device vectors.
 
-    Each cycle builds up to ``krylovdim`` fully re-orthogonalised vectors
-    from the current Ritz vector (FP64 inner products on the device, FP64
-    tridiagonal eigenproblem); converged when |beta_k y_k| < tol.
+    The basis lives on the device; every iteration applies the map, runs two
+    classical Gram-Schmidt passes against the whole basis and normalises, with
+    all scalars in device memory (``mpsh_lanczos_*``), so the host reads just
+    (alpha, beta) once per iteration to solve the tridiagonal eigenproblem in
+    FP64 and test |beta_k y_k| < tol.  With ``graphs`` each iteration index is
+    captured once in a hipGraph and replayed (the first application runs
+    eagerly, so the operator prepares itself outside the capture).
     """
-    import numpy as np
-    from .symtensors import axpby_, dot, norm, rmul_
-    x = x0.copy()
-    rmul_(x, 1.0 / norm(x))
+    kd = max(1, int(krylovdim))
+    B = _LanczosBasis(x0, kd)
+    x = x0 if x0.dtype == torch.float64 else x0.to(torch.float64)
+    lb, st = lib(), _stream()
+    check(lb.mpsh_lanczos_normalize(B.n, _ptr(x.data), B.V[0].data_ptr(), B.sc.data_ptr() + 16 * (kd + 1),
+                                    _ptr(B.part), st))
     numops, theta, res, it = 0, None, float("inf"), 0
+    warm = not graphs
     for it in range(1, maxiter + 1):
-        V = [x]
         alphas, betas = [], []
-        w = apply(x)
-        numops += 1
         y = np.array([1.0])
-        for j in range(krylovdim):
-            a = dot(V[j], w)
+        for j in range(kd):
+            B.step(apply, j, graph=graphs and warm)
+            warm = True
+            numops += 1
+            a, b = B.scalars(j)
             alphas.append(a)
-            axpby_(-a, V[j], 1.0, w)
-            if j > 0:
-                axpby_(-betas[j - 1], V[j - 1], 1.0, w)
-            for i in range(j + 1):
-                axpby_(-dot(V[i], w), V[i], 1.0, w)
-            b = norm(w)
             T = np.diag(alphas) + np.diag(betas, 1) + np.diag(betas, -1)
             evals, evecs = np.linalg.eigh(T)
             theta, y = float(evals[0]), evecs[:, 0]
             res = abs(b * y[-1])
-            if res < tol or j == krylovdim - 1 or b < 1e-14:
+            if res < tol or j == kd - 1 or b < 1e-14:
                 break
             betas.append(b)
-            rmul_(w, 1.0 / b)
-            V.append(w)
-            w = apply(w)
-            numops += 1
-        xn = V[0].copy()
-        rmul_(xn, float(y[0]))
-        for i in range(1, len(y)):
-            axpby_(float(y[i]), V[i], 1.0, xn)
-        rmul_(xn, 1.0 / norm(xn))
-        x = xn
+        B.combine(y)
         if res < tol:
             break
-    return theta, x, EigInfo(res, it, numops, res < tol)
+    out = SymTensor(x0.charge, x0.space, B.V[0, :B.n].clone(), x0.struct)
+    return theta, out, EigInfo(res, it, numops, res < tol)
 
 
 class HeffOperator:
@@ -260,11 +360,13 @@ class HeffOperator:
         return self._apply(x, "f64", self.ops64)
 
 
-def eigsolve(H: HeffOperator, x0: SymTensor, tol=1e-7, krylovdim=5, maxiter=8, refine_tol=None):
-    """Lowest eigenpair; reduced-precision cycles then (optionally) FP64 refinement cycles."""
-    theta, x, info = lanczos_lowest(H, x0, tol, krylovdim, maxiter)
+def eigsolve(H: HeffOperator, x0: SymTensor, tol=1e-7, krylovdim=5, maxiter=8, refine_tol=None, graphs=None):
+    """Lowest eigenpair; reduced-precision cycles then (optionally) FP64 refinement cycles.
+    ``graphs`` (default ``LANCZOS_GRAPHS``) captures each Lanczos iteration in a hipGraph."""
+    graphs = LANCZOS_GRAPHS if graphs is None else graphs
+    theta, x, info = lanczos_lowest(H, x0, tol, krylovdim, maxiter, graphs=graphs)
     if H.prec != "f64" and refine_tol is not None:
-        theta, x, info2 = lanczos_lowest(H.exact, x, refine_tol, krylovdim, max(1, maxiter // 2))
+        theta, x, info2 = lanczos_lowest(H.exact, x, refine_tol, krylovdim, max(1, maxiter // 2), graphs=graphs)
         info = EigInfo(info2.normres, info.numiter + info2.numiter, info.numops + info2.numops, info2.converged)
     return theta, x, info
 

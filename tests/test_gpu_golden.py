@@ -32,7 +32,8 @@ def flat_of(O):
 @pytest.mark.parametrize("path,prec,bar", [("fused", "f64", 1e-12), ("plain", None, 1e-12), ("fused", "f32", 1e-5)])
 def test_heff_vs_golden(path, prec, bar):
     T, _, _ = load_fixture("heff_xxz_L32_chi64.npz")
-    ins = [dev(T[k]) for k in ("v", "envL", "envR", "WL", "WR")]
+    dt = torch.float32 if prec == "f32" else torch.float64
+    ins = [dev(T[k], dt) for k in ("v", "envL", "envR", "WL", "WR")]
     kw = {"path": path} if prec is None else {"path": path, "prec": prec}
     out = M._applymps2site(*ins, **kw)
     want = flat_of(T["out"])
