@@ -309,6 +309,11 @@ __global__ __launch_bounds__(256) void gemm_grouped(
     int64_t krem[DEPTH];
     bool full[DEPTH];  // chunk needs no K or row masking (uniform over the work group)
     const bool rows_full = tile.m0 + BM <= p.M && tile.n0 + BN <= p.N;
+    // 16x16 sub-tiles of this wave that touch the problem (uniform over the wave)
+    const int wr0 = tile.m0 + wm * (BM / 2), wc0 = tile.n0 + wn * (BN / 2);
+    const int wvi = wr0 >= p.M ? 0 : min(SM, (p.M - wr0 + 15) / 16);
+    const int wvj = wc0 >= p.N ? 0 : min(SN, (p.N - wc0 + 15) / 16);
+    const bool wave_full = wvi == SM && wvj == SN;
     auto load = [&](auto D) {
       constexpr int d = decltype(D)::value;
       const int64_t k = k0 + VEC * q;
@@ -349,21 +354,42 @@ __global__ __launch_bounds__(256) void gemm_grouped(
       }
       __syncthreads();
       if (it + DEPTH < nch) load(D);
+      if (wave_full) {
 #pragma unroll
-      for (int kc = 0; kc < BK; kc += 4 * KL) {
-        // operands of all sub-tiles first, then the MFMAs k-slice-outer so that
-        // consecutive MFMAs accumulate into different sub-tiles (no dependent chain)
-        typename Frag<MODE>::F fa[SM], fb[SN];
+        for (int kc = 0; kc < BK; kc += 4 * KL) {
+          // operands of all sub-tiles first, then the MFMAs k-slice-outer so that
+          // consecutive MFMAs accumulate into different sub-tiles (no dependent chain)
+          typename Frag<MODE>::F fa[SM], fb[SN];
 #pragma unroll
-        for (int j = 0; j < SN; ++j) fb[j] = Frag<MODE>::load(&Bs[lb][wn * (BN / 2) + j * 16 + l16][kc + KL * g]);
+          for (int j = 0; j < SN; ++j) fb[j] = Frag<MODE>::load(&Bs[lb][wn * (BN / 2) + j * 16 + l16][kc + KL * g]);
 #pragma unroll
-        for (int i = 0; i < SM; ++i) fa[i] = Frag<MODE>::load(&As[lb][wm * (BM / 2) + i * 16 + l16][kc + KL * g]);
+          for (int i = 0; i < SM; ++i) fa[i] = Frag<MODE>::load(&As[lb][wm * (BM / 2) + i * 16 + l16][kc + KL * g]);
 #pragma unroll
-        for (int kk = 0; kk < Frag<MODE>::NK; ++kk)
+          for (int kk = 0; kk < Frag<MODE>::NK; ++kk)
+#pragma unroll
+            for (int i = 0; i < SM; ++i)
+#pragma unroll
+              for (int j = 0; j < SN; ++j) Frag<MODE>::mfma(acc[i][j], fb[j], fa[i], kk);
+        }
+      } else {
+        // edge tile: skip the 16x16 sub-tiles that lie wholly outside the problem
+#pragma unroll
+        for (int kc = 0; kc < BK; kc += 4 * KL) {
+          typename Frag<MODE>::F fa[SM], fb[SN];
+#pragma unroll
+          for (int j = 0; j < SN; ++j)
+            if (j < wvj) fb[j] = Frag<MODE>::load(&Bs[lb][wn * (BN / 2) + j * 16 + l16][kc + KL * g]);
 #pragma unroll
           for (int i = 0; i < SM; ++i)
+            if (i < wvi) fa[i] = Frag<MODE>::load(&As[lb][wm * (BM / 2) + i * 16 + l16][kc + KL * g]);
 #pragma unroll
-            for (int j = 0; j < SN; ++j) Frag<MODE>::mfma(acc[i][j], fb[j], fa[i], kk);
+          for (int kk = 0; kk < Frag<MODE>::NK; ++kk)
+#pragma unroll
+            for (int i = 0; i < SM; ++i)
+#pragma unroll
+              for (int j = 0; j < SN; ++j)
+                if (i < wvi && j < wvj) Frag<MODE>::mfma(acc[i][j], fb[j], fa[i], kk);
+        }
       }
     };
     typedef std::integral_constant<int, 0> D0;
