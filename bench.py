@@ -63,13 +63,15 @@ def calibrate_copy(mib):
     print(json.dumps({"calibrate_copy_mib": mib, "bytes_each_way": n * 4}), flush=True)
 
 
-def time_fused(plan, prec, ts, steps, warmup, stage_events=True):
-    """Prepared-operator matvecs: returns (ms_per_step, per-stage ms, out, prepare_ms)."""
+def time_fused(plan, prec, ts, steps, warmup, stage_events=True, tune=True):
+    """Prepared-operator matvecs (GEMM variants autotuned first): returns
+    (ms_per_step, per-stage ms, out, prepare_ms, tuned variants)."""
     import torch
     v, L, R, WL, WR = ts
     out = torch.empty(plan.out.numel, dtype=v.dtype, device=v.device)
     ws = plan.workspace(prec, v.device, "apply")
     prepared = plan.prepare(prec, L.data, R.data, WL.data, WR.data)
+    tuned = plan.tune(prec, prepared, v.data, out, ws, reps=5) if tune else None
     torch.cuda.synchronize()
     p0, p1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
     p0.record()
@@ -110,7 +112,7 @@ def time_fused(plan, prec, ts, steps, warmup, stage_events=True):
             for k in range(4):
                 stages[k] += e[k].elapsed_time(e[k + 1])
         stages = [s / steps for s in stages]
-    return max(total_ms, wall * 1e3) / steps, stages, out, prep_ms
+    return max(total_ms, wall * 1e3) / steps, stages, out, prep_ms, tuned
 
 
 def time_plain(ts, steps, warmup):
@@ -265,7 +267,7 @@ def main():
     ts64 = heff_workload(args.workload, dtype=torch.float64, device=dev)
     ts = [t.to(dt) for t in ts64] if dt != torch.float64 else ts64
     plan = heff_plan(*ts)
-    ms, stages, out, prep_ms = time_fused(plan, prec, ts, args.steps, args.warmup)
+    ms, stages, out, prep_ms, tuned = time_fused(plan, prec, ts, args.steps, args.warmup)
     value = 1e3 / ms
     F = plan.ref_flops
     # dominant kernel: the step-1 grouped GEMM launch (reference-order 2*M*N*K of
@@ -295,6 +297,7 @@ def main():
         "per_kernel_ms": {"pack_v": round(stages[0], 5), "step1_gemm": round(stages[1], 5),
                           "mpo_pass": round(stages[2], 5), "step4_gemm": round(stages[3], 5)},
         "prepare_ms_per_bond": round(prep_ms, 5),
+        "autotuned_gemm_variants": tuned,
         "roofline": {"bound": "mfma", "kernel": f"gemm_grouped<{ {'f64': 0, 'f32': 1, 'bf16': 2}[prec] },...> step-1 launch (v^T envL)",
                      "achieved": round(achieved / 1e12, 3), "peak": PEAK[prec] / 1e12, "unit": "TFLOP/s",
                      "frac": round(achieved / PEAK[prec], 4), "traffic": TRAFFIC.get((args.workload, prec)),
@@ -317,7 +320,7 @@ def main():
                 continue
             t2 = ts64 if p2 == "f64" else [t.to(torch.float32) for t in ts64]
             pl2 = heff_plan(*t2)
-            ms2, st2, _, _ = time_fused(pl2, p2, t2, max(10, args.steps // 4), 5)
+            ms2, st2, _, _, _ = time_fused(pl2, p2, t2, max(10, args.steps // 4), 5)
             extra[f"fused_{p2}_matvecs_per_s"] = round(1e3 / ms2, 3)
         ms_plain = time_plain(ts64, max(3, args.steps // 20), 2)
         extra["plain_f64_matvecs_per_s"] = round(1e3 / ms_plain, 3)

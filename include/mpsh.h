@@ -140,6 +140,17 @@ mpsh_status mpsh_heff_apply_prepared(mpsh_heff plan, mpsh_dtype dt, const void* 
  * no reference counterpart. */
 mpsh_status mpsh_heff_apply_prepared_stages(mpsh_heff plan, mpsh_dtype dt, int32_t stages, const void* prepared,
                                             const void* v, void* out, void* workspace, void* stream);
+/* Autotuner over the plan's block-size histogram: times every candidate tile
+ * variant of the step-1 GEMM and every (tile, split-K) variant of the step-4
+ * GEMM on the plan's real problems (reps launches each, HIP events on `stream`,
+ * synchronises), keeps the fastest for dtype dt in the plan, and records it
+ * under the plan's histogram signature so that later plans with the same
+ * signature adopt it without timing (force = 1 re-times).  choice[9] (may be
+ * NULL): step-1 BM, BN, KS, 0, step-4 BM, BN, KS, split target, 1 if taken from
+ * the signature cache.  Buffers as for mpsh_heff_apply_prepared; the output
+ * and workspace contents are garbage afterwards.  No reference counterpart. */
+mpsh_status mpsh_heff_tune(mpsh_heff plan, mpsh_dtype dt, const void* prepared, const void* v, void* out,
+                           void* workspace, int32_t reps, int32_t force, int32_t* choice, void* stream);
 void mpsh_heff_destroy(mpsh_heff plan);
 
 /* ------------------------------------------------------------------------ */

@@ -78,6 +78,7 @@ _SIGS = {
     "mpsh_heff_prepare": (C.c_int, [_p, C.c_int, _p, _p, _p, _p, _p, _p]),
     "mpsh_heff_apply_prepared": (C.c_int, [_p, C.c_int, _p, _p, _p, _p, _p]),
     "mpsh_heff_apply_prepared_stages": (C.c_int, [_p, C.c_int, _i32, _p, _p, _p, _p, _p]),
+    "mpsh_heff_tune": (C.c_int, [_p, C.c_int, _p, _p, _p, _p, _i32, _i32, _pi32, _p]),
     "mpsh_heff_destroy": (None, [_p]),
     "mpsh_svd_plan": (C.c_int, [_i64, _pi64, _pi64, _pi64, C.POINTER(_p)]),
     "mpsh_svd_sizes": (C.c_int, [_p, _pi64, _pi64, _pi64, _pi64, _pi64]),

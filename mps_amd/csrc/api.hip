@@ -11,6 +11,8 @@
 #include <memory>
 #include <mutex>
 #include <string>
+#include <tuple>
+#include <cmath>
 
 #include "../../include/mpsh.h"
 #include "kernels.hpp"
@@ -219,15 +221,40 @@ struct mpsh_contract_s {
 
 struct HeffDev {
   std::unique_ptr<DevTables> t;
-  DevGemm g1, g4, g1h, g4h, g1d, g4d;  // FP32, BF16 and FP64 kernel variants
   DevMpo mpo;
   DevCopy pack_v, pack_envL, pack_envR;
 };
+
+// a GEMM kernel variant of one H_eff stage: (stage 1|4, mode, BM, BN, KS, split target)
+typedef std::tuple<int, int, int, int, int, int64_t> VariantKey;
+
+// Tuned variant choice per plan signature (process-wide): plans with the same
+// block-size histogram signature reuse the timing of the first one.
+static std::mutex g_tuned_mu;
+static std::map<std::string, std::pair<VariantKey, VariantKey>> g_tuned;
 
 struct mpsh_heff_s {
   HeffPlan p;
   std::mutex mu;
   std::map<int, HeffDev> dev;
+  // per device: kernel variants uploaded so far; per mode: the active step-1 / step-4 variant
+  std::map<std::pair<int, VariantKey>, std::pair<std::unique_ptr<DevTables>, DevGemm>> var;
+  VariantKey sel1[3], sel4[3];
+  bool tuned[3] = {false, false, false};
+  const DevGemm& variant(const VariantKey& k) {
+    std::lock_guard<std::mutex> g(mu);
+    const int d = current_device();
+    auto it = var.find({d, k});
+    if (it == var.end()) {
+      const int stage = std::get<0>(k), mode = std::get<1>(k);
+      const TileCfg c{std::get<2>(k), std::get<3>(k), std::get<4>(k)};
+      GemmPlan gp = gemm_variant(stage == 1 ? p.g1 : p.g4, c, std::get<5>(k), mode);
+      auto t = std::make_unique<DevTables>();
+      DevGemm dg = t->gemm(gp);
+      it = var.emplace(std::make_pair(d, k), std::make_pair(std::move(t), dg)).first;
+    }
+    return it->second.second;
+  }
   const HeffDev& get() {
     std::lock_guard<std::mutex> g(mu);
     int d = current_device();
@@ -235,12 +262,6 @@ struct mpsh_heff_s {
     if (it == dev.end()) {
       HeffDev h;
       h.t = std::make_unique<DevTables>();
-      h.g1 = h.t->gemm(p.g1);
-      h.g4 = h.t->gemm(p.g4);
-      h.g1h = h.t->gemm(p.g1h);
-      h.g4h = h.t->gemm(p.g4h);
-      h.g1d = h.t->gemm(p.g1d);
-      h.g4d = h.t->gemm(p.g4d);
       h.mpo.sigs = h.t->up(p.sigs);
       h.mpo.terms = h.t->up(p.terms);
       h.mpo.pairs = h.t->up(p.pairs);
@@ -428,9 +449,16 @@ mpsh_status mpsh_heff_plan(mpsh_struct v, mpsh_struct envL, mpsh_struct envR, mp
   if (!v || !envL || !envR || !WL || !WR || !plan || !out) throw Error(ERR_INVALID, "null argument");
   mpsh_heff h = new mpsh_heff_s;
   try {
-    h->p = heff_plan(v->s, envL->s, envR->s, WL->s, WR->s, tiles_from_env());
+    const HeffTiles t = tiles_from_env();
+    h->p = heff_plan(v->s, envL->s, envR->s, WL->s, WR->s, t);
     if (h->p.maxR > 64 || h->p.maxC > 64)
       throw Error(ERR_UNSUPPORTED, "combined MPO group wider than 64 channels");
+    const TileCfg* c1[3] = {&t.s1d, &t.s1, &t.s1h};
+    const TileCfg* c4[3] = {&t.s4d, &t.s4, &t.s4h};
+    for (int m = 0; m < 3; ++m) {
+      h->sel1[m] = VariantKey{1, m, c1[m]->BM, c1[m]->BN, c1[m]->KS, 0};
+      h->sel4[m] = VariantKey{4, m, c4[m]->BM, c4[m]->BN, c4[m]->KS, t.split_target};
+    }
   } catch (...) {
     delete h;
     throw;
@@ -509,10 +537,10 @@ mpsh_status mpsh_heff_apply_prepared_stages(mpsh_heff plan, mpsh_dtype dt, int32
   void* T1 = ws + p.t1_off * es;
   void* T3 = ws + p.t3_off * es;
   void* Pt = ws + p.part_off * es;
-  const bool f64 = dt == MPSH_F64;
+  const int mode = gemm_mode(dt);
   if (stages & 1) check(launch_copy(d.pack_v, es, v, vp, s), "pack v");
-  const DevGemm& g1 = f64 ? d.g1d : (dt == MPSH_BF16 ? d.g1h : d.g1);
-  const DevGemm& g4 = f64 ? d.g4d : (dt == MPSH_BF16 ? d.g4h : d.g4);
+  const DevGemm& g1 = plan->variant(plan->sel1[mode]);
+  const DevGemm& g4 = plan->variant(plan->sel4[mode]);
   if (stages & 2) check(launch_gemm(g1, gemm_mode(dt), vp, pr + p.el_off * es, T1, Pt, s), "step-1 gemm");
   if (stages & 4) check(launch_mpo_apply(d.mpo, es, pr + p.pw_off * es, T1, T3, s), "mpo");
   if (stages & 8) check(launch_gemm(g4, gemm_mode(dt), T3, pr + p.er_off * es, out, Pt, s), "step-4 gemm");
@@ -523,6 +551,117 @@ mpsh_status mpsh_heff_apply_prepared_stages(mpsh_heff plan, mpsh_dtype dt, int32
 mpsh_status mpsh_heff_apply_prepared(mpsh_heff plan, mpsh_dtype dt, const void* prepared, const void* v,
                                      void* out, void* workspace, void* stream) {
   return mpsh_heff_apply_prepared_stages(plan, dt, 15, prepared, v, out, workspace, stream);
+}
+
+// Block-size histogram signature of a plan's two GEMMs (totals and averages in
+// factor-2 buckets), the key under which tuned variants are shared between plans.
+static std::string heff_signature(const HeffPlan& p, int mode) {
+  auto q = [](double x) { return x > 0 ? (long)std::lround(std::log2(x)) : -1L; };  // factor-2 buckets
+  auto stat = [&](const GemmPlan& g) {
+    double M = 0, N = 0, K = 0, n = 0;
+    for (const GemmProb& pr : g.probs) {
+      double k = 0;
+      for (int s2 = 0; s2 < pr.nseg; ++s2) k += (double)g.segs[pr.seg0 + s2].K;
+      M += pr.M;
+      N += pr.N;
+      K += k;
+      n += 1;
+    }
+    n = std::max(n, 1.0);
+    return std::to_string(q(g.flops)) + "," + std::to_string(q(M / n)) + "," + std::to_string(q(N / n)) + "," +
+           std::to_string(q(K / n)) + "," + std::to_string(q(n));
+  };
+  return std::to_string(mode) + "|" + stat(p.g1) + "|" + stat(p.g4);
+}
+
+mpsh_status mpsh_heff_tune(mpsh_heff plan, mpsh_dtype dt, const void* prepared, const void* v, void* out,
+                           void* workspace, int32_t reps, int32_t force, int32_t* choice, void* stream) {
+  API_TRY
+  check_dtype(dt);
+  if (!plan) throw Error(ERR_INVALID, "null plan");
+  const HeffPlan& p = plan->p;
+  if ((!v && p.v.numel()) || (!out && p.out.numel()) || (!prepared && p.prep_elems) || !workspace)
+    throw Error(ERR_INVALID, "null argument");
+  need_aligned(prepared, "prepared");
+  need_aligned(workspace, "workspace");
+  const int mode = gemm_mode(dt);
+  const std::string sig = heff_signature(p, mode);
+  bool cached = false;
+  if (!force) {
+    std::lock_guard<std::mutex> g(g_tuned_mu);
+    auto it = g_tuned.find(sig);
+    if (it != g_tuned.end()) {
+      plan->sel1[mode] = it->second.first;
+      plan->sel4[mode] = it->second.second;
+      cached = true;
+    }
+  }
+  if (!cached && p.g1.probs.size() + p.g4.probs.size() > 0) {
+    hipStream_t s = (hipStream_t)stream;
+    const HeffDev& d = plan->get();
+    const int es = elem_size(dt);
+    const char* pr = (const char*)prepared;
+    char* ws = (char*)workspace;
+    void* vp = ws + p.vp_off * es;
+    void* T1 = ws + p.t1_off * es;
+    void* T3 = ws + p.t3_off * es;
+    void* Pt = ws + p.part_off * es;
+    reps = std::max(1, std::min(reps, 50));
+    hipEvent_t e0, e1;
+    check(hipEventCreate(&e0), "hipEventCreate");
+    check(hipEventCreate(&e1), "hipEventCreate");
+    auto timeit = [&](const DevGemm& g, const void* A, const void* B, void* C) {
+      check(launch_gemm(g, mode, A, B, C, Pt, s), "tune gemm");  // warm
+      check(hipEventRecord(e0, s), "hipEventRecord");
+      for (int r = 0; r < reps; ++r) check(launch_gemm(g, mode, A, B, C, Pt, s), "tune gemm");
+      check(hipEventRecord(e1, s), "hipEventRecord");
+      check(hipEventSynchronize(e1), "hipEventSynchronize");
+      float ms = 0;
+      check(hipEventElapsedTime(&ms, e0, e1), "hipEventElapsedTime");
+      return ms;
+    };
+    check(launch_copy(d.pack_v, es, v, vp, s), "pack v");
+    float best = 1e30f;
+    for (const TileCfg& c : kTuneTiles) {
+      if (!tune_tile_ok(mode, c)) continue;
+      const VariantKey k{1, mode, c.BM, c.BN, c.KS, 0};
+      const float t = timeit(plan->variant(k), vp, pr + p.el_off * es, T1);
+      if (t < best) {
+        best = t;
+        plan->sel1[mode] = k;
+      }
+    }
+    check(launch_mpo_apply(d.mpo, es, pr + p.pw_off * es, T1, T3, s), "mpo");
+    best = 1e30f;
+    for (const TileCfg& c : kTuneTiles) {
+      if (!tune_tile_ok(mode, c)) continue;
+      for (int64_t sp : kTuneSplits) {
+        const VariantKey k{4, mode, c.BM, c.BN, c.KS, sp};
+        const float t = timeit(plan->variant(k), T3, pr + p.er_off * es, out);
+        if (t < best) {
+          best = t;
+          plan->sel4[mode] = k;
+        }
+      }
+    }
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    std::lock_guard<std::mutex> g(g_tuned_mu);
+    g_tuned[sig] = {plan->sel1[mode], plan->sel4[mode]};
+  }
+  plan->tuned[mode] = true;
+  if (choice) {
+    const VariantKey* ks[2] = {&plan->sel1[mode], &plan->sel4[mode]};
+    for (int i = 0; i < 2; ++i) {
+      choice[4 * i + 0] = std::get<2>(*ks[i]);
+      choice[4 * i + 1] = std::get<3>(*ks[i]);
+      choice[4 * i + 2] = std::get<4>(*ks[i]);
+      choice[4 * i + 3] = (int32_t)std::get<5>(*ks[i]);
+    }
+    choice[8] = cached ? 1 : 0;
+  }
+  return MPSH_OK;
+  API_CATCH
 }
 
 mpsh_status mpsh_heff_apply(mpsh_heff plan, mpsh_dtype dt, const void* v, const void* envL, const void* envR,

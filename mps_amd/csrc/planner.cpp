@@ -273,6 +273,15 @@ void GemmPlan::choose_splits(int64_t target, int BK, int64_t min_chunks_per_spli
   }
 }
 
+GemmPlan gemm_variant(const GemmPlan& base, const TileCfg& c, int64_t split_target, int mode) {
+  GemmPlan g = base;
+  g.BM = c.BM;
+  g.BN = c.BN;
+  g.KS = c.KS;
+  g.choose_splits(split_target, chunk_k(mode, c.KS), 32);  // target 0: no split-K
+  return g;
+}
+
 std::vector<GemmTile> GemmPlan::tiles(std::vector<int32_t>* qoff) const {
   // Panel pieces: the tiles of one (problem, split) over a contiguous range of
   // m0 rows (all n0).  Tiles of a piece share A rows (same m0) and B rows (same
@@ -930,10 +939,14 @@ HeffPlan heff_plan(const Struct& v, const Struct& envL, const Struct& envR, cons
       const int64_t vbase = vp_elems, ebase = el_elems;
       vp_elems += Mf * Kp;
       el_elems += Nf * Kp;
+      // v block (dk, d1, d2, dR) -> packed rows ordered ir + dR*(i1 + d1*i2): the
+      // right-bond index runs fastest, so the MPO pass reads T1 with unit stride
       for (size_t i = 0; i < X.size(); ++i) {
         const int64_t* z = v.size(X[i]);
-        const int64_t dims[2] = {dk, z[1] * z[2] * z[3]}, sst[2] = {1, dk}, dst[2] = {1, Kp};
-        H.pack_v.add(2, dims, sst, dst, v.offsets[X[i]], vbase + rowoff[i] * Kp);
+        const int64_t d1 = z[1], d2 = z[2], dR = z[3];
+        const int64_t dims[4] = {dk, d1, d2, dR}, sst[4] = {1, dk, dk * d1, dk * d1 * d2},
+                      dst[4] = {1, dR * Kp, dR * d1 * Kp, Kp};
+        H.pack_v.add(4, dims, sst, dst, v.offsets[X[i]], vbase + rowoff[i] * Kp);
       }
       for (size_t j = 0; j < E.size(); ++j) {
         const int64_t* z = envL.size(E[j]);
@@ -1129,10 +1142,10 @@ HeffPlan heff_plan(const Struct& v, const Struct& envL, const Struct& envR, cons
     for (size_t i = gi0; i < gi1; ++i) {
       const T1B& t = t1[i];
       if (t.db != db || t.dR != dR) throw Error(ERR_SECTOR_MISMATCH, "heff: pixel group dims differ");
-      // column order c = jw + Dw*(i1 + d1*i2); T1 element (i1 + d1 i2 + d1 d2 n, jw + Dw m)
+      // column order c = jw + Dw*(i1 + d1*i2); T1 element (n + dR (i1 + d1 i2), jw + Dw m)
       for (int64_t c = 0; c < t.Dw * t.d1 * t.d2; ++c) {
         const int64_t jw = c % t.Dw, i1 = (c / t.Dw) % t.d1, i2 = c / (t.Dw * t.d1);
-        H.inrows.push_back(MpoInRow{t.base + t.rowoff + i1 + t.d1 * i2 + t.Mf * (t.coloff + jw), t.d1 * t.d2,
+        H.inrows.push_back(MpoInRow{t.base + t.rowoff + t.dR * (i1 + t.d1 * i2) + t.Mf * (t.coloff + jw), 1,
                                     t.Mf * t.Dw});
       }
     }
@@ -1183,7 +1196,14 @@ HeffPlan heff_plan(const Struct& v, const Struct& envL, const Struct& envR, cons
   H.t1_off = up64(vp_elems);
   H.t3_off = up64(H.t1_off + H.t1_elems);
   H.part_off = up64(H.t3_off + H.t3_elems);
-  H.ws_elems = H.part_off + std::max({H.g4.part_elems, H.g4h.part_elems, H.g4d.part_elems});
+  // partials sized for the largest split-K layout any tuning candidate may pick
+  int64_t part = std::max({H.g4.part_elems, H.g4h.part_elems, H.g4d.part_elems});
+  for (int mode = 0; mode < 3; ++mode)
+    for (const TileCfg& c : kTuneTiles) {
+      if (!tune_tile_ok(mode, c)) continue;
+      for (int64_t sp : kTuneSplits) part = std::max(part, gemm_variant(H.g4, c, sp, mode).part_elems);
+    }
+  H.ws_elems = H.part_off + part;
   return H;
 }
 

@@ -289,6 +289,18 @@ struct HeffTiles {
   int64_t split_target = 1024;               // minimum work items for the step-4 GEMM
 };
 
+// Autotuner candidates for the two H_eff GEMMs (tile BM x BN x KS; step-4
+// split-K work-item targets).  FP64 uses the KS = 1 tiles up to 128 wide only.
+constexpr TileCfg kTuneTiles[] = {{64, 64, 1}, {64, 128, 1}, {128, 64, 1}, {64, 64, 2}, {128, 128, 1}, {128, 64, 2}};
+constexpr int64_t kTuneSplits[] = {512, 1024, 2048};
+inline bool tune_tile_ok(int mode, const TileCfg& c) {
+  return mode != 0 || (c.KS == 1 && c.BM * c.BN <= 128 * 64);
+}
+// K chunk of a GEMM mode (elements): 16 * KS for FP64 / FP32, 32 * KS for BF16
+inline int chunk_k(int mode, int KS) { return (mode == 2 ? 32 : 16) * KS; }
+// the step-4 GEMM of a plan with another tile and split target
+GemmPlan gemm_variant(const GemmPlan& base, const TileCfg& c, int64_t split_target, int mode);
+
 struct HeffPlan {
   Struct v, envL, envR, WL, WR, out;
   GemmPlan g1;  // A = v, B = envL, C = T1 (ws); FP32/BF16 tile variant

@@ -76,6 +76,18 @@ class HeffPlan:
                                       _ptr(prepared), stream if stream is not None else _stream()))
         return prepared
 
+    def tune(self, prec: str, prepared: torch.Tensor, v: torch.Tensor, out: torch.Tensor, ws: torch.Tensor = None,
+             reps: int = 3, force: bool = False, stream=None) -> dict:
+        """Autotune the two GEMMs' tile / split-K variants for ``prec`` on this plan's
+        real problems (``mpsh_heff_tune``); plans with the same block-size histogram
+        signature reuse the result.  Overwrites ``out`` and the workspace."""
+        ws = ws if ws is not None else self.workspace(prec, v.device, "apply")
+        ch = (L._i32 * 9)()
+        check(lib().mpsh_heff_tune(self.plan, _PREC[prec], _ptr(prepared), _ptr(v), _ptr(out), _ptr(ws), reps,
+                                   int(force), ch, stream if stream is not None else _stream()))
+        return {"step1": f"{ch[0]}x{ch[1]}x{ch[2]}", "step4": f"{ch[4]}x{ch[5]}x{ch[6]}", "split": int(ch[7]),
+                "from_cache": bool(ch[8])}
+
     def apply_prepared(self, prec: str, prepared: torch.Tensor, v: torch.Tensor, out: torch.Tensor,
                        ws: torch.Tensor = None, stream=None):
         ws = ws if ws is not None else self.workspace(prec, v.device, "apply")
@@ -220,6 +232,8 @@ class EigInfo:
 
 
 LANCZOS_GRAPHS = False  # capture each Lanczos iteration in a hipGraph (eigsolve default)
+AUTOTUNE = True  # HeffOperator tunes the GEMM variants of each new plan (signature-cached)
+AUTOTUNE_MIN_FLOPS = 2e8  # ... when one matvec costs at least this many reference FLOPs
 
 
 class _LanczosBasis:
@@ -343,9 +357,11 @@ class HeffOperator:
     def _apply(self, x: SymTensor, prec: str, ops) -> SymTensor:
         plan = heff_plan(x, *ops)
         key = (prec, x.struct.key)
+        out = torch.empty(plan.out.numel, dtype=x.dtype, device=x.device)
         if key not in self._prep:
             self._prep[key] = plan.prepare(prec, *(t.data for t in ops))
-        out = torch.empty(plan.out.numel, dtype=x.dtype, device=x.device)
+            if AUTOTUNE and plan.ref_flops >= AUTOTUNE_MIN_FLOPS:
+                plan.tune(prec, self._prep[key], x.data, out)
         plan.apply_prepared(prec, self._prep[key], x.data, out)
         return SymTensor(plan.out.charge, plan.out.spaces, out, plan.out)
 

@@ -117,3 +117,31 @@ def test_env_updates_vs_oracle(name):
     wantR = ota.mpsupdateright(oracle_of(R), oracle_of(Ar), oracle_of(WR))
     _check_structure(gotR, wantR)
     assert rel_block_err(oracle_of(gotR), wantR) <= 1e-12
+
+
+@pytest.mark.parametrize("prec,bar", [("f64", 1e-12), ("f32", 1e-5), ("bf16", 3e-2)])
+def test_autotuned_variants_match_default(prec, bar):
+    """Every variant the autotuner may pick computes the same H_eff: the tuned plan's
+    output equals the default variants' output (FP64 1e-12; reduced precisions
+    against FP64), and a second plan with the same signature reuses the choice."""
+    from mps_amd.tensornetalgs import HeffPlan
+    ts64 = _workload("xxz_L100_chi512")
+    ts = ts64 if prec == "f64" else [t.to(torch.float32) for t in ts64]
+    plan = HeffPlan(*[t.struct for t in ts])
+    prepared = plan.prepare(prec, *(t.data for t in ts[1:]))
+    dt = ts[0].dtype
+    ref = torch.empty(plan.out.numel, dtype=dt, device=DEV)
+    plan.apply_prepared(prec, prepared, ts[0].data, ref)
+    want = M._applymps2site(*ts64, prec="f64").data
+    scratch = torch.empty_like(ref)
+    choice = plan.tune(prec, prepared, ts[0].data, scratch, reps=2, force=True)
+    assert not choice["from_cache"]
+    got = torch.empty_like(ref)
+    plan.apply_prepared(prec, prepared, ts[0].data, got)
+    for x in (ref, got):
+        assert float(torch.linalg.norm(x.double() - want)) <= bar * float(torch.linalg.norm(want))
+    if prec == "f64":
+        assert float(torch.linalg.norm(got - ref)) <= 1e-13 * float(torch.linalg.norm(ref))
+    plan2 = HeffPlan(*[t.struct for t in ts])
+    prep2 = plan2.prepare(prec, *(t.data for t in ts[1:]))
+    assert plan2.tune(prec, prep2, ts[0].data, scratch)["from_cache"]
