@@ -650,6 +650,112 @@ __global__ __launch_bounds__(256) void mpo_apply(const MpoSig* __restrict__ sigs
   }
 }
 
+// MFMA form of the pass: per 16-pixel tile, out[R x 16] = W[R x C] . in[C x 16]
+// with 16x16x4 MFMAs (FP64 or FP32): lane (l16, q) feeds W[16 rt + l16][4 s + q]
+// from LDS and in[4 s + q][pixel l16] straight from T1 (pixels run along the
+// right-bond index, so 16 lanes read 16 consecutive elements).  Each wave
+// covers 64 of the work item's 256 pixels.  KSM / RTM bound the plan's channel
+// and row counts (KSM*4 >= C, RTM*16 >= R); channels past C read row C-1 and
+// meet zero W columns, rows past R meet zero W rows and are not stored, so
+// every load and MFMA is unconditional.
+template <int MODE, int KSM, int RTM>
+__global__ __launch_bounds__(256) void mpo_apply_mfma(const MpoSig* __restrict__ sigs,
+                                                      const MpoGroup* __restrict__ groups,
+                                                      const MpoInRow* __restrict__ inrows,
+                                                      const MpoOutRow* __restrict__ outrows,
+                                                      const MpoWork* __restrict__ work,
+                                                      const typename GT<MODE>::T* __restrict__ W,
+                                                      const typename GT<MODE>::T* __restrict__ T1,
+                                                      typename GT<MODE>::T* __restrict__ T3) {
+  typedef typename GT<MODE>::T T;
+  typedef typename GT<MODE>::Acc Acc;
+  constexpr int LDW = 4 * KSM + 1;
+  __shared__ T Ws[RTM * 16 * LDW];
+  __shared__ int64_t ib[4 * KSM], im[4 * KSM], ob[RTM * 16], om[RTM * 16];
+  const MpoWork wk = work[blockIdx.x];
+  const MpoGroup g = groups[wk.group];
+  const MpoSig sg = sigs[g.sig];
+  const int R = sg.R, C = sg.C;
+  if (R <= 0 || C <= 0) return;  // uniform: the whole block leaves before any barrier
+  for (int i = threadIdx.x; i < RTM * 16 * LDW; i += blockDim.x) {
+    const int r = i / LDW, c = i - r * LDW;
+    Ws[i] = (r < R && c < C) ? W[sg.w_off + (int64_t)r * C + c] : T(0);
+  }
+  for (int i = threadIdx.x; i < 4 * KSM; i += blockDim.x) {
+    const MpoInRow qr = inrows[g.in0 + (i < C ? i : C - 1)];
+    ib[i] = qr.base;
+    im[i] = qr.mstride;
+  }
+  for (int i = threadIdx.x; i < RTM * 16; i += blockDim.x) {
+    if (i < R) {
+      const MpoOutRow qr = outrows[g.out0 + i];
+      ob[i] = qr.base;
+      om[i] = qr.mstride;
+    }
+  }
+  __syncthreads();
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int l16 = lane & 15, q = lane >> 4;
+  for (int t = 0; t < 4; ++t) {
+    const int64_t p0 = (int64_t)wk.pix0 + wave * 64 + t * 16;
+    if (p0 >= g.npix) break;  // uniform over the wave
+    const int64_t p = p0 + l16;
+    const bool live = p < g.npix;
+    const int64_t pc = live ? p : 0;
+    const int64_t m = pc / g.dR, n = pc - m * g.dR;
+    T b[KSM];
+#pragma unroll
+    for (int s2 = 0; s2 < KSM; ++s2) {
+      const int c = 4 * s2 + q;
+      b[s2] = T1[ib[c] + n + m * im[c]];
+    }
+    Acc acc[RTM];
+#pragma unroll
+    for (int rt = 0; rt < RTM; ++rt)
+#pragma unroll
+      for (int r = 0; r < 4; ++r) acc[rt][r] = 0;
+#pragma unroll
+    for (int s2 = 0; s2 < KSM; ++s2)
+#pragma unroll
+      for (int rt = 0; rt < RTM; ++rt) {
+        const T a = Ws[(rt * 16 + l16) * LDW + 4 * s2 + q];
+        if constexpr (MODE == 0)
+          acc[rt] = __builtin_amdgcn_mfma_f64_16x16x4f64(a, b[s2], acc[rt], 0, 0, 0);
+        else
+          acc[rt] = __builtin_amdgcn_mfma_f32_16x16x4f32(a, b[s2], acc[rt], 0, 0, 0);
+      }
+    if (live) {
+#pragma unroll
+      for (int rt = 0; rt < RTM; ++rt)
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+          const int row = rt * 16 + (MODE == 0 ? q + 4 * r : 4 * q + r);
+          if (row < R) T3[ob[row] + n + m * om[row]] = (T)acc[rt][r];
+        }
+    }
+  }
+}
+
+template <int MODE>
+static void launch_mpo_mfma(const DevMpo& d, const void* W, const void* T1, void* T3, dim3 grid, dim3 block,
+                            hipStream_t s) {
+  typedef typename GT<MODE>::T T;
+  const int ks = (d.cmax + 3) / 4, rt = (d.rmax + 15) / 16;
+#define MPSH_MPOM(K, RT)                                                                                        \
+  hipLaunchKernelGGL((mpo_apply_mfma<MODE, K, RT>), grid, block, 0, s, d.sigs, d.groups, d.inrows, d.outrows, \
+                     d.work, (const T*)W, (const T*)T1, (T*)T3)
+  if (rt <= 2) {
+    if (ks <= 4) MPSH_MPOM(4, 2);
+    else if (ks <= 8) MPSH_MPOM(8, 2);
+    else MPSH_MPOM(16, 2);
+  } else {
+    if (ks <= 4) MPSH_MPOM(4, 4);
+    else if (ks <= 8) MPSH_MPOM(8, 4);
+    else MPSH_MPOM(16, 4);
+  }
+#undef MPSH_MPOM
+}
+
 hipError_t launch_mpo_combine(const DevMpo& d, int elem_size, const void* WL, const void* WR, void* W,
                               hipStream_t s) {
   if (d.nterms == 0) return hipSuccess;
@@ -665,6 +771,13 @@ hipError_t launch_mpo_combine(const DevMpo& d, int elem_size, const void* WL, co
 hipError_t launch_mpo_apply(const DevMpo& d, int elem_size, const void* W, const void* T1, void* T3, hipStream_t s) {
   if (d.nwork == 0) return hipSuccess;
   dim3 grid(d.nwork), block(256);
+  if (d.mfma) {
+    if (elem_size == 8)
+      launch_mpo_mfma<0>(d, W, T1, T3, grid, block, s);
+    else
+      launch_mpo_mfma<1>(d, W, T1, T3, grid, block, s);
+    return hipGetLastError();
+  }
 #define MPSH_MPO(T, CM)                                                                                  \
   hipLaunchKernelGGL((mpo_apply<T, CM>), grid, block, 0, s, d.sigs, d.groups, d.inrows, d.outrows, d.work, \
                      (const T*)W, (const T*)T1, (T*)T3)

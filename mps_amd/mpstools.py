@@ -74,8 +74,10 @@ _SM = {(0, 1): 1.0}
 _ID = {(0, 0): 1.0, (1, 1): 1.0}
 
 
-def fsm_u1mpo(L: int, terms: Sequence[Tuple[int, int, float, float]], dtype=torch.float64, device=None) -> MPOperator:
-    """MPO of H = sum over terms (i, j, Jz, Jxy), 1 <= i < j <= L, of Jz Sz_i Sz_j + Jxy/2 (S+_i S-_j + h.c.).
+def fsm_u1mpo(L: int, terms: Sequence[Tuple[int, int, float, float]], dtype=torch.float64, device=None,
+              fields: Dict[int, float] = None, const: float = 0.0) -> MPOperator:
+    """MPO of H = sum over terms (i, j, Jz, Jxy), 1 <= i < j <= L, of Jz Sz_i Sz_j + Jxy/2 (S+_i S-_j + h.c.),
+    plus on-site fields sum_i h_i Sz_i and a constant (both on the before -> after channel).
 
     Bond states per charge: 0 -> [after, Sz channels by source site, before],
     +1 -> S+ channels, -1 -> S- channels; the two boundary bonds keep only
@@ -142,6 +144,10 @@ def fsm_u1mpo(L: int, terms: Sequence[Tuple[int, int, float, float]], dtype=torc
 
         put("before", "before", _ID, 1.0)
         put("after", "after", _ID, 1.0)
+        if fields and fields.get(s, 0.0) != 0.0:
+            put("before", "after", _SZ, fields[s])
+        if const != 0.0 and s == 1:
+            put("before", "after", _ID, const)
         if s in partners:
             if has_xy[s]:
                 put("before", ("p", s), _SP, 1.0)
@@ -159,6 +165,20 @@ def fsm_u1mpo(L: int, terms: Sequence[Tuple[int, int, float, float]], dtype=torc
                 put(("z", i), "after", _SZ, jz)
         Ws.append(SymTensor.from_flat(0, space, flat, dtype, device))
     return MPOperator(Ws)
+
+
+def tv_u1mpo(L: int, t: float = 1.0, V: float = 2.0, dtype=torch.float64, device=None) -> MPOperator:
+    """Spinless-fermion t-V chain H = -t sum (c+_i c_{i+1} + h.c.) + V sum n_i n_{i+1}, open,
+    U(1) particle number.  Jordan-Wigner (occupied = physical charge 1; nearest-neighbour
+    hopping carries no string): c+_i c_{i+1} + h.c. = S+_i S-_{i+1} + h.c. and
+    n_i n_j = Sz_i Sz_j + (Sz_i + Sz_j)/2 + 1/4, i.e. Jxy = -2t, Jz = V, fields V/2 per bond
+    end, constant V/4 per bond.  (No reference counterpart: the reference has no t-V MPO.)"""
+    fields: Dict[int, float] = {}
+    for i in range(1, L):
+        fields[i] = fields.get(i, 0.0) + V / 2
+        fields[i + 1] = fields.get(i + 1, 0.0) + V / 2
+    return fsm_u1mpo(L, [(i, i + 1, V, -2.0 * t) for i in range(1, L)], dtype, device, fields=fields,
+                     const=V / 4 * (L - 1))
 
 
 def xxz_u1mpo(lx: int, delta: float = 1.0, dtype=torch.float64, device=None) -> MPOperator:

@@ -84,3 +84,44 @@ def test_truncatedsizes_matches_oracle_greedy():
                 truncatedsizes(S, maxdim=maxdim, tol=tol)
             continue
         assert truncatedsizes(S, maxdim=maxdim, tol=tol) == list(want)
+
+
+def _dense_tv(L, t, V):
+    """Spinless fermions with explicit Jordan-Wigner strings, basis (empty, occupied) per
+    site, site 1 slowest: H = -t sum (c+_i c_{i+1} + h.c.) + V sum n_i n_{i+1}."""
+    a = np.array([[0.0, 1.0], [0.0, 0.0]])
+    n1 = np.diag([0.0, 1.0])
+    P = np.diag([1.0, -1.0])
+
+    def chain(ops):
+        out = np.eye(1)
+        for o in ops:
+            out = np.kron(out, o)
+        return out
+    c = [chain([P] * i + [a] + [np.eye(2)] * (L - i - 1)) for i in range(L)]
+    n = [chain([np.eye(2)] * i + [n1] + [np.eye(2)] * (L - i - 1)) for i in range(L)]
+    H = np.zeros((2 ** L, 2 ** L))
+    for i in range(L - 1):
+        H += -t * (c[i].T @ c[i + 1] + c[i + 1].T @ c[i]) + V * n[i] @ n[i + 1]
+    return H
+
+
+@pytest.mark.parametrize("L,t,V", [(4, 1.0, 2.0), (6, 1.0, 2.0), (5, 0.7, -1.3)])
+def test_tv_mpo_equals_fermion_hamiltonian(L, t, V):
+    """The t-V MPO (Jordan-Wigner, fields and constant on the FSM) equals the fermionic
+    Hamiltonian built with explicit JW strings, matrix element by matrix element."""
+    mpo = mt.tv_u1mpo(L, t, V, device=CPU)
+    Hm = ota.mpo_dense_chain(ota.MPOperator([_to_oracle(Wt) for Wt in mpo.Ws]))
+    np.testing.assert_allclose(Hm, _dense_tv(L, t, V), atol=1e-13)
+
+
+def test_tv_free_fermion_closed_form():
+    """V = 0: the N-particle ground energy of the open chain is the sum of the N lowest
+    single-particle levels -2t cos(k pi / (L + 1))."""
+    L, N, t = 8, 3, 1.0
+    H = _dense_tv(L, t, 0.0)
+    occ = np.array([bin(i).count("1") for i in range(2 ** L)])
+    idx = np.nonzero(occ == N)[0]
+    e = np.linalg.eigvalsh(H[np.ix_(idx, idx)])[0]
+    levels = np.sort(-2 * t * np.cos(np.arange(1, L + 1) * np.pi / (L + 1)))
+    assert abs(e - levels[:N].sum()) < 1e-12
