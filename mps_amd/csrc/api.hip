@@ -273,7 +273,6 @@ struct mpsh_heff_s {
       h.mpo.nwork = (int)p.work.size();
       h.mpo.cmax = (int)p.maxC;
       h.mpo.rmax = (int)p.maxR;
-      h.mpo.mfma = std::getenv("MPSH_MPO_FMA") == nullptr;  // A/B switch to the one-thread-per-pixel form
       h.pack_v = h.t->copy(p.pack_v);
       h.pack_envL = h.t->copy(p.pack_envL);
       h.pack_envR = h.t->copy(p.pack_envR);

@@ -586,7 +586,30 @@ __global__ __launch_bounds__(256) void mpo_combine(const MpoSig* __restrict__ si
   }
 }
 
-// out[r][pix] = sum_c W[r][c] * in[c][pix], pix = n + dR*m
+// One pixel's outputs: out[r] = sum_c W[r][c] in[c] for r < R, with the C inputs
+// in registers.  CM >= C is the group's channel class; channels past C read
+// row C-1 (a real row) and meet zero W columns, so the loads are unconditional.
+template <typename T, int CM, int CMAX>
+__device__ __forceinline__ void mpo_pixel(const T* Ws, const int64_t* ib, const int64_t* ins, const int64_t* im,
+                                          const int64_t* ob, const int64_t* om, int R, int64_t m, int64_t n,
+                                          bool live, const T* __restrict__ T1, T* __restrict__ T3) {
+  T v[CM];
+#pragma unroll
+  for (int c = 0; c < CM; ++c) v[c] = T1[ib[c] + n * ins[c] + m * im[c]];
+  if (!live) return;
+  for (int r = 0; r < R; ++r) {
+    const T* wr = Ws + r * CMAX;
+    T acc = 0;
+#pragma unroll
+    for (int c = 0; c < CM; ++c) acc += wr[c] * v[c];
+    T3[ob[r] + n + m * om[r]] = acc;
+  }
+}
+
+// out[r][pix] = sum_c W[r][c] * in[c][pix], pix = n + dR*m; one thread per pixel,
+// 256 pixels of one (bra_l, ket_r) group per work group.  CMAX bounds the
+// plan's channel count (LDS layout); each group runs the smallest channel
+// class (multiple of 4) that holds its own C.
 template <typename T, int CMAX>
 __global__ __launch_bounds__(256) void mpo_apply(const MpoSig* __restrict__ sigs,
                                                  const MpoGroup* __restrict__ groups,
@@ -601,22 +624,18 @@ __global__ __launch_bounds__(256) void mpo_apply(const MpoSig* __restrict__ sigs
   const MpoGroup g = groups[wk.group];
   const MpoSig sg = sigs[g.sig];
   const int R = sg.R, C = sg.C;
+  if (R <= 0) return;  // uniform: the whole block leaves before any barrier
   for (int i = threadIdx.x; i < R * CMAX; i += blockDim.x) {
     const int r = i / CMAX, c = i - r * CMAX;
     Ws[i] = c < C ? W[sg.w_off + (int64_t)r * C + c] : T(0);
   }
-  for (int i = threadIdx.x; i < CMAX; i += blockDim.x) {
-    if (i < C) {
-      const MpoInRow q = inrows[g.in0 + i];
+  if (C > 0)
+    for (int i = threadIdx.x; i < CMAX; i += blockDim.x) {
+      const MpoInRow q = inrows[g.in0 + (i < C ? i : C - 1)];
       ib[i] = q.base;
       ins[i] = q.nstride;
       im[i] = q.mstride;
-    } else {
-      ib[i] = 0;
-      ins[i] = 0;
-      im[i] = 0;
     }
-  }
   for (int i = threadIdx.x; i < R; i += blockDim.x) {
     const MpoOutRow q = outrows[g.out0 + i];
     ob[i] = q.base;
@@ -627,133 +646,30 @@ __global__ __launch_bounds__(256) void mpo_apply(const MpoSig* __restrict__ sigs
   const bool live = p0 < g.npix;
   const int64_t p = live ? p0 : 0;
   const int64_t m = p / g.dR, n = p - m * g.dR;
-  T v[CMAX];
-  if (C > 0) {
-    // every row index is clamped to a real input row and W is zero past C, so
-    // the loads are unconditional (no wait is forced before the FMAs)
-#pragma unroll
-    for (int c = 0; c < CMAX; ++c) {
-      const int cc = c < C ? c : C - 1;
-      v[c] = T1[ib[cc] + n * ins[cc] + m * im[cc]];
-    }
-  } else {
-#pragma unroll
-    for (int c = 0; c < CMAX; ++c) v[c] = 0;
+  if (C <= 0) {  // a group no input type reaches: its T3 cells are zero
+    if (live)
+      for (int r = 0; r < R; ++r) T3[ob[r] + n + m * om[r]] = T(0);
+    return;
   }
-  if (!live) return;
-  for (int r = 0; r < R; ++r) {
-    const T* wr = Ws + r * CMAX;
-    T acc = 0;
-#pragma unroll
-    for (int c = 0; c < CMAX; ++c) acc += wr[c] * v[c];
-    T3[ob[r] + n + m * om[r]] = acc;
+  const int cls = (C + 3) / 4;  // uniform over the block
+#define MPSH_PIX(K)                                                                             \
+  case K:                                                                                       \
+    if constexpr (4 * K <= CMAX) mpo_pixel<T, 4 * K, CMAX>(Ws, ib, ins, im, ob, om, R, m, n, live, T1, T3); \
+    break;
+  switch (cls) {
+    MPSH_PIX(1)
+    MPSH_PIX(2)
+    MPSH_PIX(3)
+    MPSH_PIX(4)
+    MPSH_PIX(5)
+    MPSH_PIX(6)
+    MPSH_PIX(7)
+    MPSH_PIX(8)
+    default:
+      if constexpr (CMAX == 64) mpo_pixel<T, 64, CMAX>(Ws, ib, ins, im, ob, om, R, m, n, live, T1, T3);
+      break;
   }
-}
-
-// MFMA form of the pass: per 16-pixel tile, out[R x 16] = W[R x C] . in[C x 16]
-// with 16x16x4 MFMAs (FP64 or FP32): lane (l16, q) feeds W[16 rt + l16][4 s + q]
-// from LDS and in[4 s + q][pixel l16] straight from T1 (pixels run along the
-// right-bond index, so 16 lanes read 16 consecutive elements).  Each wave
-// covers 64 of the work item's 256 pixels.  KSM / RTM bound the plan's channel
-// and row counts (KSM*4 >= C, RTM*16 >= R); channels past C read row C-1 and
-// meet zero W columns, rows past R meet zero W rows and are not stored, so
-// every load and MFMA is unconditional.
-template <int MODE, int KSM, int RTM>
-__global__ __launch_bounds__(256) void mpo_apply_mfma(const MpoSig* __restrict__ sigs,
-                                                      const MpoGroup* __restrict__ groups,
-                                                      const MpoInRow* __restrict__ inrows,
-                                                      const MpoOutRow* __restrict__ outrows,
-                                                      const MpoWork* __restrict__ work,
-                                                      const typename GT<MODE>::T* __restrict__ W,
-                                                      const typename GT<MODE>::T* __restrict__ T1,
-                                                      typename GT<MODE>::T* __restrict__ T3) {
-  typedef typename GT<MODE>::T T;
-  typedef typename GT<MODE>::Acc Acc;
-  constexpr int LDW = 4 * KSM + 1;
-  __shared__ T Ws[RTM * 16 * LDW];
-  __shared__ int64_t ib[4 * KSM], im[4 * KSM], ob[RTM * 16], om[RTM * 16];
-  const MpoWork wk = work[blockIdx.x];
-  const MpoGroup g = groups[wk.group];
-  const MpoSig sg = sigs[g.sig];
-  const int R = sg.R, C = sg.C;
-  if (R <= 0 || C <= 0) return;  // uniform: the whole block leaves before any barrier
-  for (int i = threadIdx.x; i < RTM * 16 * LDW; i += blockDim.x) {
-    const int r = i / LDW, c = i - r * LDW;
-    Ws[i] = (r < R && c < C) ? W[sg.w_off + (int64_t)r * C + c] : T(0);
-  }
-  for (int i = threadIdx.x; i < 4 * KSM; i += blockDim.x) {
-    const MpoInRow qr = inrows[g.in0 + (i < C ? i : C - 1)];
-    ib[i] = qr.base;
-    im[i] = qr.mstride;
-  }
-  for (int i = threadIdx.x; i < RTM * 16; i += blockDim.x) {
-    if (i < R) {
-      const MpoOutRow qr = outrows[g.out0 + i];
-      ob[i] = qr.base;
-      om[i] = qr.mstride;
-    }
-  }
-  __syncthreads();
-  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-  const int l16 = lane & 15, q = lane >> 4;
-  for (int t = 0; t < 4; ++t) {
-    const int64_t p0 = (int64_t)wk.pix0 + wave * 64 + t * 16;
-    if (p0 >= g.npix) break;  // uniform over the wave
-    const int64_t p = p0 + l16;
-    const bool live = p < g.npix;
-    const int64_t pc = live ? p : 0;
-    const int64_t m = pc / g.dR, n = pc - m * g.dR;
-    T b[KSM];
-#pragma unroll
-    for (int s2 = 0; s2 < KSM; ++s2) {
-      const int c = 4 * s2 + q;
-      b[s2] = T1[ib[c] + n + m * im[c]];
-    }
-    Acc acc[RTM];
-#pragma unroll
-    for (int rt = 0; rt < RTM; ++rt)
-#pragma unroll
-      for (int r = 0; r < 4; ++r) acc[rt][r] = 0;
-#pragma unroll
-    for (int s2 = 0; s2 < KSM; ++s2)
-#pragma unroll
-      for (int rt = 0; rt < RTM; ++rt) {
-        const T a = Ws[(rt * 16 + l16) * LDW + 4 * s2 + q];
-        if constexpr (MODE == 0)
-          acc[rt] = __builtin_amdgcn_mfma_f64_16x16x4f64(a, b[s2], acc[rt], 0, 0, 0);
-        else
-          acc[rt] = __builtin_amdgcn_mfma_f32_16x16x4f32(a, b[s2], acc[rt], 0, 0, 0);
-      }
-    if (live) {
-#pragma unroll
-      for (int rt = 0; rt < RTM; ++rt)
-#pragma unroll
-        for (int r = 0; r < 4; ++r) {
-          const int row = rt * 16 + (MODE == 0 ? q + 4 * r : 4 * q + r);
-          if (row < R) T3[ob[row] + n + m * om[row]] = (T)acc[rt][r];
-        }
-    }
-  }
-}
-
-template <int MODE>
-static void launch_mpo_mfma(const DevMpo& d, const void* W, const void* T1, void* T3, dim3 grid, dim3 block,
-                            hipStream_t s) {
-  typedef typename GT<MODE>::T T;
-  const int ks = (d.cmax + 3) / 4, rt = (d.rmax + 15) / 16;
-#define MPSH_MPOM(K, RT)                                                                                        \
-  hipLaunchKernelGGL((mpo_apply_mfma<MODE, K, RT>), grid, block, 0, s, d.sigs, d.groups, d.inrows, d.outrows, \
-                     d.work, (const T*)W, (const T*)T1, (T*)T3)
-  if (rt <= 2) {
-    if (ks <= 4) MPSH_MPOM(4, 2);
-    else if (ks <= 8) MPSH_MPOM(8, 2);
-    else MPSH_MPOM(16, 2);
-  } else {
-    if (ks <= 4) MPSH_MPOM(4, 4);
-    else if (ks <= 8) MPSH_MPOM(8, 4);
-    else MPSH_MPOM(16, 4);
-  }
-#undef MPSH_MPOM
+#undef MPSH_PIX
 }
 
 hipError_t launch_mpo_combine(const DevMpo& d, int elem_size, const void* WL, const void* WR, void* W,
@@ -771,23 +687,14 @@ hipError_t launch_mpo_combine(const DevMpo& d, int elem_size, const void* WL, co
 hipError_t launch_mpo_apply(const DevMpo& d, int elem_size, const void* W, const void* T1, void* T3, hipStream_t s) {
   if (d.nwork == 0) return hipSuccess;
   dim3 grid(d.nwork), block(256);
-  if (d.mfma) {
-    if (elem_size == 8)
-      launch_mpo_mfma<0>(d, W, T1, T3, grid, block, s);
-    else
-      launch_mpo_mfma<1>(d, W, T1, T3, grid, block, s);
-    return hipGetLastError();
-  }
 #define MPSH_MPO(T, CM)                                                                                  \
   hipLaunchKernelGGL((mpo_apply<T, CM>), grid, block, 0, s, d.sigs, d.groups, d.inrows, d.outrows, d.work, \
                      (const T*)W, (const T*)T1, (T*)T3)
   if (elem_size == 8) {
-    if (d.cmax <= 16) MPSH_MPO(double, 16);
-    else if (d.cmax <= 32) MPSH_MPO(double, 32);
+    if (d.cmax <= 32) MPSH_MPO(double, 32);
     else MPSH_MPO(double, 64);
   } else {
-    if (d.cmax <= 16) MPSH_MPO(float, 16);
-    else if (d.cmax <= 32) MPSH_MPO(float, 32);
+    if (d.cmax <= 32) MPSH_MPO(float, 32);
     else MPSH_MPO(float, 64);
   }
 #undef MPSH_MPO

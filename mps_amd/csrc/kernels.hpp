@@ -44,7 +44,6 @@ struct DevMpo {
   const MpoOutRow* outrows = nullptr;
   const MpoWork* work = nullptr;
   int nterms = 0, nwork = 0, cmax = 32, rmax = 32;
-  bool mfma = true;  // MFMA form of the pass (else one thread per pixel with FMAs)
 };
 
 hipError_t launch_copy(const DevCopy& c, int elem_size, const void* src, void* dst, hipStream_t s);
