@@ -159,6 +159,7 @@ def dmrg_window(nbonds, dev):
     ta.dmrg2site_window(mps, mpo, env, 1, **kw)
     timing = []
     torch.cuda.synchronize()
+    tc0 = dict(ta.TUNE_COUNTS)
     t1 = time.perf_counter()
     energies = ta.dmrg2site_window(mps, mpo, env, nbonds, timing=timing, **kw)
     torch.cuda.synchronize()
@@ -178,6 +179,7 @@ def dmrg_window(nbonds, dev):
            "sweep_s_extrapolated": round(2 * (L - 1) * ms_bond / 1e3, 2),
            "sweep_note": "2(L-1) bulk two-site updates x measured ms per bulk update (edge bonds are cheaper)",
            "setup_s": round(setup_s, 2), "energies": [round(e, 10) for e in energies],
+           "autotune_in_window": {k: ta.TUNE_COUNTS[k] - tc0[k] for k in tc0},
            "settings": {k: c[k] for k in ("maxdim", "tol", "lanczostol", "krylovdim", "krylovmaxiter", "prec",
                                            "refine_tol")}}
     return out, state

@@ -77,9 +77,32 @@ void ckb(rocblas_status e, const char* w) {
   if (e != rocblas_status_success) throw Error(ERR_DEVICE, std::string(w) + ": rocblas status " + std::to_string(e));
 }
 
-// one rocBLAS handle per device, used under a lock (handles are not thread-safe)
+// Per device: NLANE side streams, each with its own rocBLAS handle, so the
+// size buckets' gesdd calls (latency-bound chains of small kernels) run
+// concurrently.  Used under a lock (handles are not thread-safe).
+constexpr int NLANE = 4;
+struct Lanes {
+  hipStream_t st[NLANE];
+  rocblas_handle h[NLANE];
+  hipEvent_t done[NLANE];
+  hipEvent_t fork;
+};
 std::mutex g_handle_mu;
-std::map<int, rocblas_handle> g_handles;
+std::map<int, Lanes> g_lanes;
+
+Lanes& lanes_for(int dev) {
+  auto it = g_lanes.find(dev);
+  if (it != g_lanes.end()) return it->second;
+  Lanes L{};
+  for (int i = 0; i < NLANE; ++i) {
+    ck(hipStreamCreateWithFlags(&L.st[i], hipStreamNonBlocking), "hipStreamCreate");
+    ckb(rocblas_create_handle(&L.h[i]), "rocblas_create_handle");
+    ckb(rocblas_set_stream(L.h[i], L.st[i]), "rocblas_set_stream");
+    ck(hipEventCreateWithFlags(&L.done[i], hipEventDisableTiming), "hipEventCreate");
+  }
+  ck(hipEventCreateWithFlags(&L.fork, hipEventDisableTiming), "hipEventCreate");
+  return g_lanes.emplace(dev, L).first->second;
+}
 
 }  // namespace
 
@@ -268,23 +291,25 @@ mpsh_status mpsh_svd_execute(mpsh_svd plan, const double* A, double* S, double* 
     int dev = 0;
     ck(hipGetDevice(&dev), "hipGetDevice");
     std::lock_guard<std::mutex> g(g_handle_mu);
-    rocblas_handle h;
-    auto it = g_handles.find(dev);
-    if (it == g_handles.end()) {
-      ckb(rocblas_create_handle(&h), "rocblas_create_handle");
-      g_handles[dev] = h;
-    } else {
-      h = it->second;
-    }
-    ckb(rocblas_set_stream(h, s), "rocblas_set_stream");
-    for (const Bucket& B : plan->buckets) {
-      ckb(rocsolver_dgesdd_strided_batched(h, rocblas_svect_singular, rocblas_svect_singular, (rocblas_int)B.Mp,
-                                           (rocblas_int)B.Np, ws + B.a_off, (rocblas_int)B.Mp, B.Mp * B.Np,
-                                           ws + B.s_off, B.kp, ws + B.u_off, (rocblas_int)B.Mp, B.Mp * B.kp,
-                                           ws + B.v_off, (rocblas_int)B.kp, B.kp * B.Np, info + B.info_off,
-                                           (rocblas_int)B.count),
+    Lanes& L = lanes_for(dev);
+    // fork: largest bucket first, bucket i on lane i mod NLANE; join back into s
+    ck(hipEventRecord(L.fork, s), "hipEventRecord");
+    const int nb = (int)plan->buckets.size();
+    for (int i = 0; i < nb && i < NLANE; ++i) ck(hipStreamWaitEvent(L.st[i], L.fork, 0), "hipStreamWaitEvent");
+    for (int i = 0; i < nb; ++i) {
+      const Bucket& B = plan->buckets[nb - 1 - i];
+      ckb(rocsolver_dgesdd_strided_batched(L.h[i % NLANE], rocblas_svect_singular, rocblas_svect_singular,
+                                           (rocblas_int)B.Mp, (rocblas_int)B.Np, ws + B.a_off, (rocblas_int)B.Mp,
+                                           B.Mp * B.Np, ws + B.s_off, B.kp, ws + B.u_off, (rocblas_int)B.Mp,
+                                           B.Mp * B.kp, ws + B.v_off, (rocblas_int)B.kp, B.kp * B.Np,
+                                           info + B.info_off, (rocblas_int)B.count),
           "rocsolver_dgesdd_strided_batched");
     }
+    for (int i = 0; i < nb && i < NLANE; ++i) {
+      ck(hipEventRecord(L.done[i], L.st[i]), "hipEventRecord");
+      ck(hipStreamWaitEvent(s, L.done[i], 0), "hipStreamWaitEvent");
+    }
+    rocblas_handle h = L.h[0];
     std::vector<rocblas_int> hinfo(plan->info_ints);
     ck(hipMemcpyAsync(hinfo.data(), info, hinfo.size() * 4, hipMemcpyDeviceToHost, s), "hipMemcpyAsync(info)");
     ck(hipStreamSynchronize(s), "hipStreamSynchronize");
@@ -297,12 +322,17 @@ mpsh_status mpsh_svd_execute(mpsh_svd plan, const double* A, double* S, double* 
       ck(launch_copy(cp.pad, 8, A, ws, s), "svd pad");
       DevBuf eb;  // gesvd's superdiagonal scratch (this retry path only)
       ck(hipMalloc(&eb.p, (size_t)B.count * B.kp * 8 + 8), "hipMalloc(gesvd E)");
+      // the padded inputs are rebuilt on s; lane 0 runs gesvd after them
+      ck(hipEventRecord(L.fork, s), "hipEventRecord");
+      ck(hipStreamWaitEvent(L.st[0], L.fork, 0), "hipStreamWaitEvent");
       ckb(rocsolver_dgesvd_strided_batched(h, rocblas_svect_singular, rocblas_svect_singular, (rocblas_int)B.Mp,
                                            (rocblas_int)B.Np, ws + B.a_off, (rocblas_int)B.Mp, B.Mp * B.Np,
                                            ws + B.s_off, B.kp, ws + B.u_off, (rocblas_int)B.Mp, B.Mp * B.kp,
                                            ws + B.v_off, (rocblas_int)B.kp, B.kp * B.Np, (double*)eb.p, B.kp,
                                            rocblas_outofplace, info + B.info_off, (rocblas_int)B.count),
           "rocsolver_dgesvd_strided_batched");
+      ck(hipEventRecord(L.done[0], L.st[0]), "hipEventRecord");
+      ck(hipStreamWaitEvent(s, L.done[0], 0), "hipStreamWaitEvent");
       std::vector<rocblas_int> hi(B.count);
       ck(hipMemcpyAsync(hi.data(), info + B.info_off, hi.size() * 4, hipMemcpyDeviceToHost, s), "hipMemcpyAsync");
       ck(hipStreamSynchronize(s), "hipStreamSynchronize");

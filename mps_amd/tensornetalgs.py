@@ -85,6 +85,7 @@ class HeffPlan:
         ch = (L._i32 * 9)()
         check(lib().mpsh_heff_tune(self.plan, _PREC[prec], _ptr(prepared), _ptr(v), _ptr(out), _ptr(ws), reps,
                                    int(force), ch, stream if stream is not None else _stream()))
+        TUNE_COUNTS["cached" if ch[8] else "timed"] += 1
         return {"step1": f"{ch[0]}x{ch[1]}x{ch[2]}", "step4": f"{ch[4]}x{ch[5]}x{ch[6]}", "split": int(ch[7]),
                 "from_cache": bool(ch[8])}
 
@@ -232,8 +233,9 @@ class EigInfo:
 
 
 LANCZOS_GRAPHS = False  # capture each Lanczos iteration in a hipGraph (eigsolve default)
-AUTOTUNE = True  # HeffOperator tunes the GEMM variants of each new plan (signature-cached)
+AUTOTUNE = True  # HeffOperator tunes the FP32/BF16 GEMM variants of each new plan (signature-cached)
 AUTOTUNE_MIN_FLOPS = 2e8  # ... when one matvec costs at least this many reference FLOPs
+TUNE_COUNTS = {"timed": 0, "cached": 0}  # autotuner calls that timed variants / reused a signature
 
 
 class _LanczosBasis:
@@ -360,7 +362,9 @@ class HeffOperator:
         out = torch.empty(plan.out.numel, dtype=x.dtype, device=x.device)
         if key not in self._prep:
             self._prep[key] = plan.prepare(prec, *(t.data for t in ops))
-            if AUTOTUNE and plan.ref_flops >= AUTOTUNE_MIN_FLOPS:
+            # the bulk (reduced-precision) matvecs are tuned; FP64 refinement runs few
+            # matvecs per bond, so it keeps the default variants
+            if AUTOTUNE and prec != "f64" and plan.ref_flops >= AUTOTUNE_MIN_FLOPS:
                 plan.tune(prec, self._prep[key], x.data, out)
         plan.apply_prepared(prec, self._prep[key], x.data, out)
         return SymTensor(plan.out.charge, plan.out.spaces, out, plan.out)
