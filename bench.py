@@ -47,6 +47,9 @@ def parse():
                     help="capture each Lanczos iteration of the DMRG window in a hipGraph")
     ap.add_argument("--dmrg-bonds", type=int, default=6,
                     help="timed two-site updates at chi=2048 in the bulk of the 6x24 cylinder (0: skip)")
+    ap.add_argument("--precision", default="none", choices=["none", "chain", "tv", "all"],
+                    help="precision studies: XXZ L=100 chi=512 four sweeps (chain), t-V L=128 chi=4096 "
+                         "middle-bond eigensolve (tv)")
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
                     help="only run a plain device copy of MIB MiB (FP32) for HBM-counter calibration")
     return ap.parse_args()
@@ -183,6 +186,113 @@ def dmrg_window(nbonds, dev):
            "settings": {k: c[k] for k in ("maxdim", "tol", "lanczostol", "krylovdim", "krylovmaxiter", "prec",
                                            "refine_tol")}}
     return out, state
+
+
+PREC_CHAIN = dict(L=100, N=50, chi=512, nsec=30, dmax=60, delta=1.0, sweeps=4, seed=1911, maxdim=512, tol=1e-10,
+                  lanczostol=1e-8, krylovdim=8, krylovmaxiter=2, refine_tol=1e-10)
+
+
+def precision_chain(dev):
+    """XXZ chain L=100, Delta=1, U(1) Sz, D=5, chi=512 (~30 sectors, dims <= 60): four
+    two-site DMRG sweeps from one seeded random MPS with FP64 matvecs, FP32-MFMA
+    matvecs, and FP32 matvecs + FP64 refinement (Lanczos orthogonalisation,
+    tridiagonal and SVD are FP64 in all runs).  Per sweep: the final energy
+    (FP64 <psi|H|psi> of the truncated state) and the truncation error (sum and
+    max of the discarded weight 1 - sum kept s^2 over the sweep's bonds)."""
+    import torch
+    from mps_amd import mpstools as mt
+    from mps_amd import tensornetalgs as ta
+    c = PREC_CHAIN
+    kw = {k: c[k] for k in ("maxdim", "tol", "lanczostol", "krylovdim", "krylovmaxiter")}
+    runs = {}
+    for name, prec, refine in (("f64", "f64", None), ("f32", "f32", None), ("f32_refined", "f32", c["refine_tol"])):
+        t0 = time.perf_counter()
+        mpo = mt.xxz_u1mpo(c["L"], c["delta"], device=dev)
+        mps = mt.random_mps(c["L"], c["N"], c["chi"], seed=c["seed"], nsec=c["nsec"], dmax=c["dmax"], device=dev)
+        env = ta.initialenv(mps, mpo)
+        torch.cuda.synchronize()
+        setup = time.perf_counter() - t0
+        sweeps = []
+        for k in range(c["sweeps"]):
+            log = []
+            t1 = time.perf_counter()
+            es = ta.dmrg2sitesweep(mps, mpo, env, prec=prec, refine_tol=refine, log=log, **kw)
+            torch.cuda.synchronize()
+            dw = [x["discarded_weight"] for x in log if "discarded_weight" in x]
+            sweeps.append({"energy": es[-1], "sweep_s": round(time.perf_counter() - t1, 3),
+                           "trunc_err_sum": sum(dw), "trunc_err_max": max(dw),
+                           "matvecs": sum(x["matvecs"] for x in log if "matvecs" in x)})
+            print(f"precision chain {name}: sweep {k + 1} E={es[-1]:.12f} ({sweeps[-1]['sweep_s']} s)",
+                  file=sys.stderr, flush=True)
+        runs[name] = {"setup_s": round(setup, 2), "sweeps": sweeps, "max_bond": max(mps.bonddims())}
+    e64 = runs["f64"]["sweeps"][-1]["energy"]
+    for name, r in runs.items():
+        e = r["sweeps"][-1]["energy"]
+        r["energy_rel_diff_vs_f64"] = abs(e - e64) / abs(e64)
+        r["trunc_err_sum_diff_vs_f64"] = r["sweeps"][-1]["trunc_err_sum"] - runs["f64"]["sweeps"][-1]["trunc_err_sum"]
+    return {"model": f"XXZ chain L={c['L']}, Delta={c['delta']}, N={c['N']}, MPO D=5", "chi": c["chi"],
+            "settings": {k: c[k] for k in ("sweeps", "maxdim", "tol", "lanczostol", "krylovdim", "krylovmaxiter",
+                                           "refine_tol", "nsec", "dmax", "seed")},
+            "runs": runs}
+
+
+PREC_TV = dict(L=128, N=64, chi=4096, nsec=70, dmax=400, t=1.0, V=2.0, seed=1911, lanczostol=1e-12, krylovdim=24,
+               krylovmaxiter=12, refine_tol=1e-11, maxdim=4096, tol=1e-10)
+
+
+def precision_tv(dev):
+    """Spinless-fermion t-V chain L=128, t=1, V=2, half filling, Jordan-Wigner MPO
+    D=6, chi=4096 (~70 sectors, dims <= 400): the two-site eigenproblem at the
+    middle bond of a seeded random MPS (canonical there) solved with FP64, FP32
+    and BF16-operand (FP32 accumulate) matvecs, each with and without FP64
+    refinement cycles; the eigenvalue, the FP64 Rayleigh quotient of the
+    returned vector, and the truncation error of its SVD to chi are compared
+    with the FP64 solve."""
+    import torch
+    from mps_amd import mpstools as mt
+    from mps_amd import tensornetalgs as ta
+    from mps_amd.symtensors import contract, dot, symmatrix
+    c = PREC_TV
+    t0 = time.perf_counter()
+    mpo = mt.tv_u1mpo(c["L"], c["t"], c["V"], device=dev)
+    mps = mt.random_mps(c["L"], c["N"], c["chi"], seed=c["seed"], nsec=c["nsec"], dmax=c["dmax"], device=dev)
+    print(f"precision t-V: random chi={c['chi']} MPS ({time.perf_counter() - t0:.1f} s)", file=sys.stderr, flush=True)
+    cen = c["L"] // 2
+    mt.center_at(mps, cen)
+    env = ta.initialenv(mps, mpo, at=cen)
+    AA = contract(mps.As[cen - 1], (1, 2, -1), mps.As[cen], (-1, 3, 4))
+    torch.cuda.synchronize()
+    setup = time.perf_counter() - t0
+    print(f"precision t-V: center {cen}, environments ready ({setup:.1f} s)", file=sys.stderr, flush=True)
+    args = (env[cen - 1], env[cen + 2], mpo.Ws[cen - 1], mpo.Ws[cen])
+    H64 = ta.HeffOperator(*args, "f64")
+    runs = {}
+    for name, prec, refine in (("f64", "f64", None), ("f32", "f32", None), ("f32_refined", "f32", c["refine_tol"]),
+                               ("bf16", "bf16", None), ("bf16_refined", "bf16", c["refine_tol"])):
+        H = H64 if prec == "f64" else ta.HeffOperator(*args, prec)
+        torch.cuda.synchronize()
+        t1 = time.perf_counter()
+        theta, v, info = ta.eigsolve(H, AA, c["lanczostol"], c["krylovdim"], c["krylovmaxiter"], refine)
+        torch.cuda.synchronize()
+        secs = time.perf_counter() - t1
+        rq = dot(v, H64(v)) / dot(v, v)
+        r = mt.svdtrunc(symmatrix(v, [1, 2], [3, 4]), maxdim=c["maxdim"], tol=c["tol"])
+        s = r.singular_values()
+        runs[name] = {"eigenvalue": theta, "rayleigh_f64": rq, "normres": info.normres, "matvecs": info.numops,
+                      "solve_s": round(secs, 3), "trunc_err": max(0.0, 1.0 - float((s ** 2).sum()) / float(dot(v, v))),
+                      "kept": int(r.Vl.dim())}
+        print(f"precision t-V {name}: E={theta:.14f} rq={rq:.14f} ({secs:.2f} s, {info.numops} matvecs)",
+              file=sys.stderr, flush=True)
+    e64 = runs["f64"]["rayleigh_f64"]
+    for r in runs.values():
+        r["eigenvalue_rel_diff_vs_f64"] = abs(r["eigenvalue"] - runs["f64"]["eigenvalue"]) / abs(runs["f64"]["eigenvalue"])
+        r["rayleigh_rel_diff_vs_f64"] = abs(r["rayleigh_f64"] - e64) / abs(e64)
+        r["trunc_err_diff_vs_f64"] = r["trunc_err"] - runs["f64"]["trunc_err"]
+    return {"model": f"spinless t-V chain L={c['L']}, t={c['t']}, V={c['V']}, N={c['N']}, Jordan-Wigner MPO D=6",
+            "chi": c["chi"], "bond": [cen, cen + 1], "setup_s": round(setup, 2),
+            "dim_two_site": AA.struct.numel, "settings": {k: c[k] for k in ("lanczostol", "krylovdim", "krylovmaxiter",
+                                                                    "refine_tol", "maxdim", "tol", "nsec", "dmax")},
+            "runs": runs}
 
 
 def cpu_baseline(name, budget):
@@ -336,6 +446,10 @@ def main():
         ta.LANCZOS_GRAPHS = bool(args.lanczos_graphs)
         result["dmrg"], dmrg_state = dmrg_window(args.dmrg_bonds, dev)
         result["dmrg"]["lanczos_hipgraph"] = ta.LANCZOS_GRAPHS
+    if args.precision in ("chain", "all"):
+        result["precision_chain"] = precision_chain(dev)
+    if args.precision in ("tv", "all"):
+        result["precision_tv"] = precision_tv(dev)
     if not args.no_cpu_baseline:
         rate, calls, secs, threads, oflat, vflat = cpu_baseline(args.workload, args.cpu_budget)
         e_or = float(np.dot(vflat, oflat) / np.dot(vflat, vflat))

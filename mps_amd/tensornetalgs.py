@@ -235,6 +235,7 @@ class EigInfo:
 LANCZOS_GRAPHS = False  # capture each Lanczos iteration in a hipGraph (eigsolve default)
 AUTOTUNE = True  # HeffOperator tunes the FP32/BF16 GEMM variants of each new plan (signature-cached)
 AUTOTUNE_MIN_FLOPS = 2e8  # ... when one matvec costs at least this many reference FLOPs
+STAGE_RTOL = {"f32": 1e-6}  # relative residual at which a refined eigsolve leaves its reduced-precision stage
 TUNE_COUNTS = {"timed": 0, "cached": 0}  # autotuner calls that timed variants / reused a signature
 
 
@@ -300,14 +301,14 @@ class _LanczosBasis:
 
 
 def lanczos_lowest(apply, x0: SymTensor, tol: float = 1e-7, krylovdim: int = 5, maxiter: int = 8,
-                   graphs: bool = False):
+                   graphs: bool = False, rtol: float = 0.0):
     """Restarted Lanczos for the lowest eigenpair of a Hermitian map on FP64 device vectors.
 
     The basis lives on the device; every iteration applies the map, runs two
     classical Gram-Schmidt passes against the whole basis and normalises, with
     all scalars in device memory (``mpsh_lanczos_*``), so the host reads just
     (alpha, beta) once per iteration to solve the tridiagonal eigenproblem in
-    FP64 and test |beta_k y_k| < tol.  With ``graphs`` each iteration index is
+    FP64 and test |beta_k y_k| < max(tol, rtol |theta|).  With ``graphs`` each iteration index is
     captured once in a hipGraph and replayed (the first application runs
     eagerly, so the operator prepares itself outside the capture).
     """
@@ -332,14 +333,15 @@ def lanczos_lowest(apply, x0: SymTensor, tol: float = 1e-7, krylovdim: int = 5, 
             evals, evecs = np.linalg.eigh(T)
             theta, y = float(evals[0]), evecs[:, 0]
             res = abs(b * y[-1])
-            if res < tol or j == kd - 1 or b < 1e-14:
+            stop = max(tol, rtol * abs(theta))
+            if res < stop or j == kd - 1 or b < 1e-14:
                 break
             betas.append(b)
         B.combine(y)
-        if res < tol:
+        if res < stop:
             break
     out = SymTensor(x0.charge, x0.space, B.V[0, :B.n].clone(), x0.struct)
-    return theta, out, EigInfo(res, it, numops, res < tol)
+    return theta, out, EigInfo(res, it, numops, res < stop)
 
 
 class HeffOperator:
@@ -384,7 +386,10 @@ def eigsolve(H: HeffOperator, x0: SymTensor, tol=1e-7, krylovdim=5, maxiter=8, r
     """Lowest eigenpair; reduced-precision cycles then (optionally) FP64 refinement cycles.
     ``graphs`` (default ``LANCZOS_GRAPHS``) captures each Lanczos iteration in a hipGraph."""
     graphs = LANCZOS_GRAPHS if graphs is None else graphs
-    theta, x, info = lanczos_lowest(H, x0, tol, krylovdim, maxiter, graphs=graphs)
+    # with refinement to follow, the FP32 stage stops at its own noise floor
+    # (residuals below ~1e-6 |theta| are not resolved by FP32 matvecs)
+    rtol = STAGE_RTOL.get(H.prec, 0.0) if refine_tol is not None else 0.0
+    theta, x, info = lanczos_lowest(H, x0, tol, krylovdim, maxiter, graphs=graphs, rtol=rtol)
     if H.prec != "f64" and refine_tol is not None:
         theta, x, info2 = lanczos_lowest(H.exact, x, refine_tol, krylovdim, max(1, maxiter // 2), graphs=graphs)
         info = EigInfo(info2.normres, info.numiter + info2.numiter, info.numops + info2.numops, info2.converged)
