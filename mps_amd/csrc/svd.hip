@@ -1,9 +1,24 @@
 // Batched per-block FP64 SVD on the device: the per-sector SVD of
 // svdtrunc / svd (reference src/mpsutils.jl:116-179, src/symtools.jl:1-35).
-// Blocks of different shapes are zero-padded into square shape buckets and
-// each bucket is one rocSOLVER gesdd strided-batched
-// call -- the divide-and-conquer SVD the reference gets from LAPACK gesdd.
-// Zero padding adds only zero singular values; the leading min(m, n) singular
+//
+// Default path: one-sided block Jacobi (Hestenes) on all blocks at once.  A
+// block A (m x n) is worked as W = A (m >= n) or W = A^T (m < n), r x c with
+// r >= c, columns split into blocks of w columns.  One launch per round of a
+// round-robin tournament over the column blocks: every work group loads one
+// pair of column blocks (r x 2w) into LDS and runs one cyclic sweep of plane
+// rotations over its 2w columns, each rotation computed from the current
+// columns' dot products (Rutishauser's formula), then writes the columns back
+// and applies the accumulated 2w x 2w rotation to the c x c matrix Q.  A
+// matrix is done after a sweep in which no pair had
+// |x_i.x_j| > sqrt(r) eps |x_i| |x_j|; then sigma_j = |W_j|, the left vectors
+// are W_j / sigma_j and the right vectors the columns of Q, sorted by
+// descending sigma.  Matrices with at most 2w columns are solved in one work
+// group.  Rows up to 2048 fit LDS; larger blocks use the rocSOLVER path.
+//
+// rocSOLVER path (MPSH_SVD_ALGO=gesdd, or rows > 2048): blocks are zero-padded
+// into square shape buckets and each bucket is one gesdd strided-batched call,
+// the divide-and-conquer SVD the reference gets from LAPACK gesdd.  Zero
+// padding adds only zero singular values; the leading min(m, n) singular
 // triplets of a block are read back from its padded slot.  A bucket whose
 // gesdd reports non-convergence is redone with gesvd (QR iteration), as the
 // reference retries gesdd failures with gesvd (src/symtools.jl:14-19).
@@ -12,6 +27,9 @@
 #include <rocsolver/rocsolver.h>
 
 #include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstdlib>
 #include <map>
 #include <memory>
 #include <mutex>
@@ -77,6 +95,396 @@ void ckb(rocblas_status e, const char* w) {
   if (e != rocblas_status_success) throw Error(ERR_DEVICE, std::string(w) + ": rocblas status " + std::to_string(e));
 }
 
+// ---------------------------------------------------------------------------
+// one-sided block Jacobi
+// ---------------------------------------------------------------------------
+constexpr int JT = 256;                 // threads per work group
+constexpr int64_t JAC_MAX_ROWS = 2048;  // r limit of the LDS-resident column pairs
+constexpr int JAC_LDS = 17472;          // doubles: 2w * ldx + 2w * (2w + 1) for the largest (w, r) class
+constexpr int JAC_MAX_SWEEPS = 60;
+
+struct JacMat {
+  int64_t r, c;          // work matrix W (r x c, ld r), r >= c >= 1
+  int64_t w_off, q_off;  // workspace offsets (doubles): W and Q (c x c, ld c)
+  int64_t m, n;          // the block's shape
+  int64_t u_off, s_off, v_off;  // packed outputs
+  double ctol;           // sqrt(r) * eps
+  int32_t w, nb;         // column-block width and block count (even, >= 2)
+  int32_t trans, pad;    // trans: W = A^T
+};
+struct JacTask {
+  int32_t mat, k;  // matrix (index into the JacMat table) and pair index within a round
+};
+
+// columns and width of a block width: w in {16, 8, 4} so that 2w * (r | 1) doubles fit the LDS budget
+int jac_width(int64_t r) { return r <= 512 ? 16 : (r <= 1024 ? 8 : 4); }
+
+__device__ __forceinline__ void jac_pair_of(int n, int t, int k, int& p, int& q) {
+  // round t of the circle method over n (even) players: player n-1 fixed
+  if (k == 0) {
+    p = t % (n - 1);
+    q = n - 1;
+  } else {
+    p = (t + k) % (n - 1);
+    q = (t - k + (n - 1)) % (n - 1);
+  }
+}
+
+// sum over each row of 16 lanes, returned in every lane of the row: DPP quad
+// swaps, then rotations by 4 and 8 within the row, then lane 0's sum broadcast
+// to the row (the rotation sums are added in lane-dependent order, and every
+// lane of a rotation must use bitwise the same value)
+template <int CTRL>
+__device__ __forceinline__ double dpp_mov(double v) {
+  const int lo = __builtin_amdgcn_mov_dpp(__double2loint(v), CTRL, 0xF, 0xF, false);
+  const int hi = __builtin_amdgcn_mov_dpp(__double2hiint(v), CTRL, 0xF, 0xF, false);
+  return __hiloint2double(hi, lo);
+}
+__device__ __forceinline__ double wave_row_sum(double v) {
+  v += dpp_mov<0xB1>(v);   // quad_perm [1,0,3,2]
+  v += dpp_mov<0x4E>(v);   // quad_perm [2,3,0,1]
+  v += dpp_mov<0x124>(v);  // row_ror:4
+  v += dpp_mov<0x128>(v);  // row_ror:8
+  return dpp_mov<0x150>(v);  // row_newbcast:0
+}
+
+// One pair of column blocks (p, q) per work group, X = [W_p W_q] (r x 2w) in
+// LDS.  A sweep over the pair: (a) the pairs inside p and inside q (w - 1
+// steps, circle order; only in round 0 of a sweep, where every block appears
+// once), (b) the w x w cross pairs in w steps: group k keeps column k of p in
+// registers and meets column (k + s) mod w of q at step s.  Over the nb - 1
+// rounds of a sweep every column pair of the matrix is rotated once.
+// iterate = 1 (matrices of two blocks, the pair holds every column): sweeps
+// until converged.  One wave per rotation (w waves per work group); a lane
+// holds RPL rows of a column, r <= 64 RPL.  Rows r .. 64 RPL of every column
+// are zero, so loads and stores run unmasked.  The 2w x 2w rotation R is kept
+// transposed (Rt[row * (2w + 1) + col]).
+// offmax[mat] is set to 1 when any pair of the sweep was rotated (some
+// |x_i.x_j| > ctol |x_i| |x_j|); a sweep without rotations ends the matrix.
+__device__ __forceinline__ double read_lane(double v, int l) {
+  return __hiloint2double(__builtin_amdgcn_readlane(__double2hiint(v), l),
+                          __builtin_amdgcn_readlane(__double2loint(v), l));
+}
+// sum over the 64 lanes of a wave, the same value in every lane: row sums by
+// DPP, then the four rows' sums read from lanes 0, 16, 32, 48
+__device__ __forceinline__ double wave_sum(double v) {
+  v = wave_row_sum(v);
+  return (read_lane(v, 0) + read_lane(v, 16)) + (read_lane(v, 32) + read_lane(v, 48));
+}
+
+template <int WB>
+__global__ __launch_bounds__(64 * WB) void jac_pairs(const JacMat* __restrict__ mats,
+                                                     const JacTask* __restrict__ tasks, int t, int iterate,
+                                                     double* __restrict__ ws, unsigned long long* __restrict__ offmax,
+                                                     int* __restrict__ done, int* __restrict__ flags) {
+  constexpr int NT = 64 * WB, NC = 2 * WB, GSZ = 64, RPL = WB == 16 ? 8 : (WB == 8 ? 16 : 32), LDX = GSZ * RPL + 1,
+                RTS = NC + 1;
+  static_assert(NC * LDX + NC * RTS <= JAC_LDS, "LDS budget");
+  __shared__ __align__(16) double lds[JAC_LDS];
+  __shared__ int rot_any;
+  const JacTask task = tasks[blockIdx.x];
+  const JacMat M = mats[task.mat];
+  if (done[task.mat]) return;
+  if (!iterate && t >= M.nb - 1) return;
+  const int r = (int)M.r, c = (int)M.c;
+  double* X = lds;
+  double* Rt = lds + NC * LDX;
+  int p, q;
+  jac_pair_of(M.nb, iterate ? 0 : t, task.k, p, q);
+  auto gcol = [&](int lc) -> int {
+    const int g = lc < WB ? p * WB + lc : q * WB + (lc - WB);
+    return g < c ? g : -1;
+  };
+  double* W = ws + M.w_off;
+  double* Q = ws + M.q_off;
+  const int tid = threadIdx.x;
+  for (int e = tid; e < NC * LDX; e += NT) {
+    const int lc = e / LDX, row = e - lc * LDX;
+    const int g = gcol(lc);
+    X[e] = (g >= 0 && row < r) ? W[(int64_t)g * r + row] : 0.0;
+  }
+  for (int e = tid; e < NC * RTS; e += NT) Rt[e] = (e / RTS == e % RTS) ? 1.0 : 0.0;
+  if (tid == 0) rot_any = 0;
+  __syncthreads();
+  const int grp = tid / GSZ, lane = tid % GSZ;
+  const double ctol = M.ctol;
+  const int max_sweeps = iterate ? JAC_MAX_SWEEPS : 1;
+  bool rotated = false;
+
+  // rotate columns (i, j) held in registers (rows lane + 64 u); true when rotated
+  auto rotate = [&](double (&xi)[RPL], double (&xj)[RPL], int i, int j) {
+    double a0 = 0.0, a1 = 0.0, b0 = 0.0, b1 = 0.0, g0 = 0.0, g1 = 0.0;
+#pragma unroll
+    for (int u = 0; u < RPL; u += 2) {
+      a0 = fma(xi[u], xi[u], a0);
+      b0 = fma(xj[u], xj[u], b0);
+      g0 = fma(xi[u], xj[u], g0);
+      a1 = fma(xi[u + 1], xi[u + 1], a1);
+      b1 = fma(xj[u + 1], xj[u + 1], b1);
+      g1 = fma(xi[u + 1], xj[u + 1], g1);
+    }
+    const double a = wave_sum(a0 + a1), b = wave_sum(b0 + b1), g = wave_sum(g0 + g1);
+    // rotate when |g| > ctol |x_i| |x_j|; columns of norm^2 <= 1e-280 (the
+    // matrix is scaled to entries < 1) count as zero, and so does NaN
+    if (!(a > 1e-280 && b > 1e-280 && fabs(g) > ctol * (sqrt(a) * sqrt(b)))) return false;
+    // t = tan of the angle that zeroes the pair's product (Rutishauser):
+    // t = sign(z) / (|z| + sqrt(1 + z^2)), z = (b - a) / (2g); for |z| > 1e150,
+    // t = 1 / (2z) (z^2 would overflow)
+    const double z = (b - a) / (2.0 * g);
+    const double az = fabs(z);
+    const double tt = az > 1e150 ? 0.5 / z : copysign(1.0 / (az + sqrt(fma(az, az, 1.0))), z);
+    const double cs = 1.0 / sqrt(fma(tt, tt, 1.0)), sn = cs * tt;
+#pragma unroll
+    for (int u = 0; u < RPL; ++u) {
+      const double x = xi[u], y = xj[u];
+      xi[u] = cs * x - sn * y;
+      xj[u] = sn * x + cs * y;
+    }
+    if (lane < NC) {
+      const double x = Rt[lane * RTS + i], y = Rt[lane * RTS + j];
+      Rt[lane * RTS + i] = cs * x - sn * y;
+      Rt[lane * RTS + j] = sn * x + cs * y;
+    }
+    return true;
+  };
+  auto load = [&](double (&x)[RPL], int col) {
+    const double* src = X + col * LDX + lane;
+#pragma unroll
+    for (int u = 0; u < RPL; ++u) x[u] = src[u * GSZ];
+  };
+  auto store = [&](const double (&x)[RPL], int col) {
+    double* dst = X + col * LDX + lane;
+#pragma unroll
+    for (int u = 0; u < RPL; ++u) dst[u * GSZ] = x[u];
+  };
+
+  int sweep = 0;
+  int conv = 0;
+  for (; sweep < max_sweeps; ++sweep) {
+    double xi[RPL], xj[RPL];
+    if (iterate || t == 0) {
+      // (a) pairs inside p (groups 0 .. WB/2-1) and inside q (groups WB/2 ..)
+      const int half = WB / 2, base = grp < half ? 0 : WB, k = grp < half ? grp : grp - half;
+#pragma unroll 1
+      for (int s2 = 0; s2 < WB - 1; ++s2) {
+        int i, j;
+        jac_pair_of(WB, s2, k, i, j);
+        i += base;
+        j += base;
+        load(xi, i);
+        load(xj, j);
+        if (rotate(xi, xj, i, j)) {
+          rotated = true;
+          store(xi, i);
+          store(xj, j);
+        }
+        __syncthreads();
+      }
+    }
+    // (b) cross pairs: group k keeps column k of p
+    load(xi, grp);
+    bool moved = false;
+#pragma unroll 1
+    for (int s2 = 0; s2 < WB; ++s2) {
+      const int j = WB + (grp + s2) % WB;
+      load(xj, j);
+      if (rotate(xi, xj, grp, j)) {
+        store(xj, j);
+        moved = true;
+      }
+      __syncthreads();
+    }
+    if (moved) {
+      rotated = true;
+      store(xi, grp);
+    }
+    if (rotated && lane == 0) rot_any = 1;
+    __syncthreads();
+    conv = !rot_any;
+    __syncthreads();
+    if (tid == 0) rot_any = 0;
+    rotated = false;
+    if (!iterate && !conv && tid == 0) offmax[task.mat] = 1ull;
+    if (conv) break;
+  }
+  if (iterate && tid == 0) {
+    if (conv)
+      done[task.mat] = 1;
+    else
+      flags[1] = 1;  // not converged
+  }
+  // columns back to W
+  for (int e = tid; e < NC * r; e += NT) {
+    const int lc = e / r, row = e - lc * r;
+    const int g = gcol(lc);
+    if (g >= 0) W[(int64_t)g * r + row] = X[lc * LDX + row];
+  }
+  __syncthreads();
+  // Q[:, cols] <- Q[:, cols] R, staged through the X region (c <= r rows);
+  // a work item is one row times 8 columns
+  for (int e = tid; e < NC * c; e += NT) {
+    const int lc = e / c, row = e - lc * c;
+    const int g = gcol(lc);
+    X[lc * LDX + row] = g >= 0 ? Q[(int64_t)g * c + row] : 0.0;
+  }
+  __syncthreads();
+  for (int it = tid; it < c * (NC / 8); it += NT) {
+    const int cg = it / c, row = it - cg * c;
+    double acc[8] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
+#pragma unroll 2
+    for (int li = 0; li < NC; ++li) {
+      const double x = X[li * LDX + row];
+      const double* rr = Rt + li * RTS + cg * 8;
+#pragma unroll
+      for (int k = 0; k < 8; ++k) acc[k] = fma(x, rr[k], acc[k]);
+    }
+#pragma unroll
+    for (int k = 0; k < 8; ++k) {
+      const int g = gcol(cg * 8 + k);
+      if (g >= 0) Q[(int64_t)g * c + row] = acc[k];
+    }
+  }
+}
+
+// Scale every work matrix by a power of two so its largest |entry| lies in
+// [0.5, 1): products of columns then neither underflow (blocks of 1e-120 occur
+// in DMRG) nor overflow.  sexp[mat] keeps the exponent for the finish.
+__global__ __launch_bounds__(JT) void jac_scale(const JacMat* __restrict__ mats, double* __restrict__ ws,
+                                                int* __restrict__ sexp) {
+  __shared__ double red[JT / 64];
+  __shared__ int e_sh;
+  const JacMat M = mats[blockIdx.x];
+  double* W = ws + M.w_off;
+  const int64_t n = M.r * M.c;
+  double mx = 0.0;
+  for (int64_t e = threadIdx.x; e < n; e += JT) mx = fmax(mx, fabs(W[e]));
+  for (int o = 32; o > 0; o >>= 1) mx = fmax(mx, __shfl_xor(mx, o, 64));
+  if (threadIdx.x % 64 == 0) red[threadIdx.x / 64] = mx;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    double m = 0.0;
+    for (int k = 0; k < JT / 64; ++k) m = fmax(m, red[k]);
+    int ex = 0;
+    if (m > 0.0 && m <= 1.7976931348623157e308) frexp(m, &ex);  // NaN / inf: unscaled, reported later
+    e_sh = ex;
+    sexp[blockIdx.x] = ex;
+  }
+  __syncthreads();
+  const int ex = e_sh;
+  if (ex == 0) return;
+  for (int64_t e = threadIdx.x; e < n; e += JT) W[e] = ldexp(W[e], -ex);
+}
+
+// After a sweep of rounds: matrices whose largest relative off-diagonal stayed
+// below tolerance are done; count the rest.  One thread (few matrices).
+__global__ void jac_check(const JacMat* __restrict__ mats, const int32_t* __restrict__ big, int nbig,
+                          unsigned long long* __restrict__ offmax, int* __restrict__ done, int* __restrict__ left) {
+  if (threadIdx.x != 0 || blockIdx.x != 0) return;
+  int n = 0;
+  for (int i = 0; i < nbig; ++i) {
+    const int b = big[i];
+    if (!done[b]) {
+      if (offmax[b] == 0ull)  // a sweep without rotations
+        done[b] = 1;
+      else
+        ++n;
+    }
+    offmax[b] = 0ull;
+  }
+  left[0] = n;
+}
+
+__global__ __launch_bounds__(JT) void jac_init_q(const JacMat* __restrict__ mats, double* __restrict__ ws) {
+  const JacMat M = mats[blockIdx.x];
+  double* Q = ws + M.q_off;
+  const int64_t cc = M.c * M.c;
+  for (int64_t e = threadIdx.x; e < cc; e += JT) Q[e] = (e / M.c == e % M.c) ? 1.0 : 0.0;
+}
+
+// sigma, sort (descending; ties by column index) and the packed outputs; the
+// left vector of a zero column is completed to an orthonormal set by
+// Gram-Schmidt on unit vectors.
+constexpr int JF_MAX = 2048;
+__global__ __launch_bounds__(JT) void jac_finish(const JacMat* __restrict__ mats, const double* __restrict__ ws,
+                                                 const int* __restrict__ sexp, double* __restrict__ U,
+                                                 double* __restrict__ S, double* __restrict__ Vt,
+                                                 int* __restrict__ flags) {
+  __shared__ double sig[JF_MAX];
+  __shared__ int rk[JF_MAX];
+  __shared__ double x[JF_MAX];
+  __shared__ double red[JT / 64];
+  __shared__ int nzero;
+  const JacMat M = mats[blockIdx.x];
+  const int64_t r = M.r, c = M.c, k = c;
+  const double* W = ws + M.w_off;
+  const double* Q = ws + M.q_off;
+  const int tid = threadIdx.x, wave = tid / 64, ln = tid % 64;
+  for (int64_t j = wave; j < c; j += JT / 64) {
+    double a = 0.0;
+    for (int64_t row = ln; row < r; row += 64) a = fma(W[j * r + row], W[j * r + row], a);
+    for (int o = 32; o > 0; o >>= 1) a += __shfl_xor(a, o, 64);
+    if (ln == 0) sig[j] = sqrt(a);
+  }
+  if (tid == 0) nzero = 0;
+  __syncthreads();
+  for (int64_t j = tid; j < c; j += JT) {
+    const double sj = sig[j];
+    if (!(sj == sj) || sj > 1.7e308) flags[0] = 1;
+    int n = 0;
+    for (int64_t i = 0; i < c; ++i) n += (sig[i] > sj) || (sig[i] == sj && i < j);
+    rk[j] = n;
+    if (!(sj > 1e-140)) atomicAdd(&nzero, 1);
+  }
+  __syncthreads();
+  // left vector (length r) of rank q: U column (trans 0) or Vt row (trans 1)
+  const int64_t ldl = M.trans ? k : 1, stl = M.trans ? 1 : M.m;  // element i of left vector q at base + q*stl + i*ldl
+  double* Lb = M.trans ? Vt + M.v_off : U + M.u_off;
+  double* Rb = M.trans ? U + M.u_off : Vt + M.v_off;
+  const int64_t ldr = M.trans ? 1 : k, str = M.trans ? M.m : 1;  // right vector (length c) likewise
+  for (int64_t j = 0; j < c; ++j) {
+    const double sj = sig[j];
+    const int64_t q = rk[j];
+    if (tid == 0) S[M.s_off + q] = ldexp(sj, sexp[blockIdx.x]);
+    const bool live = sj > 1e-140;  // columns the rotations treat as zero get completed below
+    const double inv = live ? 1.0 / sj : 0.0;
+    for (int64_t row = tid; row < r; row += JT) Lb[q * stl + row * ldl] = live ? W[j * r + row] * inv : 0.0;
+    for (int64_t row = tid; row < c; row += JT) Rb[q * str + row * ldr] = Q[j * c + row];
+  }
+  if (nzero == 0) return;
+  __syncthreads();
+  // completion: zero columns have the largest ranks c - nzero .. c - 1
+  for (int64_t q = c - nzero; q < c; ++q) {
+    for (int64_t cand = 0; cand < r; ++cand) {
+      for (int64_t row = tid; row < r; row += JT) x[row] = row == cand ? 1.0 : 0.0;
+      __syncthreads();
+      for (int pass = 0; pass < 2; ++pass) {
+        for (int64_t o = 0; o < q; ++o) {
+          double d = 0.0;
+          for (int64_t row = tid; row < r; row += JT) d = fma(Lb[o * stl + row * ldl], x[row], d);
+          for (int s2 = 32; s2 > 0; s2 >>= 1) d += __shfl_xor(d, s2, 64);
+          if (ln == 0) red[wave] = d;
+          __syncthreads();
+          const double dot = red[0] + red[1] + red[2] + red[3];
+          for (int64_t row = tid; row < r; row += JT) x[row] -= dot * Lb[o * stl + row * ldl];
+          __syncthreads();
+        }
+      }
+      double nn = 0.0;
+      for (int64_t row = tid; row < r; row += JT) nn = fma(x[row], x[row], nn);
+      for (int s2 = 32; s2 > 0; s2 >>= 1) nn += __shfl_xor(nn, s2, 64);
+      if (ln == 0) red[wave] = nn;
+      __syncthreads();
+      const double nrm = sqrt(red[0] + red[1] + red[2] + red[3]);
+      __syncthreads();
+      if (nrm > 0.5) {
+        for (int64_t row = tid; row < r; row += JT) Lb[q * stl + row * ldl] = x[row] / nrm;
+        __syncthreads();
+        break;
+      }
+    }
+  }
+}
+
 // Per device: NLANE side streams, each with its own rocBLAS handle, so the
 // size buckets' gesdd calls (latency-bound chains of small kernels) run
 // concurrently.  Used under a lock (handles are not thread-safe).
@@ -115,9 +523,21 @@ struct mpsh_svd_s {
   int64_t ws_doubles = 0, info_ints = 0;
   CopyPlan pad;               // block -> padded slot
   CopyPlan unpad_u, unpad_v, unpad_s;  // padded outputs -> packed U, Vt, S
+  // one-sided Jacobi path
+  std::vector<JacMat> jm;
+  std::vector<JacTask> jsmall, jbig;  // tasks of 2-block matrices / of one round of the others,
+  int32_t jsmall_cls[4] = {0, 0, 0, 0}, jbig_cls[4] = {0, 0, 0, 0};  // by width class (16, 8, 4): [cls, cls+1)
+  std::vector<int32_t> jbig_idx;      // matrices with more than two column blocks
+  int32_t jrounds = 0;                // rounds per sweep (max over matrices)
+  int64_t jac_off = 0, jac_meta_off = 0;  // workspace (doubles): W | Q, then offmax | done | left, flags
+  CopyPlan jinit;                     // blocks -> W (transposed when m < n)
   struct Dev {
     std::vector<std::unique_ptr<DevBuf>> bufs;
-    DevCopy pad, u, v, s;
+    DevCopy pad, u, v, s, jinit;
+    const JacMat* jm = nullptr;
+    const JacTask* jsmall = nullptr;
+    const JacTask* jbig = nullptr;
+    const int32_t* jbig_idx = nullptr;
   };
   std::mutex mu;
   std::map<int, std::unique_ptr<Dev>> dev;
@@ -151,11 +571,89 @@ struct mpsh_svd_s {
       D->u = upc(D->bufs, unpad_u);
       D->v = upc(D->bufs, unpad_v);
       D->s = upc(D->bufs, unpad_s);
+      D->jinit = upc(D->bufs, jinit);
+      D->jm = up(D->bufs, jm);
+      D->jsmall = up(D->bufs, jsmall);
+      D->jbig = up(D->bufs, jbig);
+      D->jbig_idx = up(D->bufs, jbig_idx);
       it = dev.emplace(d, std::move(D)).first;
     }
     return *it->second;
   }
 };
+
+namespace {
+
+// shapes of the matrices not marked done, for the error message
+std::string jacobi_unconverged(const mpsh_svd_s& P, const int* done, hipStream_t s) {
+  std::vector<int> h(P.jm.size());
+  if (hipMemcpyAsync(h.data(), done, h.size() * 4, hipMemcpyDeviceToHost, s) != hipSuccess ||
+      hipStreamSynchronize(s) != hipSuccess)
+    return "";
+  std::string out;
+  int n = 0;
+  for (size_t i = 0; i < h.size(); ++i)
+    if (!h[i] && n++ < 8)
+      out += " " + std::to_string(P.jm[i].m) + "x" + std::to_string(P.jm[i].n) + "(w" + std::to_string(P.jm[i].w) + ")";
+  return " (" + std::to_string(n) + " blocks:" + out + ")";
+}
+
+void jacobi_execute(const mpsh_svd_s& P, const mpsh_svd_s::Dev& cp, const double* A, double* S, double* U,
+                    double* Vt, double* ws, hipStream_t s) {
+  const int nj = (int)P.jm.size();
+  double* jw = ws + P.jac_off;
+  unsigned long long* offmax = (unsigned long long*)(ws + P.jac_meta_off);
+  int* done = (int*)(offmax + nj);
+  int* sexp = done + nj + (nj & 1);
+  int* left = sexp + nj + (nj & 1);
+  int* flags = left + 1;
+  ck(hipMemsetAsync(offmax, 0, (size_t)nj * 8 + (size_t)(nj + (nj & 1)) * 8 + 16, s), "hipMemsetAsync(jacobi)");
+  ck(launch_copy(cp.jinit, 8, A, jw, s), "jacobi init W");
+  hipLaunchKernelGGL(jac_scale, dim3(nj), dim3(JT), 0, s, cp.jm, jw, sexp);
+  ck(hipGetLastError(), "jac_scale");
+  hipLaunchKernelGGL(jac_init_q, dim3(nj), dim3(JT), 0, s, cp.jm, jw);
+  ck(hipGetLastError(), "jac_init_q");
+  // one launch per width class present
+  auto pairs = [&](const JacTask* tasks, const int32_t* cls, int t, int iterate) {
+    for (int k = 0; k < 3; ++k) {
+      const int n = cls[k + 1] - cls[k];
+      if (n == 0) continue;
+      const dim3 g((unsigned)n), b(k == 0 ? 1024 : (k == 1 ? 512 : 256));
+      if (k == 0)
+        hipLaunchKernelGGL(jac_pairs<16>, g, b, 0, s, cp.jm, tasks + cls[k], t, iterate, jw, offmax, done, flags);
+      else if (k == 1)
+        hipLaunchKernelGGL(jac_pairs<8>, g, b, 0, s, cp.jm, tasks + cls[k], t, iterate, jw, offmax, done, flags);
+      else
+        hipLaunchKernelGGL(jac_pairs<4>, g, b, 0, s, cp.jm, tasks + cls[k], t, iterate, jw, offmax, done, flags);
+    }
+    ck(hipGetLastError(), "jac_pairs");
+  };
+  if (!P.jsmall.empty()) pairs(cp.jsmall, P.jsmall_cls, 0, 1);
+  if (!P.jbig.empty()) {
+    int hl = 1, sweeps = 0;
+    for (int sweep = 0; sweep < JAC_MAX_SWEEPS && hl != 0; ++sweep, ++sweeps) {
+      for (int t = 0; t < P.jrounds; ++t) pairs(cp.jbig, P.jbig_cls, t, 0);
+      hipLaunchKernelGGL(jac_check, dim3(1), dim3(64), 0, s, cp.jm, cp.jbig_idx, (int)P.jbig_idx.size(), offmax,
+                         done, left);
+      ck(hipGetLastError(), "jac_check");
+      ck(hipMemcpyAsync(&hl, left, 4, hipMemcpyDeviceToHost, s), "hipMemcpyAsync(jacobi)");
+      ck(hipStreamSynchronize(s), "hipStreamSynchronize");
+    }
+    if (hl != 0) throw Error(ERR_NONFINITE, "Jacobi SVD did not converge" + jacobi_unconverged(P, done, s));
+    static const bool stats = getenv("MPSH_SVD_STATS") != nullptr;
+    if (stats) fprintf(stderr, "mpsh_svd: %d Jacobi sweeps x %d rounds\n", sweeps, P.jrounds);
+  }
+  hipLaunchKernelGGL(jac_finish, dim3(nj), dim3(JT), 0, s, cp.jm, (const double*)jw, (const int*)sexp, U, S, Vt,
+                     flags);
+  ck(hipGetLastError(), "jac_finish");
+  int hf[2] = {0, 0};
+  ck(hipMemcpyAsync(hf, flags, 8, hipMemcpyDeviceToHost, s), "hipMemcpyAsync(jacobi flags)");
+  ck(hipStreamSynchronize(s), "hipStreamSynchronize");
+  if (hf[0]) throw Error(ERR_NONFINITE, "SVD input is not finite");
+  if (hf[1]) throw Error(ERR_NONFINITE, "Jacobi SVD did not converge" + jacobi_unconverged(P, done, s));
+}
+
+}  // namespace
 
 extern "C" {
 
@@ -181,13 +679,22 @@ mpsh_status mpsh_svd_plan(int64_t nblocks, const int64_t* m, const int64_t* n, c
       P->u_total += m[b] * k;
       P->v_total += k * n[b];
     }
-    std::vector<int64_t> order(nblocks);
-    for (int64_t b = 0; b < nblocks; ++b) order[b] = b;
+    const char* algo = getenv("MPSH_SVD_ALGO");
+    const bool jacobi = !(algo && std::string(algo) == "gesdd");
+    std::vector<int64_t> order;
+    std::vector<int64_t> jac_blocks;
+    for (int64_t b = 0; b < nblocks; ++b) {
+      if (jacobi && std::max(m[b], n[b]) <= JAC_MAX_ROWS)
+        jac_blocks.push_back(b);
+      else
+        order.push_back(b);
+    }
     std::stable_sort(order.begin(), order.end(), [&](int64_t x, int64_t y) {
       return std::max(m[x], n[x]) < std::max(m[y], n[y]);
     });
-    std::vector<int64_t> Ps(nblocks);
-    for (int64_t i = 0; i < nblocks; ++i) Ps[i] = pad16(std::max(m[order[i]], n[order[i]]));
+    const int64_t nroc = (int64_t)order.size();
+    std::vector<int64_t> Ps(nroc);
+    for (int64_t i = 0; i < nroc; ++i) Ps[i] = pad16(std::max(m[order[i]], n[order[i]]));
     int64_t start = 0;
     for (int64_t end : choose_buckets(Ps)) {
       const int64_t P = Ps[end - 1];
@@ -234,6 +741,62 @@ mpsh_status mpsh_svd_plan(int64_t nblocks, const int64_t* m, const int64_t* n, c
       }
       P->buckets.push_back(B);
     }
+    // Jacobi matrices: W | Q per block, then the per-matrix flags
+    P->jac_off = w;
+    int64_t jw = 0;
+    std::vector<JacTask> small_by[3], big_by[3];
+    for (int64_t b : jac_blocks) {
+      JacMat J{};
+      J.m = m[b];
+      J.n = n[b];
+      J.trans = m[b] < n[b];
+      J.r = std::max(m[b], n[b]);
+      J.c = std::min(m[b], n[b]);
+      J.w = jac_width(J.r);
+      J.nb = (int32_t)((J.c + J.w - 1) / J.w);
+      J.nb += J.nb & 1;
+      if (J.nb < 2) J.nb = 2;
+      J.ctol = std::sqrt((double)J.r) * 2.220446049250313e-16;
+      if (2 * J.w * (J.r | 1) + 4 * J.w * J.w > JAC_LDS || J.r > JF_MAX)
+        throw Error(ERR_INVALID, "svd plan: Jacobi block exceeds the LDS budget");
+      J.u_off = P->u_off[b];
+      J.s_off = P->s_off[b];
+      J.v_off = P->v_off[b];
+      J.w_off = jw;
+      jw += J.r * J.c;
+      J.q_off = jw;
+      jw += J.c * J.c;
+      jw = (jw + 1) / 2 * 2;
+      const int32_t idx = (int32_t)P->jm.size();
+      const int cls = J.w == 16 ? 0 : (J.w == 8 ? 1 : 2);
+      if (J.nb == 2) {
+        small_by[cls].push_back({idx, 0});
+      } else {
+        P->jbig_idx.push_back(idx);
+        for (int32_t k = 0; k < J.nb / 2; ++k) big_by[cls].push_back({idx, k});
+        P->jrounds = std::max(P->jrounds, J.nb - 1);
+      }
+      if (!J.trans) {
+        const int64_t dims[2] = {J.m, J.n}, sst[2] = {1, J.m}, dst[2] = {1, J.r};
+        P->jinit.add(2, dims, sst, dst, a_off[b], J.w_off);
+      } else {  // W(i, j) = A(j, i)
+        const int64_t dims[2] = {J.n, J.m}, sst[2] = {J.m, 1}, dst[2] = {1, J.r};
+        P->jinit.add(2, dims, sst, dst, a_off[b], J.w_off);
+      }
+      P->jm.push_back(J);
+    }
+    for (int k = 0; k < 3; ++k) {
+      P->jsmall.insert(P->jsmall.end(), small_by[k].begin(), small_by[k].end());
+      P->jbig.insert(P->jbig.end(), big_by[k].begin(), big_by[k].end());
+      P->jsmall_cls[k + 1] = (int32_t)P->jsmall.size();
+      P->jbig_cls[k + 1] = (int32_t)P->jbig.size();
+    }
+    w += jw;
+    P->jac_meta_off = w;
+    const int64_t nj = (int64_t)P->jm.size();
+    w += nj + 2 * ((nj + 1) / 2) + 2;  // offmax (u64) | done (int32) | sexp (int32) | left, flags
+    w = (w + 1) / 2 * 2;
+    P->jinit.finalize();
     P->ws_doubles = w;
     P->info_ints = info;
     P->pad.finalize();
@@ -284,6 +847,8 @@ mpsh_status mpsh_svd_execute(mpsh_svd plan, const double* A, double* S, double* 
     double* ws = (double*)workspace;
     rocblas_int* info = (rocblas_int*)(ws + plan->ws_doubles);
     const mpsh_svd_s::Dev& cp = plan->get();
+    if (!plan->jm.empty()) jacobi_execute(*plan, cp, A, S, U, Vt, ws, s);
+    if (plan->buckets.empty()) return MPSH_OK;
     // zero the padded inputs, then place every block in its slot
     for (const Bucket& B : plan->buckets)
       ck(hipMemsetAsync(ws + B.a_off, 0, (size_t)B.count * B.Mp * B.Np * 8, s), "hipMemsetAsync");

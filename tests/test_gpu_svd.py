@@ -4,8 +4,12 @@ Pinned by mathematics, not by the oracle's formula: U diag(S) Vh reconstructs
 each block, U and Vh have orthonormal columns / rows, S is descending and
 equals LAPACK's singular values; svdtrunc keeps exactly the oracle's counts
 (mpsutils.jl:89-112 greedy, absolute tol) and its u*s*v equals the oracle's
-truncated product (gauge-invariant).  FP64 bar: 1e-12 relative.
+truncated product (gauge-invariant).  FP64 bar: 1e-12 relative.  The
+one-sided Jacobi path (default) and the rocSOLVER gesdd path
+(``MPSH_SVD_ALGO=gesdd``) are checked against each other and against LAPACK.
 """
+import os
+
 import numpy as np
 import pytest
 import torch
@@ -89,3 +93,113 @@ def test_svdtrunc_matches_oracle(maxdim, tol):
     got = M.array(r.usv())
     want = ost.array(usv_or)
     assert np.linalg.norm(got - want) <= 1e-12 * np.linalg.norm(want)
+
+
+def _blocks_with(A, algo):
+    """_svd_blocks through a fresh plan built with MPSH_SVD_ALGO=algo (plans read it when built)."""
+    old = os.environ.get("MPSH_SVD_ALGO")
+    os.environ["MPSH_SVD_ALGO"] = algo
+    try:
+        P = mt._SvdPlan(A.struct)
+    finally:
+        if old is None:
+            del os.environ["MPSH_SVD_ALGO"]
+        else:
+            os.environ["MPSH_SVD_ALGO"] = old
+    saved = mt._SVD_PLANS.get(A.struct)
+    mt._SVD_PLANS[A.struct] = P
+    try:
+        return mt._svd_blocks(A)
+    finally:
+        if saved is None:
+            del mt._SVD_PLANS[A.struct]
+        else:
+            mt._SVD_PLANS[A.struct] = saved
+
+
+def _check_facts(blocks, facts, rtol=1e-12):
+    for key, U, S, Vh in facts:
+        b = blocks[key]
+        m, n = b.shape
+        k = min(m, n)
+        U, S, Vh = U.cpu().numpy(), S.cpu().numpy(), Vh.cpu().numpy()
+        assert U.shape == (m, k) and S.shape == (k,) and Vh.shape == (k, n)
+        assert np.all(np.diff(S) <= 0)
+        s_ref = np.linalg.svd(b, compute_uv=False)
+        scale = max(s_ref[0], 1e-300)
+        np.testing.assert_allclose(S, s_ref, rtol=rtol, atol=rtol * scale)
+        assert np.linalg.norm((U * S) @ Vh - b) <= rtol * max(np.linalg.norm(b), 1e-300)
+        np.testing.assert_allclose(U.T @ U, np.eye(k), atol=rtol)
+        np.testing.assert_allclose(Vh @ Vh.T, np.eye(k), atol=rtol)
+
+
+@pytest.mark.parametrize("shape", range(len(SHAPES)))
+def test_jacobi_and_gesdd_paths_agree(shape):
+    rows, cols = SHAPES[shape]
+    A = _matrix(rows, cols, seed=31 + shape)
+    blocks = A.blocks()
+    fj = _blocks_with(A, "jacobi")
+    fg = _blocks_with(A, "gesdd")
+    _check_facts(blocks, fj)
+    _check_facts(blocks, fg)
+    for (kj, _, Sj, _), (kg, _, Sg, _) in zip(fj, fg):
+        assert kj == kg
+        np.testing.assert_allclose(Sj.cpu().numpy(), Sg.cpu().numpy(), rtol=1e-12, atol=1e-12 * float(Sg[0]))
+
+
+def test_jacobi_graded_spectrum_relative_accuracy():
+    """Singular values spanning 1e-13..1: the Jacobi path keeps them to 1e-12 of the largest."""
+    A = _matrix([(q, 60 + 30 * q) for q in range(4)], [(q, 70 + 25 * q) for q in range(4)], seed=41, scale=0.8)
+    blocks = A.blocks()
+    _check_facts(blocks, _blocks_with(A, "jacobi"))
+
+
+def test_jacobi_rank_deficient_and_zero_blocks():
+    """Exactly zero singular values: the completed singular vectors stay orthonormal."""
+    A = _matrix([(0, 40), (1, 12), (2, 70)], [(0, 25), (1, 12), (2, 90)], seed=43)
+    blocks = A.blocks()
+    keys = list(blocks.keys())
+    blocks[keys[0]] = np.zeros_like(blocks[keys[0]])                  # all-zero block
+    b = blocks[keys[2]]
+    b[:, 45:] = b[:, :45] @ np.random.default_rng(3).standard_normal((45, b.shape[1] - 45))  # rank 45 of 70
+    blocks[keys[1]][:, 5] = 0.0                                         # one zero column
+    Z = M.SymTensor.from_blocks(A.charge, A.space, blocks, dtype=torch.float64, device=DEV)
+    facts = _blocks_with(Z, "jacobi")
+    for key, U, S, Vh in facts:
+        bb = blocks[key]
+        k = min(bb.shape)
+        U, S, Vh = U.cpu().numpy(), S.cpu().numpy(), Vh.cpu().numpy()
+        np.testing.assert_allclose(U.T @ U, np.eye(k), atol=1e-12)
+        np.testing.assert_allclose(Vh @ Vh.T, np.eye(k), atol=1e-12)
+        assert np.linalg.norm((U * S) @ Vh - bb) <= 1e-12 * max(1.0, np.linalg.norm(bb))
+        s_ref = np.linalg.svd(bb, compute_uv=False)
+        np.testing.assert_allclose(S, s_ref, atol=1e-12 * max(1.0, s_ref[0]))
+    assert np.all(facts[0][2].cpu().numpy() == 0.0)
+
+
+def test_jacobi_wide_blocks_and_rocsolver_fallback():
+    """Blocks with more than 2048 rows go to gesdd inside the same plan; w = 8 and 4 classes."""
+    A = _matrix([(0, 600), (1, 1500), (2, 2060)], [(0, 700), (1, 40), (2, 24)], seed=47)
+    blocks = A.blocks()
+    _check_facts(blocks, _blocks_with(A, "jacobi"))
+
+
+def test_svd_nonfinite_input_raises():
+    A = _matrix([(0, 50), (1, 80)], [(0, 60), (1, 90)], seed=53)
+    blocks = A.blocks()
+    k1 = list(blocks.keys())[1]
+    blocks[k1][3, 7] = np.nan
+    Z = M.SymTensor.from_blocks(A.charge, A.space, blocks, dtype=torch.float64, device=DEV)
+    with pytest.raises(Exception, match="not finite"):
+        _blocks_with(Z, "jacobi")
+
+
+def test_jacobi_extreme_block_scales():
+    """Blocks scaled to 1e-120 .. 1e150 (DMRG two-site blocks far from the mean
+    charge are ~1e-120): each block's factors match LAPACK relative to its own scale."""
+    A = _matrix([(q, [2, 2, 7, 40, 2][q]) for q in range(5)], [(q, [2, 3, 7, 35, 2][q]) for q in range(5)], seed=59)
+    blocks = A.blocks()
+    for key, f in zip(list(blocks.keys()), (1e-120, 1e-60, 1e-140, 1e150, 3e-200)):
+        blocks[key] = blocks[key] * f
+    Z = M.SymTensor.from_blocks(A.charge, A.space, blocks, dtype=torch.float64, device=DEV)
+    _check_facts(blocks, _blocks_with(Z, "jacobi"))
