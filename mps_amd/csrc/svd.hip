@@ -155,9 +155,9 @@ __device__ __forceinline__ double wave_row_sum(double v) {
 // registers and meets column (k + s) mod w of q at step s.  Over the nb - 1
 // rounds of a sweep every column pair of the matrix is rotated once.
 // iterate = 1 (matrices of two blocks, the pair holds every column): sweeps
-// until converged.  One wave per rotation (w waves per work group); a lane
-// holds RPL rows of a column, r <= 64 RPL.  Rows r .. 64 RPL of every column
-// are zero, so loads and stores run unmasked.  The 2w x 2w rotation R is kept
+// until converged.  GSZ lanes per rotation (w GSZ threads per work group); a
+// lane holds RPL rows of a column, r <= GSZ RPL.  Rows r .. GSZ RPL of every
+// column are zero, so loads and stores run unmasked.  The 2w x 2w rotation R is kept
 // transposed (Rt[row * (2w + 1) + col]).
 // offmax[mat] is set to 1 when any pair of the sweep was rotated (some
 // |x_i.x_j| > ctol |x_i| |x_j|); a sweep without rotations ends the matrix.
@@ -172,13 +172,26 @@ __device__ __forceinline__ double wave_sum(double v) {
   return (read_lane(v, 0) + read_lane(v, 16)) + (read_lane(v, 32) + read_lane(v, 48));
 }
 
-template <int WB>
-__global__ __launch_bounds__(64 * WB) void jac_pairs(const JacMat* __restrict__ mats,
-                                                     const JacTask* __restrict__ tasks, int t, int iterate,
-                                                     double* __restrict__ ws, unsigned long long* __restrict__ offmax,
-                                                     int* __restrict__ done, int* __restrict__ flags) {
-  constexpr int NT = 64 * WB, NC = 2 * WB, GSZ = 64, RPL = WB == 16 ? 8 : (WB == 8 ? 16 : 32), LDX = GSZ * RPL + 1,
-                RTS = NC + 1;
+template <int GSZ>
+__device__ __forceinline__ double group_sum(double v) {
+  if (GSZ == 64) return wave_sum(v);
+  v = wave_row_sum(v);                           // rows of 16, uniform per row
+  if (GSZ >= 32) v += __shfl_xor(v, 16, 64);     // commutative: uniform per group
+  return v;
+}
+
+template <int WB, int GSZ>
+__global__ __launch_bounds__(WB * GSZ) void jac_pairs(const JacMat* __restrict__ mats,
+                                                      const JacTask* __restrict__ tasks, int t, int iterate,
+                                                      double* __restrict__ ws,
+                                                      unsigned long long* __restrict__ offmax,
+                                                      int* __restrict__ done, int* __restrict__ flags) {
+  // GSZ lanes per rotation (64: one wave; 32: two rotations share a wave and
+  // its scalar instructions -- measured no faster, the launches use 64); a
+  // lane holds RPL rows of a column
+  constexpr int NT = WB * GSZ, NC = 2 * WB, RPL = (WB == 16 ? 512 : (WB == 8 ? 1024 : 2048)) / GSZ,
+                LDX = GSZ * RPL + 1, RTS = NC + 1;
+  static_assert(NC <= GSZ, "one Rt row per lane of a group");
   static_assert(NC * LDX + NC * RTS <= JAC_LDS, "LDS budget");
   __shared__ __align__(16) double lds[JAC_LDS];
   __shared__ int rot_any;
@@ -223,29 +236,32 @@ __global__ __launch_bounds__(64 * WB) void jac_pairs(const JacMat* __restrict__ 
       b1 = fma(xj[u + 1], xj[u + 1], b1);
       g1 = fma(xi[u + 1], xj[u + 1], g1);
     }
-    const double a = wave_sum(a0 + a1), b = wave_sum(b0 + b1), g = wave_sum(g0 + g1);
+    const double a = group_sum<GSZ>(a0 + a1), b = group_sum<GSZ>(b0 + b1), g = group_sum<GSZ>(g0 + g1);
     // rotate when |g| > ctol |x_i| |x_j|; columns of norm^2 <= 1e-280 (the
     // matrix is scaled to entries < 1) count as zero, and so does NaN
-    if (!(a > 1e-280 && b > 1e-280 && fabs(g) > ctol * (sqrt(a) * sqrt(b)))) return false;
+    const bool rot = a > 1e-280 && b > 1e-280 && fabs(g) > ctol * (sqrt(a) * sqrt(b));
+    if (!__any(rot)) return false;
     // t = tan of the angle that zeroes the pair's product (Rutishauser):
     // t = sign(z) / (|z| + sqrt(1 + z^2)), z = (b - a) / (2g); for |z| > 1e150,
-    // t = 1 / (2z) (z^2 would overflow)
-    const double z = (b - a) / (2.0 * g);
+    // t = 1 / (2z) (z^2 would overflow).  A group that does not rotate in a
+    // wave that does applies the identity.
+    const double z = (b - a) / (2.0 * (rot ? g : 1.0));
     const double az = fabs(z);
     const double tt = az > 1e150 ? 0.5 / z : copysign(1.0 / (az + sqrt(fma(az, az, 1.0))), z);
-    const double cs = 1.0 / sqrt(fma(tt, tt, 1.0)), sn = cs * tt;
+    const double c0 = 1.0 / sqrt(fma(tt, tt, 1.0));
+    const double cs = rot ? c0 : 1.0, sn = rot ? c0 * tt : 0.0;
 #pragma unroll
     for (int u = 0; u < RPL; ++u) {
       const double x = xi[u], y = xj[u];
       xi[u] = cs * x - sn * y;
       xj[u] = sn * x + cs * y;
     }
-    if (lane < NC) {
+    if (rot && lane < NC) {
       const double x = Rt[lane * RTS + i], y = Rt[lane * RTS + j];
       Rt[lane * RTS + i] = cs * x - sn * y;
       Rt[lane * RTS + j] = sn * x + cs * y;
     }
-    return true;
+    return rot;
   };
   auto load = [&](double (&x)[RPL], int col) {
     const double* src = X + col * LDX + lane;
@@ -298,7 +314,7 @@ __global__ __launch_bounds__(64 * WB) void jac_pairs(const JacMat* __restrict__ 
       rotated = true;
       store(xi, grp);
     }
-    if (rotated && lane == 0) rot_any = 1;
+    if (rotated) rot_any = 1;
     __syncthreads();
     conv = !rot_any;
     __syncthreads();
@@ -618,13 +634,16 @@ void jacobi_execute(const mpsh_svd_s& P, const mpsh_svd_s::Dev& cp, const double
     for (int k = 0; k < 3; ++k) {
       const int n = cls[k + 1] - cls[k];
       if (n == 0) continue;
-      const dim3 g((unsigned)n), b(k == 0 ? 1024 : (k == 1 ? 512 : 256));
+      const dim3 g((unsigned)n);
       if (k == 0)
-        hipLaunchKernelGGL(jac_pairs<16>, g, b, 0, s, cp.jm, tasks + cls[k], t, iterate, jw, offmax, done, flags);
+        hipLaunchKernelGGL((jac_pairs<16, 64>), g, dim3(1024), 0, s, cp.jm, tasks + cls[k], t, iterate, jw, offmax,
+                           done, flags);
       else if (k == 1)
-        hipLaunchKernelGGL(jac_pairs<8>, g, b, 0, s, cp.jm, tasks + cls[k], t, iterate, jw, offmax, done, flags);
+        hipLaunchKernelGGL((jac_pairs<8, 64>), g, dim3(512), 0, s, cp.jm, tasks + cls[k], t, iterate, jw, offmax,
+                           done, flags);
       else
-        hipLaunchKernelGGL(jac_pairs<4>, g, b, 0, s, cp.jm, tasks + cls[k], t, iterate, jw, offmax, done, flags);
+        hipLaunchKernelGGL((jac_pairs<4, 64>), g, dim3(256), 0, s, cp.jm, tasks + cls[k], t, iterate, jw, offmax,
+                           done, flags);
     }
     ck(hipGetLastError(), "jac_pairs");
   };
