@@ -45,8 +45,10 @@ def parse():
                     help="skip the GEMM autotuner (variants from MPSH_HEFF_TILES or the defaults); for profiling")
     ap.add_argument("--lanczos-graphs", type=int, default=0, choices=[0, 1],
                     help="capture each Lanczos iteration of the DMRG window in a hipGraph")
-    ap.add_argument("--dmrg-bonds", type=int, default=6,
+    ap.add_argument("--dmrg-bonds", type=int, default=0,
                     help="timed two-site updates at chi=2048 in the bulk of the 6x24 cylinder (0: skip)")
+    ap.add_argument("--dmrg-sweep", type=int, default=1, choices=[0, 1],
+                    help="time one full two-site DMRG sweep at chi=2048 on the 6x24 cylinder")
     ap.add_argument("--precision", default="none", choices=["none", "chain", "tv", "all"],
                     help="precision studies: XXZ L=100 chi=512 four sweeps (chain), t-V L=128 chi=4096 "
                          "middle-bond eigensolve (tv)")
@@ -295,6 +297,53 @@ def precision_tv(dev):
             "runs": runs}
 
 
+def dmrg_sweep(dev):
+    """One full two-site DMRG sweep (L->R->L, dmrg2sitesweep!) at chi=2048 on the
+    6x24 Heisenberg cylinder from a seeded random U(1) MPS canonical at site 1:
+    wall time of the sweep with its stage totals; the final energy is checked
+    against an independent device <psi|H|psi> (measure) of the swept MPS."""
+    import torch
+    from mps_amd import mpstools as mt
+    from mps_amd import tensornetalgs as ta
+    from mps_amd.symtensors import contract
+    c = DMRG_CFG
+    L, N = c["W"] * c["Lx"], c["W"] * c["Lx"] // 2
+    t0 = time.perf_counter()
+    mpo = mt.heisenberg_cylinder_mpo(c["W"], c["Lx"], device=dev)
+    mps = mt.random_mps(L, N, c["chi"], seed=c["seed"], nsec=c["nsec"], dmax=c["dmax"], device=dev)
+    env = ta.initialenv(mps, mpo)
+    torch.cuda.synchronize()
+    setup_s = time.perf_counter() - t0
+    print(f"dmrg sweep: random chi={c['chi']} MPS and environments ready ({setup_s:.1f} s)", file=sys.stderr,
+          flush=True)
+    kw = dict(maxdim=c["maxdim"], tol=c["tol"], lanczostol=c["lanczostol"], krylovdim=c["krylovdim"],
+              krylovmaxiter=c["krylovmaxiter"], prec=c["prec"], refine_tol=c["refine_tol"])
+    stages, log = {}, []
+    tc0 = dict(ta.TUNE_COUNTS)
+    torch.cuda.synchronize()
+    t1 = time.perf_counter()
+    energies = ta.dmrg2sitesweep(mps, mpo, env, log=log, stage_ms=stages, **kw)
+    torch.cuda.synchronize()
+    sweep_s = time.perf_counter() - t1
+    print(f"dmrg sweep: done ({sweep_s:.1f} s)", file=sys.stderr, flush=True)
+    e_measure = mt.measure(mps, mpo)
+    updates = sum(1 for x in log if "matvecs" in x)
+    AA = contract(mps.As[0], (1, 2, -1), mps.As[1], (-1, 3, 4))
+    state = (AA, env[0], env[3], mpo.Ws[0], mpo.Ws[1])
+    out = {"model": "Heisenberg 6x24 cylinder (snake), L=144, N=72, MPO D=20", "chi": c["chi"],
+           "sweep_s": round(sweep_s, 3), "setup_s": round(setup_s, 2), "bond_updates": updates,
+           "ms_per_update": round(1e3 * sweep_s / max(1, updates), 3),
+           "stage_s": {k: round(v / 1e3, 3) for k, v in stages.items()},
+           "matvecs": sum(x["matvecs"] for x in log if "matvecs" in x),
+           "max_bond": max(mps.bonddims()), "energy_final": energies[-1], "energy_first": energies[0],
+           "energy_measure": e_measure, "energy_rel_diff_vs_measure": abs(energies[-1] - e_measure) / abs(e_measure),
+           "max_discarded_weight": max(x["discarded_weight"] for x in log if "discarded_weight" in x),
+           "autotune_in_sweep": {k: ta.TUNE_COUNTS[k] - tc0[k] for k in tc0},
+           "settings": {k: c[k] for k in ("maxdim", "tol", "lanczostol", "krylovdim", "krylovmaxiter", "prec",
+                                           "refine_tol")}}
+    return out, state
+
+
 def cpu_baseline(name, budget):
     """Oracle (numpy restatement of the reference algorithm) on the host cores."""
     import numpy as np
@@ -446,6 +495,11 @@ def main():
         ta.LANCZOS_GRAPHS = bool(args.lanczos_graphs)
         result["dmrg"], dmrg_state = dmrg_window(args.dmrg_bonds, dev)
         result["dmrg"]["lanczos_hipgraph"] = ta.LANCZOS_GRAPHS
+    sweep_state = None
+    if args.dmrg_sweep:
+        from mps_amd import tensornetalgs as ta
+        ta.LANCZOS_GRAPHS = bool(args.lanczos_graphs)
+        result["dmrg_sweep"], sweep_state = dmrg_sweep(dev)
     if args.precision in ("chain", "all"):
         result["precision_chain"] = precision_chain(dev)
     if args.precision in ("tv", "all"):
@@ -463,6 +517,8 @@ def main():
         result["f64_energy_rel_err_vs_cpu_oracle"] = abs(e64 - e_or) / abs(e_or)
         if dmrg_state is not None:
             result["dmrg"]["energy_vs_cpu_oracle"] = dmrg_energy_check(dmrg_state)
+        if sweep_state is not None:
+            result["dmrg_sweep"]["energy_vs_cpu_oracle"] = dmrg_energy_check(sweep_state)
     print(json.dumps(result), flush=True)
 
 

@@ -172,6 +172,21 @@ __device__ __forceinline__ double wave_sum(double v) {
   return (read_lane(v, 0) + read_lane(v, 16)) + (read_lane(v, 32) + read_lane(v, 48));
 }
 
+// 1/x and 1/sqrt(x): the hardware estimates refined by two Newton steps (each
+// squares the relative error).  The rotation is orthogonal to rounding because
+// cs is the refined 1/sqrt(1 + t^2) and sn = cs t.
+__device__ __forceinline__ double rcp_nr(double x) {
+  double r = __builtin_amdgcn_rcp(x);
+  r = fma(r, fma(-x, r, 1.0), r);
+  return fma(r, fma(-x, r, 1.0), r);
+}
+__device__ __forceinline__ double rsq_nr(double x) {
+  double y = __builtin_amdgcn_rsq(x);
+  const double h = 0.5 * x;
+  y = fma(y, fma(-h * y, y, 0.5), y);
+  return fma(y, fma(-h * y, y, 0.5), y);
+}
+
 template <int GSZ>
 __device__ __forceinline__ double group_sum(double v) {
   if (GSZ == 64) return wave_sum(v);
@@ -239,16 +254,20 @@ __global__ __launch_bounds__(WB * GSZ) void jac_pairs(const JacMat* __restrict__
     const double a = group_sum<GSZ>(a0 + a1), b = group_sum<GSZ>(b0 + b1), g = group_sum<GSZ>(g0 + g1);
     // rotate when |g| > ctol |x_i| |x_j|; columns of norm^2 <= 1e-280 (the
     // matrix is scaled to entries < 1) count as zero, and so does NaN
-    const bool rot = a > 1e-280 && b > 1e-280 && fabs(g) > ctol * (sqrt(a) * sqrt(b));
+    // (the threshold test needs no full precision: raw hardware estimates)
+    const bool rot = a > 1e-280 && b > 1e-280 &&
+                     fabs(g) * __builtin_amdgcn_rsq(a) * __builtin_amdgcn_rsq(b) > ctol;
     if (!__any(rot)) return false;
     // t = tan of the angle that zeroes the pair's product (Rutishauser):
-    // t = sign(z) / (|z| + sqrt(1 + z^2)), z = (b - a) / (2g); for |z| > 1e150,
-    // t = 1 / (2z) (z^2 would overflow).  A group that does not rotate in a
-    // wave that does applies the identity.
-    const double z = (b - a) / (2.0 * (rot ? g : 1.0));
+    // t = sign(z) / (|z| + sqrt(1 + z^2)), z = (b - a) / (2g); for |z| > 1e150
+    // sqrt(1 + z^2) is taken as |z| (z^2 would overflow), t = 1 / (2z).  A
+    // group that does not rotate in a wave that does applies the identity.
+    const double z = (b - a) * rcp_nr(2.0 * (rot ? g : 1.0));
     const double az = fabs(z);
-    const double tt = az > 1e150 ? 0.5 / z : copysign(1.0 / (az + sqrt(fma(az, az, 1.0))), z);
-    const double c0 = 1.0 / sqrt(fma(tt, tt, 1.0));
+    const double w1 = fma(az, az, 1.0);
+    const double sq = az > 1e150 ? az : w1 * rsq_nr(w1);
+    const double tt = copysign(rcp_nr(az + sq), z);
+    const double c0 = rsq_nr(fma(tt, tt, 1.0));
     const double cs = rot ? c0 : 1.0, sn = rot ? c0 * tt : 0.0;
 #pragma unroll
     for (int u = 0; u < RPL; ++u) {

@@ -13,6 +13,8 @@ from __future__ import annotations
 
 from typing import Dict, Optional
 
+from collections import OrderedDict
+
 import numpy as np
 import torch
 
@@ -38,7 +40,6 @@ class HeffPlan:
         self.ref_elems = int(st[1])
         self.step_flops = [int(st[2 + i]) for i in range(4)]
         self.step_elems = [int(st[6 + i]) for i in range(4)]
-        self._ws: Dict = {}
 
     def workspace_bytes(self, prec: str) -> int:
         """Workspace of the one-shot apply (prepared operator included)."""
@@ -51,13 +52,10 @@ class HeffPlan:
         return int(lib().mpsh_heff_prepared_size(self.plan, _PREC[prec]))
 
     def workspace(self, prec: str, device, kind: str = "oneshot") -> torch.Tensor:
-        key = (prec, device, kind)
-        ws = self._ws.get(key)
-        if ws is None:
-            n = self.workspace_bytes(prec) if kind == "oneshot" else self.apply_workspace_bytes(prec)
-            ws = torch.empty(max(16, n), dtype=torch.uint8, device=device)
-            self._ws[key] = ws
-        return ws
+        """Scratch of the one-shot apply ("oneshot") or of the prepared apply ("apply"):
+        a view of the device's shared H_eff scratch (applies run in stream order)."""
+        n = self.workspace_bytes(prec) if kind == "oneshot" else self.apply_workspace_bytes(prec)
+        return _scratch(device, max(16, n))
 
     def apply(self, prec: str, v: torch.Tensor, envL: torch.Tensor, envR: torch.Tensor, WL: torch.Tensor,
               WR: torch.Tensor, out: torch.Tensor, ws: torch.Tensor = None, stream=None):
@@ -110,7 +108,29 @@ class HeffPlan:
             pass
 
 
-_HEFF: Dict = {}
+# One H_eff scratch buffer per device, shared by every plan: a sweep meets a new
+# bond structure at every bond, and per-plan buffers (~1 GB each at chi=2048)
+# would pile up.  The buffer only grows; a replaced buffer is kept alive so a
+# pointer captured earlier (hipGraph replay) never points at freed memory.
+_SCRATCH: Dict = {}
+_SCRATCH_OLD: list = []
+
+
+def _scratch(device, nbytes: int) -> torch.Tensor:
+    device = torch.device(device)
+    buf = _SCRATCH.get(device)
+    if buf is None or buf.numel() < nbytes:
+        if buf is not None:
+            _SCRATCH_OLD.append(buf)
+        buf = torch.empty(int(nbytes * 1.25) + 256, dtype=torch.uint8, device=device)
+        _SCRATCH[device] = buf
+    return buf[:nbytes]
+
+
+# plans (host and device tables, a few hundred KB each at chi=2048), least
+# recently used dropped beyond HEFF_PLAN_CACHE
+HEFF_PLAN_CACHE = 512
+_HEFF: "OrderedDict" = OrderedDict()
 
 
 def heff_plan(v: SymTensor, envL: SymTensor, envR: SymTensor, WL: SymTensor, WR: SymTensor) -> HeffPlan:
@@ -119,6 +139,10 @@ def heff_plan(v: SymTensor, envL: SymTensor, envR: SymTensor, WL: SymTensor, WR:
     if p is None:
         p = HeffPlan(v.struct, envL.struct, envR.struct, WL.struct, WR.struct)
         _HEFF[key] = p
+        while len(_HEFF) > HEFF_PLAN_CACHE:
+            _HEFF.popitem(last=False)
+    else:
+        _HEFF.move_to_end(key)
     return p
 
 
@@ -473,15 +497,19 @@ def dmrg2site_window(mps, mpo, env, nbonds: int, maxdim: int = 200, tol: float =
 
 def dmrg2sitesweep(mps, mpo, env, maxdim: int = 200, tol: float = 1e-9, lanczostol: float = 1e-7,
                    krylovdim: int = 5, krylovmaxiter: int = 8, verbose: bool = False, prec: str = "f64",
-                   refine_tol: float = None, log: list = None):
+                   refine_tol: float = None, log: list = None, stage_ms: dict = None):
     """``dmrg2sitesweep!(mps, mpo, env; ...)`` (dmrg.jl:58-179) with every contraction on the device.
 
     Keeps the reference's energy list including quirk Q6 (each R->L energy
     pushed twice, dmrg.jl:132,156).  ``prec`` is the H_eff arithmetic of the
     Lanczos matvecs; with ``refine_tol`` the eigenpair of a reduced-precision
     solve is refined by FP64 Lanczos cycles.  Energies after truncation are
-    always FP64 matvecs.
+    always FP64 matvecs.  With ``stage_ms`` a dict, the wall milliseconds of
+    the eigensolves, truncations (SVD), energies and the rest (two-site tensors
+    and environment updates) are accumulated into it (the device is
+    synchronised at stage boundaries).
     """
+    import time
     from .mpstools import svdtrunc
     from .symtensors import dot, splitleg, symmatrix
     lx = len(mps)
@@ -489,8 +517,20 @@ def dmrg2sitesweep(mps, mpo, env, maxdim: int = 200, tol: float = 1e-9, lanczost
         raise ValueError("The center of MPS has to be 1 for dmrg L->R->L")
     Ws = mpo.Ws
     energies = []
+    timed = stage_ms is not None
+    sync = torch.cuda.synchronize if (timed and mps.As[0].data.is_cuda) else (lambda: None)
 
-    def solve(AA, l):
+    def stage(name, fn, *a):
+        if not timed:
+            return fn(*a)
+        sync()
+        t0 = time.perf_counter()
+        out = fn(*a)
+        sync()
+        stage_ms[name] = stage_ms.get(name, 0.0) + 1e3 * (time.perf_counter() - t0)
+        return out
+
+    def _solve(AA, l):
         H = HeffOperator(env[l - 1], env[l + 2], Ws[l - 1], Ws[l], prec)
         theta, v, info = eigsolve(H, AA, lanczostol, krylovdim, krylovmaxiter, refine_tol)
         if log is not None:
@@ -499,7 +539,7 @@ def dmrg2sitesweep(mps, mpo, env, maxdim: int = 200, tol: float = 1e-9, lanczost
             print(f"bond {l},{l + 1}: energy {theta} normres {info.normres} #applications {info.numops}")
         return v
 
-    def trunc(v, AA):
+    def _trunc(v, AA):
         r = svdtrunc(symmatrix(v, [1, 2], [3, 4]), maxdim=maxdim, tol=tol)
         if log is not None:
             s = r.singular_values()
@@ -508,8 +548,17 @@ def dmrg2sitesweep(mps, mpo, env, maxdim: int = 200, tol: float = 1e-9, lanczost
         AA2 = splitleg(splitleg(r.usv(), 2, AA.space[2:4]), 1, AA.space[0:2])
         return r, AA2
 
-    def energy(AA2, l):
+    def _energy(AA2, l):
         return dot(AA2, _applymps2site(AA2, env[l - 1], env[l + 2], Ws[l - 1], Ws[l]))
+
+    def solve(AA, l):
+        return stage("eigsolve", _solve, AA, l)
+
+    def trunc(v, AA):
+        return stage("svdtrunc", _trunc, v, AA)
+
+    def energy(AA2, l):
+        return stage("energy", _energy, AA2, l)
 
     A = mps.As[0]
     for l in range(1, lx - 1):
