@@ -433,6 +433,213 @@ __global__ __launch_bounds__(256) void gemm_grouped(
   }
 }
 
+// ---------------------------------------------------------------------------
+// FP32 grouped GEMM with operands streamed to LDS by DMA (global_load_lds
+// dwordx4: no register staging), a 4-stage LDS ring (loads run three K chunks
+// ahead) and one barrier per chunk.  Same problem / segment / tile / split-K /
+// output conventions as gemm_grouped (tile variant KS = 0).  In LDS a stage
+// holds BM + BN operand rows of 16 floats; lane l of wave w fills the 16-byte
+// slot (row, q) = (w*16 + l/4 + 64 i, l%4) with the K group q ^ ((row >> 2) & 3),
+// so the MFMA operand reads (16 rows at one K group) hit distinct banks.  K
+// groups past a segment's K are read from the row start and zeroed in LDS
+// (only in a segment's last chunk), as are the entries past K of the last group.
+// ---------------------------------------------------------------------------
+template <int N>
+__device__ __forceinline__ void wait_vmcnt() {
+  static_assert(N < 64, "vmcnt");
+  __builtin_amdgcn_s_waitcnt((N & 15) | (((N >> 4) & 3) << 14) | (7 << 4) | (15 << 8));
+}
+__device__ __forceinline__ void wait_lgkmcnt0() { __builtin_amdgcn_s_waitcnt(15 | (3 << 14) | (7 << 4)); }
+// a barrier without the compiler's release fence (which would wait for every
+// outstanding vector memory operation, the DMA loads of later chunks included);
+// the callers wait for the counters they need first
+__device__ __forceinline__ void barrier_raw() { asm volatile("s_barrier" ::: "memory"); }
+
+typedef __attribute__((address_space(3))) void* lds_void_ptr;
+
+template <int BM, int BN>
+__global__ __launch_bounds__(256) void gemm_dma(
+    const GemmProb* __restrict__ probs, const GemmSeg* __restrict__ segs, const int64_t* __restrict__ atab,
+    const int64_t* __restrict__ btab, const int64_t* __restrict__ crow, const int64_t* __restrict__ cld,
+    const GemmTile* __restrict__ tiles, XcdQueues xq, const float* __restrict__ A, const float* __restrict__ B,
+    float* __restrict__ C, float* __restrict__ P) {
+  constexpr int BK = 16, NA = BM / 64, NB = BN / 64, SM = BM / 32, SN = BN / 32;
+  constexpr int STG = (BM + BN) * BK;  // floats per stage
+  // the four stages are distinct LDS variables so the compiler can tell a DMA
+  // into one stage from the operand reads of another
+  __shared__ __attribute__((aligned(16))) float S0[STG], S1[STG], S2[STG], S3[STG];
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const int wm = wave >> 1, wn = wave & 1;
+  const int g = lane >> 4, l16 = lane & 15;
+  const int lrow = wave * 16 + (lane >> 2), lq = lane & 3;  // + 64 i for the i-th slot of a lane
+  const int lkl = lq ^ ((lrow >> 2) & 3);  // (64 i + lrow) >> 2 & 3 does not depend on i
+  int offa[SM], offb[SN];  // operand read offsets (floats) within a stage
+#pragma unroll
+  for (int i = 0; i < SM; ++i) {
+    const int r = wm * (BM / 2) + i * 16 + l16;
+    offa[i] = (r * 4 + (g ^ ((r >> 2) & 3))) * 4;
+  }
+#pragma unroll
+  for (int j = 0; j < SN; ++j) {
+    const int r = wn * (BN / 2) + j * 16 + l16;
+    offb[j] = BM * BK + (r * 4 + (g ^ ((r >> 2) & 3))) * 4;
+  }
+  typedef std::integral_constant<int, 0> I0;
+  typedef std::integral_constant<int, 1> I1;
+  typedef std::integral_constant<int, 2> I2;
+  typedef std::integral_constant<int, 3> I3;
+  auto stage = [&](auto ST) -> float* {
+    constexpr int st = decltype(ST)::value;
+    return st == 0 ? S0 : st == 1 ? S1 : st == 2 ? S2 : S3;
+  };
+
+  const int xcd = blockIdx.x % NXCD, gq = gridDim.x / NXCD;
+  const int tend = xq.off[xcd + 1];
+  for (int t = xq.off[xcd] + blockIdx.x / NXCD; t < tend; t += gq) {
+    const GemmTile tile = tiles[t];
+    const GemmProb p = probs[tile.prob];
+    floatx4 acc[SM][SN];
+#pragma unroll
+    for (int i = 0; i < SM; ++i)
+#pragma unroll
+      for (int j = 0; j < SN; ++j) acc[i][j] = floatx4{0, 0, 0, 0};
+    int64_t ctot = 0;
+    for (int s = 0; s < p.nseg; ++s) ctot += (segs[p.seg0 + s].K + BK - 1) / BK;
+    const int64_t cb = ctot * tile.split / p.nsplit, ce = ctot * (tile.split + 1) / p.nsplit;
+    const int nch = (int)(ce - cb);
+    // issue side: segment, k offset, this lane's row bases
+    int seg = 0;
+    int64_t k0 = 0, K = 0, cpos = 0;
+    for (; seg < p.nseg; ++seg) {
+      const int64_t nc = (segs[p.seg0 + seg].K + BK - 1) / BK;
+      if (cpos + nc > cb) {
+        k0 = (cb - cpos) * BK;
+        break;
+      }
+      cpos += nc;
+    }
+    int64_t ab[NA], bb[NB];
+    auto bases = [&]() {
+      const GemmSeg sg = segs[p.seg0 + seg];
+      K = sg.K;
+#pragma unroll
+      for (int i = 0; i < NA; ++i) {
+        const int64_t r = tile.m0 + lrow + 64 * i < p.M ? tile.m0 + lrow + 64 * i : 0;
+        ab[i] = sg.atab0 >= 0 ? atab[sg.atab0 + r] : sg.a_off + r * sg.lda;
+      }
+#pragma unroll
+      for (int i = 0; i < NB; ++i) {
+        const int64_t r = tile.n0 + lrow + 64 * i < p.N ? tile.n0 + lrow + 64 * i : 0;
+        bb[i] = sg.btab0 >= 0 ? btab[sg.btab0 + r] : sg.b_off + r * sg.ldb;
+      }
+    };
+    if (nch > 0) bases();
+    int krem[4];  // K entries left at the start of the chunk held by each stage (>= 16: no tail)
+    auto issue = [&](auto ST) {
+      constexpr int st = decltype(ST)::value;
+      float* sp = stage(ST);
+      const int64_t kg = k0 + 4 * lkl;
+      const int64_t ko = kg < K ? kg : 0;  // groups past K: read the row start, zeroed later
+      krem[st] = (int)std::min<int64_t>(K - k0, BK);
+#pragma unroll
+      for (int i = 0; i < NA; ++i)
+        __builtin_amdgcn_global_load_lds((const void*)(A + ab[i] + ko), (lds_void_ptr)(sp + (i * 256 + wave * 64) * 4),
+                                         16, 0, 0);
+#pragma unroll
+      for (int i = 0; i < NB; ++i)
+        __builtin_amdgcn_global_load_lds((const void*)(B + bb[i] + ko),
+                                         (lds_void_ptr)(sp + BM * BK + (i * 256 + wave * 64) * 4), 16, 0, 0);
+      k0 += BK;
+      if (k0 >= K && seg + 1 < p.nseg) {
+        ++seg;
+        k0 = 0;
+        bases();
+      }
+    };
+    auto compute = [&](auto ST) {
+      constexpr int st = decltype(ST)::value;
+      const float* sp = stage(ST);
+      if (krem[st] < BK) {
+        // segment tail: zero this lane's own slots past K, then make them visible
+        float* wp = stage(ST);
+        const int kr = krem[st];
+#pragma unroll
+        for (int i = 0; i < NA + NB; ++i) {
+          float* slot = wp + (i < NA ? 0 : BM * BK) + ((i < NA ? i : i - NA) * 256 + wave * 64 + lane) * 4;
+#pragma unroll
+          for (int e = 0; e < 4; ++e)
+            if (4 * lkl + e >= kr) slot[e] = 0.0f;
+        }
+        wait_lgkmcnt0();
+        barrier_raw();
+      }
+      floatx4 fa[SM], fb[SN];
+#pragma unroll
+      for (int j = 0; j < SN; ++j) fb[j] = *(const floatx4*)(sp + offb[j]);
+#pragma unroll
+      for (int i = 0; i < SM; ++i) fa[i] = *(const floatx4*)(sp + offa[i]);
+#pragma unroll
+      for (int kk = 0; kk < 4; ++kk)
+#pragma unroll
+        for (int i = 0; i < SM; ++i)
+#pragma unroll
+          for (int j = 0; j < SN; ++j)
+            acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x4f32(fb[j][kk], fa[i][kk], acc[i][j], 0, 0, 0);
+    };
+    // chunk it: wait for its DMA (the chunks after it may stay in flight), barrier
+    // (every wave's part landed; every wave is done with chunk it-1, whose stage
+    // (it+3)%4 is refilled now), issue chunk it+3, compute chunk it
+    auto iter = [&](auto ST, int it) {
+      constexpr int st = decltype(ST)::value;
+      if (nch - 1 - it >= 2)
+        wait_vmcnt<2 * (NA + NB)>();
+      else if (nch - 1 - it == 1)
+        wait_vmcnt<NA + NB>();
+      else
+        wait_vmcnt<0>();
+      barrier_raw();
+      if (it + 3 < nch) issue(std::integral_constant<int, (st + 3) % 4>{});
+      compute(ST);
+    };
+    if (0 < nch) issue(I0{});
+    if (1 < nch) issue(I1{});
+    if (2 < nch) issue(I2{});
+    for (int it = 0; it < nch; it += 4) {
+      iter(I0{}, it);
+      if (it + 1 < nch) iter(I1{}, it + 1);
+      if (it + 2 < nch) iter(I2{}, it + 2);
+      if (it + 3 < nch) iter(I3{}, it + 3);
+    }
+    // epilogue as in gemm_grouped (FP32 form: register r is column 4g + r)
+    const bool split = p.nsplit > 1;
+    float* Pp = P + (split ? p.part_off + (int64_t)tile.split * p.M * p.N : 0);
+#pragma unroll
+    for (int i = 0; i < SM; ++i) {
+      const int m = tile.m0 + wm * (BM / 2) + i * 16 + l16;
+      if (m >= p.M) continue;
+      int64_t cb_ = m, cl_ = p.M;
+      if (!split) {
+        if (p.crow0 >= 0) {
+          cb_ = crow[p.crow0 + m];
+          cl_ = cld[p.crow0 + m];
+        } else {
+          cb_ = p.c_off + m;
+          cl_ = p.ldc;
+        }
+      }
+      float* Cb = split ? Pp : C;
+#pragma unroll
+      for (int j = 0; j < SN; ++j)
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+          const int n = tile.n0 + wn * (BN / 2) + j * 16 + 4 * g + r;
+          if (n < p.N) Cb[cb_ + (int64_t)n * cl_] = acc[i][j][r];
+        }
+    }
+    barrier_raw();  // the next tile refills the stages
+  }
+}
+
 // C(m, n) = sum_s P[part_off + s*M*N + m + M*n] for every split problem
 template <typename T>
 __global__ __launch_bounds__(256) void splitk_reduce(const RedDesc* __restrict__ reds, int nred,
@@ -499,8 +706,25 @@ static void launch_variant(const DevGemm& gm, const void* A, const void* B, void
                      gm.crow, gm.cld, gm.tiles, gm.q, (const T*)A, (const T*)B, (T*)C, (T*)P);
 }
 
+template <int BM, int BN>
+static void launch_variant_dma(const DevGemm& gm, const void* A, const void* B, void* C, void* P, hipStream_t s) {
+  static int occ = 0;
+  if (occ == 0) {
+    int nb = 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, gemm_dma<BM, BN>, 256, 0) != hipSuccess || nb <= 0) nb = 2;
+    occ = nb;
+  }
+  int qmax = 0;
+  for (int x = 0; x < NXCD; ++x) qmax = std::max(qmax, gm.q.off[x + 1] - gm.q.off[x]);
+  const int per = std::min(std::max(1, num_cu() / NXCD) * occ, qmax);
+  hipLaunchKernelGGL((gemm_dma<BM, BN>), dim3((unsigned)(per * NXCD)), dim3(256), 0, s, gm.probs, gm.segs, gm.atab,
+                     gm.btab, gm.crow, gm.cld, gm.tiles, gm.q, (const float*)A, (const float*)B, (float*)C,
+                     (float*)P);
+}
+
 // Compiled tile variants: (BM, BN) in {64x64, 128x64, 64x128, 128x128} with K
-// chunk multiplier KS in {1, 2} (FP64: KS = 1 only).
+// chunk multiplier KS in {1, 2} (FP64: KS = 1 only); FP32 also KS = 0, the
+// DMA-streamed kernel (K chunk 16) for 64x64, 128x64 and 64x128.
 template <int MODE>
 static hipError_t launch_mode(const DevGemm& gm, const void* A, const void* B, void* C, void* P, hipStream_t s) {
   const int key = gm.BM * 10000 + gm.BN * 10 + gm.KS;
@@ -526,6 +750,18 @@ static hipError_t launch_mode(const DevGemm& gm, const void* A, const void* B, v
       MPSH_V(128, 64, 2)
       MPSH_V(64, 128, 2)
       MPSH_V(128, 128, 2)
+      case 64 * 10000 + 64 * 10:
+        if (MODE != 1) return hipErrorInvalidValue;
+        launch_variant_dma<64, 64>(gm, A, B, C, P, s);
+        break;
+      case 128 * 10000 + 64 * 10:
+        if (MODE != 1) return hipErrorInvalidValue;
+        launch_variant_dma<128, 64>(gm, A, B, C, P, s);
+        break;
+      case 64 * 10000 + 128 * 10:
+        if (MODE != 1) return hipErrorInvalidValue;
+        launch_variant_dma<64, 128>(gm, A, B, C, P, s);
+        break;
       default:
         return hipErrorInvalidValue;
     }

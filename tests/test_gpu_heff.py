@@ -145,3 +145,32 @@ def test_autotuned_variants_match_default(prec, bar):
     plan2 = HeffPlan(*[t.struct for t in ts])
     prep2 = plan2.prepare(prec, *(t.data for t in ts[1:]))
     assert plan2.tune(prec, prep2, ts[0].data, scratch)["from_cache"]
+
+
+@pytest.mark.parametrize("name", ["xxz_L32_chi64", "xxz_L100_chi512", "cyl6x24_chi2048"])
+@pytest.mark.parametrize("tiles", [(64, 64), (128, 64), (64, 128)])
+def test_dma_gemm_variants_bitwise(name, tiles):
+    """The FP32 GEMM with DMA-streamed operands (tile variant KS = 0) sums the same
+    products in the same order as the register-staged kernel with K chunk 16
+    (KS = 1): step 1 and step 4 (K segments, K tails, split-K) are bitwise equal."""
+    import os
+    from mps_amd.tensornetalgs import HeffPlan
+    ts = [t.to(torch.float32) for t in _workload(name)]
+    bm, bn = tiles
+    outs = []
+    old = os.environ.get("MPSH_HEFF_TILES")
+    try:
+        for ks in (1, 0):
+            os.environ["MPSH_HEFF_TILES"] = f"s1={bm}x{bn}x{ks},s4={bm}x{bn}x{ks},split=1024"
+            plan = HeffPlan(*[t.struct for t in ts])
+            prepared = plan.prepare("f32", *(t.data for t in ts[1:]))
+            out = torch.full((plan.out.numel,), float("nan"), dtype=torch.float32, device=DEV)
+            plan.apply_prepared("f32", prepared, ts[0].data, out)
+            outs.append(out)
+    finally:
+        if old is None:
+            os.environ.pop("MPSH_HEFF_TILES", None)
+        else:
+            os.environ["MPSH_HEFF_TILES"] = old
+    assert torch.isfinite(outs[1]).all()
+    assert torch.equal(outs[0], outs[1])
