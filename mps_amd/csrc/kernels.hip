@@ -457,6 +457,14 @@ __device__ __forceinline__ void barrier_raw() { asm volatile("s_barrier" ::: "me
 
 typedef __attribute__((address_space(3))) void* lds_void_ptr;
 
+// a work-group-uniform value in scalar registers
+__device__ __forceinline__ int32_t uniform_i32(int32_t x) { return __builtin_amdgcn_readfirstlane(x); }
+__device__ __forceinline__ int64_t uniform_i64(int64_t x) {
+  const uint32_t lo = __builtin_amdgcn_readfirstlane((uint32_t)(x & 0xffffffffu));
+  const uint32_t hi = __builtin_amdgcn_readfirstlane((uint32_t)((uint64_t)x >> 32));
+  return (int64_t)(((uint64_t)hi << 32) | lo);
+}
+
 template <int BM, int BN>
 __global__ __launch_bounds__(256) void gemm_dma(
     const GemmProb* __restrict__ probs, const GemmSeg* __restrict__ segs, const int64_t* __restrict__ atab,
@@ -496,22 +504,38 @@ __global__ __launch_bounds__(256) void gemm_dma(
   const int xcd = blockIdx.x % NXCD, gq = gridDim.x / NXCD;
   const int tend = xq.off[xcd + 1];
   for (int t = xq.off[xcd] + blockIdx.x / NXCD; t < tend; t += gq) {
-    const GemmTile tile = tiles[t];
-    const GemmProb p = probs[tile.prob];
+    // tile and problem fields moved to scalar registers here: the loop below then
+    // uses no register written by a pending vector load, and the compiler places
+    // no full vmcnt wait (which would also drain the DMA of later chunks) in it
+    GemmTile tile = tiles[t];
+    tile.prob = uniform_i32(tile.prob);
+    tile.m0 = uniform_i32(tile.m0);
+    tile.n0 = uniform_i32(tile.n0);
+    tile.split = uniform_i32(tile.split);
+    GemmProb p = probs[tile.prob];
+    p.M = uniform_i32(p.M);
+    p.N = uniform_i32(p.N);
+    p.seg0 = uniform_i32(p.seg0);
+    p.nseg = uniform_i32(p.nseg);
+    p.nsplit = uniform_i32(p.nsplit);
+    p.crow0 = uniform_i64(p.crow0);
+    p.part_off = uniform_i64(p.part_off);
+    p.c_off = uniform_i64(p.c_off);
+    p.ldc = uniform_i64(p.ldc);
     floatx4 acc[SM][SN];
 #pragma unroll
     for (int i = 0; i < SM; ++i)
 #pragma unroll
       for (int j = 0; j < SN; ++j) acc[i][j] = floatx4{0, 0, 0, 0};
     int64_t ctot = 0;
-    for (int s = 0; s < p.nseg; ++s) ctot += (segs[p.seg0 + s].K + BK - 1) / BK;
+    for (int s = 0; s < p.nseg; ++s) ctot += (uniform_i64(segs[p.seg0 + s].K) + BK - 1) / BK;
     const int64_t cb = ctot * tile.split / p.nsplit, ce = ctot * (tile.split + 1) / p.nsplit;
     const int nch = (int)(ce - cb);
     // issue side: segment, k offset, this lane's row bases
     int seg = 0;
     int64_t k0 = 0, K = 0, cpos = 0;
     for (; seg < p.nseg; ++seg) {
-      const int64_t nc = (segs[p.seg0 + seg].K + BK - 1) / BK;
+      const int64_t nc = (uniform_i64(segs[p.seg0 + seg].K) + BK - 1) / BK;
       if (cpos + nc > cb) {
         k0 = (cb - cpos) * BK;
         break;
@@ -520,17 +544,44 @@ __global__ __launch_bounds__(256) void gemm_dma(
     }
     int64_t ab[NA], bb[NB];
     auto bases = [&]() {
-      const GemmSeg sg = segs[p.seg0 + seg];
+      GemmSeg sg = segs[p.seg0 + seg];
+      sg.K = uniform_i64(sg.K);
+      sg.atab0 = uniform_i64(sg.atab0);
+      sg.btab0 = uniform_i64(sg.btab0);
+      sg.a_off = uniform_i64(sg.a_off);
+      sg.lda = uniform_i64(sg.lda);
+      sg.b_off = uniform_i64(sg.b_off);
+      sg.ldb = uniform_i64(sg.ldb);
       K = sg.K;
+      // rows from tables are waited for here, inside their branch, so no row
+      // base is a pending load in the chunk loop
+      if (sg.atab0 >= 0) {
 #pragma unroll
-      for (int i = 0; i < NA; ++i) {
-        const int64_t r = tile.m0 + lrow + 64 * i < p.M ? tile.m0 + lrow + 64 * i : 0;
-        ab[i] = sg.atab0 >= 0 ? atab[sg.atab0 + r] : sg.a_off + r * sg.lda;
+        for (int i = 0; i < NA; ++i) {
+          const int64_t r = tile.m0 + lrow + 64 * i < p.M ? tile.m0 + lrow + 64 * i : 0;
+          ab[i] = atab[sg.atab0 + r];
+        }
+        wait_vmcnt<0>();
+      } else {
+#pragma unroll
+        for (int i = 0; i < NA; ++i) {
+          const int64_t r = tile.m0 + lrow + 64 * i < p.M ? tile.m0 + lrow + 64 * i : 0;
+          ab[i] = sg.a_off + r * sg.lda;
+        }
       }
+      if (sg.btab0 >= 0) {
 #pragma unroll
-      for (int i = 0; i < NB; ++i) {
-        const int64_t r = tile.n0 + lrow + 64 * i < p.N ? tile.n0 + lrow + 64 * i : 0;
-        bb[i] = sg.btab0 >= 0 ? btab[sg.btab0 + r] : sg.b_off + r * sg.ldb;
+        for (int i = 0; i < NB; ++i) {
+          const int64_t r = tile.n0 + lrow + 64 * i < p.N ? tile.n0 + lrow + 64 * i : 0;
+          bb[i] = btab[sg.btab0 + r];
+        }
+        wait_vmcnt<0>();
+      } else {
+#pragma unroll
+        for (int i = 0; i < NB; ++i) {
+          const int64_t r = tile.n0 + lrow + 64 * i < p.N ? tile.n0 + lrow + 64 * i : 0;
+          bb[i] = sg.b_off + r * sg.ldb;
+        }
       }
     };
     if (nch > 0) bases();
