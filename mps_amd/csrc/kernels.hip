@@ -439,7 +439,7 @@ __global__ __launch_bounds__(256) void gemm_grouped(
 // ahead) and one barrier per chunk.  Same problem / segment / tile / split-K /
 // output conventions as gemm_grouped (tile variant KS = 0).  In LDS a stage
 // holds BM + BN operand rows of 16 floats; lane l of wave w fills the 16-byte
-// slot (row, q) = (w*16 + l/4 + 64 i, l%4) with the K group q ^ ((row >> 2) & 3),
+// slot (row, q) = (w*16 + l/4 + 64 i, l%4) with the K group q ^ dma_swz(row),
 // so the MFMA operand reads (16 rows at one K group) hit distinct banks.  K
 // groups past a segment's K are read from the row start and zeroed in LDS
 // (only in a segment's last chunk), as are the entries past K of the last group.
@@ -456,6 +456,11 @@ __device__ __forceinline__ void wait_lgkmcnt0() { __builtin_amdgcn_s_waitcnt(15 
 __device__ __forceinline__ void barrier_raw() { asm volatile("s_barrier" ::: "memory"); }
 
 typedef __attribute__((address_space(3))) void* lds_void_ptr;
+
+// K-group swizzle of LDS row r (rows of 16 floats = 64 bytes): 8 consecutive
+// rows read at one K group hit 8 distinct 16-byte bank groups of a 128-byte
+// bank line, and 16 consecutive rows 16 distinct groups of a 256-byte line
+__device__ __forceinline__ int dma_swz(int r) { return ((r >> 2) ^ (r & 2)) & 3; }
 
 // a work-group-uniform value in scalar registers
 __device__ __forceinline__ int32_t uniform_i32(int32_t x) { return __builtin_amdgcn_readfirstlane(x); }
@@ -480,17 +485,17 @@ __global__ __launch_bounds__(256) void gemm_dma(
   const int wm = wave >> 1, wn = wave & 1;
   const int g = lane >> 4, l16 = lane & 15;
   const int lrow = wave * 16 + (lane >> 2), lq = lane & 3;  // + 64 i for the i-th slot of a lane
-  const int lkl = lq ^ ((lrow >> 2) & 3);  // (64 i + lrow) >> 2 & 3 does not depend on i
+  const int lkl = lq ^ dma_swz(lrow);  // dma_swz(64 i + lrow) does not depend on i
   int offa[SM], offb[SN];  // operand read offsets (floats) within a stage
 #pragma unroll
   for (int i = 0; i < SM; ++i) {
     const int r = wm * (BM / 2) + i * 16 + l16;
-    offa[i] = (r * 4 + (g ^ ((r >> 2) & 3))) * 4;
+    offa[i] = (r * 4 + (g ^ dma_swz(r))) * 4;
   }
 #pragma unroll
   for (int j = 0; j < SN; ++j) {
     const int r = wn * (BN / 2) + j * 16 + l16;
-    offb[j] = BM * BK + (r * 4 + (g ^ ((r >> 2) & 3))) * 4;
+    offb[j] = BM * BK + (r * 4 + (g ^ dma_swz(r))) * 4;
   }
   typedef std::integral_constant<int, 0> I0;
   typedef std::integral_constant<int, 1> I1;
