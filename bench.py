@@ -28,7 +28,7 @@ HBM_PEAK = 8.0e12
 METRIC = "H_eff matvecs/sec + DMRG sweep time at χ=2048; |ΔE| vs FP64 CPU oracle"
 # HBM bytes per step-1 GEMM launch: rocprofv3 FETCH_SIZE x 2.000 + WRITE_SIZE x 1.000
 # (scales from a 256 MiB device copy), profiles/heff_cyl6x24_chi2048_f32_hbm_traffic.txt
-TRAFFIC = {("cyl6x24_chi2048", "f32"): 268.6e6}  # autotuned step-1 variant 64x64x1
+TRAFFIC = {("cyl6x24_chi2048", "f32"): 287.6e6}  # autotuned step-1 variant gemm_dma<64, 128>
 
 
 def parse():
@@ -120,6 +120,24 @@ def time_fused(plan, prec, ts, steps, warmup, stage_events=True, tune=True):
                 stages[k] += e[k].elapsed_time(e[k + 1])
         stages = [s / steps for s in stages]
     return max(total_ms, wall * 1e3) / steps, stages, out, prep_ms, tuned
+
+
+def step1_kernel(prec, tuned):
+    """Name of the step-1 GEMM kernel variant: the autotuned one, else the one
+    MPSH_HEFF_TILES or the defaults select (FP32 64x128 DMA-streamed, BF16 128x64,
+    FP64 64x64)."""
+    key = {"f32": "s1", "bf16": "s1h", "f64": "s1d"}[prec]
+    cfg = {"f32": "64x128x0", "bf16": "128x64x1", "f64": "64x64x1"}[prec]
+    if tuned:
+        cfg = tuned["step1"]
+    else:
+        for item in os.environ.get("MPSH_HEFF_TILES", "").split(","):
+            if item.startswith(key + "="):
+                cfg = item.split("=", 1)[1]
+    parts = (cfg.split("x") + ["1"])[:3]
+    bm, bn, ks = parts
+    mode = {"f64": 0, "f32": 1, "bf16": 2}[prec]
+    return f"gemm_dma<{bm}, {bn}>" if ks == "0" else f"gemm_grouped<{mode}, {bm}, {bn}, {ks}>"
 
 
 def time_plain(ts, steps, warmup):
@@ -461,7 +479,7 @@ def main():
                           "mpo_pass": round(stages[2], 5), "step4_gemm": round(stages[3], 5)},
         "prepare_ms_per_bond": round(prep_ms, 5),
         "autotuned_gemm_variants": tuned,
-        "roofline": {"bound": "mfma", "kernel": f"gemm_grouped<{ {'f64': 0, 'f32': 1, 'bf16': 2}[prec] },...> step-1 launch (v^T envL)",
+        "roofline": {"bound": "mfma", "kernel": f"{step1_kernel(prec, tuned)} step-1 launch (v^T envL)",
                      "achieved": round(achieved / 1e12, 3), "peak": PEAK[prec] / 1e12, "unit": "TFLOP/s",
                      "frac": round(achieved / PEAK[prec], 4), "traffic": TRAFFIC.get((args.workload, prec)),
                      "flops_per_launch": s1_flops, "avg_launch_ms": round(stages[1], 5),

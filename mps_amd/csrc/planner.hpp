@@ -283,7 +283,7 @@ struct TileCfg {
   int BM = 64, BN = 64, KS = 1;
 };
 struct HeffTiles {
-  TileCfg s1{64, 128, 1}, s4{64, 64, 2};     // FP32
+  TileCfg s1{64, 128, 0}, s4{64, 64, 0};     // FP32 (DMA-streamed operands)
   TileCfg s1h{128, 64, 1}, s4h{64, 64, 2};   // BF16 operands
   TileCfg s1d{64, 64, 1}, s4d{64, 64, 1};    // FP64
   int64_t split_target = 1024;               // minimum work items for the step-4 GEMM
