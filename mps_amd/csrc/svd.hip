@@ -102,6 +102,7 @@ constexpr int JT = 256;                 // threads per work group
 constexpr int64_t JAC_MAX_ROWS = 2048;  // r limit of the LDS-resident column pairs
 constexpr int JAC_LDS = 17472;          // doubles: 2w * ldx + 2w * (2w + 1) for the largest (w, r) class
 constexpr int JAC_MAX_SWEEPS = 60;
+constexpr double JAC_SETTLED = 1e-10;  // a sweep whose rotations all stay below this ends the matrix
 
 struct JacMat {
   int64_t r, c;          // work matrix W (r x c, ld r), r >= c >= 1
@@ -159,8 +160,13 @@ __device__ __forceinline__ double wave_row_sum(double v) {
 // lane holds RPL rows of a column, r <= GSZ RPL.  Rows r .. GSZ RPL of every
 // column are zero, so loads and stores run unmasked.  The 2w x 2w rotation R is kept
 // transposed (Rt[row * (2w + 1) + col]).
-// offmax[mat] is set to 1 when any pair of the sweep was rotated (some
-// |x_i.x_j| > ctol |x_i| |x_j|); a sweep without rotations ends the matrix.
+// offmax[mat] is set to 1 when a pair of the sweep was rotated at a relative
+// product |x_i.x_j| / (|x_i| |x_j|) above JAC_SETTLED.  A sweep whose rotations
+// all stayed below that ends the matrix: a later rotation (i, k) by an angle
+// <= 1e-10 changes x_i by at most 1e-10 |x_k| in the direction of a column k
+// whose product with x_j is itself <= 1e-10 relative, so every pair ends the
+// sweep with a relative product of order c 1e-20 (below ctol = sqrt(r) eps for
+// c up to 2048), and the sweep that would only confirm this is not run.
 __device__ __forceinline__ double read_lane(double v, int l) {
   return __hiloint2double(__builtin_amdgcn_readlane(__double2hiint(v), l),
                           __builtin_amdgcn_readlane(__double2loint(v), l));
@@ -237,7 +243,7 @@ __global__ __launch_bounds__(WB * GSZ) void jac_pairs(const JacMat* __restrict__
   const int grp = tid / GSZ, lane = tid % GSZ;
   const double ctol = M.ctol;
   const int max_sweeps = iterate ? JAC_MAX_SWEEPS : 1;
-  bool rotated = false;
+  bool unsettled = false;
 
   // rotate columns (i, j) held in registers (rows lane + 64 u); true when rotated
   auto rotate = [&](double (&xi)[RPL], double (&xj)[RPL], int i, int j) {
@@ -255,8 +261,9 @@ __global__ __launch_bounds__(WB * GSZ) void jac_pairs(const JacMat* __restrict__
     // rotate when |g| > ctol |x_i| |x_j|; columns of norm^2 <= 1e-280 (the
     // matrix is scaled to entries < 1) count as zero, and so does NaN
     // (the threshold test needs no full precision: raw hardware estimates)
-    const bool rot = a > 1e-280 && b > 1e-280 &&
-                     fabs(g) * __builtin_amdgcn_rsq(a) * __builtin_amdgcn_rsq(b) > ctol;
+    const double off = fabs(g) * __builtin_amdgcn_rsq(a) * __builtin_amdgcn_rsq(b);
+    const bool rot = a > 1e-280 && b > 1e-280 && off > ctol;
+    if (rot && off > JAC_SETTLED) unsettled = true;
     if (!__any(rot)) return false;
     // t = tan of the angle that zeroes the pair's product (Rutishauser):
     // t = sign(z) / (|z| + sqrt(1 + z^2)), z = (b - a) / (2g); for |z| > 1e150
@@ -309,7 +316,6 @@ __global__ __launch_bounds__(WB * GSZ) void jac_pairs(const JacMat* __restrict__
         load(xi, i);
         load(xj, j);
         if (rotate(xi, xj, i, j)) {
-          rotated = true;
           store(xi, i);
           store(xj, j);
         }
@@ -329,16 +335,13 @@ __global__ __launch_bounds__(WB * GSZ) void jac_pairs(const JacMat* __restrict__
       }
       __syncthreads();
     }
-    if (moved) {
-      rotated = true;
-      store(xi, grp);
-    }
-    if (rotated) rot_any = 1;
+    if (moved) store(xi, grp);
+    if (unsettled) rot_any = 1;
     __syncthreads();
     conv = !rot_any;
     __syncthreads();
     if (tid == 0) rot_any = 0;
-    rotated = false;
+    unsettled = false;
     if (!iterate && !conv && tid == 0) offmax[task.mat] = 1ull;
     if (conv) break;
   }
