@@ -107,11 +107,14 @@ constexpr double JAC_SETTLED = 1e-10;  // a sweep whose rotations all stay below
 struct JacMat {
   int64_t r, c;          // work matrix W (r x c, ld r), r >= c >= 1
   int64_t w_off, q_off;  // workspace offsets (doubles): W and Q (c x c, ld c)
+  int64_t w0_off;        // the scaled block before preconditioning (== w_off when not preconditioned)
+  int64_t g_off, gld;    // preconditioned: Gram / eigenvector slot (gld x gld) and its leading dimension
   int64_t m, n;          // the block's shape
   int64_t u_off, s_off, v_off;  // packed outputs
   double ctol;           // sqrt(r) * eps
   int32_t w, nb;         // column-block width and block count (even, >= 2)
-  int32_t trans, pad;    // trans: W = A^T
+  int32_t trans, pre;    // trans: W = A^T; pre: preconditioned by the Gram eigenvectors
+  int32_t g_info, pad;   // preconditioned: index of the syevd info entry
 };
 struct JacTask {
   int32_t mat, k;  // matrix (index into the JacMat table) and pair index within a round
@@ -119,6 +122,48 @@ struct JacTask {
 
 // columns and width of a block width: w in {16, 8, 4} so that 2w * (r | 1) doubles fit the LDS budget
 int jac_width(int64_t r) { return r <= 512 ? 16 : (r <= 1024 ? 8 : 4); }
+
+// Preconditioning.  Cyclic one-sided Jacobi converges only linearly for a
+// long while on the graded spectra of DMRG two-site blocks (27-30 sweeps for
+// 900 x 800 blocks whose singular values span 1 .. 1e-9, against 13 for
+// Gaussian blocks).  Matrices with at least JAC_PRE_MIN columns are first
+// rotated by the eigenvectors V of their Gram matrix W^T W (rocSOLVER syevd,
+// FP64, batched over padded size buckets): the columns of W V are orthogonal
+// up to eps |W|^2 / (sigma_i sigma_j) relative, and the Jacobi sweeps that
+// follow (started from Q = V) restore full relative orthogonality in 5-7
+// sweeps.  V enters only as an orthogonal change of basis (syevd's vectors are
+// orthonormal to rounding), so the singular values and vectors keep the
+// accuracy of the Jacobi sweeps.  A matrix whose syevd reports failure keeps
+// Q = I and is swept unpreconditioned.
+constexpr int64_t JAC_PRE_MIN = 96;
+// syevd cost model (MI355X, rocSOLVER 3.30, ms): a bucket of `count` padded
+// P x P matrices costs ~ 0.022 P (the sequential Householder and merge steps)
+// + 1.64e-9 count P^3 (16 matrices of 800: 31 ms, one: 18 ms; 16 of 400: 12 ms)
+std::vector<int64_t> choose_gram_buckets(const std::vector<int64_t>& P) {
+  const int64_t n = (int64_t)P.size();
+  std::vector<double> best(n + 1, 0.0);
+  std::vector<int64_t> from(n + 1, 0);
+  for (int64_t j = 1; j <= n; ++j) {
+    best[j] = 1e300;
+    const double p = (double)P[j - 1];
+    for (int64_t i = 0; i < j; ++i) {
+      const double c = best[i] + 0.022 * p + 1.64e-9 * (double)(j - i) * p * p * p;
+      if (c < best[j]) {
+        best[j] = c;
+        from[j] = i;
+      }
+    }
+  }
+  std::vector<int64_t> ends;
+  for (int64_t j = n; j > 0; j = from[j]) ends.push_back(j);
+  std::reverse(ends.begin(), ends.end());
+  return ends;
+}
+struct GramBucket {
+  int64_t P, count;       // padded order and number of matrices
+  int64_t g_off, d_off, e_off;  // (jacobi region, doubles) slots P x P, eigenvalues, syevd scratch
+  int32_t mat_start, info_start;  // first entry in the bucket-ordered matrix list / info array
+};
 
 __device__ __forceinline__ void jac_pair_of(int n, int t, int k, int& p, int& q) {
   // round t of the circle method over n (even) players: player n-1 fixed
@@ -215,7 +260,7 @@ __global__ __launch_bounds__(WB * GSZ) void jac_pairs(const JacMat* __restrict__
   static_assert(NC <= GSZ, "one Rt row per lane of a group");
   static_assert(NC * LDX + NC * RTS <= JAC_LDS, "LDS budget");
   __shared__ __align__(16) double lds[JAC_LDS];
-  __shared__ int rot_any;
+  __shared__ int rot_any, touched;
   const JacTask task = tasks[blockIdx.x];
   const JacMat M = mats[task.mat];
   if (done[task.mat]) return;
@@ -238,7 +283,10 @@ __global__ __launch_bounds__(WB * GSZ) void jac_pairs(const JacMat* __restrict__
     X[e] = (g >= 0 && row < r) ? W[(int64_t)g * r + row] : 0.0;
   }
   for (int e = tid; e < NC * RTS; e += NT) Rt[e] = (e / RTS == e % RTS) ? 1.0 : 0.0;
-  if (tid == 0) rot_any = 0;
+  if (tid == 0) {
+    rot_any = 0;
+    touched = 0;
+  }
   __syncthreads();
   const int grp = tid / GSZ, lane = tid % GSZ;
   const double ctol = M.ctol;
@@ -318,6 +366,7 @@ __global__ __launch_bounds__(WB * GSZ) void jac_pairs(const JacMat* __restrict__
         if (rotate(xi, xj, i, j)) {
           store(xi, i);
           store(xj, j);
+          touched = 1;
         }
         __syncthreads();
       }
@@ -332,6 +381,7 @@ __global__ __launch_bounds__(WB * GSZ) void jac_pairs(const JacMat* __restrict__
       if (rotate(xi, xj, grp, j)) {
         store(xj, j);
         moved = true;
+        touched = 1;
       }
       __syncthreads();
     }
@@ -351,6 +401,9 @@ __global__ __launch_bounds__(WB * GSZ) void jac_pairs(const JacMat* __restrict__
     else
       flags[1] = 1;  // not converged
   }
+  // nothing rotated: W and Q are unchanged (the last barrier of the sweep
+  // loop orders every write of `touched` before this read)
+  if (!touched) return;
   // columns back to W
   for (int e = tid; e < NC * r; e += NT) {
     const int lc = e / r, row = e - lc * r;
@@ -392,7 +445,7 @@ __global__ __launch_bounds__(JT) void jac_scale(const JacMat* __restrict__ mats,
   __shared__ double red[JT / 64];
   __shared__ int e_sh;
   const JacMat M = mats[blockIdx.x];
-  double* W = ws + M.w_off;
+  double* W = ws + M.w0_off;
   const int64_t n = M.r * M.c;
   double mx = 0.0;
   for (int64_t e = threadIdx.x; e < n; e += JT) mx = fmax(mx, fabs(W[e]));
@@ -414,22 +467,77 @@ __global__ __launch_bounds__(JT) void jac_scale(const JacMat* __restrict__ mats,
 }
 
 // After a sweep of rounds: matrices whose largest relative off-diagonal stayed
-// below tolerance are done; count the rest.  One thread (few matrices).
+// below tolerance are done; count the rest (left[0]) and the rounds a sweep of
+// the rest takes (left[3]: the largest nb - 1).  One thread (few matrices).
 __global__ void jac_check(const JacMat* __restrict__ mats, const int32_t* __restrict__ big, int nbig,
                           unsigned long long* __restrict__ offmax, int* __restrict__ done, int* __restrict__ left) {
   if (threadIdx.x != 0 || blockIdx.x != 0) return;
-  int n = 0;
+  int n = 0, rounds = 0;
   for (int i = 0; i < nbig; ++i) {
     const int b = big[i];
     if (!done[b]) {
-      if (offmax[b] == 0ull)  // a sweep without rotations
+      if (offmax[b] == 0ull) {  // a sweep without rotations
         done[b] = 1;
-      else
+      } else {
         ++n;
+        rounds = max(rounds, mats[b].nb - 1);
+      }
     }
     offmax[b] = 0ull;
   }
   left[0] = n;
+  left[3] = rounds;
+}
+
+// Gram slots of one bucket (after the syrk wrote the upper triangles of the
+// c x c Gram matrices): padded rows / columns c .. P get -1 on the diagonal
+// and 0 elsewhere, so the padding's eigenpairs (-1, unit vectors) lie below
+// every eigenvalue of the Gram matrix (>= 0) and stay apart from them.  A slot
+// with a non-finite entry is replaced by the identity (the sweeps then report
+// the input as not finite).
+__global__ __launch_bounds__(JT) void jac_gram_pad(const JacMat* __restrict__ mats, const int32_t* __restrict__ list,
+                                                   double* __restrict__ jw, int64_t P) {
+  __shared__ int bad;
+  const JacMat M = mats[list[blockIdx.x]];
+  double* G = jw + M.g_off;
+  const int64_t c = M.c;
+  if (threadIdx.x == 0) bad = 0;
+  __syncthreads();
+  int b = 0;
+  for (int64_t e = threadIdx.x; e < c * c; e += JT) {
+    const int64_t j = e / c, i = e - j * c;
+    if (i <= j) {
+      const double x = G[i + j * P];
+      b |= !(x == x) || fabs(x) > 1.7976931348623157e308;
+    }
+  }
+  if (b) bad = 1;
+  __syncthreads();
+  if (bad) {
+    for (int64_t e = threadIdx.x; e < P * P; e += JT) G[e] = (e / P == e % P) ? 1.0 : 0.0;
+    return;
+  }
+  for (int64_t e = threadIdx.x; e < P * P; e += JT) {
+    const int64_t j = e / P, i = e - j * P;
+    if (i >= c || j >= c) G[e] = (i == j) ? -1.0 : 0.0;
+  }
+}
+
+// Q <- the eigenvectors of the Gram matrix by descending eigenvalue (syevd
+// returns them ascending, the padding's -1 first): Q(:, j) = V(0:c, P-1-j).
+// A matrix whose syevd failed keeps Q = I.
+__global__ __launch_bounds__(JT) void jac_take_v(const JacMat* __restrict__ mats, const int32_t* __restrict__ list,
+                                                 double* __restrict__ jw, const rocblas_int* __restrict__ info,
+                                                 int64_t P) {
+  const JacMat M = mats[list[blockIdx.x]];
+  if (info[M.g_info] != 0) return;
+  const double* G = jw + M.g_off;
+  double* Q = jw + M.q_off;
+  const int64_t c = M.c;
+  for (int64_t e = threadIdx.x; e < c * c; e += JT) {
+    const int64_t j = e / c, i = e - j * c;
+    Q[e] = G[i + (P - 1 - j) * P];
+  }
 }
 
 __global__ __launch_bounds__(JT) void jac_init_q(const JacMat* __restrict__ mats, double* __restrict__ ws) {
@@ -567,6 +675,8 @@ struct mpsh_svd_s {
   int32_t jsmall_cls[4] = {0, 0, 0, 0}, jbig_cls[4] = {0, 0, 0, 0};  // by width class (16, 8, 4): [cls, cls+1)
   std::vector<int32_t> jbig_idx;      // matrices with more than two column blocks
   int32_t jrounds = 0;                // rounds per sweep (max over matrices)
+  std::vector<GramBucket> gb;         // preconditioning: syevd size buckets (ascending P)
+  std::vector<int32_t> gmat;          // their matrices, bucket by bucket
   int64_t jac_off = 0, jac_meta_off = 0;  // workspace (doubles): W | Q, then offmax | done | left, flags
   CopyPlan jinit;                     // blocks -> W (transposed when m < n)
   struct Dev {
@@ -576,6 +686,7 @@ struct mpsh_svd_s {
     const JacTask* jsmall = nullptr;
     const JacTask* jbig = nullptr;
     const int32_t* jbig_idx = nullptr;
+    const int32_t* gmat = nullptr;
   };
   std::mutex mu;
   std::map<int, std::unique_ptr<Dev>> dev;
@@ -614,6 +725,7 @@ struct mpsh_svd_s {
       D->jsmall = up(D->bufs, jsmall);
       D->jbig = up(D->bufs, jbig);
       D->jbig_idx = up(D->bufs, jbig_idx);
+      D->gmat = up(D->bufs, gmat);
       it = dev.emplace(d, std::move(D)).first;
     }
     return *it->second;
@@ -651,6 +763,51 @@ void jacobi_execute(const mpsh_svd_s& P, const mpsh_svd_s::Dev& cp, const double
   ck(hipGetLastError(), "jac_scale");
   hipLaunchKernelGGL(jac_init_q, dim3(nj), dim3(JT), 0, s, cp.jm, jw);
   ck(hipGetLastError(), "jac_init_q");
+  if (!P.gb.empty()) {
+    // preconditioning, size buckets spread over the side lanes (largest first):
+    // G = W0^T W0 (upper), padding, syevd, Q = eigenvectors, W = W0 Q
+    rocblas_int* info = (rocblas_int*)(ws + P.ws_doubles);
+    int dev = 0;
+    ck(hipGetDevice(&dev), "hipGetDevice");
+    std::lock_guard<std::mutex> g(g_handle_mu);
+    Lanes& L = lanes_for(dev);
+    ck(hipEventRecord(L.fork, s), "hipEventRecord");
+    const int ngb = (int)P.gb.size();
+    for (int i = 0; i < ngb && i < NLANE; ++i) ck(hipStreamWaitEvent(L.st[i], L.fork, 0), "hipStreamWaitEvent");
+    const double one = 1.0, zero = 0.0;
+    for (int i = 0; i < ngb; ++i) {
+      const GramBucket& B = P.gb[ngb - 1 - i];
+      rocblas_handle h = L.h[i % NLANE];
+      hipStream_t ls = L.st[i % NLANE];
+      for (int64_t k = 0; k < B.count; ++k) {
+        const JacMat& J = P.jm[P.gmat[B.mat_start + k]];
+        ckb(rocblas_dsyrk(h, rocblas_fill_upper, rocblas_operation_transpose, (rocblas_int)J.c, (rocblas_int)J.r,
+                          &one, jw + J.w0_off, (rocblas_int)J.r, &zero, jw + J.g_off, (rocblas_int)B.P),
+            "rocblas_dsyrk");
+      }
+      hipLaunchKernelGGL(jac_gram_pad, dim3((unsigned)B.count), dim3(JT), 0, ls, cp.jm, cp.gmat + B.mat_start, jw,
+                         B.P);
+      ck(hipGetLastError(), "jac_gram_pad");
+      ckb(rocsolver_dsyevd_strided_batched(h, rocblas_evect_original, rocblas_fill_upper, (rocblas_int)B.P,
+                                           jw + B.g_off, (rocblas_int)B.P, B.P * B.P, jw + B.d_off, B.P,
+                                           jw + B.e_off, B.P, info + B.info_start, (rocblas_int)B.count),
+          "rocsolver_dsyevd_strided_batched");
+      hipLaunchKernelGGL(jac_take_v, dim3((unsigned)B.count), dim3(JT), 0, ls, cp.jm, cp.gmat + B.mat_start, jw,
+                         (const rocblas_int*)info, B.P);
+      ck(hipGetLastError(), "jac_take_v");
+      for (int64_t k = 0; k < B.count; ++k) {
+        const JacMat& J = P.jm[P.gmat[B.mat_start + k]];
+        ckb(rocblas_dgemm(h, rocblas_operation_none, rocblas_operation_none, (rocblas_int)J.r, (rocblas_int)J.c,
+                          (rocblas_int)J.c, &one, jw + J.w0_off, (rocblas_int)J.r, jw + J.q_off, (rocblas_int)J.c,
+                          &zero, jw + J.w_off, (rocblas_int)J.r),
+            "rocblas_dgemm");
+      }
+    }
+    for (int i = 0; i < ngb && i < NLANE; ++i) {
+      ck(hipEventRecord(L.done[i], L.st[i]), "hipEventRecord");
+      ck(hipStreamWaitEvent(s, L.done[i], 0), "hipStreamWaitEvent");
+    }
+  }
   // one launch per width class present
   auto pairs = [&](const JacTask* tasks, const int32_t* cls, int t, int iterate) {
     for (int k = 0; k < 3; ++k) {
@@ -671,18 +828,25 @@ void jacobi_execute(const mpsh_svd_s& P, const mpsh_svd_s::Dev& cp, const double
   };
   if (!P.jsmall.empty()) pairs(cp.jsmall, P.jsmall_cls, 0, 1);
   if (!P.jbig.empty()) {
-    int hl = 1, sweeps = 0;
+    // a sweep runs the rounds of the largest matrix not yet done
+    int hl = 1, sweeps = 0, rounds = P.jrounds, total_rounds = 0;
     for (int sweep = 0; sweep < JAC_MAX_SWEEPS && hl != 0; ++sweep, ++sweeps) {
-      for (int t = 0; t < P.jrounds; ++t) pairs(cp.jbig, P.jbig_cls, t, 0);
+      for (int t = 0; t < rounds; ++t) pairs(cp.jbig, P.jbig_cls, t, 0);
+      total_rounds += rounds;
       hipLaunchKernelGGL(jac_check, dim3(1), dim3(64), 0, s, cp.jm, cp.jbig_idx, (int)P.jbig_idx.size(), offmax,
                          done, left);
       ck(hipGetLastError(), "jac_check");
-      ck(hipMemcpyAsync(&hl, left, 4, hipMemcpyDeviceToHost, s), "hipMemcpyAsync(jacobi)");
+      int h4[4] = {0, 0, 0, 0};
+      ck(hipMemcpyAsync(h4, left, 16, hipMemcpyDeviceToHost, s), "hipMemcpyAsync(jacobi)");
       ck(hipStreamSynchronize(s), "hipStreamSynchronize");
+      hl = h4[0];
+      rounds = h4[3];
     }
     if (hl != 0) throw Error(ERR_NONFINITE, "Jacobi SVD did not converge" + jacobi_unconverged(P, done, s));
     static const bool stats = getenv("MPSH_SVD_STATS") != nullptr;
-    if (stats) fprintf(stderr, "mpsh_svd: %d Jacobi sweeps x %d rounds\n", sweeps, P.jrounds);
+    if (stats)
+      fprintf(stderr, "mpsh_svd: %d Jacobi sweeps, %d rounds (%d per full sweep), %d preconditioned in %d buckets\n",
+              sweeps, total_rounds, P.jrounds, (int)P.gmat.size(), (int)P.gb.size());
   }
   hipLaunchKernelGGL(jac_finish, dim3(nj), dim3(JT), 0, s, cp.jm, (const double*)jw, (const int*)sexp, U, S, Vt,
                      flags);
@@ -722,6 +886,8 @@ mpsh_status mpsh_svd_plan(int64_t nblocks, const int64_t* m, const int64_t* n, c
     }
     const char* algo = getenv("MPSH_SVD_ALGO");
     const bool jacobi = !(algo && std::string(algo) == "gesdd");
+    const char* pre_env = getenv("MPSH_SVD_PRECOND");
+    const bool precond = !(pre_env && std::string(pre_env) == "0");
     std::vector<int64_t> order;
     std::vector<int64_t> jac_blocks;
     for (int64_t b = 0; b < nblocks; ++b) {
@@ -803,10 +969,16 @@ mpsh_status mpsh_svd_plan(int64_t nblocks, const int64_t* m, const int64_t* n, c
       J.u_off = P->u_off[b];
       J.s_off = P->s_off[b];
       J.v_off = P->v_off[b];
+      J.pre = precond && J.nb > 2 && J.c >= JAC_PRE_MIN;
       J.w_off = jw;
       jw += J.r * J.c;
       J.q_off = jw;
       jw += J.c * J.c;
+      J.w0_off = J.w_off;
+      if (J.pre) {
+        J.w0_off = jw;
+        jw += J.r * J.c;
+      }
       jw = (jw + 1) / 2 * 2;
       const int32_t idx = (int32_t)P->jm.size();
       const int cls = J.w == 16 ? 0 : (J.w == 8 ? 1 : 2);
@@ -819,12 +991,46 @@ mpsh_status mpsh_svd_plan(int64_t nblocks, const int64_t* m, const int64_t* n, c
       }
       if (!J.trans) {
         const int64_t dims[2] = {J.m, J.n}, sst[2] = {1, J.m}, dst[2] = {1, J.r};
-        P->jinit.add(2, dims, sst, dst, a_off[b], J.w_off);
+        P->jinit.add(2, dims, sst, dst, a_off[b], J.w0_off);
       } else {  // W(i, j) = A(j, i)
         const int64_t dims[2] = {J.n, J.m}, sst[2] = {J.m, 1}, dst[2] = {1, J.r};
-        P->jinit.add(2, dims, sst, dst, a_off[b], J.w_off);
+        P->jinit.add(2, dims, sst, dst, a_off[b], J.w0_off);
       }
       P->jm.push_back(J);
+    }
+    {
+      // preconditioning buckets over the matrices sorted by column count
+      std::vector<int32_t> pre;
+      for (int32_t i = 0; i < (int32_t)P->jm.size(); ++i)
+        if (P->jm[i].pre) pre.push_back(i);
+      std::stable_sort(pre.begin(), pre.end(), [&](int32_t x, int32_t y) { return P->jm[x].c < P->jm[y].c; });
+      std::vector<int64_t> Pg(pre.size());
+      for (size_t i = 0; i < pre.size(); ++i) Pg[i] = pad16(P->jm[pre[i]].c);
+      int64_t start = 0;
+      for (int64_t end : choose_gram_buckets(Pg)) {
+        GramBucket B{};
+        B.P = Pg[end - 1];
+        B.count = end - start;
+        B.g_off = jw;
+        jw += B.count * B.P * B.P;
+        B.d_off = jw;
+        jw += B.count * B.P;
+        B.e_off = jw;
+        jw += B.count * B.P;
+        jw = (jw + 1) / 2 * 2;
+        B.mat_start = (int32_t)P->gmat.size();
+        B.info_start = (int32_t)info;
+        for (int64_t i = start; i < end; ++i) {
+          JacMat& J = P->jm[pre[i]];
+          J.g_off = B.g_off + (i - start) * B.P * B.P;
+          J.gld = B.P;
+          J.g_info = (int32_t)(info + (i - start));
+          P->gmat.push_back(pre[i]);
+        }
+        info += B.count;
+        P->gb.push_back(B);
+        start = end;
+      }
     }
     for (int k = 0; k < 3; ++k) {
       P->jsmall.insert(P->jsmall.end(), small_by[k].begin(), small_by[k].end());

@@ -323,27 +323,45 @@ def _assemble(charge, space, parts: Dict[Tuple[int, int], torch.Tensor], dtype, 
 
 
 class SVDResult:
-    """u, s, v of ``svdtrunc`` kept as device blocks; products are formed on request."""
+    """u, s, v of ``svdtrunc`` kept as device blocks; products are formed on request.
 
-    def __init__(self, M: SymTensor, keys, U, S, Vh, Vl: U1Space, Vr: U1Space):
+    The kept singular values are views of one device vector (``S_pack``) and
+    are also held on the host (``S_host``, copied once for the truncation
+    decision), so norms and normalisation need no device round trips."""
+
+    def __init__(self, M: SymTensor, keys, U, S, Vh, Vl: U1Space, Vr: U1Space, S_pack=None, S_host=None):
         self.M, self.keys, self.U, self.S, self.Vh, self.Vl, self.Vr = M, keys, U, S, Vh, Vl, Vr
+        self.S_pack, self.S_host = S_pack, S_host
+        self._cache = {}
 
     def singular_values(self) -> np.ndarray:
         if not self.S:
             return np.zeros(0)
+        if self.S_host is not None:
+            return np.sort(np.concatenate(self.S_host))[::-1]
         return np.sort(torch.cat(self.S).double().cpu().numpy())[::-1]
 
     def norm_s(self) -> float:
+        if self.S_host is not None:
+            return math.sqrt(sum(float(np.dot(s, s)) for s in self.S_host))
         return math.sqrt(sum(float((s.double() ** 2).sum()) for s in self.S))
 
     def normalize_s(self):
         """``normalize!(s)`` (symtensor.jl:325)."""
         nrm = self.norm_s()
-        self.S = [s / nrm for s in self.S]
+        if self.S_pack is not None:
+            self.S_pack.div_(nrm)  # the per-block views follow
+            self.S_host = [s / nrm for s in self.S_host]
+        else:
+            self.S = [s / nrm for s in self.S]
+        self._cache.clear()
 
     def u(self) -> SymTensor:
-        return _assemble(0, (self.M.space[0], self.Vl.dual()),
-                         {(k[0], k[0]): U for k, U in zip(self.keys, self.U)}, self.M.dtype, self.M.device)
+        if "u" not in self._cache:
+            self._cache["u"] = _assemble(0, (self.M.space[0], self.Vl.dual()),
+                                         {(k[0], k[0]): U for k, U in zip(self.keys, self.U)}, self.M.dtype,
+                                         self.M.device)
+        return self._cache["u"]
 
     def v(self) -> SymTensor:
         return _assemble(0, (self.Vr.dual(), self.M.space[1]),
@@ -357,20 +375,19 @@ class SVDResult:
 
     def sv(self) -> SymTensor:
         """s*v (legs (Vl, space2))."""
-        return _assemble(self.M.charge, (self.Vl, self.M.space[1]),
-                         {(k[0], k[1]): s[:, None] * V for k, V, s in zip(self.keys, self.Vh, self.S)},
-                         self.M.dtype, self.M.device)
+        if "sv" not in self._cache:
+            self._cache["sv"] = _assemble(self.M.charge, (self.Vl, self.M.space[1]),
+                                          {(k[0], k[1]): s[:, None] * V for k, V, s in
+                                           zip(self.keys, self.Vh, self.S)}, self.M.dtype, self.M.device)
+        return self._cache["sv"]
 
     def usv(self) -> SymTensor:
-        """u*s*v: same structure as the factorised matrix (all its sectors; dropped ones are zero)."""
-        kept = {k: (U * s) @ V for k, U, s, V in zip(self.keys, self.U, self.S, self.Vh)}
-        sec, siz, off = self.M.struct.tables()
-        data = torch.zeros(self.M.struct.numel, dtype=self.M.dtype, device=self.M.device)
-        for b in range(self.M.struct.nblocks):
-            key = (int(sec[b][0]), int(sec[b][1]))
-            if key in kept:
-                data[off[b]:off[b + 1]] = kept[key].t().reshape(-1)
-        return SymTensor(self.M.charge, self.M.space, data, self.M.struct)
+        """u*s*v: same structure as the factorised matrix (all its sectors; dropped
+        ones are zero), one device contraction of u and s*v."""
+        if not self.keys:
+            data = torch.zeros(self.M.struct.numel, dtype=self.M.dtype, device=self.M.device)
+            return SymTensor(self.M.charge, self.M.space, data, self.M.struct)
+        return contract(self.u(), (1, -1), self.sv(), (-1, 2))
 
 
 def svdtrunc(M: SymTensor, maxdim: int = None, tol: float = 1e-12) -> SVDResult:
@@ -378,39 +395,53 @@ def svdtrunc(M: SymTensor, maxdim: int = None, tol: float = 1e-12) -> SVDResult:
     if maxdim is None:
         maxdim = max(V.dim() for V in M.space)
     facts = _svd_blocks(M)
-    svals = np.split(torch.cat([f[2] for f in facts]).cpu().numpy(), np.cumsum([len(f[2]) for f in facts])[:-1]) \
-        if facts else []
-    ns = truncatedsizes(svals, maxdim=maxdim, tol=tol) if facts else []
-    keys, Us, Ss, Vs, midl, midr = [], [], [], [], [], []
     dt = M.dtype
-    for (key, U, S, Vh), k in zip(facts, ns):
+    if facts:
+        S_cat = torch.cat([f[2] for f in facts])
+        offs = np.concatenate([[0], np.cumsum([len(f[2]) for f in facts])])
+        svals = np.split(S_cat.cpu().numpy(), offs[1:-1])
+        S_pack = S_cat if S_cat.dtype == dt else S_cat.to(dt)
+    else:
+        svals, offs, S_pack = [], [0], None
+    ns = truncatedsizes(svals, maxdim=maxdim, tol=tol) if facts else []
+    keys, Us, Ss, Vs, S_host, midl, midr = [], [], [], [], [], [], []
+    for b, ((key, U, S, Vh), k) in enumerate(zip(facts, ns)):
         if k <= 0:
             continue
         keys.append(key)
         Us.append(U[:, :k].to(dt))
-        Ss.append(S[:k].to(dt))
+        Ss.append(S_pack[int(offs[b]):int(offs[b]) + k])
+        S_host.append(svals[b][:k].copy())
         Vs.append(Vh[:k, :].to(dt))
         midl.append((key[0], k))
         midr.append((key[1], k))
     Vl = U1Space(*midl, isdual=M.space[0].isdual)
     Vr = U1Space(*midr, isdual=M.space[1].isdual)
-    return SVDResult(M, keys, Us, Ss, Vs, Vl, Vr)
+    return SVDResult(M, keys, Us, Ss, Vs, Vl, Vr, S_pack=S_pack, S_host=S_host)
 
 
 def svd_(M: SymTensor) -> SVDResult:
     """``_svd_(A)`` (symtools.jl:1-35): untruncated per-sector SVD."""
-    keys, Us, Ss, Vs, midl, midr = [], [], [], [], [], []
+    keys, Us, Ss, Vs, S_host, midl, midr = [], [], [], [], [], [], []
     dt = M.dtype
-    for key, U, S, Vh in _svd_blocks(M):
+    facts = _svd_blocks(M)
+    S_pack = None
+    if facts:
+        S_cat = torch.cat([f[2] for f in facts])
+        offs = np.concatenate([[0], np.cumsum([len(f[2]) for f in facts])])
+        svals = np.split(S_cat.cpu().numpy(), offs[1:-1])
+        S_pack = S_cat if S_cat.dtype == dt else S_cat.to(dt)
+    for b, (key, U, S, Vh) in enumerate(facts):
         keys.append(key)
         Us.append(U.to(dt))
-        Ss.append(S.to(dt))
+        Ss.append(S_pack[int(offs[b]):int(offs[b + 1])])
+        S_host.append(svals[b].copy())
         Vs.append(Vh.to(dt))
         midl.append((key[0], S.numel()))
         midr.append((key[1], S.numel()))
     Vl = U1Space(*midl, isdual=M.space[0].isdual)
     Vr = U1Space(*midr, isdual=M.space[1].isdual)
-    return SVDResult(M, keys, Us, Ss, Vs, Vl, Vr)
+    return SVDResult(M, keys, Us, Ss, Vs, Vl, Vr, S_pack=S_pack, S_host=S_host if facts else None)
 
 
 # ---------------------------------------------------------------------------
