@@ -115,6 +115,7 @@ struct JacMat {
   int32_t w, nb;         // column-block width and block count (even, >= 2)
   int32_t trans, pre;    // trans: W = A^T; pre: preconditioned by the Gram eigenvectors
   int32_t g_info, pad;   // preconditioned: index of the syevd info entry
+  int64_t sg_off, rk_off;  // sigma (c doubles) and rank (c int32) of the finish
 };
 struct JacTask {
   int32_t mat, k;  // matrix (index into the JacMat table) and pair index within a round
@@ -439,31 +440,58 @@ __global__ __launch_bounds__(WB * GSZ) void jac_pairs(const JacMat* __restrict__
 
 // Scale every work matrix by a power of two so its largest |entry| lies in
 // [0.5, 1): products of columns then neither underflow (blocks of 1e-120 occur
-// in DMRG) nor overflow.  sexp[mat] keeps the exponent for the finish.
-__global__ __launch_bounds__(JT) void jac_scale(const JacMat* __restrict__ mats, double* __restrict__ ws,
-                                                int* __restrict__ sexp) {
+// in DMRG) nor overflow.  Two passes over JSPLIT slices per matrix: the largest
+// |entry| (atomic max of the non-negative doubles' bit patterns, which order
+// like the values) and a flag for non-finite entries, then the scaling.
+// sexp[mat] keeps the exponent for the finish; NaN / inf leave the matrix
+// unscaled (reported by the finish).
+constexpr int JSPLIT = 32;
+__device__ __forceinline__ void slice_of(int64_t n, int64_t& b0, int64_t& b1) {
+  const int64_t per = (n + JSPLIT - 1) / JSPLIT;
+  b0 = (int64_t)blockIdx.y * per;
+  b1 = b0 + per < n ? b0 + per : n;
+}
+__global__ __launch_bounds__(JT) void jac_absmax(const JacMat* __restrict__ mats, const double* __restrict__ jw,
+                                                 unsigned long long* __restrict__ amax, int* __restrict__ bad) {
   __shared__ double red[JT / 64];
-  __shared__ int e_sh;
+  __shared__ int nf_any;
   const JacMat M = mats[blockIdx.x];
-  double* W = ws + M.w0_off;
-  const int64_t n = M.r * M.c;
+  const double* W = jw + M.w0_off;
+  int64_t b0, b1;
+  slice_of(M.r * M.c, b0, b1);
+  if (threadIdx.x == 0) nf_any = 0;
+  __syncthreads();
   double mx = 0.0;
-  for (int64_t e = threadIdx.x; e < n; e += JT) mx = fmax(mx, fabs(W[e]));
+  int nf = 0;
+  for (int64_t e = b0 + threadIdx.x; e < b1; e += JT) {
+    const double x = W[e];
+    mx = fmax(mx, fabs(x));
+    nf |= !(x == x) || fabs(x) > 1.7976931348623157e308;
+  }
   for (int o = 32; o > 0; o >>= 1) mx = fmax(mx, __shfl_xor(mx, o, 64));
   if (threadIdx.x % 64 == 0) red[threadIdx.x / 64] = mx;
+  if (nf) nf_any = 1;
   __syncthreads();
   if (threadIdx.x == 0) {
     double m = 0.0;
     for (int k = 0; k < JT / 64; ++k) m = fmax(m, red[k]);
-    int ex = 0;
-    if (m > 0.0 && m <= 1.7976931348623157e308) frexp(m, &ex);  // NaN / inf: unscaled, reported later
-    e_sh = ex;
-    sexp[blockIdx.x] = ex;
+    if (m > 0.0) atomicMax(&amax[blockIdx.x], (unsigned long long)__double_as_longlong(m));
+    if (nf_any) atomicOr(&bad[blockIdx.x], 1);
   }
-  __syncthreads();
-  const int ex = e_sh;
+}
+__global__ __launch_bounds__(JT) void jac_rescale(const JacMat* __restrict__ mats, double* __restrict__ jw,
+                                                  const unsigned long long* __restrict__ amax,
+                                                  int* __restrict__ sexp) {
+  const JacMat M = mats[blockIdx.x];
+  const double m = __longlong_as_double((long long)amax[blockIdx.x]);
+  int ex = 0;
+  if (m > 0.0 && m <= 1.7976931348623157e308) frexp(m, &ex);
+  if (blockIdx.y == 0 && threadIdx.x == 0) sexp[blockIdx.x] = ex;
   if (ex == 0) return;
-  for (int64_t e = threadIdx.x; e < n; e += JT) W[e] = ldexp(W[e], -ex);
+  double* W = jw + M.w0_off;
+  int64_t b0, b1;
+  slice_of(M.r * M.c, b0, b1);
+  for (int64_t e = b0 + threadIdx.x; e < b1; e += JT) W[e] = ldexp(W[e], -ex);
 }
 
 // After a sweep of rounds: matrices whose largest relative off-diagonal stayed
@@ -492,40 +520,40 @@ __global__ void jac_check(const JacMat* __restrict__ mats, const int32_t* __rest
 // Gram slots of one bucket (after the syrk wrote the upper triangles of the
 // c x c Gram matrices): padded rows / columns c .. P get -1 on the diagonal
 // and 0 elsewhere, so the padding's eigenpairs (-1, unit vectors) lie below
-// every eigenvalue of the Gram matrix (>= 0) and stay apart from them.  A slot
-// with a non-finite entry is replaced by the identity (the sweeps then report
-// the input as not finite).
+// every eigenvalue of the Gram matrix (>= 0) and stay apart from them.  The
+// slot of a matrix with a non-finite entry is set to the identity (the sweeps
+// then report the input as not finite).  JSPLIT slices per slot.
 __global__ __launch_bounds__(JT) void jac_gram_pad(const JacMat* __restrict__ mats, const int32_t* __restrict__ list,
-                                                   double* __restrict__ jw, int64_t P) {
-  __shared__ int bad;
-  const JacMat M = mats[list[blockIdx.x]];
+                                                   const int* __restrict__ bad, double* __restrict__ jw, int64_t P) {
+  const int mi = list[blockIdx.x];
+  const JacMat M = mats[mi];
   double* G = jw + M.g_off;
   const int64_t c = M.c;
-  if (threadIdx.x == 0) bad = 0;
-  __syncthreads();
-  int b = 0;
-  for (int64_t e = threadIdx.x; e < c * c; e += JT) {
-    const int64_t j = e / c, i = e - j * c;
-    if (i <= j) {
-      const double x = G[i + j * P];
-      b |= !(x == x) || fabs(x) > 1.7976931348623157e308;
-    }
-  }
-  if (b) bad = 1;
-  __syncthreads();
-  if (bad) {
-    for (int64_t e = threadIdx.x; e < P * P; e += JT) G[e] = (e / P == e % P) ? 1.0 : 0.0;
+  int64_t b0, b1;
+  if (bad[mi]) {
+    slice_of(P * P, b0, b1);
+    for (int64_t e = b0 + threadIdx.x; e < b1; e += JT) G[e] = (e / P == e % P) ? 1.0 : 0.0;
     return;
   }
-  for (int64_t e = threadIdx.x; e < P * P; e += JT) {
-    const int64_t j = e / P, i = e - j * P;
-    if (i >= c || j >= c) G[e] = (i == j) ? -1.0 : 0.0;
+  // columns c .. P-1 (all rows), then rows c .. P-1 of columns 0 .. c-1
+  const int64_t n1 = (P - c) * P, n2 = (P - c) * c;
+  slice_of(n1 + n2, b0, b1);
+  for (int64_t e = b0 + threadIdx.x; e < b1; e += JT) {
+    int64_t i, j;
+    if (e < n1) {
+      j = c + e / P;
+      i = e - (j - c) * P;
+    } else {
+      j = (e - n1) / (P - c);
+      i = c + (e - n1) - j * (P - c);
+    }
+    G[i + j * P] = (i == j) ? -1.0 : 0.0;
   }
 }
 
 // Q <- the eigenvectors of the Gram matrix by descending eigenvalue (syevd
 // returns them ascending, the padding's -1 first): Q(:, j) = V(0:c, P-1-j).
-// A matrix whose syevd failed keeps Q = I.
+// A matrix whose syevd failed keeps Q = I.  JSPLIT slices per matrix.
 __global__ __launch_bounds__(JT) void jac_take_v(const JacMat* __restrict__ mats, const int32_t* __restrict__ list,
                                                  double* __restrict__ jw, const rocblas_int* __restrict__ info,
                                                  int64_t P) {
@@ -534,7 +562,9 @@ __global__ __launch_bounds__(JT) void jac_take_v(const JacMat* __restrict__ mats
   const double* G = jw + M.g_off;
   double* Q = jw + M.q_off;
   const int64_t c = M.c;
-  for (int64_t e = threadIdx.x; e < c * c; e += JT) {
+  int64_t b0, b1;
+  slice_of(c * c, b0, b1);
+  for (int64_t e = b0 + threadIdx.x; e < b1; e += JT) {
     const int64_t j = e / c, i = e - j * c;
     Q[e] = G[i + (P - 1 - j) * P];
   }
@@ -547,59 +577,77 @@ __global__ __launch_bounds__(JT) void jac_init_q(const JacMat* __restrict__ mats
   for (int64_t e = threadIdx.x; e < cc; e += JT) Q[e] = (e / M.c == e % M.c) ? 1.0 : 0.0;
 }
 
-// sigma, sort (descending; ties by column index) and the packed outputs; the
-// left vector of a zero column is completed to an orthonormal set by
-// Gram-Schmidt on unit vectors.
+// The finish, in four launches: sigma_j = |W_j| (one wave per column), the
+// rank of each column by descending sigma (ties by column index) with S and the
+// count of zero columns, the left / right vectors in rank order (one wave per
+// column), and the completion of zero columns' left vectors to an orthonormal
+// set by Gram-Schmidt on unit vectors (one work group per matrix that has any).
 constexpr int JF_MAX = 2048;
-__global__ __launch_bounds__(JT) void jac_finish(const JacMat* __restrict__ mats, const double* __restrict__ ws,
-                                                 const int* __restrict__ sexp, double* __restrict__ U,
-                                                 double* __restrict__ S, double* __restrict__ Vt,
-                                                 int* __restrict__ flags) {
-  __shared__ double sig[JF_MAX];
-  __shared__ int rk[JF_MAX];
-  __shared__ double x[JF_MAX];
-  __shared__ double red[JT / 64];
-  __shared__ int nzero;
+constexpr int JF_COLS = JT / 64;  // columns per work group (one per wave)
+__global__ __launch_bounds__(JT) void jac_sigma(const JacMat* __restrict__ mats, double* __restrict__ jw) {
   const JacMat M = mats[blockIdx.x];
+  const int64_t j = (int64_t)blockIdx.y * JF_COLS + threadIdx.x / 64;
+  if (j >= M.c) return;
+  const int ln = threadIdx.x % 64;
+  const double* W = jw + M.w_off + j * M.r;
+  double a = 0.0;
+  for (int64_t row = ln; row < M.r; row += 64) a = fma(W[row], W[row], a);
+  for (int o = 32; o > 0; o >>= 1) a += __shfl_xor(a, o, 64);
+  if (ln == 0) jw[M.sg_off + j] = sqrt(a);
+}
+__global__ __launch_bounds__(JT) void jac_rank(const JacMat* __restrict__ mats, double* __restrict__ jw,
+                                               const int* __restrict__ sexp, double* __restrict__ S,
+                                               int* __restrict__ nzero, int* __restrict__ flags) {
+  __shared__ double sig[JF_MAX];
+  const JacMat M = mats[blockIdx.x];
+  const int64_t c = M.c;
+  if ((int64_t)blockIdx.y * JT >= c) return;
+  for (int64_t i = threadIdx.x; i < c; i += JT) sig[i] = jw[M.sg_off + i];
+  __syncthreads();
+  const int64_t j = (int64_t)blockIdx.y * JT + threadIdx.x;
+  if (j >= c) return;
+  const double sj = sig[j];
+  if (!(sj == sj) || sj > 1.7e308) flags[0] = 1;
+  int n = 0;
+  for (int64_t i = 0; i < c; ++i) n += (sig[i] > sj) || (sig[i] == sj && i < j);
+  ((int*)(jw + M.rk_off))[j] = n;
+  S[M.s_off + n] = ldexp(sj, sexp[blockIdx.x]);
+  if (!(sj > 1e-140)) atomicAdd(&nzero[blockIdx.x], 1);
+}
+// left vector (length r) of rank q: U column (trans 0) or Vt row (trans 1); right vector likewise
+__global__ __launch_bounds__(JT) void jac_vectors(const JacMat* __restrict__ mats, const double* __restrict__ jw,
+                                                  double* __restrict__ U, double* __restrict__ Vt) {
+  const JacMat M = mats[blockIdx.x];
+  const int64_t j = (int64_t)blockIdx.y * JF_COLS + threadIdx.x / 64;
+  if (j >= M.c) return;
+  const int ln = threadIdx.x % 64;
   const int64_t r = M.r, c = M.c, k = c;
-  const double* W = ws + M.w_off;
-  const double* Q = ws + M.q_off;
-  const int tid = threadIdx.x, wave = tid / 64, ln = tid % 64;
-  for (int64_t j = wave; j < c; j += JT / 64) {
-    double a = 0.0;
-    for (int64_t row = ln; row < r; row += 64) a = fma(W[j * r + row], W[j * r + row], a);
-    for (int o = 32; o > 0; o >>= 1) a += __shfl_xor(a, o, 64);
-    if (ln == 0) sig[j] = sqrt(a);
-  }
-  if (tid == 0) nzero = 0;
-  __syncthreads();
-  for (int64_t j = tid; j < c; j += JT) {
-    const double sj = sig[j];
-    if (!(sj == sj) || sj > 1.7e308) flags[0] = 1;
-    int n = 0;
-    for (int64_t i = 0; i < c; ++i) n += (sig[i] > sj) || (sig[i] == sj && i < j);
-    rk[j] = n;
-    if (!(sj > 1e-140)) atomicAdd(&nzero, 1);
-  }
-  __syncthreads();
-  // left vector (length r) of rank q: U column (trans 0) or Vt row (trans 1)
-  const int64_t ldl = M.trans ? k : 1, stl = M.trans ? 1 : M.m;  // element i of left vector q at base + q*stl + i*ldl
+  const double* W = jw + M.w_off;
+  const double* Q = jw + M.q_off;
+  const int64_t ldl = M.trans ? k : 1, stl = M.trans ? 1 : M.m;
   double* Lb = M.trans ? Vt + M.v_off : U + M.u_off;
   double* Rb = M.trans ? U + M.u_off : Vt + M.v_off;
-  const int64_t ldr = M.trans ? 1 : k, str = M.trans ? M.m : 1;  // right vector (length c) likewise
-  for (int64_t j = 0; j < c; ++j) {
-    const double sj = sig[j];
-    const int64_t q = rk[j];
-    if (tid == 0) S[M.s_off + q] = ldexp(sj, sexp[blockIdx.x]);
-    const bool live = sj > 1e-140;  // columns the rotations treat as zero get completed below
-    const double inv = live ? 1.0 / sj : 0.0;
-    for (int64_t row = tid; row < r; row += JT) Lb[q * stl + row * ldl] = live ? W[j * r + row] * inv : 0.0;
-    for (int64_t row = tid; row < c; row += JT) Rb[q * str + row * ldr] = Q[j * c + row];
-  }
-  if (nzero == 0) return;
-  __syncthreads();
-  // completion: zero columns have the largest ranks c - nzero .. c - 1
-  for (int64_t q = c - nzero; q < c; ++q) {
+  const int64_t ldr = M.trans ? 1 : k, str = M.trans ? M.m : 1;
+  const double sj = jw[M.sg_off + j];
+  const int64_t q = ((const int*)(jw + M.rk_off))[j];
+  const bool live = sj > 1e-140;  // columns the rotations treat as zero get completed below
+  const double inv = live ? 1.0 / sj : 0.0;
+  for (int64_t row = ln; row < r; row += 64) Lb[q * stl + row * ldl] = live ? W[j * r + row] * inv : 0.0;
+  for (int64_t row = ln; row < c; row += 64) Rb[q * str + row * ldr] = Q[j * c + row];
+}
+// completion: zero columns have the largest ranks c - nzero .. c - 1
+__global__ __launch_bounds__(JT) void jac_complete(const JacMat* __restrict__ mats, const int* __restrict__ nzero,
+                                                   double* __restrict__ U, double* __restrict__ Vt) {
+  __shared__ double x[JF_MAX];
+  __shared__ double red[JT / 64];
+  const int nz = nzero[blockIdx.x];
+  if (nz == 0) return;
+  const JacMat M = mats[blockIdx.x];
+  const int64_t r = M.r, c = M.c, k = c;
+  const int tid = threadIdx.x, wave = tid / 64, ln = tid % 64;
+  const int64_t ldl = M.trans ? k : 1, stl = M.trans ? 1 : M.m;
+  double* Lb = M.trans ? Vt + M.v_off : U + M.u_off;
+  for (int64_t q = c - nz; q < c; ++q) {
     for (int64_t cand = 0; cand < r; ++cand) {
       for (int64_t row = tid; row < r; row += JT) x[row] = row == cand ? 1.0 : 0.0;
       __syncthreads();
@@ -675,6 +723,7 @@ struct mpsh_svd_s {
   int32_t jsmall_cls[4] = {0, 0, 0, 0}, jbig_cls[4] = {0, 0, 0, 0};  // by width class (16, 8, 4): [cls, cls+1)
   std::vector<int32_t> jbig_idx;      // matrices with more than two column blocks
   int32_t jrounds = 0;                // rounds per sweep (max over matrices)
+  int64_t jcmax = 0;                  // largest column count of the Jacobi matrices
   std::vector<GramBucket> gb;         // preconditioning: syevd size buckets (ascending P)
   std::vector<int32_t> gmat;          // their matrices, bucket by bucket
   int64_t jac_off = 0, jac_meta_off = 0;  // workspace (doubles): W | Q, then offmax | done | left, flags
@@ -752,15 +801,22 @@ void jacobi_execute(const mpsh_svd_s& P, const mpsh_svd_s::Dev& cp, const double
                     double* Vt, double* ws, hipStream_t s) {
   const int nj = (int)P.jm.size();
   double* jw = ws + P.jac_off;
+  // meta: offmax | amax (u64) | done | sexp | bad | nzero (int32, even counts) | left, flags[2], rounds
   unsigned long long* offmax = (unsigned long long*)(ws + P.jac_meta_off);
-  int* done = (int*)(offmax + nj);
-  int* sexp = done + nj + (nj & 1);
-  int* left = sexp + nj + (nj & 1);
+  unsigned long long* amax = offmax + nj;
+  const int nje = nj + (nj & 1);
+  int* done = (int*)(amax + nj);
+  int* sexp = done + nje;
+  int* bad = sexp + nje;
+  int* nzero = bad + nje;
+  int* left = nzero + nje;
   int* flags = left + 1;
-  ck(hipMemsetAsync(offmax, 0, (size_t)nj * 8 + (size_t)(nj + (nj & 1)) * 8 + 16, s), "hipMemsetAsync(jacobi)");
+  ck(hipMemsetAsync(offmax, 0, (size_t)nj * 16 + (size_t)nje * 16 + 16, s), "hipMemsetAsync(jacobi)");
   ck(launch_copy(cp.jinit, 8, A, jw, s), "jacobi init W");
-  hipLaunchKernelGGL(jac_scale, dim3(nj), dim3(JT), 0, s, cp.jm, jw, sexp);
-  ck(hipGetLastError(), "jac_scale");
+  hipLaunchKernelGGL(jac_absmax, dim3(nj, JSPLIT), dim3(JT), 0, s, cp.jm, (const double*)jw, amax, bad);
+  ck(hipGetLastError(), "jac_absmax");
+  hipLaunchKernelGGL(jac_rescale, dim3(nj, JSPLIT), dim3(JT), 0, s, cp.jm, jw, (const unsigned long long*)amax, sexp);
+  ck(hipGetLastError(), "jac_rescale");
   hipLaunchKernelGGL(jac_init_q, dim3(nj), dim3(JT), 0, s, cp.jm, jw);
   ck(hipGetLastError(), "jac_init_q");
   if (!P.gb.empty()) {
@@ -785,15 +841,15 @@ void jacobi_execute(const mpsh_svd_s& P, const mpsh_svd_s::Dev& cp, const double
                           &one, jw + J.w0_off, (rocblas_int)J.r, &zero, jw + J.g_off, (rocblas_int)B.P),
             "rocblas_dsyrk");
       }
-      hipLaunchKernelGGL(jac_gram_pad, dim3((unsigned)B.count), dim3(JT), 0, ls, cp.jm, cp.gmat + B.mat_start, jw,
-                         B.P);
+      hipLaunchKernelGGL(jac_gram_pad, dim3((unsigned)B.count, JSPLIT), dim3(JT), 0, ls, cp.jm, cp.gmat + B.mat_start,
+                         (const int*)bad, jw, B.P);
       ck(hipGetLastError(), "jac_gram_pad");
       ckb(rocsolver_dsyevd_strided_batched(h, rocblas_evect_original, rocblas_fill_upper, (rocblas_int)B.P,
                                            jw + B.g_off, (rocblas_int)B.P, B.P * B.P, jw + B.d_off, B.P,
                                            jw + B.e_off, B.P, info + B.info_start, (rocblas_int)B.count),
           "rocsolver_dsyevd_strided_batched");
-      hipLaunchKernelGGL(jac_take_v, dim3((unsigned)B.count), dim3(JT), 0, ls, cp.jm, cp.gmat + B.mat_start, jw,
-                         (const rocblas_int*)info, B.P);
+      hipLaunchKernelGGL(jac_take_v, dim3((unsigned)B.count, JSPLIT), dim3(JT), 0, ls, cp.jm, cp.gmat + B.mat_start,
+                         jw, (const rocblas_int*)info, B.P);
       ck(hipGetLastError(), "jac_take_v");
       for (int64_t k = 0; k < B.count; ++k) {
         const JacMat& J = P.jm[P.gmat[B.mat_start + k]];
@@ -848,9 +904,16 @@ void jacobi_execute(const mpsh_svd_s& P, const mpsh_svd_s::Dev& cp, const double
       fprintf(stderr, "mpsh_svd: %d Jacobi sweeps, %d rounds (%d per full sweep), %d preconditioned in %d buckets\n",
               sweeps, total_rounds, P.jrounds, (int)P.gmat.size(), (int)P.gb.size());
   }
-  hipLaunchKernelGGL(jac_finish, dim3(nj), dim3(JT), 0, s, cp.jm, (const double*)jw, (const int*)sexp, U, S, Vt,
-                     flags);
-  ck(hipGetLastError(), "jac_finish");
+  const unsigned ycols = (unsigned)((P.jcmax + JF_COLS - 1) / JF_COLS);
+  hipLaunchKernelGGL(jac_sigma, dim3(nj, ycols), dim3(JT), 0, s, cp.jm, jw);
+  ck(hipGetLastError(), "jac_sigma");
+  hipLaunchKernelGGL(jac_rank, dim3(nj, (unsigned)((P.jcmax + JT - 1) / JT)), dim3(JT), 0, s, cp.jm, jw,
+                     (const int*)sexp, S, nzero, flags);
+  ck(hipGetLastError(), "jac_rank");
+  hipLaunchKernelGGL(jac_vectors, dim3(nj, ycols), dim3(JT), 0, s, cp.jm, (const double*)jw, U, Vt);
+  ck(hipGetLastError(), "jac_vectors");
+  hipLaunchKernelGGL(jac_complete, dim3(nj), dim3(JT), 0, s, cp.jm, (const int*)nzero, U, Vt);
+  ck(hipGetLastError(), "jac_complete");
   int hf[2] = {0, 0};
   ck(hipMemcpyAsync(hf, flags, 8, hipMemcpyDeviceToHost, s), "hipMemcpyAsync(jacobi flags)");
   ck(hipStreamSynchronize(s), "hipStreamSynchronize");
@@ -979,7 +1042,12 @@ mpsh_status mpsh_svd_plan(int64_t nblocks, const int64_t* m, const int64_t* n, c
         J.w0_off = jw;
         jw += J.r * J.c;
       }
+      J.sg_off = jw;
+      jw += J.c;
+      J.rk_off = jw;
+      jw += (J.c + 1) / 2;
       jw = (jw + 1) / 2 * 2;
+      P->jcmax = std::max(P->jcmax, J.c);
       const int32_t idx = (int32_t)P->jm.size();
       const int cls = J.w == 16 ? 0 : (J.w == 8 ? 1 : 2);
       if (J.nb == 2) {
@@ -1041,7 +1109,7 @@ mpsh_status mpsh_svd_plan(int64_t nblocks, const int64_t* m, const int64_t* n, c
     w += jw;
     P->jac_meta_off = w;
     const int64_t nj = (int64_t)P->jm.size();
-    w += nj + 2 * ((nj + 1) / 2) + 2;  // offmax (u64) | done (int32) | sexp (int32) | left, flags
+    w += 2 * nj + 4 * ((nj + 1) / 2) + 2;  // offmax, amax (u64) | done, sexp, bad, nzero (int32) | left, flags, rounds
     w = (w + 1) / 2 * 2;
     P->jinit.finalize();
     P->ws_doubles = w;

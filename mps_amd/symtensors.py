@@ -178,7 +178,9 @@ _DT = {torch.float64: L.MPSH_F64, torch.float32: L.MPSH_F32}
 
 
 def _stream():
-    return torch.cuda.current_stream().cuda_stream
+    """The raw handle of torch's current stream on the current device (the
+    low-level accessors: this runs several times per matvec)."""
+    return torch._C._cuda_getCurrentRawStream(torch._C._cuda_getDevice())
 
 
 def _ptr(t: torch.Tensor):

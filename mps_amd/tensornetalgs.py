@@ -265,7 +265,8 @@ TUNE_COUNTS = {"timed": 0, "cached": 0}  # autotuner calls that timed variants /
 
 class _LanczosBasis:
     """Device-resident Lanczos basis: V[0..kdim] (FP64 rows), the device scalars of
-    one iteration and the partial-sum scratch of the ``mpsh_lanczos_*`` kernels."""
+    each iteration (block j: c1 | c2 | beta), their pinned host copies and the
+    partial-sum scratch of the ``mpsh_lanczos_*`` kernels."""
 
     def __init__(self, x: SymTensor, kdim: int):
         self.like = x
@@ -273,8 +274,13 @@ class _LanczosBasis:
         self.kdim = kdim
         dev = x.device
         self.ld = max(1, self.n)
+        kd = kdim + 1
+        self.blk = 2 * kd + 2  # doubles per iteration block: c1 (kd) | c2 (kd) | beta | pad
         self.V = torch.empty((kdim + 1, self.ld), dtype=torch.float64, device=dev)
-        self.sc = torch.zeros(2 * (kdim + 1) + 2, dtype=torch.float64, device=dev)  # c1 | c2 | beta
+        # blocks 0..kdim-1: iterations; block kdim: the start / Ritz normalisations
+        self.sc = torch.zeros((kdim + 1, self.blk), dtype=torch.float64, device=dev)
+        self.host = torch.zeros((kdim + 1, self.blk), dtype=torch.float64, pin_memory=dev.type == "cuda")
+        self.events = [torch.cuda.Event() if dev.type == "cuda" else None for _ in range(kdim)]
         self.part = torch.empty((kdim + 1) * L.LZ_PARTS, dtype=torch.float64, device=dev)
         self.graphs = {}
         self.outs = {}
@@ -282,46 +288,59 @@ class _LanczosBasis:
     def vec(self, j) -> SymTensor:
         return SymTensor(self.like.charge, self.like.space, self.V[j, :self.n], self.like.struct)
 
+    def _blk(self, j) -> int:
+        return self.sc[j].data_ptr()
+
     def iterate(self, apply, j):
         """w = apply(V_j); two classical Gram-Schmidt passes against V_0..V_j (scalars
-        c1, c2 on the device); V_{j+1} = w / ||w||.  No host synchronisation."""
+        c1, c2 of block j on the device); V_{j+1} = w / ||w||.  No host synchronisation."""
         w = apply(self.vec(j))
         lb, st, k, kd = lib(), _stream(), j + 1, self.kdim + 1
-        V, wp, sc = _ptr(self.V), _ptr(w.data), self.sc.data_ptr()
-        check(lb.mpsh_lanczos_dots(k, self.n, V, self.ld, wp, sc, _ptr(self.part), st))
-        check(lb.mpsh_lanczos_update(k, self.n, V, self.ld, sc, -1.0, wp, st))
-        check(lb.mpsh_lanczos_dots(k, self.n, V, self.ld, wp, sc + 8 * kd, _ptr(self.part), st))
-        check(lb.mpsh_lanczos_update(k, self.n, V, self.ld, sc + 8 * kd, -1.0, wp, st))
-        check(lb.mpsh_lanczos_normalize(self.n, wp, self.V[j + 1].data_ptr(), sc + 16 * kd, _ptr(self.part), st))
+        V, wp, b = _ptr(self.V), _ptr(w.data), self._blk(j)
+        check(lb.mpsh_lanczos_dots(k, self.n, V, self.ld, wp, b, _ptr(self.part), st))
+        check(lb.mpsh_lanczos_update(k, self.n, V, self.ld, b, -1.0, wp, st))
+        check(lb.mpsh_lanczos_dots(k, self.n, V, self.ld, wp, b + 8 * kd, _ptr(self.part), st))
+        check(lb.mpsh_lanczos_update(k, self.n, V, self.ld, b + 8 * kd, -1.0, wp, st))
+        check(lb.mpsh_lanczos_normalize(self.n, wp, self.V[j + 1].data_ptr(), b + 16 * kd, _ptr(self.part), st))
         return w
 
     def step(self, apply, j, graph: bool):
+        """Enqueue iteration j and the copy of its scalars to the host (no wait)."""
         if not graph:
             self.outs[j] = self.iterate(apply, j)
-            return
-        g = self.graphs.get(j)
-        if g is None:
-            g = torch.cuda.CUDAGraph()
-            with torch.cuda.graph(g):
-                self.outs[j] = self.iterate(apply, j)
-            self.graphs[j] = g
-        g.replay()
+        else:
+            g = self.graphs.get(j)
+            if g is None:
+                g = torch.cuda.CUDAGraph()
+                with torch.cuda.graph(g):
+                    self.outs[j] = self.iterate(apply, j)
+                self.graphs[j] = g
+            g.replay()
+        if self.events[j] is not None:
+            self.host[j].copy_(self.sc[j], non_blocking=True)
+            self.events[j].record()
 
     def scalars(self, j):
-        """(alpha_j, beta_j) after iteration j (one device-to-host copy)."""
-        s = self.sc.cpu().numpy()
+        """(alpha_j, beta_j) of iteration j: waits for that iteration only."""
         kd = self.kdim + 1
+        if self.events[j] is not None:
+            self.events[j].synchronize()
+            s = self.host[j].numpy()
+        else:
+            s = self.sc[j].cpu().numpy()
         return float(s[j] + s[kd + j]), float(s[2 * kd])
+
+    def normalize_into_v0(self, x_ptr: int) -> None:
+        check(lib().mpsh_lanczos_normalize(self.n, x_ptr, self.V[0].data_ptr(), self._blk(self.kdim) + 16 * (self.kdim + 1),
+                                           _ptr(self.part), _stream()))
 
     def combine(self, y) -> None:
         """V_0 <- normalised sum_i y_i V_i (the Ritz vector)."""
         k = len(y)
         c = torch.tensor(np.asarray(y, dtype=np.float64), device=self.V.device)
         x = torch.zeros(self.ld, dtype=torch.float64, device=self.V.device)
-        lb, st = lib(), _stream()
-        check(lb.mpsh_lanczos_update(k, self.n, _ptr(self.V), self.ld, _ptr(c), 1.0, _ptr(x), st))
-        check(lb.mpsh_lanczos_normalize(self.n, _ptr(x), self.V[0].data_ptr(), self.sc.data_ptr() + 16 * (self.kdim + 1),
-                                        _ptr(self.part), st))
+        check(lib().mpsh_lanczos_update(k, self.n, _ptr(self.V), self.ld, _ptr(c), 1.0, _ptr(x), _stream()))
+        self.normalize_into_v0(_ptr(x))
 
 
 def lanczos_lowest(apply, x0: SymTensor, tol: float = 1e-7, krylovdim: int = 5, maxiter: int = 8,
@@ -330,37 +349,50 @@ def lanczos_lowest(apply, x0: SymTensor, tol: float = 1e-7, krylovdim: int = 5, 
 
     The basis lives on the device; every iteration applies the map, runs two
     classical Gram-Schmidt passes against the whole basis and normalises, with
-    all scalars in device memory (``mpsh_lanczos_*``), so the host reads just
-    (alpha, beta) once per iteration to solve the tridiagonal eigenproblem in
-    FP64 and test |beta_k y_k| < max(tol, rtol |theta|).  With ``graphs`` each iteration index is
+    all scalars in device memory (``mpsh_lanczos_*``); the host reads (alpha,
+    beta) of each iteration from a pinned copy to solve the tridiagonal
+    eigenproblem in FP64 and test |beta_k y_k| < max(tol, rtol |theta|).
+    Iteration j+1 is enqueued before the host waits for iteration j's scalars,
+    so the device never idles on the host's test; it is skipped when the
+    residuals' geometric trend predicts that iteration j converges (an
+    iteration enqueued past convergence is discarded: it writes only V_{j+2}
+    and its own scalar block).  With ``graphs`` each iteration index is
     captured once in a hipGraph and replayed (the first application runs
     eagerly, so the operator prepares itself outside the capture).
     """
     kd = max(1, int(krylovdim))
     B = _LanczosBasis(x0, kd)
     x = x0 if x0.dtype == torch.float64 else x0.to(torch.float64)
-    lb, st = lib(), _stream()
-    check(lb.mpsh_lanczos_normalize(B.n, _ptr(x.data), B.V[0].data_ptr(), B.sc.data_ptr() + 16 * (kd + 1),
-                                    _ptr(B.part), st))
+    B.normalize_into_v0(_ptr(x.data))
     numops, theta, res, it = 0, None, float("inf"), 0
     warm = not graphs
+    stop = tol
     for it in range(1, maxiter + 1):
-        alphas, betas = [], []
+        alphas, betas, hist = [], [], []
         y = np.array([1.0])
+        launched = 0
         for j in range(kd):
-            B.step(apply, j, graph=graphs and warm)
-            warm = True
-            numops += 1
+            if launched <= j:
+                B.step(apply, j, graph=graphs and warm)
+                warm = True
+                launched = j + 1
+            predicted = hist[-1] * hist[-1] / hist[-2] if len(hist) >= 2 and hist[-2] > 0 else float("inf")
+            if j + 1 < kd and predicted >= stop:
+                B.step(apply, j + 1, graph=graphs and warm)
+                launched = j + 2
             a, b = B.scalars(j)
+            numops += 1
             alphas.append(a)
             T = np.diag(alphas) + np.diag(betas, 1) + np.diag(betas, -1)
             evals, evecs = np.linalg.eigh(T)
             theta, y = float(evals[0]), evecs[:, 0]
             res = abs(b * y[-1])
             stop = max(tol, rtol * abs(theta))
+            hist.append(res)
             if res < stop or j == kd - 1 or b < 1e-14:
                 break
             betas.append(b)
+        numops += launched - len(alphas)  # applications enqueued past the exit
         B.combine(y)
         if res < stop:
             break

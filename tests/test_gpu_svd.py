@@ -95,17 +95,20 @@ def test_svdtrunc_matches_oracle(maxdim, tol):
     assert np.linalg.norm(got - want) <= 1e-12 * np.linalg.norm(want)
 
 
-def _blocks_with(A, algo):
-    """_svd_blocks through a fresh plan built with MPSH_SVD_ALGO=algo (plans read it when built)."""
-    old = os.environ.get("MPSH_SVD_ALGO")
-    os.environ["MPSH_SVD_ALGO"] = algo
+def _blocks_with(A, algo, precond=True):
+    """_svd_blocks through a fresh plan built with MPSH_SVD_ALGO=algo and
+    MPSH_SVD_PRECOND (plans read both when built)."""
+    env = {"MPSH_SVD_ALGO": algo, "MPSH_SVD_PRECOND": "1" if precond else "0"}
+    old = {k: os.environ.get(k) for k in env}
+    os.environ.update(env)
     try:
         P = mt._SvdPlan(A.struct)
     finally:
-        if old is None:
-            del os.environ["MPSH_SVD_ALGO"]
-        else:
-            os.environ["MPSH_SVD_ALGO"] = old
+        for k, v in old.items():
+            if v is None:
+                del os.environ[k]
+            else:
+                os.environ[k] = v
     saved = mt._SVD_PLANS.get(A.struct)
     mt._SVD_PLANS[A.struct] = P
     try:
@@ -154,14 +157,18 @@ def test_jacobi_graded_spectrum_relative_accuracy():
     _check_facts(blocks, _blocks_with(A, "jacobi"))
 
 
-def test_jacobi_rank_deficient_and_zero_blocks():
-    """Exactly zero singular values: the completed singular vectors stay orthonormal."""
-    A = _matrix([(0, 40), (1, 12), (2, 70)], [(0, 25), (1, 12), (2, 90)], seed=43)
+@pytest.mark.parametrize("scale", [1, 3])
+def test_jacobi_rank_deficient_and_zero_blocks(scale):
+    """Exactly zero singular values: the completed singular vectors stay orthonormal.
+    scale 3: blocks of 120+ columns, which take the Gram-preconditioned path."""
+    A = _matrix([(0, 40 * scale), (1, 12), (2, 70 * scale)], [(0, 25 if scale == 1 else 110), (1, 12), (2, 90 * scale)],
+                seed=43)
     blocks = A.blocks()
     keys = list(blocks.keys())
     blocks[keys[0]] = np.zeros_like(blocks[keys[0]])                  # all-zero block
     b = blocks[keys[2]]
-    b[:, 45:] = b[:, :45] @ np.random.default_rng(3).standard_normal((45, b.shape[1] - 45))  # rank 45 of 70
+    rk = 45 * scale
+    b[:, rk:] = b[:, :rk] @ np.random.default_rng(3).standard_normal((rk, b.shape[1] - rk))  # rank 45 scale
     blocks[keys[1]][:, 5] = 0.0                                         # one zero column
     Z = M.SymTensor.from_blocks(A.charge, A.space, blocks, dtype=torch.float64, device=DEV)
     facts = _blocks_with(Z, "jacobi")
@@ -184,8 +191,9 @@ def test_jacobi_wide_blocks_and_rocsolver_fallback():
     _check_facts(blocks, _blocks_with(A, "jacobi"))
 
 
-def test_svd_nonfinite_input_raises():
-    A = _matrix([(0, 50), (1, 80)], [(0, 60), (1, 90)], seed=53)
+@pytest.mark.parametrize("scale", [1, 3])
+def test_svd_nonfinite_input_raises(scale):
+    A = _matrix([(0, 50 * scale), (1, 80 * scale)], [(0, 60 * scale), (1, 90 * scale)], seed=53)
     blocks = A.blocks()
     k1 = list(blocks.keys())[1]
     blocks[k1][3, 7] = np.nan
@@ -194,12 +202,39 @@ def test_svd_nonfinite_input_raises():
         _blocks_with(Z, "jacobi")
 
 
-def test_jacobi_extreme_block_scales():
+@pytest.mark.parametrize("big", [40, 160])
+def test_jacobi_extreme_block_scales(big):
     """Blocks scaled to 1e-120 .. 1e150 (DMRG two-site blocks far from the mean
     charge are ~1e-120): each block's factors match LAPACK relative to its own scale."""
-    A = _matrix([(q, [2, 2, 7, 40, 2][q]) for q in range(5)], [(q, [2, 3, 7, 35, 2][q]) for q in range(5)], seed=59)
+    A = _matrix([(q, [2, 2, 7, big, 2][q]) for q in range(5)], [(q, [2, 3, 7, big - 5, 2][q]) for q in range(5)],
+                seed=59)
     blocks = A.blocks()
     for key, f in zip(list(blocks.keys()), (1e-120, 1e-60, 1e-140, 1e150, 3e-200)):
         blocks[key] = blocks[key] * f
     Z = M.SymTensor.from_blocks(A.charge, A.space, blocks, dtype=torch.float64, device=DEV)
     _check_facts(blocks, _blocks_with(Z, "jacobi"))
+
+
+def test_preconditioned_and_plain_jacobi_agree():
+    """Graded blocks of 100-400 columns (a DMRG-like spectrum spanning 1 .. 1e-9):
+    the Gram-preconditioned sweeps (default) and the plain sweeps
+    (MPSH_SVD_PRECOND=0) both match LAPACK to 1e-12 of the largest singular value
+    and reconstruct the block; their singular values agree to the same bar."""
+    rows = [(0, 100), (1, 260), (2, 410)]
+    cols = [(0, 120), (1, 240), (2, 400)]
+    A = _matrix(rows, cols, seed=61)
+    blocks = A.blocks()
+    rng = np.random.default_rng(7)
+    for k, b in blocks.items():
+        m, n = b.shape
+        u, _ = np.linalg.qr(rng.standard_normal((m, min(m, n))))
+        v, _ = np.linalg.qr(rng.standard_normal((n, min(m, n))))
+        s = np.logspace(0, -9, min(m, n))
+        blocks[k] = (u * s) @ v.T
+    Z = M.SymTensor.from_blocks(A.charge, A.space, blocks, dtype=torch.float64, device=DEV)
+    fp = _blocks_with(Z, "jacobi", precond=True)
+    f0 = _blocks_with(Z, "jacobi", precond=False)
+    _check_facts(blocks, fp)
+    _check_facts(blocks, f0)
+    for (_, _, Sp, _), (_, _, S0, _) in zip(fp, f0):
+        np.testing.assert_allclose(Sp.cpu().numpy(), S0.cpu().numpy(), rtol=1e-12, atol=1e-12 * float(S0[0]))
