@@ -434,15 +434,19 @@ __global__ __launch_bounds__(256) void gemm_grouped(
 }
 
 // ---------------------------------------------------------------------------
-// FP32 grouped GEMM with operands streamed to LDS by DMA (global_load_lds
-// dwordx4: no register staging), a 4-stage LDS ring (loads run three K chunks
-// ahead) and one barrier per chunk.  Same problem / segment / tile / split-K /
-// output conventions as gemm_grouped (tile variant KS = 0).  In LDS a stage
-// holds BM + BN operand rows of 16 floats; lane l of wave w fills the 16-byte
-// slot (row, q) = (w*16 + l/4 + 64 i, l%4) with the K group q ^ dma_swz(row),
-// so the MFMA operand reads (16 rows at one K group) hit distinct banks.  K
-// groups past a segment's K are read from the row start and zeroed in LDS
-// (only in a segment's last chunk), as are the entries past K of the last group.
+// FP32 / FP64 grouped GEMM with operands streamed to LDS by DMA
+// (global_load_lds dwordx4: no register staging), a 4-stage LDS ring (loads run
+// three K chunks ahead) and one barrier per chunk.  Same problem / segment /
+// tile / split-K / output conventions as gemm_grouped (tile variant KS = 0).
+// In LDS a stage holds BM + BN operand rows of 64 bytes (a K chunk of 16 floats
+// or 8 doubles) in four 16-byte K groups; lane l of wave w fills the slot
+// (row, q) = (w*16 + l/4 + 64 i, l%4) with the K group q ^ dma_swz(row), so
+// the MFMA operand reads (16 rows at one K group) hit distinct banks.  A lane
+// of MFMA lane group g reads K group g of its row and feeds one MFMA per
+// element (FP32 16x16x4: 4, FP64 16x16x4: 2), the same K permutation for A
+// and B.  K groups past a segment's K are read from the row start and zeroed in
+// LDS (only in a segment's last chunk), as are the entries past K of the last
+// group.
 // ---------------------------------------------------------------------------
 template <int N>
 __device__ __forceinline__ void wait_vmcnt() {
@@ -470,38 +474,68 @@ __device__ __forceinline__ int64_t uniform_i64(int64_t x) {
   return (int64_t)(((uint64_t)hi << 32) | lo);
 }
 
-template <int BM, int BN>
+template <int MODE>
+struct DmaT;
+template <>
+struct DmaT<1> {  // FP32: 4 floats per K group, MFMA 16x16x4 f32 (output column 4g + r)
+  typedef float T;
+  typedef floatx4 F;
+  typedef floatx4 Acc;
+  static constexpr int EG = 4;
+  static __device__ __forceinline__ void mfma(Acc& c, const F& b, const F& a, int kk) {
+    c = __builtin_amdgcn_mfma_f32_16x16x4f32(b[kk], a[kk], c, 0, 0, 0);
+  }
+  static __device__ __forceinline__ int col(int g, int r) { return 4 * g + r; }
+};
+template <>
+struct DmaT<0> {  // FP64: 2 doubles per K group, MFMA 16x16x4 f64 (output column g + 4r)
+  typedef double T;
+  typedef double2 F;
+  typedef doublex4 Acc;
+  static constexpr int EG = 2;
+  static __device__ __forceinline__ void mfma(Acc& c, const F& b, const F& a, int kk) {
+    c = __builtin_amdgcn_mfma_f64_16x16x4f64(kk == 0 ? b.x : b.y, kk == 0 ? a.x : a.y, c, 0, 0, 0);
+  }
+  static __device__ __forceinline__ int col(int g, int r) { return g + 4 * r; }
+};
+
+template <int MODE, int BM, int BN>
 __global__ __launch_bounds__(256) void gemm_dma(
     const GemmProb* __restrict__ probs, const GemmSeg* __restrict__ segs, const int64_t* __restrict__ atab,
     const int64_t* __restrict__ btab, const int64_t* __restrict__ crow, const int64_t* __restrict__ cld,
-    const GemmTile* __restrict__ tiles, XcdQueues xq, const float* __restrict__ A, const float* __restrict__ B,
-    float* __restrict__ C, float* __restrict__ P) {
-  constexpr int BK = 16, NA = BM / 64, NB = BN / 64, SM = BM / 32, SN = BN / 32;
-  constexpr int STG = (BM + BN) * BK;  // floats per stage
+    const GemmTile* __restrict__ tiles, XcdQueues xq, const typename DmaT<MODE>::T* __restrict__ A,
+    const typename DmaT<MODE>::T* __restrict__ B, typename DmaT<MODE>::T* __restrict__ C,
+    typename DmaT<MODE>::T* __restrict__ P) {
+  typedef typename DmaT<MODE>::T T;
+  typedef typename DmaT<MODE>::F F;
+  typedef typename DmaT<MODE>::Acc Acc;
+  constexpr int EG = DmaT<MODE>::EG;  // elements per 16-byte K group
+  constexpr int BK = 4 * EG, NA = BM / 64, NB = BN / 64, SM = BM / 32, SN = BN / 32;
+  constexpr int STG = (BM + BN) * BK;  // elements per stage
   // the four stages are distinct LDS variables so the compiler can tell a DMA
   // into one stage from the operand reads of another
-  __shared__ __attribute__((aligned(16))) float S0[STG], S1[STG], S2[STG], S3[STG];
+  __shared__ __attribute__((aligned(16))) T S0[STG], S1[STG], S2[STG], S3[STG];
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   const int wm = wave >> 1, wn = wave & 1;
   const int g = lane >> 4, l16 = lane & 15;
   const int lrow = wave * 16 + (lane >> 2), lq = lane & 3;  // + 64 i for the i-th slot of a lane
   const int lkl = lq ^ dma_swz(lrow);  // dma_swz(64 i + lrow) does not depend on i
-  int offa[SM], offb[SN];  // operand read offsets (floats) within a stage
+  int offa[SM], offb[SN];  // operand read offsets (elements) within a stage
 #pragma unroll
   for (int i = 0; i < SM; ++i) {
     const int r = wm * (BM / 2) + i * 16 + l16;
-    offa[i] = (r * 4 + (g ^ dma_swz(r))) * 4;
+    offa[i] = (r * 4 + (g ^ dma_swz(r))) * EG;
   }
 #pragma unroll
   for (int j = 0; j < SN; ++j) {
     const int r = wn * (BN / 2) + j * 16 + l16;
-    offb[j] = BM * BK + (r * 4 + (g ^ dma_swz(r))) * 4;
+    offb[j] = BM * BK + (r * 4 + (g ^ dma_swz(r))) * EG;
   }
   typedef std::integral_constant<int, 0> I0;
   typedef std::integral_constant<int, 1> I1;
   typedef std::integral_constant<int, 2> I2;
   typedef std::integral_constant<int, 3> I3;
-  auto stage = [&](auto ST) -> float* {
+  auto stage = [&](auto ST) -> T* {
     constexpr int st = decltype(ST)::value;
     return st == 0 ? S0 : st == 1 ? S1 : st == 2 ? S2 : S3;
   };
@@ -527,11 +561,13 @@ __global__ __launch_bounds__(256) void gemm_dma(
     p.part_off = uniform_i64(p.part_off);
     p.c_off = uniform_i64(p.c_off);
     p.ldc = uniform_i64(p.ldc);
-    floatx4 acc[SM][SN];
+    Acc acc[SM][SN];
 #pragma unroll
     for (int i = 0; i < SM; ++i)
 #pragma unroll
-      for (int j = 0; j < SN; ++j) acc[i][j] = floatx4{0, 0, 0, 0};
+      for (int j = 0; j < SN; ++j)
+#pragma unroll
+        for (int r = 0; r < 4; ++r) acc[i][j][r] = 0;
     int64_t ctot = 0;
     for (int s = 0; s < p.nseg; ++s) ctot += (uniform_i64(segs[p.seg0 + s].K) + BK - 1) / BK;
     const int64_t cb = ctot * tile.split / p.nsplit, ce = ctot * (tile.split + 1) / p.nsplit;
@@ -590,21 +626,21 @@ __global__ __launch_bounds__(256) void gemm_dma(
       }
     };
     if (nch > 0) bases();
-    int krem[4];  // K entries left at the start of the chunk held by each stage (>= 16: no tail)
+    int krem[4];  // K entries left at the start of the chunk held by each stage (>= BK: no tail)
     auto issue = [&](auto ST) {
       constexpr int st = decltype(ST)::value;
-      float* sp = stage(ST);
-      const int64_t kg = k0 + 4 * lkl;
+      T* sp = stage(ST);
+      const int64_t kg = k0 + EG * lkl;
       const int64_t ko = kg < K ? kg : 0;  // groups past K: read the row start, zeroed later
       krem[st] = (int)std::min<int64_t>(K - k0, BK);
 #pragma unroll
       for (int i = 0; i < NA; ++i)
-        __builtin_amdgcn_global_load_lds((const void*)(A + ab[i] + ko), (lds_void_ptr)(sp + (i * 256 + wave * 64) * 4),
-                                         16, 0, 0);
+        __builtin_amdgcn_global_load_lds((const void*)(A + ab[i] + ko),
+                                         (lds_void_ptr)(sp + (i * 256 + wave * 64) * EG), 16, 0, 0);
 #pragma unroll
       for (int i = 0; i < NB; ++i)
         __builtin_amdgcn_global_load_lds((const void*)(B + bb[i] + ko),
-                                         (lds_void_ptr)(sp + BM * BK + (i * 256 + wave * 64) * 4), 16, 0, 0);
+                                         (lds_void_ptr)(sp + BM * BK + (i * 256 + wave * 64) * EG), 16, 0, 0);
       k0 += BK;
       if (k0 >= K && seg + 1 < p.nseg) {
         ++seg;
@@ -614,33 +650,32 @@ __global__ __launch_bounds__(256) void gemm_dma(
     };
     auto compute = [&](auto ST) {
       constexpr int st = decltype(ST)::value;
-      const float* sp = stage(ST);
+      const T* sp = stage(ST);
       if (krem[st] < BK) {
         // segment tail: zero this lane's own slots past K, then make them visible
-        float* wp = stage(ST);
+        T* wp = stage(ST);
         const int kr = krem[st];
 #pragma unroll
         for (int i = 0; i < NA + NB; ++i) {
-          float* slot = wp + (i < NA ? 0 : BM * BK) + ((i < NA ? i : i - NA) * 256 + wave * 64 + lane) * 4;
+          T* slot = wp + (i < NA ? 0 : BM * BK) + ((i < NA ? i : i - NA) * 256 + wave * 64 + lane) * EG;
 #pragma unroll
-          for (int e = 0; e < 4; ++e)
-            if (4 * lkl + e >= kr) slot[e] = 0.0f;
+          for (int e = 0; e < EG; ++e)
+            if (EG * lkl + e >= kr) slot[e] = (T)0;
         }
         wait_lgkmcnt0();
         barrier_raw();
       }
-      floatx4 fa[SM], fb[SN];
+      F fa[SM], fb[SN];
 #pragma unroll
-      for (int j = 0; j < SN; ++j) fb[j] = *(const floatx4*)(sp + offb[j]);
+      for (int j = 0; j < SN; ++j) fb[j] = *(const F*)(sp + offb[j]);
 #pragma unroll
-      for (int i = 0; i < SM; ++i) fa[i] = *(const floatx4*)(sp + offa[i]);
+      for (int i = 0; i < SM; ++i) fa[i] = *(const F*)(sp + offa[i]);
 #pragma unroll
-      for (int kk = 0; kk < 4; ++kk)
+      for (int kk = 0; kk < EG; ++kk)
 #pragma unroll
         for (int i = 0; i < SM; ++i)
 #pragma unroll
-          for (int j = 0; j < SN; ++j)
-            acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x4f32(fb[j][kk], fa[i][kk], acc[i][j], 0, 0, 0);
+          for (int j = 0; j < SN; ++j) DmaT<MODE>::mfma(acc[i][j], fb[j], fa[i], kk);
     };
     // chunk it: wait for its DMA (the chunks after it may stay in flight), barrier
     // (every wave's part landed; every wave is done with chunk it-1, whose stage
@@ -666,9 +701,9 @@ __global__ __launch_bounds__(256) void gemm_dma(
       if (it + 2 < nch) iter(I2{}, it + 2);
       if (it + 3 < nch) iter(I3{}, it + 3);
     }
-    // epilogue as in gemm_grouped (FP32 form: register r is column 4g + r)
+    // epilogue as in gemm_grouped
     const bool split = p.nsplit > 1;
-    float* Pp = P + (split ? p.part_off + (int64_t)tile.split * p.M * p.N : 0);
+    T* Pp = P + (split ? p.part_off + (int64_t)tile.split * p.M * p.N : 0);
 #pragma unroll
     for (int i = 0; i < SM; ++i) {
       const int m = tile.m0 + wm * (BM / 2) + i * 16 + l16;
@@ -683,12 +718,12 @@ __global__ __launch_bounds__(256) void gemm_dma(
           cl_ = p.ldc;
         }
       }
-      float* Cb = split ? Pp : C;
+      T* Cb = split ? Pp : C;
 #pragma unroll
       for (int j = 0; j < SN; ++j)
 #pragma unroll
         for (int r = 0; r < 4; ++r) {
-          const int n = tile.n0 + wn * (BN / 2) + j * 16 + 4 * g + r;
+          const int n = tile.n0 + wn * (BN / 2) + j * 16 + DmaT<MODE>::col(g, r);
           if (n < p.N) Cb[cb_ + (int64_t)n * cl_] = acc[i][j][r];
         }
     }
@@ -762,25 +797,27 @@ static void launch_variant(const DevGemm& gm, const void* A, const void* B, void
                      gm.crow, gm.cld, gm.tiles, gm.q, (const T*)A, (const T*)B, (T*)C, (T*)P);
 }
 
-template <int BM, int BN>
+template <int MODE, int BM, int BN>
 static void launch_variant_dma(const DevGemm& gm, const void* A, const void* B, void* C, void* P, hipStream_t s) {
+  typedef typename DmaT<MODE>::T T;
   static int occ = 0;
   if (occ == 0) {
     int nb = 0;
-    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, gemm_dma<BM, BN>, 256, 0) != hipSuccess || nb <= 0) nb = 2;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, gemm_dma<MODE, BM, BN>, 256, 0) != hipSuccess || nb <= 0)
+      nb = 2;
     occ = nb;
   }
   int qmax = 0;
   for (int x = 0; x < NXCD; ++x) qmax = std::max(qmax, gm.q.off[x + 1] - gm.q.off[x]);
   const int per = std::min(std::max(1, num_cu() / NXCD) * occ, qmax);
-  hipLaunchKernelGGL((gemm_dma<BM, BN>), dim3((unsigned)(per * NXCD)), dim3(256), 0, s, gm.probs, gm.segs, gm.atab,
-                     gm.btab, gm.crow, gm.cld, gm.tiles, gm.q, (const float*)A, (const float*)B, (float*)C,
-                     (float*)P);
+  hipLaunchKernelGGL((gemm_dma<MODE, BM, BN>), dim3((unsigned)(per * NXCD)), dim3(256), 0, s, gm.probs, gm.segs,
+                     gm.atab, gm.btab, gm.crow, gm.cld, gm.tiles, gm.q, (const T*)A, (const T*)B, (T*)C, (T*)P);
 }
 
 // Compiled tile variants: (BM, BN) in {64x64, 128x64, 64x128, 128x128} with K
-// chunk multiplier KS in {1, 2} (FP64: KS = 1 only); FP32 also KS = 0, the
-// DMA-streamed kernel (K chunk 16) for 64x64, 128x64 and 64x128.
+// chunk multiplier KS in {1, 2} (FP64: KS = 1 only); FP32 and FP64 also KS = 0,
+// the DMA-streamed kernel (K chunk 16 floats / 8 doubles) for 64x64, 128x64 and
+// 64x128.
 template <int MODE>
 static hipError_t launch_mode(const DevGemm& gm, const void* A, const void* B, void* C, void* P, hipStream_t s) {
   const int key = gm.BM * 10000 + gm.BN * 10 + gm.KS;
@@ -793,6 +830,15 @@ static hipError_t launch_mode(const DevGemm& gm, const void* A, const void* B, v
       MPSH_V(64, 64, 1)
       MPSH_V(128, 64, 1)
       MPSH_V(64, 128, 1)
+      case 64 * 10000 + 64 * 10:
+        launch_variant_dma<0, 64, 64>(gm, A, B, C, P, s);
+        break;
+      case 128 * 10000 + 64 * 10:
+        launch_variant_dma<0, 128, 64>(gm, A, B, C, P, s);
+        break;
+      case 64 * 10000 + 128 * 10:
+        launch_variant_dma<0, 64, 128>(gm, A, B, C, P, s);
+        break;
       default:
         return hipErrorInvalidValue;
     }
@@ -808,15 +854,15 @@ static hipError_t launch_mode(const DevGemm& gm, const void* A, const void* B, v
       MPSH_V(128, 128, 2)
       case 64 * 10000 + 64 * 10:
         if (MODE != 1) return hipErrorInvalidValue;
-        launch_variant_dma<64, 64>(gm, A, B, C, P, s);
+        launch_variant_dma<1, 64, 64>(gm, A, B, C, P, s);
         break;
       case 128 * 10000 + 64 * 10:
         if (MODE != 1) return hipErrorInvalidValue;
-        launch_variant_dma<128, 64>(gm, A, B, C, P, s);
+        launch_variant_dma<1, 128, 64>(gm, A, B, C, P, s);
         break;
       case 64 * 10000 + 128 * 10:
         if (MODE != 1) return hipErrorInvalidValue;
-        launch_variant_dma<64, 128>(gm, A, B, C, P, s);
+        launch_variant_dma<1, 64, 128>(gm, A, B, C, P, s);
         break;
       default:
         return hipErrorInvalidValue;

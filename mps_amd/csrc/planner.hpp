@@ -285,23 +285,24 @@ struct TileCfg {
 struct HeffTiles {
   TileCfg s1{64, 128, 0}, s4{64, 64, 0};     // FP32 (DMA-streamed operands)
   TileCfg s1h{128, 64, 1}, s4h{64, 64, 2};   // BF16 operands
-  TileCfg s1d{64, 64, 1}, s4d{64, 64, 1};    // FP64
+  TileCfg s1d{64, 128, 0}, s4d{64, 64, 0};   // FP64 (DMA-streamed operands)
   int64_t split_target = 1024;               // minimum work items for the step-4 GEMM
 };
 
 // Autotuner candidates for the two H_eff GEMMs (tile BM x BN x KS; step-4
-// split-K work-item targets).  FP64 uses the KS = 1 tiles up to 128 wide only;
-// KS = 0 (DMA-streamed operands) is FP32 only.
+// split-K work-item targets).  FP64 uses the KS = 1 tiles up to 128 wide and
+// KS = 0; KS = 0 (DMA-streamed operands) is FP32 / FP64 only.
 constexpr TileCfg kTuneTiles[] = {{64, 64, 1}, {64, 128, 1}, {128, 64, 1}, {64, 64, 2}, {128, 128, 1},
                                   {128, 64, 2}, {64, 64, 0},  {128, 64, 0}, {64, 128, 0}};
 constexpr int64_t kTuneSplits[] = {512, 1024, 2048};
-// KS = 0: the FP32 kernel with DMA-streamed operands (K chunk 16), tiles up to 128 x 64.
+// KS = 0: the kernel with DMA-streamed operands (FP32 / FP64), tiles up to 128 x 64.
 inline bool tune_tile_ok(int mode, const TileCfg& c) {
-  if (c.KS == 0) return mode == 1 && c.BM * c.BN <= 128 * 64;
+  if (c.KS == 0) return mode != 2 && c.BM * c.BN <= 128 * 64;
   return mode != 0 || (c.KS == 1 && c.BM * c.BN <= 128 * 64);
 }
-// K chunk of a GEMM mode (elements): 16 * KS for FP64 / FP32, 32 * KS for BF16; 16 for KS = 0
-inline int chunk_k(int mode, int KS) { return KS == 0 ? 16 : (mode == 2 ? 32 : 16) * KS; }
+// K chunk of a GEMM mode (elements): 16 * KS for FP64 / FP32, 32 * KS for BF16;
+// KS = 0: 16 floats / 8 doubles
+inline int chunk_k(int mode, int KS) { return KS == 0 ? (mode == 0 ? 8 : 16) : (mode == 2 ? 32 : 16) * KS; }
 // the step-4 GEMM of a plan with another tile and split target
 GemmPlan gemm_variant(const GemmPlan& base, const TileCfg& c, int64_t split_target, int mode);
 

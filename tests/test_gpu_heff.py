@@ -174,3 +174,34 @@ def test_dma_gemm_variants_bitwise(name, tiles):
             os.environ["MPSH_HEFF_TILES"] = old
     assert torch.isfinite(outs[1]).all()
     assert torch.equal(outs[0], outs[1])
+
+
+@pytest.mark.parametrize("name", ["xxz_L32_chi64", "xxz_L100_chi512", "cyl6x24_chi2048"])
+@pytest.mark.parametrize("tiles", [(64, 64), (128, 64), (64, 128)])
+def test_dma_gemm_f64_variants(name, tiles):
+    """The FP64 GEMM with DMA-streamed operands (K chunk 8 doubles) against the
+    register-staged FP64 kernel (K chunk 16): the same products summed in another
+    order, so step 1 and step 4 (K segments, K tails, split-K) agree to FP64
+    rounding (1e-13 relative)."""
+    import os
+    from mps_amd.tensornetalgs import HeffPlan
+    ts = _workload(name)
+    bm, bn = tiles
+    outs = []
+    old = os.environ.get("MPSH_HEFF_TILES")
+    try:
+        for ks in (1, 0):
+            os.environ["MPSH_HEFF_TILES"] = f"s1d={bm}x{bn}x{ks},s4d={bm}x{bn}x{ks},split=1024"
+            plan = HeffPlan(*[t.struct for t in ts])
+            prepared = plan.prepare("f64", *(t.data for t in ts[1:]))
+            out = torch.full((plan.out.numel,), float("nan"), dtype=torch.float64, device=DEV)
+            plan.apply_prepared("f64", prepared, ts[0].data, out)
+            outs.append(out)
+    finally:
+        if old is None:
+            os.environ.pop("MPSH_HEFF_TILES", None)
+        else:
+            os.environ["MPSH_HEFF_TILES"] = old
+    assert torch.isfinite(outs[1]).all()
+    err = float(torch.linalg.vector_norm(outs[1] - outs[0]) / torch.linalg.vector_norm(outs[0]))
+    assert err < 1e-13, err
