@@ -127,15 +127,19 @@ int jac_width(int64_t r) { return r <= 512 ? 16 : (r <= 1024 ? 8 : 4); }
 // Preconditioning.  Cyclic one-sided Jacobi converges only linearly for a
 // long while on the graded spectra of DMRG two-site blocks (27-30 sweeps for
 // 900 x 800 blocks whose singular values span 1 .. 1e-9, against 13 for
-// Gaussian blocks).  Matrices with at least JAC_PRE_MIN columns are first
-// rotated by the eigenvectors V of their Gram matrix W^T W (rocSOLVER syevd,
-// FP64, batched over padded size buckets): the columns of W V are orthogonal
-// up to eps |W|^2 / (sigma_i sigma_j) relative, and the Jacobi sweeps that
-// follow (started from Q = V) restore full relative orthogonality in 5-7
-// sweeps.  V enters only as an orthogonal change of basis (syevd's vectors are
-// orthonormal to rounding), so the singular values and vectors keep the
-// accuracy of the Jacobi sweeps.  A matrix whose syevd reports failure keeps
-// Q = I and is swept unpreconditioned.
+// Gaussian blocks).  Large matrices are first rotated by the eigenvectors V of
+// their Gram matrix W^T W (rocSOLVER syevd, FP64, batched over padded size
+// buckets): the columns of W V are orthogonal up to eps |W|^2 / (sigma_i
+// sigma_j) relative, and the Jacobi sweeps that follow (started from Q = V)
+// restore full relative orthogonality in 5-7 sweeps.  V enters only as an
+// orthogonal change of basis (syevd's vectors are orthonormal to rounding), so
+// the singular values and vectors keep the accuracy of the Jacobi sweeps.  A
+// matrix whose syevd reports failure keeps Q = I and is swept
+// unpreconditioned.  Every matrix of at least JAC_PRE_MIN columns is
+// preconditioned: leaving the smaller ones plain (their ~30 sweeps run after
+// the large matrices are done, each sweep a host synchronisation and a launch
+// per round) measured slower than their syevd buckets, which run on the side
+// streams next to the largest one.
 constexpr int64_t JAC_PRE_MIN = 96;
 // syevd cost model (MI355X, rocSOLVER 3.30, ms): a bucket of `count` padded
 // P x P matrices costs ~ 0.022 P (the sequential Householder and merge steps)

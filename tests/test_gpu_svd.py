@@ -157,12 +157,13 @@ def test_jacobi_graded_spectrum_relative_accuracy():
     _check_facts(blocks, _blocks_with(A, "jacobi"))
 
 
-@pytest.mark.parametrize("scale", [1, 3])
+@pytest.mark.parametrize("scale", [1, 5])
 def test_jacobi_rank_deficient_and_zero_blocks(scale):
     """Exactly zero singular values: the completed singular vectors stay orthonormal.
-    scale 3: blocks of 120+ columns, which take the Gram-preconditioned path."""
-    A = _matrix([(0, 40 * scale), (1, 12), (2, 70 * scale)], [(0, 25 if scale == 1 else 110), (1, 12), (2, 90 * scale)],
-                seed=43)
+    scale 5: the zero and rank-deficient blocks have 200+ columns and take the
+    Gram-preconditioned path."""
+    A = _matrix([(0, 40 * scale), (1, 12), (2, 70 * scale)],
+                [(0, 25 if scale == 1 else 40 * scale + 10), (1, 12), (2, 90 * scale)], seed=43)
     blocks = A.blocks()
     keys = list(blocks.keys())
     blocks[keys[0]] = np.zeros_like(blocks[keys[0]])                  # all-zero block
@@ -202,7 +203,7 @@ def test_svd_nonfinite_input_raises(scale):
         _blocks_with(Z, "jacobi")
 
 
-@pytest.mark.parametrize("big", [40, 160])
+@pytest.mark.parametrize("big", [40, 220])
 def test_jacobi_extreme_block_scales(big):
     """Blocks scaled to 1e-120 .. 1e150 (DMRG two-site blocks far from the mean
     charge are ~1e-120): each block's factors match LAPACK relative to its own scale."""
