@@ -210,6 +210,12 @@ __device__ __forceinline__ double wave_row_sum(double v) {
 // lane holds RPL rows of a column, r <= GSZ RPL.  Rows r .. GSZ RPL of every
 // column are zero, so loads and stores run unmasked.  The 2w x 2w rotation R is kept
 // transposed (Rt[row * (2w + 1) + col]).
+// Squared column norms live in LDS (nrm): computed from the columns at the
+// start of every sweep of the work group (every round, for the round
+// launches), then carried through each rotation by |x_i'|^2 = |x_i|^2 - t g,
+// |x_j'|^2 = |x_j|^2 + t g (t the rotation's tangent, g = x_i.x_j), so a
+// rotation needs one dot product instead of three; a norm that falls below
+// 1/8 of its old value (cancellation) is recomputed from the column.
 // offmax[mat] is set to 1 when a pair of the sweep was rotated at a relative
 // product |x_i.x_j| / (|x_i| |x_j|) above JAC_SETTLED.  A sweep whose rotations
 // all stayed below that ends the matrix: a later rotation (i, k) by an angle
@@ -265,6 +271,7 @@ __global__ __launch_bounds__(WB * GSZ) void jac_pairs(const JacMat* __restrict__
   static_assert(NC <= GSZ, "one Rt row per lane of a group");
   static_assert(NC * LDX + NC * RTS <= JAC_LDS, "LDS budget");
   __shared__ __align__(16) double lds[JAC_LDS];
+  __shared__ double nrm[NC];
   __shared__ int rot_any, touched;
   const JacTask task = tasks[blockIdx.x];
   const JacMat M = mats[task.mat];
@@ -282,10 +289,27 @@ __global__ __launch_bounds__(WB * GSZ) void jac_pairs(const JacMat* __restrict__
   double* W = ws + M.w_off;
   double* Q = ws + M.q_off;
   const int tid = threadIdx.x;
-  for (int e = tid; e < NC * LDX; e += NT) {
-    const int lc = e / LDX, row = e - lc * LDX;
-    const int g = gcol(lc);
-    X[e] = (g >= 0 && row < r) ? W[(int64_t)g * r + row] : 0.0;
+  // staging loops keep JLB global loads in flight per thread (one work group
+  // per CU: a load-then-store loop would wait out every load's latency); loads
+  // are unpredicated from clamped addresses, out-of-range values zeroed after
+  constexpr int JLB = 8;
+  {
+    constexpr int NX = NC * LDX;
+    for (int e0 = tid; e0 < NX; e0 += NT * JLB) {
+      double v[JLB];
+#pragma unroll
+      for (int u = 0; u < JLB; ++u) {
+        const int e = e0 + u * NT;
+        const int lc = e / LDX, row = e - lc * LDX;
+        const int g = e < NX ? gcol(lc) : -1;
+        const bool in = g >= 0 && row < r;
+        const double x = W[in ? (int64_t)g * r + row : 0];
+        v[u] = in ? x : 0.0;
+      }
+#pragma unroll
+      for (int u = 0; u < JLB; ++u)
+        if (e0 + u * NT < NX) X[e0 + u * NT] = v[u];
+    }
   }
   for (int e = tid; e < NC * RTS; e += NT) Rt[e] = (e / RTS == e % RTS) ? 1.0 : 0.0;
   if (tid == 0) {
@@ -298,19 +322,24 @@ __global__ __launch_bounds__(WB * GSZ) void jac_pairs(const JacMat* __restrict__
   const int max_sweeps = iterate ? JAC_MAX_SWEEPS : 1;
   bool unsettled = false;
 
-  // rotate columns (i, j) held in registers (rows lane + 64 u); true when rotated
-  auto rotate = [&](double (&xi)[RPL], double (&xj)[RPL], int i, int j) {
-    double a0 = 0.0, a1 = 0.0, b0 = 0.0, b1 = 0.0, g0 = 0.0, g1 = 0.0;
+  auto sumsq = [&](const double (&x)[RPL]) -> double {
+    double s0 = 0.0, s1 = 0.0;
 #pragma unroll
     for (int u = 0; u < RPL; u += 2) {
-      a0 = fma(xi[u], xi[u], a0);
-      b0 = fma(xj[u], xj[u], b0);
+      s0 = fma(x[u], x[u], s0);
+      s1 = fma(x[u + 1], x[u + 1], s1);
+    }
+    return group_sum<GSZ>(s0 + s1);
+  };
+  // rotate columns (i, j) held in registers (rows lane + 64 u); true when rotated
+  auto rotate = [&](double (&xi)[RPL], double (&xj)[RPL], int i, int j) {
+    double g0 = 0.0, g1 = 0.0;
+#pragma unroll
+    for (int u = 0; u < RPL; u += 2) {
       g0 = fma(xi[u], xj[u], g0);
-      a1 = fma(xi[u + 1], xi[u + 1], a1);
-      b1 = fma(xj[u + 1], xj[u + 1], b1);
       g1 = fma(xi[u + 1], xj[u + 1], g1);
     }
-    const double a = group_sum<GSZ>(a0 + a1), b = group_sum<GSZ>(b0 + b1), g = group_sum<GSZ>(g0 + g1);
+    const double a = nrm[i], b = nrm[j], g = group_sum<GSZ>(g0 + g1);
     // rotate when |g| > ctol |x_i| |x_j|; columns of norm^2 <= 1e-280 (the
     // matrix is scaled to entries < 1) count as zero, and so does NaN
     // (the threshold test needs no full precision: raw hardware estimates)
@@ -340,6 +369,21 @@ __global__ __launch_bounds__(WB * GSZ) void jac_pairs(const JacMat* __restrict__
       Rt[lane * RTS + i] = cs * x - sn * y;
       Rt[lane * RTS + j] = sn * x + cs * y;
     }
+    // the norms after the rotation (uniform over the group, as a, b, g, tt are)
+    double na = rot ? fma(-tt, g, a) : a, nb = rot ? fma(tt, g, b) : b;
+    const bool ra = rot && na < 0.125 * a, rb = rot && nb < 0.125 * b;
+    if (__any(ra)) {
+      const double e = sumsq(xi);
+      if (ra) na = e;
+    }
+    if (__any(rb)) {
+      const double e = sumsq(xj);
+      if (rb) nb = e;
+    }
+    if (rot && lane == 0) {
+      nrm[i] = na;
+      nrm[j] = nb;
+    }
     return rot;
   };
   auto load = [&](double (&x)[RPL], int col) {
@@ -357,6 +401,14 @@ __global__ __launch_bounds__(WB * GSZ) void jac_pairs(const JacMat* __restrict__
   int conv = 0;
   for (; sweep < max_sweeps; ++sweep) {
     double xi[RPL], xj[RPL];
+    // exact squared norms of the 2w columns: group k takes columns k and k + WB
+#pragma unroll
+    for (int h = 0; h < 2; ++h) {
+      load(xi, grp + h * WB);
+      const double e = sumsq(xi);
+      if (lane == 0) nrm[grp + h * WB] = e;
+    }
+    __syncthreads();
     if (iterate || t == 0) {
       // (a) pairs inside p (groups 0 .. WB/2-1) and inside q (groups WB/2 ..)
       const int half = WB / 2, base = grp < half ? 0 : WB, k = grp < half ? grp : grp - half;
@@ -418,10 +470,24 @@ __global__ __launch_bounds__(WB * GSZ) void jac_pairs(const JacMat* __restrict__
   __syncthreads();
   // Q[:, cols] <- Q[:, cols] R, staged through the X region (c <= r rows);
   // a work item is one row times 8 columns
-  for (int e = tid; e < NC * c; e += NT) {
-    const int lc = e / c, row = e - lc * c;
-    const int g = gcol(lc);
-    X[lc * LDX + row] = g >= 0 ? Q[(int64_t)g * c + row] : 0.0;
+  {
+    const int NQ = NC * c;
+    for (int e0 = tid; e0 < NQ; e0 += NT * JLB) {
+      double v[JLB];
+      int dst[JLB];
+#pragma unroll
+      for (int u = 0; u < JLB; ++u) {
+        const int e = e0 + u * NT;
+        const int lc = e / c, row = e - lc * c;
+        const int g = e < NQ ? gcol(lc) : -1;
+        const double x = Q[g >= 0 ? (int64_t)g * c + row : 0];
+        v[u] = g >= 0 ? x : 0.0;
+        dst[u] = e < NQ ? lc * LDX + row : -1;
+      }
+#pragma unroll
+      for (int u = 0; u < JLB; ++u)
+        if (dst[u] >= 0) X[dst[u]] = v[u];
+    }
   }
   __syncthreads();
   for (int it = tid; it < c * (NC / 8); it += NT) {
