@@ -112,7 +112,7 @@ void need_aligned(const void* p, const char* what) {
 }
 
 // Optional override of the H_eff kernel variants, read at plan creation:
-// MPSH_HEFF_TILES="s1=64x128x1,s4=64x64x2,s1h=128x64x1,s4h=64x64x2,s1d=64x64x1,s4d=64x64x1,split=1024"
+// MPSH_HEFF_TILES="s1=64x128x1,s4=64x64x2,s1h=128x64x1,s4h=64x64x2,s1d=64x64x1,s4d=64x64x1,split=1024,splitd=2048"
 HeffTiles tiles_from_env() {
   HeffTiles t;
   const char* e = std::getenv("MPSH_HEFF_TILES");
@@ -128,6 +128,10 @@ HeffTiles tiles_from_env() {
     std::string k = item.substr(0, eq), v = item.substr(eq + 1);
     if (k == "split") {
       t.split_target = std::atoll(v.c_str());
+      continue;
+    }
+    if (k == "splitd") {
+      t.split_target_d = std::atoll(v.c_str());
       continue;
     }
     int bm = 0, bn = 0, ks = 1;
@@ -458,7 +462,7 @@ mpsh_status mpsh_heff_plan(mpsh_struct v, mpsh_struct envL, mpsh_struct envR, mp
     const TileCfg* c4[3] = {&t.s4d, &t.s4, &t.s4h};
     for (int m = 0; m < 3; ++m) {
       h->sel1[m] = VariantKey{1, m, c1[m]->BM, c1[m]->BN, c1[m]->KS, 0};
-      h->sel4[m] = VariantKey{4, m, c4[m]->BM, c4[m]->BN, c4[m]->KS, t.split_target};
+      h->sel4[m] = VariantKey{4, m, c4[m]->BM, c4[m]->BN, c4[m]->KS, m == 0 ? t.split_target_d : t.split_target};
     }
   } catch (...) {
     delete h;

@@ -435,8 +435,8 @@ __global__ __launch_bounds__(256) void gemm_grouped(
 
 // ---------------------------------------------------------------------------
 // FP32 / FP64 grouped GEMM with operands streamed to LDS by DMA
-// (global_load_lds dwordx4: no register staging), a 4-stage LDS ring (loads run
-// three K chunks ahead) and one barrier per chunk.  Same problem / segment /
+// (global_load_lds dwordx4: no register staging), an LDS ring of 4 (FP32) or 3
+// (FP64) stages (loads run 3 or 2 K chunks ahead) and one barrier per chunk.  Same problem / segment /
 // tile / split-K / output conventions as gemm_grouped (tile variant KS = 0).
 // In LDS a stage holds BM + BN operand rows of 64 bytes (a K chunk of 16 floats
 // or 8 doubles) in four 16-byte K groups; lane l of wave w fills the slot
@@ -499,7 +499,11 @@ struct DmaT<0> {  // FP64: 2 doubles per K group, MFMA 16x16x4 f64 (output colum
   static __device__ __forceinline__ int col(int g, int r) { return g + 4 * r; }
 };
 
-template <int MODE, int BM, int BN>
+// NS: LDS ring depth (loads run NS - 1 chunks ahead).  Measured on the H_eff
+// GEMMs (autotuned tiles): FP32 is faster with 4 stages (3547 vs 3491
+// matvecs/s), FP64 with 3 (2120 vs 2050: a smaller ring leaves LDS for one
+// more resident work group per CU).
+template <int MODE, int BM, int BN, int NS = (MODE == 0 ? 3 : 4)>
 __global__ __launch_bounds__(256) void gemm_dma(
     const GemmProb* __restrict__ probs, const GemmSeg* __restrict__ segs, const int64_t* __restrict__ atab,
     const int64_t* __restrict__ btab, const int64_t* __restrict__ crow, const int64_t* __restrict__ cld,
@@ -514,7 +518,8 @@ __global__ __launch_bounds__(256) void gemm_dma(
   constexpr int STG = (BM + BN) * BK;  // elements per stage
   // the four stages are distinct LDS variables so the compiler can tell a DMA
   // into one stage from the operand reads of another
-  __shared__ __attribute__((aligned(16))) T S0[STG], S1[STG], S2[STG], S3[STG];
+  static_assert(NS == 3 || NS == 4, "LDS ring of 3 or 4 stages");
+  __shared__ __attribute__((aligned(16))) T S0[STG], S1[STG], S2[STG], S3[NS == 4 ? STG : 1];
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   const int wm = wave >> 1, wn = wave & 1;
   const int g = lane >> 4, l16 = lane & 15;
@@ -682,24 +687,31 @@ __global__ __launch_bounds__(256) void gemm_dma(
     // (it+3)%4 is refilled now), issue chunk it+3, compute chunk it
     auto iter = [&](auto ST, int it) {
       constexpr int st = decltype(ST)::value;
-      if (nch - 1 - it >= 2)
-        wait_vmcnt<2 * (NA + NB)>();
-      else if (nch - 1 - it == 1)
-        wait_vmcnt<NA + NB>();
-      else
-        wait_vmcnt<0>();
+      if constexpr (NS == 4) {
+        if (nch - 1 - it >= 2)
+          wait_vmcnt<2 * (NA + NB)>();
+        else if (nch - 1 - it == 1)
+          wait_vmcnt<NA + NB>();
+        else
+          wait_vmcnt<0>();
+      } else {
+        if (nch - 1 - it >= 1)
+          wait_vmcnt<NA + NB>();
+        else
+          wait_vmcnt<0>();
+      }
       barrier_raw();
-      if (it + 3 < nch) issue(std::integral_constant<int, (st + 3) % 4>{});
+      if (it + NS - 1 < nch) issue(std::integral_constant<int, (st + NS - 1) % NS>{});
       compute(ST);
     };
     if (0 < nch) issue(I0{});
     if (1 < nch) issue(I1{});
-    if (2 < nch) issue(I2{});
-    for (int it = 0; it < nch; it += 4) {
+    if (NS == 4 && 2 < nch) issue(I2{});
+    for (int it = 0; it < nch; it += NS) {
       iter(I0{}, it);
       if (it + 1 < nch) iter(I1{}, it + 1);
       if (it + 2 < nch) iter(I2{}, it + 2);
-      if (it + 3 < nch) iter(I3{}, it + 3);
+      if (NS == 4 && it + 3 < nch) iter(I3{}, it + 3);
     }
     // epilogue as in gemm_grouped
     const bool split = p.nsplit > 1;

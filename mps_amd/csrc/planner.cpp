@@ -1181,7 +1181,7 @@ HeffPlan heff_plan(const Struct& v, const Struct& envL, const Struct& envR, cons
   // K chunk: 16 * KS for FP32 / FP64, 32 * KS for BF16 (chunk_k)
   H.g4.choose_splits(tiles.split_target, chunk_k(1, tiles.s4.KS), 32);
   H.g4h.choose_splits(tiles.split_target, chunk_k(2, tiles.s4h.KS), 32);
-  H.g4d.choose_splits(tiles.split_target, chunk_k(0, tiles.s4d.KS), 32);
+  H.g4d.choose_splits(tiles.split_target_d, chunk_k(0, tiles.s4d.KS), 32);
   H.pack_v.finalize();
   H.pack_envL.finalize();
   H.pack_envR.finalize();

@@ -286,7 +286,8 @@ struct HeffTiles {
   TileCfg s1{64, 128, 0}, s4{64, 64, 0};     // FP32 (DMA-streamed operands)
   TileCfg s1h{128, 64, 1}, s4h{64, 64, 2};   // BF16 operands
   TileCfg s1d{64, 128, 0}, s4d{64, 64, 0};   // FP64 (DMA-streamed operands)
-  int64_t split_target = 1024;               // minimum work items for the step-4 GEMM
+  int64_t split_target = 1024;               // minimum work items for the step-4 GEMM (FP32, BF16)
+  int64_t split_target_d = 2048;             // ... FP64 (not autotuned; measured best on the H_eff shapes)
 };
 
 // Autotuner candidates for the two H_eff GEMMs (tile BM x BN x KS; step-4
