@@ -257,6 +257,77 @@ __device__ __forceinline__ double group_sum(double v) {
   return v;
 }
 
+// squared norm of a column held as RPL rows per lane (one wave), in every lane
+template <int RPL>
+__device__ __forceinline__ double jac_sumsq(const double (&x)[RPL]) {
+  double s0 = 0.0, s1 = 0.0;
+#pragma unroll
+  for (int u = 0; u < RPL; u += 2) {
+    s0 = fma(x[u], x[u], s0);
+    s1 = fma(x[u + 1], x[u + 1], s1);
+  }
+  return wave_sum(s0 + s1);
+}
+
+// One rotation of columns (i, j) held in registers by one wave (rows lane +
+// 64 u); nrm: the columns' squared norms (LDS), Rt: the transposed accumulated
+// rotation (LDS, row stride RTS, NC rows).  True when rotated.
+template <int RPL, int NC, int RTS>
+__device__ __forceinline__ bool jac_rotate(double (&xi)[RPL], double (&xj)[RPL], int i, int j, double* nrm,
+                                           double* Rt, int lane, double ctol, bool& unsettled) {
+  double g0 = 0.0, g1 = 0.0;
+#pragma unroll
+  for (int u = 0; u < RPL; u += 2) {
+    g0 = fma(xi[u], xj[u], g0);
+    g1 = fma(xi[u + 1], xj[u + 1], g1);
+  }
+  const double a = nrm[i], b = nrm[j], g = wave_sum(g0 + g1);
+  // rotate when |g| > ctol |x_i| |x_j|; columns of norm^2 <= 1e-280 (the
+  // matrix is scaled to entries < 1) count as zero, and so does NaN
+  // (the threshold test needs no full precision: raw hardware estimates)
+  const double off = fabs(g) * __builtin_amdgcn_rsq(a) * __builtin_amdgcn_rsq(b);
+  const bool rot = a > 1e-280 && b > 1e-280 && off > ctol;
+  if (rot && off > JAC_SETTLED) unsettled = true;
+  if (!__any(rot)) return false;
+  // t = tan of the angle that zeroes the pair's product (Rutishauser):
+  // t = sign(z) / (|z| + sqrt(1 + z^2)), z = (b - a) / (2g); for |z| > 1e150
+  // sqrt(1 + z^2) is taken as |z| (z^2 would overflow), t = 1 / (2z).
+  const double z = (b - a) * rcp_nr(2.0 * (rot ? g : 1.0));
+  const double az = fabs(z);
+  const double w1 = fma(az, az, 1.0);
+  const double sq = az > 1e150 ? az : w1 * rsq_nr(w1);
+  const double tt = copysign(rcp_nr(az + sq), z);
+  const double c0 = rsq_nr(fma(tt, tt, 1.0));
+  const double cs = rot ? c0 : 1.0, sn = rot ? c0 * tt : 0.0;
+#pragma unroll
+  for (int u = 0; u < RPL; ++u) {
+    const double x = xi[u], y = xj[u];
+    xi[u] = cs * x - sn * y;
+    xj[u] = sn * x + cs * y;
+  }
+  if (rot && lane < NC) {
+    const double x = Rt[lane * RTS + i], y = Rt[lane * RTS + j];
+    Rt[lane * RTS + i] = cs * x - sn * y;
+    Rt[lane * RTS + j] = sn * x + cs * y;
+  }
+  // the norms after the rotation (uniform over the wave, as a, b, g, tt are)
+  double na = rot ? fma(-tt, g, a) : a, nb = rot ? fma(tt, g, b) : b;
+  const bool ra = rot && na < 0.125 * a, rb = rot && nb < 0.125 * b;
+  if (__any(ra)) {
+    const double e = jac_sumsq<RPL>(xi);
+    if (ra) na = e;
+  }
+  if (__any(rb)) {
+    const double e = jac_sumsq<RPL>(xj);
+    if (rb) nb = e;
+  }
+  if (rot && lane == 0) {
+    nrm[i] = na;
+    nrm[j] = nb;
+  }
+  return rot;
+}
+
 template <int WB, int GSZ>
 __global__ __launch_bounds__(WB * GSZ) void jac_pairs(const JacMat* __restrict__ mats,
                                                       const JacTask* __restrict__ tasks, int t, int iterate,
@@ -269,6 +340,7 @@ __global__ __launch_bounds__(WB * GSZ) void jac_pairs(const JacMat* __restrict__
   constexpr int NT = WB * GSZ, NC = 2 * WB, RPL = (WB == 16 ? 512 : (WB == 8 ? 1024 : 2048)) / GSZ,
                 LDX = GSZ * RPL + 1, RTS = NC + 1;
   static_assert(NC <= GSZ, "one Rt row per lane of a group");
+  static_assert(GSZ == 64, "one wave per rotation");
   static_assert(NC * LDX + NC * RTS <= JAC_LDS, "LDS budget");
   __shared__ __align__(16) double lds[JAC_LDS];
   __shared__ double nrm[NC];
@@ -289,27 +361,25 @@ __global__ __launch_bounds__(WB * GSZ) void jac_pairs(const JacMat* __restrict__
   double* W = ws + M.w_off;
   double* Q = ws + M.q_off;
   const int tid = threadIdx.x;
-  // staging loops keep JLB global loads in flight per thread (one work group
-  // per CU: a load-then-store loop would wait out every load's latency); loads
-  // are unpredicated from clamped addresses, out-of-range values zeroed after
-  constexpr int JLB = 8;
-  {
-    constexpr int NX = NC * LDX;
-    for (int e0 = tid; e0 < NX; e0 += NT * JLB) {
-      double v[JLB];
+  // staging, write-back and Q loops run column by column: wave k takes local
+  // columns k and k + WB, lane l rows l + 64 u (coalesced, no per-element
+  // index division); loads are unpredicated from clamped addresses with
+  // out-of-range values zeroed after, all of a column in flight at once
+  const int grp = tid / GSZ, lane = tid % GSZ;
 #pragma unroll
-      for (int u = 0; u < JLB; ++u) {
-        const int e = e0 + u * NT;
-        const int lc = e / LDX, row = e - lc * LDX;
-        const int g = e < NX ? gcol(lc) : -1;
-        const bool in = g >= 0 && row < r;
-        const double x = W[in ? (int64_t)g * r + row : 0];
-        v[u] = in ? x : 0.0;
-      }
+  for (int h = 0; h < 2; ++h) {
+    const int lc = grp + h * WB, g = gcol(lc);
+    const double* src = W + (int64_t)(g >= 0 ? g : 0) * r;
+    double v[RPL];
 #pragma unroll
-      for (int u = 0; u < JLB; ++u)
-        if (e0 + u * NT < NX) X[e0 + u * NT] = v[u];
+    for (int u = 0; u < RPL; ++u) {
+      const int row = lane + GSZ * u;
+      const bool in = g >= 0 && row < r;
+      const double x = src[in ? row : 0];
+      v[u] = in ? x : 0.0;
     }
+#pragma unroll
+    for (int u = 0; u < RPL; ++u) X[lc * LDX + lane + GSZ * u] = v[u];
   }
   for (int e = tid; e < NC * RTS; e += NT) Rt[e] = (e / RTS == e % RTS) ? 1.0 : 0.0;
   if (tid == 0) {
@@ -317,74 +387,14 @@ __global__ __launch_bounds__(WB * GSZ) void jac_pairs(const JacMat* __restrict__
     touched = 0;
   }
   __syncthreads();
-  const int grp = tid / GSZ, lane = tid % GSZ;
   const double ctol = M.ctol;
   const int max_sweeps = iterate ? JAC_MAX_SWEEPS : 1;
   bool unsettled = false;
 
-  auto sumsq = [&](const double (&x)[RPL]) -> double {
-    double s0 = 0.0, s1 = 0.0;
-#pragma unroll
-    for (int u = 0; u < RPL; u += 2) {
-      s0 = fma(x[u], x[u], s0);
-      s1 = fma(x[u + 1], x[u + 1], s1);
-    }
-    return group_sum<GSZ>(s0 + s1);
-  };
+  auto sumsq = [&](const double (&x)[RPL]) -> double { return jac_sumsq<RPL>(x); };
   // rotate columns (i, j) held in registers (rows lane + 64 u); true when rotated
   auto rotate = [&](double (&xi)[RPL], double (&xj)[RPL], int i, int j) {
-    double g0 = 0.0, g1 = 0.0;
-#pragma unroll
-    for (int u = 0; u < RPL; u += 2) {
-      g0 = fma(xi[u], xj[u], g0);
-      g1 = fma(xi[u + 1], xj[u + 1], g1);
-    }
-    const double a = nrm[i], b = nrm[j], g = group_sum<GSZ>(g0 + g1);
-    // rotate when |g| > ctol |x_i| |x_j|; columns of norm^2 <= 1e-280 (the
-    // matrix is scaled to entries < 1) count as zero, and so does NaN
-    // (the threshold test needs no full precision: raw hardware estimates)
-    const double off = fabs(g) * __builtin_amdgcn_rsq(a) * __builtin_amdgcn_rsq(b);
-    const bool rot = a > 1e-280 && b > 1e-280 && off > ctol;
-    if (rot && off > JAC_SETTLED) unsettled = true;
-    if (!__any(rot)) return false;
-    // t = tan of the angle that zeroes the pair's product (Rutishauser):
-    // t = sign(z) / (|z| + sqrt(1 + z^2)), z = (b - a) / (2g); for |z| > 1e150
-    // sqrt(1 + z^2) is taken as |z| (z^2 would overflow), t = 1 / (2z).  A
-    // group that does not rotate in a wave that does applies the identity.
-    const double z = (b - a) * rcp_nr(2.0 * (rot ? g : 1.0));
-    const double az = fabs(z);
-    const double w1 = fma(az, az, 1.0);
-    const double sq = az > 1e150 ? az : w1 * rsq_nr(w1);
-    const double tt = copysign(rcp_nr(az + sq), z);
-    const double c0 = rsq_nr(fma(tt, tt, 1.0));
-    const double cs = rot ? c0 : 1.0, sn = rot ? c0 * tt : 0.0;
-#pragma unroll
-    for (int u = 0; u < RPL; ++u) {
-      const double x = xi[u], y = xj[u];
-      xi[u] = cs * x - sn * y;
-      xj[u] = sn * x + cs * y;
-    }
-    if (rot && lane < NC) {
-      const double x = Rt[lane * RTS + i], y = Rt[lane * RTS + j];
-      Rt[lane * RTS + i] = cs * x - sn * y;
-      Rt[lane * RTS + j] = sn * x + cs * y;
-    }
-    // the norms after the rotation (uniform over the group, as a, b, g, tt are)
-    double na = rot ? fma(-tt, g, a) : a, nb = rot ? fma(tt, g, b) : b;
-    const bool ra = rot && na < 0.125 * a, rb = rot && nb < 0.125 * b;
-    if (__any(ra)) {
-      const double e = sumsq(xi);
-      if (ra) na = e;
-    }
-    if (__any(rb)) {
-      const double e = sumsq(xj);
-      if (rb) nb = e;
-    }
-    if (rot && lane == 0) {
-      nrm[i] = na;
-      nrm[j] = nb;
-    }
-    return rot;
+    return jac_rotate<RPL, NC, RTS>(xi, xj, i, j, nrm, Rt, lane, ctol, unsettled);
   };
   auto load = [&](double (&x)[RPL], int col) {
     const double* src = X + col * LDX + lane;
@@ -462,50 +472,53 @@ __global__ __launch_bounds__(WB * GSZ) void jac_pairs(const JacMat* __restrict__
   // loop orders every write of `touched` before this read)
   if (!touched) return;
   // columns back to W
-  for (int e = tid; e < NC * r; e += NT) {
-    const int lc = e / r, row = e - lc * r;
-    const int g = gcol(lc);
-    if (g >= 0) W[(int64_t)g * r + row] = X[lc * LDX + row];
+#pragma unroll
+  for (int h = 0; h < 2; ++h) {
+    const int lc = grp + h * WB, g = gcol(lc);
+    if (g >= 0) {
+      double* dst = W + (int64_t)g * r;
+#pragma unroll
+      for (int u = 0; u < RPL; ++u) {
+        const int row = lane + GSZ * u;
+        if (row < r) dst[row] = X[lc * LDX + row];
+      }
+    }
   }
   __syncthreads();
   // Q[:, cols] <- Q[:, cols] R, staged through the X region (c <= r rows);
   // a work item is one row times 8 columns
-  {
-    const int NQ = NC * c;
-    for (int e0 = tid; e0 < NQ; e0 += NT * JLB) {
-      double v[JLB];
-      int dst[JLB];
 #pragma unroll
-      for (int u = 0; u < JLB; ++u) {
-        const int e = e0 + u * NT;
-        const int lc = e / c, row = e - lc * c;
-        const int g = e < NQ ? gcol(lc) : -1;
-        const double x = Q[g >= 0 ? (int64_t)g * c + row : 0];
-        v[u] = g >= 0 ? x : 0.0;
-        dst[u] = e < NQ ? lc * LDX + row : -1;
-      }
+  for (int h = 0; h < 2; ++h) {
+    const int lc = grp + h * WB, g = gcol(lc);
+    const double* src = Q + (int64_t)(g >= 0 ? g : 0) * c;
+    double v[RPL];
 #pragma unroll
-      for (int u = 0; u < JLB; ++u)
-        if (dst[u] >= 0) X[dst[u]] = v[u];
+    for (int u = 0; u < RPL; ++u) {
+      const int row = lane + GSZ * u;
+      const bool in = g >= 0 && row < c;
+      const double x = src[in ? row : 0];
+      v[u] = in ? x : 0.0;
     }
+#pragma unroll
+    for (int u = 0; u < RPL; ++u) X[lc * LDX + lane + GSZ * u] = v[u];
   }
   __syncthreads();
-  for (int it = tid; it < c * (NC / 8); it += NT) {
-    const int cg = it / c, row = it - cg * c;
-    double acc[8] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
+  for (int cg = 0; cg < NC / 8; ++cg)
+    for (int row = tid; row < c; row += NT) {
+      double acc[8] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
 #pragma unroll 2
-    for (int li = 0; li < NC; ++li) {
-      const double x = X[li * LDX + row];
-      const double* rr = Rt + li * RTS + cg * 8;
+      for (int li = 0; li < NC; ++li) {
+        const double x = X[li * LDX + row];
+        const double* rr = Rt + li * RTS + cg * 8;
 #pragma unroll
-      for (int k = 0; k < 8; ++k) acc[k] = fma(x, rr[k], acc[k]);
-    }
+        for (int k = 0; k < 8; ++k) acc[k] = fma(x, rr[k], acc[k]);
+      }
 #pragma unroll
-    for (int k = 0; k < 8; ++k) {
-      const int g = gcol(cg * 8 + k);
-      if (g >= 0) Q[(int64_t)g * c + row] = acc[k];
+      for (int k = 0; k < 8; ++k) {
+        const int g = gcol(cg * 8 + k);
+        if (g >= 0) Q[(int64_t)g * c + row] = acc[k];
+      }
     }
-  }
 }
 
 // Scale every work matrix by a power of two so its largest |entry| lies in
@@ -1093,6 +1106,7 @@ mpsh_status mpsh_svd_plan(int64_t nblocks, const int64_t* m, const int64_t* n, c
       J.r = std::max(m[b], n[b]);
       J.c = std::min(m[b], n[b]);
       J.w = jac_width(J.r);
+      const int cls = J.w == 16 ? 0 : (J.w == 8 ? 1 : 2);
       J.nb = (int32_t)((J.c + J.w - 1) / J.w);
       J.nb += J.nb & 1;
       if (J.nb < 2) J.nb = 2;
@@ -1119,7 +1133,6 @@ mpsh_status mpsh_svd_plan(int64_t nblocks, const int64_t* m, const int64_t* n, c
       jw = (jw + 1) / 2 * 2;
       P->jcmax = std::max(P->jcmax, J.c);
       const int32_t idx = (int32_t)P->jm.size();
-      const int cls = J.w == 16 ? 0 : (J.w == 8 ? 1 : 2);
       if (J.nb == 2) {
         small_by[cls].push_back({idx, 0});
       } else {
