@@ -249,14 +249,6 @@ __device__ __forceinline__ double rsq_nr(double x) {
   return fma(y, fma(-h * y, y, 0.5), y);
 }
 
-template <int GSZ>
-__device__ __forceinline__ double group_sum(double v) {
-  if (GSZ == 64) return wave_sum(v);
-  v = wave_row_sum(v);                           // rows of 16, uniform per row
-  if (GSZ >= 32) v += __shfl_xor(v, 16, 64);     // commutative: uniform per group
-  return v;
-}
-
 // squared norm of a column held as RPL rows per lane (one wave), in every lane
 template <int RPL>
 __device__ __forceinline__ double jac_sumsq(const double (&x)[RPL]) {
@@ -334,9 +326,9 @@ __global__ __launch_bounds__(WB * GSZ) void jac_pairs(const JacMat* __restrict__
                                                       double* __restrict__ ws,
                                                       unsigned long long* __restrict__ offmax,
                                                       int* __restrict__ done, int* __restrict__ flags) {
-  // GSZ lanes per rotation (64: one wave; 32: two rotations share a wave and
-  // its scalar instructions -- measured no faster, the launches use 64); a
-  // lane holds RPL rows of a column
+  // one wave (GSZ = 64 lanes) per rotation -- two or four rotations per wave
+  // sharing its scalar instructions measured no faster; a lane holds RPL rows
+  // of a column
   constexpr int NT = WB * GSZ, NC = 2 * WB, RPL = (WB == 16 ? 512 : (WB == 8 ? 1024 : 2048)) / GSZ,
                 LDX = GSZ * RPL + 1, RTS = NC + 1;
   static_assert(NC <= GSZ, "one Rt row per lane of a group");
