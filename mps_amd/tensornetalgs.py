@@ -257,7 +257,12 @@ class EigInfo:
 
 
 LANCZOS_GRAPHS = False  # capture each Lanczos iteration in a hipGraph (eigsolve default)
-AUTOTUNE = True  # HeffOperator tunes the FP32/BF16 GEMM variants of each new plan (signature-cached)
+# HeffOperator tunes the FP32/BF16 GEMM variants of each new plan (signature-cached)
+# when set.  Off by default: in a chi=2048 sweep each of ~30 new signatures costs
+# ~34 ms of timing runs while the default variants are close to the tuned ones
+# (measured sweep 17.8 s tuned, 16.8 s untuned); runs of many sweeps, which reuse
+# the cached signatures, may turn it on.
+AUTOTUNE = False
 AUTOTUNE_MIN_FLOPS = 2e8  # ... when one matvec costs at least this many reference FLOPs
 STAGE_RTOL = {"f32": 1e-6}  # relative residual at which a refined eigsolve leaves its reduced-precision stage
 TUNE_COUNTS = {"timed": 0, "cached": 0}  # autotuner calls that timed variants / reused a signature
