@@ -28,7 +28,7 @@ HBM_PEAK = 8.0e12
 METRIC = "H_eff matvecs/sec + DMRG sweep time at χ=2048; |ΔE| vs FP64 CPU oracle"
 # HBM bytes per step-1 GEMM launch: rocprofv3 FETCH_SIZE x 2.000 + WRITE_SIZE x 1.000
 # (scales from a 256 MiB device copy), profiles/heff_cyl6x24_chi2048_f32_hbm_traffic.txt
-TRAFFIC = {("cyl6x24_chi2048", "f32"): 287.6e6}  # autotuned step-1 variant gemm_dma<64, 128>
+TRAFFIC = {("cyl6x24_chi2048", "f32"): 288.4e6}  # autotuned step-1 variant gemm_dma<1, 64, 128, 4>
 
 
 def parse():
@@ -124,10 +124,10 @@ def time_fused(plan, prec, ts, steps, warmup, stage_events=True, tune=True):
 
 def step1_kernel(prec, tuned):
     """Name of the step-1 GEMM kernel variant: the autotuned one, else the one
-    MPSH_HEFF_TILES or the defaults select (FP32 64x128 DMA-streamed, BF16 128x64,
-    FP64 64x64)."""
+    MPSH_HEFF_TILES or the defaults select (FP32 and FP64 64x128 DMA-streamed,
+    BF16 128x64); DMA kernels carry their LDS ring depth (FP32 4, FP64 3)."""
     key = {"f32": "s1", "bf16": "s1h", "f64": "s1d"}[prec]
-    cfg = {"f32": "64x128x0", "bf16": "128x64x1", "f64": "64x64x1"}[prec]
+    cfg = {"f32": "64x128x0", "bf16": "128x64x1", "f64": "64x128x0"}[prec]
     if tuned:
         cfg = tuned["step1"]
     else:
@@ -137,7 +137,8 @@ def step1_kernel(prec, tuned):
     parts = (cfg.split("x") + ["1"])[:3]
     bm, bn, ks = parts
     mode = {"f64": 0, "f32": 1, "bf16": 2}[prec]
-    return f"gemm_dma<{bm}, {bn}>" if ks == "0" else f"gemm_grouped<{mode}, {bm}, {bn}, {ks}>"
+    return (f"gemm_dma<{mode}, {bm}, {bn}, {3 if mode == 0 else 4}>" if ks == "0"
+            else f"gemm_grouped<{mode}, {bm}, {bn}, {ks}>")
 
 
 def time_plain(ts, steps, warmup):
