@@ -390,19 +390,24 @@ class SVDResult:
         return contract(self.u(), (1, -1), self.sv(), (-1, 2))
 
 
+def _pack_s(facts, dt):
+    """The blocks' singular values as one device vector of dtype dt (S_pack),
+    its block offsets, and the FP64 values on the host (one copy)."""
+    if not facts:
+        return None, [0], []
+    S_cat = torch.cat([f[2] for f in facts])
+    offs = [0] + np.cumsum([len(f[2]) for f in facts]).tolist()
+    svals = np.split(S_cat.cpu().numpy(), offs[1:-1])
+    return (S_cat if S_cat.dtype == dt else S_cat.to(dt)), offs, svals
+
+
 def svdtrunc(M: SymTensor, maxdim: int = None, tol: float = 1e-12) -> SVDResult:
     """``svdtrunc(A; maxdim, tol)`` (mpsutils.jl:116-179) on a rank-2 SymTensor."""
     if maxdim is None:
         maxdim = max(V.dim() for V in M.space)
     facts = _svd_blocks(M)
     dt = M.dtype
-    if facts:
-        S_cat = torch.cat([f[2] for f in facts])
-        offs = np.concatenate([[0], np.cumsum([len(f[2]) for f in facts])])
-        svals = np.split(S_cat.cpu().numpy(), offs[1:-1])
-        S_pack = S_cat if S_cat.dtype == dt else S_cat.to(dt)
-    else:
-        svals, offs, S_pack = [], [0], None
+    S_pack, offs, svals = _pack_s(facts, dt)
     ns = truncatedsizes(svals, maxdim=maxdim, tol=tol) if facts else []
     keys, Us, Ss, Vs, S_host, midl, midr = [], [], [], [], [], [], []
     for b, ((key, U, S, Vh), k) in enumerate(zip(facts, ns)):
@@ -410,7 +415,7 @@ def svdtrunc(M: SymTensor, maxdim: int = None, tol: float = 1e-12) -> SVDResult:
             continue
         keys.append(key)
         Us.append(U[:, :k].to(dt))
-        Ss.append(S_pack[int(offs[b]):int(offs[b]) + k])
+        Ss.append(S_pack[offs[b]:offs[b] + k])
         S_host.append(svals[b][:k].copy())
         Vs.append(Vh[:k, :].to(dt))
         midl.append((key[0], k))
@@ -425,16 +430,11 @@ def svd_(M: SymTensor) -> SVDResult:
     keys, Us, Ss, Vs, S_host, midl, midr = [], [], [], [], [], [], []
     dt = M.dtype
     facts = _svd_blocks(M)
-    S_pack = None
-    if facts:
-        S_cat = torch.cat([f[2] for f in facts])
-        offs = np.concatenate([[0], np.cumsum([len(f[2]) for f in facts])])
-        svals = np.split(S_cat.cpu().numpy(), offs[1:-1])
-        S_pack = S_cat if S_cat.dtype == dt else S_cat.to(dt)
+    S_pack, offs, svals = _pack_s(facts, dt)
     for b, (key, U, S, Vh) in enumerate(facts):
         keys.append(key)
         Us.append(U.to(dt))
-        Ss.append(S_pack[int(offs[b]):int(offs[b + 1])])
+        Ss.append(S_pack[offs[b]:offs[b + 1]])
         S_host.append(svals[b].copy())
         Vs.append(Vh.to(dt))
         midl.append((key[0], S.numel()))
