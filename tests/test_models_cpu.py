@@ -125,3 +125,24 @@ def test_tv_free_fermion_closed_form():
     e = np.linalg.eigvalsh(H[np.ix_(idx, idx)])[0]
     levels = np.sort(-2 * t * np.cos(np.arange(1, L + 1) * np.pi / (L + 1)))
     assert abs(e - levels[:N].sum()) < 1e-12
+
+
+def test_pack_s_offsets_and_host_copy():
+    """svdtrunc / svd_ keep the blocks' singular values as views of one packed
+    device vector (normalised in place) plus one host copy: offsets, views and
+    the host values agree, and an empty factorisation packs to nothing."""
+    import torch
+    from mps_amd import mpstools as mt
+    rng = np.random.default_rng(3)
+    sv = [np.sort(rng.random(k))[::-1].copy() for k in (3, 1, 5)]
+    facts = [((q, q), None, torch.tensor(s), None) for q, s in enumerate(sv)]
+    for dt in (torch.float64, torch.float32):
+        S_pack, offs, svals = mt._pack_s(facts, dt)
+        assert S_pack.dtype == dt and offs == [0, 3, 4, 9]
+        for b, s in enumerate(sv):
+            np.testing.assert_array_equal(svals[b], s)
+            view = S_pack[offs[b]:offs[b + 1]]
+            np.testing.assert_allclose(view.double().numpy(), s, rtol=1e-7)
+        S_pack.div_(2.0)  # the per-block views follow an in-place normalisation
+        np.testing.assert_allclose(S_pack[offs[2]:offs[3]].double().numpy(), sv[2] / 2, rtol=1e-7)
+    assert mt._pack_s([], torch.float64) == (None, [0], [])

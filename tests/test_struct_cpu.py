@@ -196,3 +196,4 @@ def test_heff_plan_structure_and_flops_cfg1():
             step_flops(T3, (1, 2, 3, -1, -2), O["R"], (-2, -1, 4))]
     assert hp.step_flops == want
     assert hp.ref_flops == sum(want)
+
