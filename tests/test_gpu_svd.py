@@ -239,3 +239,22 @@ def test_preconditioned_and_plain_jacobi_agree():
     _check_facts(blocks, f0)
     for (_, _, Sp, _), (_, _, S0, _) in zip(fp, f0):
         np.testing.assert_allclose(Sp.cpu().numpy(), S0.cpu().numpy(), rtol=1e-12, atol=1e-12 * float(S0[0]))
+
+
+@pytest.mark.parametrize("rows,cols", [((924, 300), (792, 280)), ((1500, 120), (300, 110))])
+def test_jacobi_dmrg_sized_graded_blocks(rows, cols):
+    """Block sizes of a chi=2048 DMRG sweep (924 x 792: w = 8 columns per block,
+    rows 1500: w = 4) with spectra graded from 1 to 1e-9 like the sweep's
+    two-site matrices: the Gram-preconditioned Jacobi path matches LAPACK to
+    1e-12 of the largest singular value, reconstructs each block and returns
+    orthonormal factors."""
+    A = _matrix([(0, rows[0]), (1, rows[1])], [(0, cols[0]), (1, cols[1])], seed=67)
+    blocks = A.blocks()
+    rng = np.random.default_rng(11)
+    for k, b in blocks.items():
+        m, n = b.shape
+        u, _ = np.linalg.qr(rng.standard_normal((m, min(m, n))))
+        v, _ = np.linalg.qr(rng.standard_normal((n, min(m, n))))
+        blocks[k] = (u * np.logspace(0, -9, min(m, n))) @ v.T
+    Z = M.SymTensor.from_blocks(A.charge, A.space, blocks, dtype=torch.float64, device=DEV)
+    _check_facts(blocks, _blocks_with(Z, "jacobi"))
