@@ -146,3 +146,43 @@ def test_pack_s_offsets_and_host_copy():
         S_pack.div_(2.0)  # the per-block views follow an in-place normalisation
         np.testing.assert_allclose(S_pack[offs[2]:offs[3]].double().numpy(), sv[2] / 2, rtol=1e-7)
     assert mt._pack_s([], torch.float64) == (None, [0], [])
+
+
+def _svd_ws(m, n, env):
+    """Workspace bytes of a host-side mpsh_svd_plan for blocks m[i] x n[i] under env."""
+    import os
+    from mps_amd import _lib as L
+    from mps_amd._lib import check, lib
+    old = {k: os.environ.get(k) for k in env}
+    os.environ.update(env)
+    try:
+        offs = [0]
+        for a, b in zip(m, n):
+            offs.append(offs[-1] + a * b)
+        h = L._p()
+        check(lib().mpsh_svd_plan(len(m), L.i64arr(m), L.i64arr(n), L.i64arr(offs[:-1]), h))
+        ws = int(lib().mpsh_svd_workspace(h))
+        lib().mpsh_svd_destroy(h)
+        return ws
+    finally:
+        for k, v in old.items():
+            if v is None:
+                del os.environ[k]
+            else:
+                os.environ[k] = v
+
+
+def test_svd_plan_reserves_gram_preconditioner_workspace():
+    """The Jacobi plan (host side, no device) reserves for every block of at
+    least 96 columns a copy of the scaled block and a padded Gram / eigenvector
+    slot with its eigenvalue and syevd scratch vectors; the 900 x 800 and
+    500 x 400 blocks share one 800 x 800 size bucket (the bucket cost model
+    prefers one syevd call), the 60 x 50 block gets nothing, and
+    MPSH_SVD_PRECOND=0 drops the whole reservation."""
+    m, n = [900, 500, 60], [800, 400, 50]
+    pre = _svd_ws(m, n, {"MPSH_SVD_ALGO": "jacobi", "MPSH_SVD_PRECOND": "1"})
+    plain = _svd_ws(m, n, {"MPSH_SVD_ALGO": "jacobi", "MPSH_SVD_PRECOND": "0"})
+    want = 8 * (900 * 800 + 500 * 400 + 2 * 800 * 800 + 2 * 800 + 2 * 800) + 4 * 2
+    assert want <= pre - plain <= want + 128
+    small = _svd_ws([60], [50], {"MPSH_SVD_ALGO": "jacobi", "MPSH_SVD_PRECOND": "1"})
+    assert small == _svd_ws([60], [50], {"MPSH_SVD_ALGO": "jacobi", "MPSH_SVD_PRECOND": "0"})
