@@ -205,3 +205,44 @@ def test_dmrg_tv_chain_reaches_ed(prec):
         es = ta.dmrg2sitesweep(mps, mpo, env, **kw)
     print(f"t-V {prec}: E = {es[-1]:.12f} (ED {e_ed:.12f})", flush=True)
     assert abs(es[-1] - e_ed) < 1e-9 * abs(e_ed)
+
+
+def test_dmrg_sweep_with_preconditioned_svd():
+    """XXZ chain L=20, N=10 from a seeded random MPS at chi=256: the two-site
+    blocks reach 100+ columns, so the truncations run the Gram-preconditioned
+    Jacobi SVD.  One sweep with it and one with plain Jacobi sweeps
+    (MPSH_SVD_PRECOND=0, fresh SVD plans) from the same start give the same
+    energies to 1e-10 relative (maxdim above every bond dimension, so only the
+    absolute tol truncates and no cut can fall inside a degenerate multiplet),
+    and each sweep's last energy equals <psi|H|psi> of its MPS to 1e-12."""
+    import os
+    L, N = 20, 10
+    runs, widest = [], [0]
+    old = os.environ.get("MPSH_SVD_PRECOND")
+    svd_blocks = mt._svd_blocks
+
+    def recording(M):  # the widest block (columns of W = min(m, n)) any truncation factorises
+        facts = svd_blocks(M)
+        widest[0] = max([widest[0]] + [min(U.shape[0], Vh.shape[1]) for _, U, _, Vh in facts])
+        return facts
+    mt._svd_blocks = recording
+    try:
+        for flag in ("1", "0"):
+            os.environ["MPSH_SVD_PRECOND"] = flag
+            mt._SVD_PLANS.clear()
+            mps = mt.random_mps(L, N, 256, seed=29, device=DEV)
+            mpo = mt.xxz_u1mpo(L, 1.0, device=DEV)
+            env = ta.initialenv(mps, mpo)
+            es = ta.dmrg2sitesweep(mps, mpo, env, maxdim=4096, tol=1e-13, lanczostol=1e-10, krylovdim=8,
+                                   krylovmaxiter=4)
+            assert abs(es[-1] - mt.measure(mps, mpo)) < 1e-12 * abs(es[-1])
+            runs.append(np.asarray(es))
+    finally:
+        mt._svd_blocks = svd_blocks
+        mt._SVD_PLANS.clear()
+        if old is None:
+            os.environ.pop("MPSH_SVD_PRECOND", None)
+        else:
+            os.environ["MPSH_SVD_PRECOND"] = old
+    assert widest[0] >= 96, widest[0]  # the preconditioned class was exercised
+    np.testing.assert_allclose(runs[0], runs[1], rtol=1e-10, atol=0)
