@@ -70,7 +70,7 @@ struct DevTables {
   DevCopy copy(const CopyPlan& c) {
     DevCopy d;
     d.descs = up(c.descs);
-    d.tile_start = up(c.tile_start);
+    d.tile_map = up(c.tile_map);
     d.ndesc = (int)c.descs.size();
     d.ntiles = c.ntiles;
     return d;

@@ -26,21 +26,15 @@ typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
 // ---------------------------------------------------------------------------
 // copy
 // ---------------------------------------------------------------------------
+// One work group per tile; the plan's tile map names the tile's descriptor and
+// its index within it, so a work group starts with one table load.
 template <typename T>
 __global__ __launch_bounds__(256) void copy_kernel(const CopyDesc* __restrict__ descs,
-                                                   const int64_t* __restrict__ ts, int ndesc,
+                                                   const int32_t* __restrict__ tmap,
                                                    const T* __restrict__ src, T* __restrict__ dst) {
-  const int64_t bid = blockIdx.x;
-  int lo = 0, hi = ndesc - 1;
-  while (lo < hi) {
-    const int mid = (lo + hi + 1) >> 1;
-    if (ts[mid] <= bid)
-      lo = mid;
-    else
-      hi = mid - 1;
-  }
-  const CopyDesc* d = descs + lo;
-  int64_t t = bid - ts[lo];
+  const int2 tm = ((const int2*)tmap)[blockIdx.x];
+  const CopyDesc* d = descs + tm.x;
+  int64_t t = tm.y;
   const int ndim = d->ndim;
   const int64_t d0 = d->dims[0], d1 = ndim > 1 ? d->dims[1] : 1;
   const int64_t s0 = d->src_st[0], s1 = ndim > 1 ? d->src_st[1] : 0;
@@ -80,14 +74,25 @@ __global__ __launch_bounds__(256) void copy_kernel(const CopyDesc* __restrict__ 
       sb += i * d->src_st[k];
       db += i * d->dst_st[k];
     }
+    // all sixteen loads of a thread are issued before its first store, so a
+    // tile is one memory round trip rather than sixteen
     const uint32_t ud0 = (uint32_t)d0;
-    for (int i = threadIdx.x; i < 4096; i += 256) {
-      const int64_t e = tt * 4096 + i;
-      if (e >= n01) break;
-      const uint32_t ue = (uint32_t)e;
-      const uint32_t y = ue / ud0, x = ue - y * ud0;
-      dst[db + x * w0 + y * w1] = src[sb + x * s0 + y * s1];
+    const int64_t e0 = tt * 4096 + threadIdx.x;
+    T val[16];
+    int64_t to[16];
+#pragma unroll
+    for (int u = 0; u < 16; ++u) {
+      const int64_t e = e0 + 256 * u;
+      if (e < n01) {
+        const uint32_t ue = (uint32_t)e;
+        const uint32_t y = ue / ud0, x = ue - y * ud0;
+        val[u] = src[sb + x * s0 + y * s1];
+        to[u] = db + x * w0 + y * w1;
+      }
     }
+#pragma unroll
+    for (int u = 0; u < 16; ++u)
+      if (e0 + 256 * u < n01) dst[to[u]] = val[u];
   }
 }
 
@@ -95,10 +100,10 @@ hipError_t launch_copy(const DevCopy& c, int elem_size, const void* src, void* d
   if (c.ntiles == 0) return hipSuccess;
   dim3 grid((unsigned)c.ntiles), block(256);
   if (elem_size == 8)
-    hipLaunchKernelGGL(copy_kernel<double>, grid, block, 0, s, c.descs, c.tile_start, c.ndesc,
+    hipLaunchKernelGGL(copy_kernel<double>, grid, block, 0, s, c.descs, c.tile_map,
                        (const double*)src, (double*)dst);
   else
-    hipLaunchKernelGGL(copy_kernel<float>, grid, block, 0, s, c.descs, c.tile_start, c.ndesc,
+    hipLaunchKernelGGL(copy_kernel<float>, grid, block, 0, s, c.descs, c.tile_map,
                        (const float*)src, (float*)dst);
   return hipGetLastError();
 }

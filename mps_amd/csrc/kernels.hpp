@@ -11,7 +11,7 @@ namespace mpsh {
 
 struct DevCopy {
   const CopyDesc* descs = nullptr;
-  const int64_t* tile_start = nullptr;
+  const int32_t* tile_map = nullptr;  // per tile: (descriptor, tile within it)
   int ndesc = 0;
   int64_t ntiles = 0;
 };

@@ -226,6 +226,13 @@ void CopyPlan::finalize() {
     tile_start[i + 1] = tile_start[i] + t;
   }
   ntiles = tile_start.back();
+  if (ntiles > INT32_MAX) throw Error(ERR_UNSUPPORTED, "copy with more than 2^31 tiles");
+  tile_map.resize(2 * ntiles);
+  for (size_t i = 0; i < descs.size(); ++i)
+    for (int64_t t = tile_start[i]; t < tile_start[i + 1]; ++t) {
+      tile_map[2 * t] = (int32_t)i;
+      tile_map[2 * t + 1] = (int32_t)(t - tile_start[i]);
+    }
 }
 
 void GemmPlan::add_seg_tab(int64_t K, const std::vector<int64_t>& arows, const std::vector<int64_t>& brows) {

@@ -111,6 +111,7 @@ struct CopyDesc {
 struct CopyPlan {
   std::vector<CopyDesc> descs;
   std::vector<int64_t> tile_start;  // ndesc + 1, prefix of 64x64 tiles
+  std::vector<int32_t> tile_map;    // per tile: (descriptor, tile within it)
   int64_t ntiles = 0;
   int64_t elems = 0;
   void add(int ndim, const int64_t* dims, const int64_t* sst, const int64_t* dst, int64_t soff,

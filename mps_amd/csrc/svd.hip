@@ -831,7 +831,7 @@ struct mpsh_svd_s {
   DevCopy upc(std::vector<std::unique_ptr<DevBuf>>& bufs, const CopyPlan& c) {
     DevCopy d;
     d.descs = up(bufs, c.descs);
-    d.tile_start = up(bufs, c.tile_start);
+    d.tile_map = up(bufs, c.tile_map);
     d.ndesc = (int)c.descs.size();
     d.ntiles = c.ntiles;
     return d;
