@@ -1119,16 +1119,26 @@ __global__ __launch_bounds__(256) void lz_dots_partial(int k, int64_t n, const d
   // block b handles a contiguous slice of n for up to 8 basis vectors at a time
   const int64_t per = (n + LZ_PARTS - 1) / LZ_PARTS;
   const int64_t e0 = (int64_t)blockIdx.x * per, e1 = e0 + per < n ? e0 + per : n;
+  // one work group per CU holds only four waves, so each thread issues the
+  // loads of four elements of every basis vector before its first FMA
   for (int i0 = 0; i0 < k; i0 += 8) {
     const int ni = k - i0 < 8 ? k - i0 : 8;
     double acc[8];
 #pragma unroll
     for (int t = 0; t < 8; ++t) acc[t] = 0.0;
-    for (int64_t e = e0 + threadIdx.x; e < e1; e += 256) {
-      const double we = w[e];
+    for (int64_t e = e0 + threadIdx.x; e < e1; e += 1024) {
+      double we[4], vv[8][4];
+#pragma unroll
+      for (int u = 0; u < 4; ++u) we[u] = e + 256 * u < e1 ? w[e + 256 * u] : 0.0;
 #pragma unroll
       for (int t = 0; t < 8; ++t)
-        if (t < ni) acc[t] += V[(int64_t)(i0 + t) * ldv + e] * we;
+#pragma unroll
+        for (int u = 0; u < 4; ++u)
+          vv[t][u] = t < ni && e + 256 * u < e1 ? V[(int64_t)(i0 + t) * ldv + e + 256 * u] : 0.0;
+#pragma unroll
+      for (int t = 0; t < 8; ++t)
+#pragma unroll
+        for (int u = 0; u < 4; ++u) acc[t] += vv[t][u] * we[u];
     }
 #pragma unroll
     for (int t = 0; t < 8; ++t) sh[t][threadIdx.x] = acc[t];
