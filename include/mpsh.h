@@ -174,6 +174,20 @@ mpsh_status mpsh_svd_execute(mpsh_svd plan, const double* A, double* S, double* 
                              void* stream);
 void mpsh_svd_destroy(mpsh_svd plan);
 
+/* The truncated factors of svdtrunc as packed SymTensors in one launch: the    */
+/* kept leading columns of each U block, the kept leading rows of each Vt block */
+/* (src/mpsutils.jl:159-161, assembled at :171-176), optionally times the kept  */
+/* singular values (u*s, s*v).  FP64 device buffers.  table (device, 8-byte     */
+/* aligned) = nblk records of MPSH_GATHER_FIELDS int64                          */
+/*   {src_off, rs, cs, rows, cols, dst_off, sc_off, smode}                      */
+/* followed by ntiles int64 (block << 32 | tile): every block's                 */
+/* ceil(rows*cols / 4096) tiles, 0-based.  For each block and i < rows, j < cols */
+/*   dst[dst_off + i + rows*j] = src[src_off + i*rs + j*cs] * f,                */
+/*   f = 1 (smode 0), scale[sc_off + i] (1), scale[sc_off + j] (2).             */
+#define MPSH_GATHER_FIELDS 8
+mpsh_status mpsh_gather_blocks(int32_t nblk, int64_t ntiles, const int64_t* table, const double* src,
+                               const double* scale, double* dst, void* stream);
+
 /* ------------------------------------------------------------------------ */
 /* Block-wise vector operations on packed buffers (KrylovKit's interface on   */
 /* SymTensors, src/symtensor.jl:266-323).  n = number of elements.            */

@@ -14,6 +14,8 @@ LIB_PATH = os.path.join(_HERE, "libmpsh.so")
 MPSH_F64, MPSH_F32, MPSH_BF16 = 0, 1, 2
 DOT_PARTIALS = 1024
 LZ_PARTS = 256  # MPSH_LZ_PARTS
+GATHER_FIELDS = 8  # MPSH_GATHER_FIELDS
+GATHER_TILE = 4096  # elements per work group of mpsh_gather_blocks
 
 STATUS_NAMES = {
     0: "OK", 1: "INVALID", 2: "SPACE_MISMATCH", 3: "SECTOR_MISMATCH", 4: "NOT_SIMILAR",
@@ -92,6 +94,7 @@ _SIGS = {
     "mpsh_axpby": (C.c_int, [C.c_int, _i64, _dbl, _p, _dbl, _p, _p]),
     "mpsh_scal": (C.c_int, [C.c_int, _i64, _dbl, _p, _p]),
     "mpsh_convert": (C.c_int, [C.c_int, _p, C.c_int, _p, _i64, _p]),
+    "mpsh_gather_blocks": (C.c_int, [C.c_int32, _i64, _p, _p, _p, _p, _p]),
 }
 
 _lib = None

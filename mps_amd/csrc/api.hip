@@ -755,4 +755,15 @@ mpsh_status mpsh_convert(mpsh_dtype src_dt, const void* src, mpsh_dtype dst_dt, 
   API_CATCH
 }
 
+mpsh_status mpsh_gather_blocks(int32_t nblk, int64_t ntiles, const int64_t* table, const double* src,
+                               const double* scale, double* dst, void* stream) {
+  API_TRY
+  if (nblk < 0 || ntiles < 0 || ntiles > INT32_MAX || (ntiles && (!table || !src || !dst)))
+    throw Error(ERR_INVALID, "bad gather arguments");
+  if (((uintptr_t)table) & 7) throw Error(ERR_INVALID, "table must be 8-byte aligned");
+  check(launch_gather_blocks(nblk, ntiles, table, src, scale, dst, (hipStream_t)stream), "gather blocks");
+  return MPSH_OK;
+  API_CATCH
+}
+
 }  // extern "C"

@@ -109,6 +109,47 @@ hipError_t launch_copy(const DevCopy& c, int elem_size, const void* src, void* d
 }
 
 // ---------------------------------------------------------------------------
+// block gather (truncated SVD factors): table = nblk records of GATHER_FIELDS
+// int64 {src_off, rs, cs, rows, cols, dst_off, sc_off, smode}, then one int64
+// per 4096-element tile: (block << 32) | tile within the block.
+//   dst[dst_off + i + rows*j] = src[src_off + i*rs + j*cs] * f,
+//   f = 1 (smode 0), scale[sc_off + i] (smode 1), scale[sc_off + j] (smode 2)
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(256) void gather_blocks_kernel(int nblk, const int64_t* __restrict__ table,
+                                                            const double* __restrict__ src,
+                                                            const double* __restrict__ scale,
+                                                            double* __restrict__ dst) {
+  const int64_t tm = table[(int64_t)nblk * GATHER_FIELDS + blockIdx.x];
+  const int64_t* d = table + (tm >> 32) * GATHER_FIELDS;
+  const int64_t so = d[0], rs = d[1], cs = d[2], rows = d[3], cols = d[4], doff = d[5], sco = d[6];
+  const int smode = (int)d[7];
+  const int64_t n = rows * cols;
+  const int64_t e0 = (tm & 0xffffffffLL) * 4096 + threadIdx.x;
+  double val[16];
+#pragma unroll
+  for (int u = 0; u < 16; ++u) {
+    const int64_t e = e0 + 256 * u;
+    if (e < n) {
+      const int64_t j = e / rows, i = e - j * rows;
+      double f = 1.0;
+      if (smode == 1) f = scale[sco + i];
+      if (smode == 2) f = scale[sco + j];
+      val[u] = src[so + i * rs + j * cs] * f;
+    }
+  }
+#pragma unroll
+  for (int u = 0; u < 16; ++u)
+    if (e0 + 256 * u < n) dst[doff + e0 + 256 * u] = val[u];
+}
+
+hipError_t launch_gather_blocks(int nblk, int64_t ntiles, const int64_t* table, const double* src,
+                                const double* scale, double* dst, hipStream_t s) {
+  if (ntiles == 0) return hipSuccess;
+  hipLaunchKernelGGL(gather_blocks_kernel, dim3((unsigned)ntiles), dim3(256), 0, s, nblk, table, src, scale, dst);
+  return hipGetLastError();
+}
+
+// ---------------------------------------------------------------------------
 // grouped GEMM
 //   C[M x N] (col-major, ldc) = sum_s A_s^T B_s, A_s(k,m) = A[a_off + k + m*lda],
 //   B_s(k,n) = B[b_off + k + n*ldb].  Tile 64x64, 4 waves in 2x2, each wave

@@ -58,6 +58,9 @@ hipError_t launch_dot(int elem_size, int64_t n, const void* x, const void* y, do
 hipError_t launch_axpby(int elem_size, int64_t n, double a, const void* x, double b, void* y, hipStream_t s);
 hipError_t launch_scal(int elem_size, int64_t n, double a, void* x, hipStream_t s);
 hipError_t launch_convert(int src_size, const void* src, int dst_size, void* dst, int64_t n, hipStream_t s);
+constexpr int GATHER_FIELDS = 8;
+hipError_t launch_gather_blocks(int nblk, int64_t ntiles, const int64_t* table, const double* src,
+                                const double* scale, double* dst, hipStream_t s);
 hipError_t launch_lz_dots(int k, int64_t n, const double* V, int64_t ldv, const double* w, double* c, double* part,
                           hipStream_t s);
 hipError_t launch_lz_update(int k, int64_t n, const double* V, int64_t ldv, const double* c, double sign, double* w,

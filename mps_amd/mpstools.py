@@ -310,14 +310,57 @@ def truncatedsizes(S: List[np.ndarray], maxdim: int = 200, tol: float = 1e-14) -
     return [int(x) for x in np.bincount(blk[take], minlength=len(S))]
 
 
-def _assemble(charge, space, parts: Dict[Tuple[int, int], torch.Tensor], dtype, device) -> SymTensor:
-    """Pack column-major blocks (dict sector -> matrix) into a SymTensor of (charge, space)."""
+def _root(t: torch.Tensor) -> torch.Tensor:
+    return t if t._base is None else t._base
+
+
+def _gather_table(blocks, src: torch.Tensor, sc: torch.Tensor, smode: int) -> Tuple[np.ndarray, int]:
+    """``mpsh_gather_blocks`` table for blocks [(matrix view of src, scale view of sc or None, dst_off)]."""
+    nb = len(blocks)
+    rec = np.zeros((nb, L.GATHER_FIELDS), dtype=np.int64)
+    tiles = []
+    for b, (x, sv, doff) in enumerate(blocks):
+        rows, cols = x.shape
+        rec[b] = ((x.data_ptr() - src.data_ptr()) // 8, x.stride(0), x.stride(1), rows, cols, doff,
+                  0 if sv is None else (sv.data_ptr() - sc.data_ptr()) // 8, 0 if sv is None else smode)
+        nt = (rows * cols + L.GATHER_TILE - 1) // L.GATHER_TILE
+        tiles.append((b << 32) + np.arange(nt, dtype=np.int64))
+    t = np.concatenate(tiles) if tiles else np.zeros(0, dtype=np.int64)
+    return np.concatenate([rec.reshape(-1), t]), len(t)
+
+
+def _assemble(charge, space, parts: Dict[Tuple[int, int], torch.Tensor], dtype, device,
+              scales: Dict[Tuple[int, int], torch.Tensor] = None, smode: int = 0) -> SymTensor:
+    """Pack column-major blocks (dict sector -> matrix) into a SymTensor of (charge, space);
+    with ``scales``, each block's rows (smode 1) or columns (smode 2) are multiplied by
+    the sector's vector first.  FP64 views of one device buffer (the SVD's packed
+    factors) are gathered in one ``mpsh_gather_blocks`` launch."""
     st = get_struct(charge, space)
     sec, siz, off = st.tables()
+    keys = [(int(sec[b][0]), int(sec[b][1])) for b in range(st.nblocks)]
+    mats = [parts[k] for k in keys]
+    scs = [None] * len(keys) if scales is None else [scales[k] for k in keys]
+    src = _root(mats[0]) if mats else None
+    sc = _root(scs[0]) if mats and scales is not None else None
+    fast = (bool(mats) and dtype == torch.float64 and src.is_cuda
+            and all(x.dtype == torch.float64 and _root(x) is src for x in mats)
+            and (scales is None or all(v.dtype == torch.float64 and _root(v) is sc for v in scs)))
+    if fast:
+        for b, x in enumerate(mats):
+            if x.shape[0] * x.shape[1] != off[b + 1] - off[b]:
+                raise ValueError(f"block {keys[b]} has shape {tuple(x.shape)}, the structure expects "
+                                 f"{off[b + 1] - off[b]} elements")
+        table, ntiles = _gather_table(list(zip(mats, scs, off[:-1])), src, sc, smode)
+        dev_table = torch.from_numpy(table).pin_memory().to(src.device, non_blocking=True)
+        data = torch.empty(st.numel, dtype=dtype, device=src.device)
+        check(lib().mpsh_gather_blocks(len(mats), ntiles, _ptr(dev_table), _ptr(src),
+                                       None if sc is None else _ptr(sc), _ptr(data), _stream()))
+        return SymTensor(charge, space, data, st)
     data = torch.empty(st.numel, dtype=dtype, device=device)
     for b in range(st.nblocks):
-        key = (int(sec[b][0]), int(sec[b][1]))
-        blk = parts[key]
+        blk = mats[b]
+        if scales is not None:
+            blk = scs[b][:, None] * blk if smode == 1 else blk * scs[b]
         data[off[b]:off[b + 1]] = blk.t().reshape(-1)
     return SymTensor(charge, space, data, st)
 
@@ -370,15 +413,16 @@ class SVDResult:
     def us(self) -> SymTensor:
         """u*s (charge of s, legs (space1, Vr))."""
         return _assemble(self.M.charge, (self.M.space[0], self.Vr),
-                         {(k[0], k[1]): U * s for k, U, s in zip(self.keys, self.U, self.S)}, self.M.dtype,
-                         self.M.device)
+                         {(k[0], k[1]): U for k, U in zip(self.keys, self.U)}, self.M.dtype, self.M.device,
+                         scales={(k[0], k[1]): s for k, s in zip(self.keys, self.S)}, smode=2)
 
     def sv(self) -> SymTensor:
         """s*v (legs (Vl, space2))."""
         if "sv" not in self._cache:
             self._cache["sv"] = _assemble(self.M.charge, (self.Vl, self.M.space[1]),
-                                          {(k[0], k[1]): s[:, None] * V for k, V, s in
-                                           zip(self.keys, self.Vh, self.S)}, self.M.dtype, self.M.device)
+                                          {(k[0], k[1]): V for k, V in zip(self.keys, self.Vh)}, self.M.dtype,
+                                          self.M.device,
+                                          scales={(k[0], k[1]): s for k, s in zip(self.keys, self.S)}, smode=1)
         return self._cache["sv"]
 
     def usv(self) -> SymTensor:

@@ -95,6 +95,30 @@ def test_svdtrunc_matches_oracle(maxdim, tol):
     assert np.linalg.norm(got - want) <= 1e-12 * np.linalg.norm(want)
 
 
+@pytest.mark.parametrize("charge,maxdim,normalize", [(0, 12, False), (1, 30, True), (1, 10_000, False)])
+def test_truncated_factors_gathered_in_one_launch(charge, maxdim, normalize):
+    """u, v, u*s, s*v come from one mpsh_gather_blocks launch each; every block
+    equals the truncated factor built block by block with torch (bit for bit:
+    one IEEE product per element either way)."""
+    A = _matrix([(q, 9 + 2 * q) for q in range(5)], [(q, 14 - q) for q in range(5)], charge=charge, seed=22,
+                scale=0.6)
+    r = mt.svdtrunc(A, maxdim=maxdim, tol=1e-13)
+    if normalize:
+        r.normalize_s()
+    assert r.keys and any(U.shape[1] < min(U.shape[0], Vh.shape[1]) for U, Vh in zip(r.U, r.Vh)) == (maxdim < 100)
+    want = {"u": {}, "v": {}, "us": {}, "sv": {}}
+    for k, U, s, Vh in zip(r.keys, r.U, r.S, r.Vh):
+        want["u"][(k[0], k[0])] = U.cpu().numpy()
+        want["v"][(k[1], k[1])] = Vh.cpu().numpy()
+        want["us"][(k[0], k[1])] = (U * s).cpu().numpy()
+        want["sv"][(k[0], k[1])] = (s[:, None] * Vh).cpu().numpy()
+    for name in want:
+        got = getattr(r, name)().blocks()
+        assert set(got) == set(want[name]), name
+        for key, blk in got.items():
+            assert np.array_equal(blk, want[name][key]), (name, key)
+
+
 def _blocks_with(A, algo, precond=True):
     """_svd_blocks through a fresh plan built with MPSH_SVD_ALGO=algo and
     MPSH_SVD_PRECOND (plans read both when built)."""

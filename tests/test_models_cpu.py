@@ -186,3 +186,48 @@ def test_svd_plan_reserves_gram_preconditioner_workspace():
     assert want <= pre - plain <= want + 128
     small = _svd_ws([60], [50], {"MPSH_SVD_ALGO": "jacobi", "MPSH_SVD_PRECOND": "1"})
     assert small == _svd_ws([60], [50], {"MPSH_SVD_ALGO": "jacobi", "MPSH_SVD_PRECOND": "0"})
+
+
+def test_gather_table_replayed_on_host_builds_the_truncated_factors():
+    """mpsh_gather_blocks' table (records + tile list), replayed here with the
+    kernel's formula, rebuilds u (column prefix), v (strided row prefix) and the
+    s-scaled products block by block; the tiles cover every element once."""
+    from mps_amd import _lib as L
+    g = torch.Generator().manual_seed(7)
+    buf = torch.randn(40_000, generator=g, dtype=torch.float64)
+    sbuf = torch.rand(200, generator=g, dtype=torch.float64)
+    # (m x k) column-major U blocks cut to their first kk columns; (k x n) Vh blocks cut to kk rows
+    shapes = [(70, 60, 45, 90), (5, 5, 5, 3), (130, 40, 1, 77)]
+    off = so = 0
+    U, Vh, S = [], [], []
+    for m, k, kk, n in shapes:
+        U.append(buf[off:off + m * k].view(k, m).t()[:, :kk])
+        off += m * k
+        Vh.append(buf[off:off + k * n].view(n, k).t()[:kk, :])
+        off += k * n
+        S.append(sbuf[so:so + kk])
+        so += kk
+    for mats, scs, smode in ((U, None, 0), (Vh, None, 0), (U, S, 2), (Vh, S, 1)):
+        doffs = np.cumsum([0] + [x.numel() for x in mats])
+        table, ntiles = mt._gather_table(list(zip(mats, scs or [None] * 3, doffs[:-1])), buf,
+                                         None if scs is None else sbuf, smode)
+        nb = len(mats)
+        rec = table[:nb * L.GATHER_FIELDS].reshape(nb, L.GATHER_FIELDS)
+        tiles = table[nb * L.GATHER_FIELDS:]
+        assert len(tiles) == ntiles == sum(-(-x.numel() // L.GATHER_TILE) for x in mats)
+        src, sc = buf.numpy(), sbuf.numpy()
+        dst = np.full(int(doffs[-1]), np.nan)
+        hits = np.zeros(int(doffs[-1]), dtype=int)
+        for t in tiles:
+            s_off, rs, cs, rows, cols, d_off, sc_off, sm = rec[int(t) >> 32]
+            e = (int(t) & 0xFFFFFFFF) * L.GATHER_TILE + np.arange(L.GATHER_TILE)
+            e = e[e < rows * cols]
+            j, i = e // rows, e % rows
+            f = 1.0 if sm == 0 else sc[sc_off + (i if sm == 1 else j)]
+            dst[d_off + e] = src[s_off + i * rs + j * cs] * f
+            hits[d_off + e] += 1
+        assert (hits == 1).all()
+        for b, x in enumerate(mats):
+            want = x if scs is None else (x * scs[b] if smode == 2 else scs[b][:, None] * x)
+            got = dst[doffs[b]:doffs[b + 1]].reshape(x.shape, order="F")
+            assert np.array_equal(got, want.numpy()), (smode, b)
