@@ -24,6 +24,31 @@ static thread_local std::string g_err;
 
 namespace mpsh {
 void set_last_error(const std::string& m) { g_err = m; }
+
+// Plan tables are uploaded on a per-device non-blocking stream of the
+// library's own and waited for there: a plain hipMemcpy runs on the null
+// stream, which first waits for every kernel queued on the caller's streams.
+hipError_t upload_tables(void* dst, const void* src, size_t bytes) {
+  static std::mutex mu;
+  static std::map<int, hipStream_t> streams;
+  int d = 0;
+  hipError_t e = hipGetDevice(&d);
+  if (e != hipSuccess) return e;
+  hipStream_t s;
+  {
+    std::lock_guard<std::mutex> g(mu);
+    auto it = streams.find(d);
+    if (it == streams.end()) {
+      e = hipStreamCreateWithFlags(&s, hipStreamNonBlocking);
+      if (e != hipSuccess) return e;
+      it = streams.emplace(d, s).first;
+    }
+    s = it->second;
+  }
+  e = hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, s);
+  if (e != hipSuccess) return e;
+  return hipStreamSynchronize(s);
+}
 }  // namespace mpsh
 
 static mpsh_status fail(int code, const std::string& m) {
@@ -62,7 +87,7 @@ struct DevTables {
     if (v.empty()) return nullptr;
     auto b = std::make_unique<DevBuf>();
     check(hipMalloc(&b->p, v.size() * sizeof(T)), "hipMalloc(plan tables)");
-    check(hipMemcpy(b->p, v.data(), v.size() * sizeof(T), hipMemcpyHostToDevice), "hipMemcpy(plan tables)");
+    check(upload_tables(b->p, v.data(), v.size() * sizeof(T)), "upload(plan tables)");
     const T* r = (const T*)b->p;
     bufs.push_back(std::move(b));
     return r;

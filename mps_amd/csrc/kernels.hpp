@@ -18,6 +18,8 @@ struct DevCopy {
 
 // thread-local message returned by mpsh_last_error (defined in api.hip)
 void set_last_error(const std::string& m);
+// synchronous host-to-device copy of plan tables that does not wait for other streams' work (api.hip)
+hipError_t upload_tables(void* dst, const void* src, size_t bytes);
 
 struct XcdQueues {
   int32_t off[NXCD + 1];

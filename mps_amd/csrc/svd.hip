@@ -823,7 +823,7 @@ struct mpsh_svd_s {
     if (v.empty()) return nullptr;
     auto b = std::make_unique<DevBuf>();
     ck(hipMalloc(&b->p, v.size() * sizeof(T)), "hipMalloc(svd tables)");
-    ck(hipMemcpy(b->p, v.data(), v.size() * sizeof(T), hipMemcpyHostToDevice), "hipMemcpy(svd tables)");
+    ck(upload_tables(b->p, v.data(), v.size() * sizeof(T)), "upload(svd tables)");
     const T* r = (const T*)b->p;
     bufs.push_back(std::move(b));
     return r;
