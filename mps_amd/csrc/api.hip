@@ -116,7 +116,16 @@ struct DevTables {
     d.ntiles = (int)t.size();
     d.nred = (int)r.size() - 1;
     d.red_groups = r.back().start;
-    if (d.nred > 0) d.reds = up(r);
+    if (d.nred > 0) {
+      d.reds = up(r);
+      std::vector<int32_t> rmap(2 * d.red_groups);
+      for (int i = 0; i < d.nred; ++i)
+        for (int64_t gi = r[i].start; gi < r[i + 1].start; ++gi) {
+          rmap[2 * gi] = r[i].prob;
+          rmap[2 * gi + 1] = (int32_t)(gi - r[i].start);
+        }
+      d.red_map = up(rmap);
+    }
     d.BM = g.BM;
     d.BN = g.BN;
     d.KS = g.KS;

@@ -791,33 +791,36 @@ __global__ __launch_bounds__(256) void gemm_dma(
 
 // C(m, n) = sum_s P[part_off + s*M*N + m + M*n] for every split problem
 template <typename T>
-__global__ __launch_bounds__(256) void splitk_reduce(const RedDesc* __restrict__ reds, int nred,
+// One work group per 1024 outputs of a split problem, named by the plan's
+// reduction map.  Each output sums its partials in split order; a thread's four
+// outputs are summed side by side, so their loads of one split are in flight
+// together.
+__global__ __launch_bounds__(256) void splitk_reduce(const int32_t* __restrict__ rmap,
                                                      const GemmProb* __restrict__ probs,
                                                      const int64_t* __restrict__ crow,
                                                      const int64_t* __restrict__ cld, const T* __restrict__ P,
                                                      T* __restrict__ C) {
-  const int64_t bid = blockIdx.x;
-  int lo = 0, hi = nred - 1;
-  while (lo < hi) {
-    const int mid = (lo + hi + 1) >> 1;
-    if (reds[mid].start <= bid)
-      lo = mid;
-    else
-      hi = mid - 1;
-  }
-  const GemmProb p = probs[reds[lo].prob];
+  const int2 rm = ((const int2*)rmap)[blockIdx.x];
+  const GemmProb p = probs[rm.x];
   const int64_t MN = (int64_t)p.M * p.N;
-  const int64_t e0 = (bid - reds[lo].start) * 1024;
-  for (int i = threadIdx.x; i < 1024; i += 256) {
-    const int64_t e = e0 + i;
-    if (e >= MN) break;
-    const int64_t n = e / p.M, m = e - n * p.M;
-    T acc = 0;
-    for (int s = 0; s < p.nsplit; ++s) acc += P[p.part_off + s * MN + e];
-    if (p.crow0 >= 0)
-      C[crow[p.crow0 + m] + n * cld[p.crow0 + m]] = acc;
-    else
-      C[p.c_off + m + n * p.ldc] = acc;
+  const int64_t e0 = (int64_t)rm.y * 1024 + threadIdx.x;
+  T acc[4] = {0, 0, 0, 0};
+  for (int s = 0; s < p.nsplit; ++s) {
+    const T* ps = P + p.part_off + s * MN;
+#pragma unroll
+    for (int u = 0; u < 4; ++u)
+      if (e0 + 256 * u < MN) acc[u] += ps[e0 + 256 * u];
+  }
+#pragma unroll
+  for (int u = 0; u < 4; ++u) {
+    const int64_t e = e0 + 256 * u;
+    if (e < MN) {
+      const int64_t n = e / p.M, m = e - n * p.M;
+      if (p.crow0 >= 0)
+        C[crow[p.crow0 + m] + n * cld[p.crow0 + m]] = acc[u];
+      else
+        C[p.c_off + m + n * p.ldc] = acc[u];
+    }
   }
 }
 
@@ -943,10 +946,10 @@ hipError_t launch_gemm(const DevGemm& gm, int mode, const void* A, const void* B
     e = launch_mode<2>(gm, A, B, C, P, s);
   if (e != hipSuccess || gm.nred == 0) return e;
   if (mode == 0)
-    hipLaunchKernelGGL(splitk_reduce<double>, dim3((unsigned)gm.red_groups), dim3(256), 0, s, gm.reds, gm.nred,
+    hipLaunchKernelGGL(splitk_reduce<double>, dim3((unsigned)gm.red_groups), dim3(256), 0, s, gm.red_map,
                        gm.probs, gm.crow, gm.cld, (const double*)P, (double*)C);
   else
-    hipLaunchKernelGGL(splitk_reduce<float>, dim3((unsigned)gm.red_groups), dim3(256), 0, s, gm.reds, gm.nred,
+    hipLaunchKernelGGL(splitk_reduce<float>, dim3((unsigned)gm.red_groups), dim3(256), 0, s, gm.red_map,
                        gm.probs, gm.crow, gm.cld, (const float*)P, (float*)C);
   return hipGetLastError();
 }

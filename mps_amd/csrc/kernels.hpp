@@ -31,6 +31,7 @@ struct DevGemm {
   const int64_t *atab = nullptr, *btab = nullptr, *crow = nullptr, *cld = nullptr;
   const GemmTile* tiles = nullptr;
   const RedDesc* reds = nullptr;
+  const int32_t* red_map = nullptr;  // per reduction work group: (problem, group within it)
   int ntiles = 0, nred = 0;
   int64_t red_groups = 0;
   int BM = 64, BN = 64, KS = 1;  // tile and K-chunk multiplier
