@@ -197,3 +197,16 @@ def test_heff_plan_structure_and_flops_cfg1():
     assert hp.step_flops == want
     assert hp.ref_flops == sum(want)
 
+
+
+def test_gather_blocks_rejects_bad_arguments_on_the_host():
+    """mpsh_gather_blocks checks its arguments before any device call: a negative
+    block count, missing buffers and a table not aligned to 8 bytes are
+    MPSH_ERR_INVALID with a message; an empty gather is a no-op."""
+    lb = _lib.lib()
+    assert lb.mpsh_gather_blocks(-1, 0, None, None, None, None, None) == 1
+    assert b"gather" in lb.mpsh_last_error()
+    assert lb.mpsh_gather_blocks(1, 1, None, None, None, None, None) == 1
+    assert lb.mpsh_gather_blocks(1, 1, 4099, 4096, None, 8192, None) == 1
+    assert b"aligned" in lb.mpsh_last_error()
+    assert lb.mpsh_gather_blocks(0, 0, None, None, None, None, None) == 0
