@@ -233,7 +233,9 @@ struct mpsh_releg_s {
 struct ContractDev {
   std::unique_ptr<DevTables> t;
   DevCopy ga, gb, sc;
-  DevGemm gm;
+  DevGemm gm;   // register-staged operands (BF16)
+  DevGemm gmd;  // the same tables with DMA-streamed operands (FP64, FP32): the
+                // fused operands' rows are padded to 4 elements, 16-byte aligned
 };
 
 struct mpsh_contract_s {
@@ -251,6 +253,8 @@ struct mpsh_contract_s {
       c.gb = c.t->copy(p.FB.gather);
       c.sc = c.t->copy(p.scatter);
       c.gm = c.t->gemm(p.gemm);
+      c.gmd = c.gm;
+      c.gmd.KS = 0;
       it = dev.emplace(d, std::move(c)).first;
     }
     return it->second;
@@ -469,7 +473,7 @@ mpsh_status mpsh_contract_execute(mpsh_contract plan, mpsh_dtype dt, const void*
   char* ws = (char*)workspace;
   check(launch_copy(d.ga, es, A, ws, s), "gather A");
   check(launch_copy(d.gb, es, B, ws, s), "gather B");
-  check(launch_gemm(d.gm, gemm_mode(dt), ws + plan->p.fa_off * es, ws + plan->p.fb_off * es,
+  check(launch_gemm(dt == MPSH_BF16 ? d.gm : d.gmd, gemm_mode(dt), ws + plan->p.fa_off * es, ws + plan->p.fb_off * es,
                     ws + plan->p.fc_off * es, nullptr, s),
         "grouped gemm");
   check(launch_copy(d.sc, es, ws, C, s), "scatter C");
