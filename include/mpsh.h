@@ -184,6 +184,9 @@ void mpsh_svd_destroy(mpsh_svd plan);
 /* ceil(rows*cols / 4096) tiles, 0-based.  For each block and i < rows, j < cols */
 /*   dst[dst_off + i + rows*j] = src[src_off + i*rs + j*cs] * f,                */
 /*   f = 1 (smode 0), scale[sc_off + i] (1), scale[sc_off + j] (2).             */
+/* The table is read on the device only: the caller guarantees that every     */
+/* record stays inside src, scale and dst, and passes scale = NULL only when   */
+/* every smode is 0.                                                          */
 #define MPSH_GATHER_FIELDS 8
 mpsh_status mpsh_gather_blocks(int32_t nblk, int64_t ntiles, const int64_t* table, const double* src,
                                const double* scale, double* dst, void* stream);
