@@ -28,7 +28,7 @@ HBM_PEAK = 8.0e12
 METRIC = "H_eff matvecs/sec + DMRG sweep time at χ=2048; |ΔE| vs FP64 CPU oracle"
 # HBM bytes per step-1 GEMM launch: rocprofv3 FETCH_SIZE x 2.000 + WRITE_SIZE x 1.000
 # (scales from a 256 MiB device copy), profiles/heff_cyl6x24_chi2048_f32_hbm_traffic.txt
-TRAFFIC = {("cyl6x24_chi2048", "f32"): 288.4e6}  # autotuned step-1 variant gemm_dma<1, 64, 128, 4>
+TRAFFIC = {("cyl6x24_chi2048", "f32"): 228.8e6}  # autotuned step-1 variant gemm_dma<1, 64, 128, 4>
 
 
 def parse():

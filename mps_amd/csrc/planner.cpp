@@ -293,7 +293,8 @@ std::vector<GemmTile> GemmPlan::tiles(std::vector<int32_t>* qoff) const {
   // Panel pieces: the tiles of one (problem, split) over a contiguous range of
   // m0 rows (all n0).  Tiles of a piece share A rows (same m0) and B rows (same
   // n0), so a piece is placed whole on one XCD; large (problem, split) units are
-  // cut by m0 rows into pieces of at most a quarter of one XCD's share.
+  // cut by m0 rows into pieces of at most a quarter of one XCD's share.  Inside
+  // a piece the tiles run along the larger operand's rows in the outer loop.
   struct Piece {
     int64_t work;
     int32_t prob, split, mr0, mr1;
@@ -329,8 +330,16 @@ std::vector<GemmTile> GemmPlan::tiles(std::vector<int32_t>* qoff) const {
     const int x = (int)(std::min_element(load.begin(), load.end()) - load.begin());
     load[x] += pc.work;
     const GemmProb& q = probs[pc.prob];
-    for (int32_t r = pc.mr0; r < pc.mr1; ++r)
-      for (int n0 = 0; n0 < q.N; n0 += BN) queue[x].push_back(GemmTile{pc.prob, r * BM, n0, pc.split});
+    // the larger operand is walked once in the outer loop, the smaller one stays
+    // in L2: column panels (n0 outer) when the piece's B rows outnumber its A rows
+    const int64_t arows = (int64_t)(pc.mr1 - pc.mr0) * BM;
+    if ((int64_t)q.N > arows) {
+      for (int n0 = 0; n0 < q.N; n0 += BN)
+        for (int32_t r = pc.mr0; r < pc.mr1; ++r) queue[x].push_back(GemmTile{pc.prob, r * BM, n0, pc.split});
+    } else {
+      for (int32_t r = pc.mr0; r < pc.mr1; ++r)
+        for (int n0 = 0; n0 < q.N; n0 += BN) queue[x].push_back(GemmTile{pc.prob, r * BM, n0, pc.split});
+    }
   }
   std::vector<GemmTile> out;
   if (qoff) qoff->assign(NXCD + 1, 0);
