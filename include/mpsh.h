@@ -154,6 +154,53 @@ mpsh_status mpsh_heff_tune(mpsh_heff plan, mpsh_dtype dt, const void* prepared, 
 void mpsh_heff_destroy(mpsh_heff plan);
 
 /* ------------------------------------------------------------------------ */
+/* Environment updates  _mpsupdateleft(lenv, A, W)  (src/apply.jl:49-58) and    */
+/* _mpsupdateright(renv, A, W)  (src/apply.jl:17-25), the reference's three     */
+/* contracts each, called from initialenv / dmrg2sitesweep! (src/dmrg.jl:29,33, */
+/* :103,141,175).  Fused plan: the H_eff's step-1 grouped GEMM (env x A) and     */
+/* combined-MPO pass over one site, then one grouped GEMM against the bra       */
+/* dual(A) (A's own data, no conjugation: src/symtensor.jl:201-212).  The right */
+/* update runs as the left update of the mirror image A(r, p, l),               */
+/* W(w_r, p_out, w_l, p_in).  The plan returns the output structure; outputs    */
+/* and block order equal the reference chain's.  FP64 and FP32 (BF16: error).   */
+/* stats[4]: reference-chain FLOPs, reference-chain elements moved, this        */
+/* engine's GEMM FLOPs, its MPO-pass FLOPs.                                     */
+/* ------------------------------------------------------------------------ */
+typedef struct mpsh_envupdate_s* mpsh_envupdate;
+
+mpsh_status mpsh_envupdate_left_plan(mpsh_struct lenv, mpsh_struct A, mpsh_struct W, mpsh_envupdate* plan,
+                                     mpsh_struct* out);
+mpsh_status mpsh_envupdate_right_plan(mpsh_struct renv, mpsh_struct A, mpsh_struct W, mpsh_envupdate* plan,
+                                      mpsh_struct* out);
+size_t mpsh_envupdate_workspace(mpsh_envupdate plan, mpsh_dtype dt); /* bytes, 16-byte aligned buffer */
+mpsh_status mpsh_envupdate_stats(mpsh_envupdate plan, int64_t* stats);
+mpsh_status mpsh_envupdate_execute(mpsh_envupdate plan, mpsh_dtype dt, const void* env, const void* A,
+                                   const void* W, void* out, void* workspace, void* stream);
+void mpsh_envupdate_destroy(mpsh_envupdate plan);
+
+/* ------------------------------------------------------------------------ */
+/* Adjoint of E(v, envL, envR) = <v|H_eff v> (H_eff of _applymps2site,          */
+/* src/apply.jl:93-105; the reference has no AD): the gradients of the          */
+/* un-normalised quadratic form w.r.t. the environments,                        */
+/*   gL[a,m,b] = sum v[a,p,s,c] WL[m,q,n,p] WR[n,r,k,s] envR[c,k,e] v[b,q,r,e],  */
+/*   gR[c,k,e] = sum envL[a,m,b] v[a,p,s,c] WL[m,q,n,p] WR[n,r,k,s] v[b,q,r,e],  */
+/* each with the structure of the dual of its variable (same sectors and block  */
+/* order as envL / envR).  gR is the two-site environment core on the operands, */
+/* gL the same core on their mirror image.  The gradient w.r.t. v is           */
+/* 2 H_eff v (mpsh_heff_apply).  FP64 and FP32.  stats as for mpsh_envupdate.   */
+/* ------------------------------------------------------------------------ */
+typedef struct mpsh_heff_adjoint_s* mpsh_heff_adjoint;
+
+mpsh_status mpsh_heff_adjoint_plan(mpsh_struct v, mpsh_struct envL, mpsh_struct envR, mpsh_struct WL,
+                                   mpsh_struct WR, mpsh_heff_adjoint* plan, mpsh_struct* gL, mpsh_struct* gR);
+size_t mpsh_heff_adjoint_workspace(mpsh_heff_adjoint plan, mpsh_dtype dt);
+mpsh_status mpsh_heff_adjoint_stats(mpsh_heff_adjoint plan, int64_t* stats);
+mpsh_status mpsh_heff_adjoint_execute(mpsh_heff_adjoint plan, mpsh_dtype dt, const void* v, const void* envL,
+                                      const void* envR, const void* WL, const void* WR, void* gL, void* gR,
+                                      void* workspace, void* stream);
+void mpsh_heff_adjoint_destroy(mpsh_heff_adjoint plan);
+
+/* ------------------------------------------------------------------------ */
 /* Batched per-block FP64 SVD: the per-fused-sector SVD of svdtrunc / svd     */
 /* (src/mpsutils.jl:116-179; LAPACK gesdd, retried with gesvd on failure,     */
 /* src/symtools.jl:14-19).  Block b is the m[b] x n[b] column-major matrix at */

@@ -82,6 +82,17 @@ _SIGS = {
     "mpsh_heff_apply_prepared_stages": (C.c_int, [_p, C.c_int, _i32, _p, _p, _p, _p, _p]),
     "mpsh_heff_tune": (C.c_int, [_p, C.c_int, _p, _p, _p, _p, _i32, _i32, _pi32, _p]),
     "mpsh_heff_destroy": (None, [_p]),
+    "mpsh_envupdate_left_plan": (C.c_int, [_p, _p, _p, C.POINTER(_p), C.POINTER(_p)]),
+    "mpsh_envupdate_right_plan": (C.c_int, [_p, _p, _p, C.POINTER(_p), C.POINTER(_p)]),
+    "mpsh_envupdate_workspace": (_sz, [_p, C.c_int]),
+    "mpsh_envupdate_stats": (C.c_int, [_p, _pi64]),
+    "mpsh_envupdate_execute": (C.c_int, [_p, C.c_int, _p, _p, _p, _p, _p, _p]),
+    "mpsh_envupdate_destroy": (None, [_p]),
+    "mpsh_heff_adjoint_plan": (C.c_int, [_p, _p, _p, _p, _p, C.POINTER(_p), C.POINTER(_p), C.POINTER(_p)]),
+    "mpsh_heff_adjoint_workspace": (_sz, [_p, C.c_int]),
+    "mpsh_heff_adjoint_stats": (C.c_int, [_p, _pi64]),
+    "mpsh_heff_adjoint_execute": (C.c_int, [_p, C.c_int, _p, _p, _p, _p, _p, _p, _p, _p, _p]),
+    "mpsh_heff_adjoint_destroy": (None, [_p]),
     "mpsh_svd_plan": (C.c_int, [_i64, _pi64, _pi64, _pi64, C.POINTER(_p)]),
     "mpsh_svd_sizes": (C.c_int, [_p, _pi64, _pi64, _pi64, _pi64, _pi64]),
     "mpsh_svd_workspace": (C.c_size_t, [_p]),

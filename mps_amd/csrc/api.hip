@@ -267,6 +267,36 @@ struct HeffDev {
   DevCopy pack_v, pack_envL, pack_envR;
 };
 
+DevMpo upload_mpo(DevTables& t, const Step13& p) {
+  DevMpo m;
+  m.sigs = t.up(p.sigs);
+  m.terms = t.up(p.terms);
+  m.pairs = t.up(p.pairs);
+  m.groups = t.up(p.groups);
+  m.inrows = t.up(p.inrows);
+  m.outrows = t.up(p.outrows);
+  m.work = t.up(p.work);
+  m.nterms = (int)p.terms.size();
+  m.nwork = (int)p.work.size();
+  m.cmax = (int)p.maxC;
+  m.rmax = (int)p.maxR;
+  return m;
+}
+
+// device tables of one environment core
+struct EnvCoreDev {
+  DevCopy pack_ket, pack_env, pack_bra;
+  DevMpo mpo;
+  DevGemm g1, gF, gF64;
+};
+struct EnvDev {
+  std::unique_ptr<DevTables> t;
+  EnvCoreDev core[2];
+  DevCopy perm[2][3];
+  const double* wr_f64 = nullptr;  // identity WR of one-site plans
+  const float* wr_f32 = nullptr;
+};
+
 // a GEMM kernel variant of one H_eff stage: (stage 1|4, mode, BM, BN, KS, split target)
 typedef std::tuple<int, int, int, int, int, int64_t> VariantKey;
 
@@ -304,17 +334,7 @@ struct mpsh_heff_s {
     if (it == dev.end()) {
       HeffDev h;
       h.t = std::make_unique<DevTables>();
-      h.mpo.sigs = h.t->up(p.sigs);
-      h.mpo.terms = h.t->up(p.terms);
-      h.mpo.pairs = h.t->up(p.pairs);
-      h.mpo.groups = h.t->up(p.groups);
-      h.mpo.inrows = h.t->up(p.inrows);
-      h.mpo.outrows = h.t->up(p.outrows);
-      h.mpo.work = h.t->up(p.work);
-      h.mpo.nterms = (int)p.terms.size();
-      h.mpo.nwork = (int)p.work.size();
-      h.mpo.cmax = (int)p.maxC;
-      h.mpo.rmax = (int)p.maxR;
+      h.mpo = upload_mpo(*h.t, p);
       h.pack_v = h.t->copy(p.pack_v);
       h.pack_envL = h.t->copy(p.pack_envL);
       h.pack_envR = h.t->copy(p.pack_envR);
@@ -323,6 +343,82 @@ struct mpsh_heff_s {
     return it->second;
   }
 };
+
+// environment updates and the H_eff adjoint share one plan object
+struct mpsh_env_s {
+  EnvPlan p;
+  std::mutex mu;
+  std::map<int, EnvDev> dev;
+  const EnvDev& get() {
+    std::lock_guard<std::mutex> g(mu);
+    int d = current_device();
+    auto it = dev.find(d);
+    if (it == dev.end()) {
+      EnvDev e;
+      e.t = std::make_unique<DevTables>();
+      for (int c = 0; c < p.ncore; ++c) {
+        const EnvCore& C = p.core[c];
+        EnvCoreDev& D = e.core[c];
+        D.pack_ket = e.t->copy(C.pack_v);
+        D.pack_env = e.t->copy(C.pack_envL);
+        D.pack_bra = e.t->copy(C.pack_bra);
+        D.mpo = upload_mpo(*e.t, C);
+        D.g1 = e.t->gemm(C.g1);
+        D.gF = e.t->gemm(C.gF);
+        D.gF64 = e.t->gemm(p.core_f64[c].gF);
+        for (int k = 0; k < 3; ++k)
+          if (p.has_perm[c][k]) e.perm[c][k] = e.t->copy(p.perm[c][k].copy);
+      }
+      if (p.wr_is_identity) {
+        e.wr_f64 = e.t->up(p.wr_identity);
+        e.wr_f32 = e.t->up(std::vector<float>(p.wr_identity.begin(), p.wr_identity.end()));
+      }
+      it = dev.emplace(d, std::move(e)).first;
+    }
+    return it->second;
+  }
+};
+struct mpsh_envupdate_s : mpsh_env_s {};
+struct mpsh_heff_adjoint_s : mpsh_env_s {};
+
+// One environment core on device buffers: packs, combined MPO, step-1 GEMM, MPO
+// pass, final GEMM against the bra (the ket's data).
+static void run_env_core(const mpsh_env_s& h, const EnvDev& d, int c, mpsh_dtype dt, const void* env,
+                         const void* ket, const void* WL, const void* WR, void* out, char* ws, hipStream_t s) {
+  const EnvCore& C = dt == MPSH_F64 ? h.p.core_f64[c] : h.p.core[c];
+  const EnvCoreDev& D = d.core[c];
+  const int es = elem_size(dt), mode = gemm_mode(dt);
+  check(launch_copy(D.pack_ket, es, ket, ws + C.kp_off * es, s), "env: pack ket");
+  check(launch_copy(D.pack_env, es, env, ws + C.ep_off * es, s), "env: pack env");
+  check(launch_copy(D.pack_bra, es, ket, ws + C.bp_off * es, s), "env: pack bra");
+  check(launch_mpo_combine(D.mpo, es, WL, WR, ws + C.w_off * es, s), "env: mpo combine");
+  check(launch_gemm(D.g1, mode, ws + C.kp_off * es, ws + C.ep_off * es, ws + C.t1_off * es, ws + C.part_off * es, s),
+        "env: step-1 gemm");
+  check(launch_mpo_apply(D.mpo, es, ws + C.w_off * es, ws + C.t1_off * es, ws + C.t3_off * es, s), "env: mpo");
+  check(launch_gemm(dt == MPSH_F64 ? D.gF64 : D.gF, mode, ws + C.t3_off * es, ws + C.bp_off * es, out,
+                    ws + C.part_off * es, s),
+        "env: final gemm");
+}
+
+static void env_check_dtype(mpsh_dtype dt) {
+  if (dt != MPSH_F64 && dt != MPSH_F32) throw Error(ERR_UNSUPPORTED, "environment plans run in FP64 or FP32");
+}
+
+static size_t env_workspace(const mpsh_env_s* h, mpsh_dtype dt) {
+  return h ? (size_t)std::max<int64_t>(h->p.ws_elems, 1) * (size_t)elem_size(dt) + 16 : 0;
+}
+
+static void env_stats(const mpsh_env_s* h, int64_t* st) {
+  if (!h || !st) throw Error(ERR_INVALID, "null argument");
+  st[0] = h->p.ref_flops;
+  st[1] = h->p.ref_elems;
+  st[2] = 0;
+  st[3] = 0;
+  for (int c = 0; c < h->p.ncore; ++c) {
+    st[2] += h->p.core[c].g1.flops + h->p.core[c].gF.flops;
+    st[3] += h->p.core[c].mpo_flops;
+  }
+}
 
 extern "C" {
 
@@ -718,6 +814,129 @@ mpsh_status mpsh_heff_apply(mpsh_heff plan, mpsh_dtype dt, const void* v, const 
 }
 
 void mpsh_heff_destroy(mpsh_heff plan) { delete plan; }
+
+// ---------------------------------------------------------------------------
+// environment updates
+// ---------------------------------------------------------------------------
+static mpsh_status envupdate_plan_c(int side, mpsh_struct env, mpsh_struct A, mpsh_struct W, mpsh_envupdate* plan,
+                                    mpsh_struct* out) {
+  API_TRY
+  if (!env || !A || !W || !plan || !out) throw Error(ERR_INVALID, "null argument");
+  auto h = std::make_unique<mpsh_envupdate_s>();
+  h->p = envupdate_plan(side, env->s, A->s, W->s);
+  *out = wrap(h->p.outs[0]);
+  *plan = h.release();
+  return MPSH_OK;
+  API_CATCH
+}
+
+mpsh_status mpsh_envupdate_left_plan(mpsh_struct env, mpsh_struct A, mpsh_struct W, mpsh_envupdate* plan,
+                                     mpsh_struct* out) {
+  return envupdate_plan_c(0, env, A, W, plan, out);
+}
+
+mpsh_status mpsh_envupdate_right_plan(mpsh_struct env, mpsh_struct A, mpsh_struct W, mpsh_envupdate* plan,
+                                      mpsh_struct* out) {
+  return envupdate_plan_c(1, env, A, W, plan, out);
+}
+
+size_t mpsh_envupdate_workspace(mpsh_envupdate plan, mpsh_dtype dt) { return env_workspace(plan, dt); }
+
+mpsh_status mpsh_envupdate_stats(mpsh_envupdate plan, int64_t* stats) {
+  API_TRY
+  env_stats(plan, stats);
+  return MPSH_OK;
+  API_CATCH
+}
+
+mpsh_status mpsh_envupdate_execute(mpsh_envupdate plan, mpsh_dtype dt, const void* env, const void* A,
+                                   const void* W, void* out, void* workspace, void* stream) {
+  API_TRY
+  env_check_dtype(dt);
+  if (!plan) throw Error(ERR_INVALID, "null plan");
+  const EnvPlan& p = plan->p;
+  if ((!env && p.ins[0].numel()) || (!A && p.ins[1].numel()) || (!W && p.ins[2].numel()) ||
+      (!out && p.outs[0].numel()) || !workspace)
+    throw Error(ERR_INVALID, "null argument");
+  need_aligned(workspace, "workspace");
+  const EnvDev& d = plan->get();
+  hipStream_t s = (hipStream_t)stream;
+  const int es = elem_size(dt);
+  char* ws = (char*)workspace;
+  const void* ket = A;
+  const void* WL = W;
+  if (p.has_perm[0][0]) {
+    check(launch_copy(d.perm[0][0], es, A, ws + p.perm_off[0][0] * es, s), "env: mirror A");
+    ket = ws + p.perm_off[0][0] * es;
+  }
+  if (p.has_perm[0][1]) {
+    check(launch_copy(d.perm[0][1], es, W, ws + p.perm_off[0][1] * es, s), "env: mirror W");
+    WL = ws + p.perm_off[0][1] * es;
+  }
+  const void* WR = dt == MPSH_F64 ? (const void*)d.wr_f64 : (const void*)d.wr_f32;
+  run_env_core(*plan, d, 0, dt, env, ket, WL, WR, out, ws, s);
+  return MPSH_OK;
+  API_CATCH
+}
+
+void mpsh_envupdate_destroy(mpsh_envupdate plan) { delete plan; }
+
+// ---------------------------------------------------------------------------
+// H_eff adjoint
+// ---------------------------------------------------------------------------
+mpsh_status mpsh_heff_adjoint_plan(mpsh_struct v, mpsh_struct envL, mpsh_struct envR, mpsh_struct WL,
+                                   mpsh_struct WR, mpsh_heff_adjoint* plan, mpsh_struct* gL, mpsh_struct* gR) {
+  API_TRY
+  if (!v || !envL || !envR || !WL || !WR || !plan || !gL || !gR) throw Error(ERR_INVALID, "null argument");
+  auto h = std::make_unique<mpsh_heff_adjoint_s>();
+  h->p = heff_adjoint_plan(v->s, envL->s, envR->s, WL->s, WR->s);
+  *gL = wrap(h->p.outs[0]);
+  *gR = wrap(h->p.outs[1]);
+  *plan = h.release();
+  return MPSH_OK;
+  API_CATCH
+}
+
+size_t mpsh_heff_adjoint_workspace(mpsh_heff_adjoint plan, mpsh_dtype dt) { return env_workspace(plan, dt); }
+
+mpsh_status mpsh_heff_adjoint_stats(mpsh_heff_adjoint plan, int64_t* stats) {
+  API_TRY
+  env_stats(plan, stats);
+  return MPSH_OK;
+  API_CATCH
+}
+
+mpsh_status mpsh_heff_adjoint_execute(mpsh_heff_adjoint plan, mpsh_dtype dt, const void* v, const void* envL,
+                                      const void* envR, const void* WL, const void* WR, void* gL, void* gR,
+                                      void* workspace, void* stream) {
+  API_TRY
+  env_check_dtype(dt);
+  if (!plan) throw Error(ERR_INVALID, "null plan");
+  const EnvPlan& p = plan->p;
+  if ((!v && p.ins[0].numel()) || (!envL && p.ins[1].numel()) || (!envR && p.ins[2].numel()) ||
+      (!WL && p.ins[3].numel()) || (!WR && p.ins[4].numel()) || (!gL && p.outs[0].numel()) ||
+      (!gR && p.outs[1].numel()) || !workspace)
+    throw Error(ERR_INVALID, "null argument");
+  need_aligned(workspace, "workspace");
+  const EnvDev& d = plan->get();
+  hipStream_t s = (hipStream_t)stream;
+  const int es = elem_size(dt);
+  char* ws = (char*)workspace;
+  // dE/denvR: the two-site core on the operands as given
+  run_env_core(*plan, d, 0, dt, envL, v, WL, WR, gR, ws, s);
+  // dE/denvL: the two-site core of the mirror image
+  const void* src[3] = {v, WR, WL};
+  void* mir[3];
+  for (int k = 0; k < 3; ++k) {
+    mir[k] = ws + p.perm_off[1][k] * es;
+    check(launch_copy(d.perm[1][k], es, src[k], mir[k], s), "adjoint: mirror operand");
+  }
+  run_env_core(*plan, d, 1, dt, envR, mir[0], mir[1], mir[2], gL, ws, s);
+  return MPSH_OK;
+  API_CATCH
+}
+
+void mpsh_heff_adjoint_destroy(mpsh_heff_adjoint plan) { delete plan; }
 
 // ---------------------------------------------------------------------------
 // vector ops

@@ -990,8 +990,8 @@ __global__ __launch_bounds__(256) void mpo_combine(const MpoSig* __restrict__ si
 // row C-1 (a real row) and meet zero W columns, so the loads are unconditional.
 template <typename T, int CM, int CMAX>
 __device__ __forceinline__ void mpo_pixel(const T* Ws, const int64_t* ib, const int64_t* ins, const int64_t* im,
-                                          const int64_t* ob, const int64_t* om, int R, int64_t m, int64_t n,
-                                          bool live, const T* __restrict__ T1, T* __restrict__ T3) {
+                                          const int64_t* ob, const int64_t* on, const int64_t* om, int R, int64_t m,
+                                          int64_t n, bool live, const T* __restrict__ T1, T* __restrict__ T3) {
   T v[CM];
 #pragma unroll
   for (int c = 0; c < CM; ++c) v[c] = T1[ib[c] + n * ins[c] + m * im[c]];
@@ -1001,11 +1001,12 @@ __device__ __forceinline__ void mpo_pixel(const T* Ws, const int64_t* ib, const 
     T acc = 0;
 #pragma unroll
     for (int c = 0; c < CM; ++c) acc += wr[c] * v[c];
-    T3[ob[r] + n + m * om[r]] = acc;
+    T3[ob[r] + n * on[r] + m * om[r]] = acc;
   }
 }
 
-// out[r][pix] = sum_c W[r][c] * in[c][pix], pix = n + dR*m; one thread per pixel,
+// out[r][pix] = sum_c W[r][c] * in[c][pix], pix = n + dR*m (output row r of pixel
+// (m, n) at ob[r] + n*on[r] + m*om[r]); one thread per pixel,
 // 256 pixels of one (bra_l, ket_r) group per work group.  CMAX bounds the
 // plan's channel count (LDS layout); each group runs the smallest channel
 // class (multiple of 4) that holds its own C.
@@ -1018,7 +1019,7 @@ __global__ __launch_bounds__(256) void mpo_apply(const MpoSig* __restrict__ sigs
                                                  const T* __restrict__ T1, T* __restrict__ T3) {
   __shared__ __attribute__((aligned(16))) T Ws[64 * CMAX];
   __shared__ int64_t ib[CMAX], ins[CMAX], im[CMAX];
-  __shared__ int64_t ob[64], om[64];
+  __shared__ int64_t ob[64], on[64], om[64];
   const MpoWork wk = work[blockIdx.x];
   const MpoGroup g = groups[wk.group];
   const MpoSig sg = sigs[g.sig];
@@ -1038,6 +1039,7 @@ __global__ __launch_bounds__(256) void mpo_apply(const MpoSig* __restrict__ sigs
   for (int i = threadIdx.x; i < R; i += blockDim.x) {
     const MpoOutRow q = outrows[g.out0 + i];
     ob[i] = q.base;
+    on[i] = q.nstride;
     om[i] = q.mstride;
   }
   __syncthreads();
@@ -1047,13 +1049,13 @@ __global__ __launch_bounds__(256) void mpo_apply(const MpoSig* __restrict__ sigs
   const int64_t m = p / g.dR, n = p - m * g.dR;
   if (C <= 0) {  // a group no input type reaches: its T3 cells are zero
     if (live)
-      for (int r = 0; r < R; ++r) T3[ob[r] + n + m * om[r]] = T(0);
+      for (int r = 0; r < R; ++r) T3[ob[r] + n * on[r] + m * om[r]] = T(0);
     return;
   }
   const int cls = (C + 3) / 4;  // uniform over the block
 #define MPSH_PIX(K)                                                                             \
   case K:                                                                                       \
-    if constexpr (4 * K <= CMAX) mpo_pixel<T, 4 * K, CMAX>(Ws, ib, ins, im, ob, om, R, m, n, live, T1, T3); \
+    if constexpr (4 * K <= CMAX) mpo_pixel<T, 4 * K, CMAX>(Ws, ib, ins, im, ob, on, om, R, m, n, live, T1, T3); \
     break;
   switch (cls) {
     MPSH_PIX(1)
@@ -1065,7 +1067,7 @@ __global__ __launch_bounds__(256) void mpo_apply(const MpoSig* __restrict__ sigs
     MPSH_PIX(7)
     MPSH_PIX(8)
     default:
-      if constexpr (CMAX == 64) mpo_pixel<T, 64, CMAX>(Ws, ib, ins, im, ob, om, R, m, n, live, T1, T3);
+      if constexpr (CMAX == 64) mpo_pixel<T, 64, CMAX>(Ws, ib, ins, im, ob, on, om, R, m, n, live, T1, T3);
       break;
   }
 #undef MPSH_PIX

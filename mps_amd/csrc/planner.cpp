@@ -871,189 +871,118 @@ RelegPlan symmatrix_releg(const Struct& A, const std::vector<int>& rows, const s
 }
 
 // ---------------------------------------------------------------------------
-// Fused H_eff plan
+// Fused H_eff plan and the environment plans built on its first stages
 // ---------------------------------------------------------------------------
 typedef std::vector<int64_t> Key;
 
 static void need_pair(const Leg& a, const Leg& b, const char* what) {
   if (!isdual_pair(a, b)) throw Error(ERR_SPACE_MISMATCH, std::string("not contractible! ") + what);
 }
+static void need_same(const Leg& a, const Leg& b, const char* what) {
+  if (a != b) throw Error(ERR_SPACE_MISMATCH, std::string("not contractible! ") + what);
+}
 
-HeffPlan heff_plan(const Struct& v, const Struct& envL, const Struct& envR, const Struct& WL, const Struct& WR,
-                   const HeffTiles& tiles) {
-  if (v.rank() != 4 || envL.rank() != 3 || envR.rank() != 3 || WL.rank() != 4 || WR.rank() != 4)
-    throw Error(ERR_INVALID, "heff: ranks must be v 4, envL 3, envR 3, WL 4, WR 4");
-  need_pair(envL.legs[0], v.legs[0], "(envL ket, v left)");
-  need_pair(envL.legs[1], WL.legs[0], "(envL mpo, WL left)");
-  need_pair(v.legs[1], WL.legs[3], "(v p1, WL p_in)");
-  need_pair(WL.legs[2], WR.legs[0], "(WL right, WR left)");
-  need_pair(v.legs[2], WR.legs[3], "(v p2, WR p_in)");
-  need_pair(WR.legs[2], envR.legs[1], "(WR right, envR mpo)");
-  need_pair(v.legs[3], envR.legs[0], "(v right, envR ket)");
-  HeffPlan H;
-  H.v = v;
-  H.envL = envL;
-  H.envR = envR;
-  H.WL = WL;
-  H.WR = WR;
-  const int64_t charge = envL.charge + v.charge + WL.charge + WR.charge + envR.charge;
-  H.out = make_struct(charge, {envL.legs[2], WL.legs[1], WR.legs[1], envR.legs[2]});
+static int64_t up4(int64_t x) { return (x + 3) / 4 * 4; }
+static int64_t up64(int64_t x) { return (x + 63) / 64 * 64; }  // 256-byte (FP32) region starts
 
-  // reference-order cost from the reference's own four contracts
-  {
-    ContractCost P1 = contract_cost(envL, {-1, 2, 1}, v, {-1, 3, 4, 5});
-    ContractCost P2 = contract_cost(P1.C, {1, -1, -2, 4, 5}, WL, {-1, 2, 3, -2});
-    ContractCost P3 = contract_cost(P2.C, {1, 2, -1, -2, 5}, WR, {-1, 3, 4, -2});
-    ContractCost P4 = contract_cost(P3.C, {1, 2, 3, -1, -2}, envR, {-2, -1, 4});
-    const ContractCost* Ps[4] = {&P1, &P2, &P3, &P4};
-    const Struct* ins[4][2] = {{&envL, &v}, {&P1.C, &WL}, {&P2.C, &WR}, {&P3.C, &envR}};
-    for (int s = 0; s < 4; ++s) {
-      H.step_flops[s] = Ps[s]->flops;
-      H.step_elems[s] = ins[s][0]->numel() + ins[s][1]->numel() + Ps[s]->C.numel();
-      H.ref_flops += H.step_flops[s];
-      H.ref_elems += H.step_elems[s];
+namespace {
+// One stored block pair of the step-1 GEMM output T1_k = v_k^T envL_k: key (b, w, s1, s2, r).
+struct T1B {
+  Key key;
+  int64_t base, rowoff, coloff, Mf, db, Dw, d1, d2, dR;
+};
+// One T3 cell (b, s1o, s2o, x, r): the MPO pass writes output row rr = jx + Dx*(i1o + d1o*i2o)
+// of pixel (m, n) (m the bra_l index, n the ket_r index) at
+//   base + jx*sx + (i1o + d1o*i2o)*so + n*sn + m*sm.
+struct Cell {
+  int64_t base, sx, so, sn, sm, db, d1o, d2o, Dx, dR;
+};
+}  // namespace
+
+// ---- step 1: one fused GEMM per ket charge k (the reference's fused sizes),
+// T1_k = [v blocks (k,.)]^T x [envL blocks (k,.)], stored column-major (ld M_k
+// rounded up to 32).  Operand rows (v block columns, envL block columns) are
+// packed into K-contiguous rows of stride Kp = roundup(dk, 4): v per matvec,
+// envL per bond.  Fills H.g1, H.pack_v, H.pack_envL, H.t1_elems, H.vp_elems,
+// H.el_elems and the T1 block list (sorted by (b, r, w, s1, s2)).
+static void step1_layout(Step13& H, const Struct& v, const Struct& envL, std::vector<T1B>& t1) {
+  std::map<int64_t, std::vector<int64_t>> vby, eby;
+  for (int64_t x = 0; x < v.nblocks; ++x) vby[v.sector(x)[0]].push_back(x);
+  for (int64_t e = 0; e < envL.nblocks; ++e) eby[envL.sector(e)[0]].push_back(e);
+  int64_t off = 0;
+  H.vp_elems = H.el_elems = 0;
+  for (auto& kv : eby) {
+    auto itv = vby.find(kv.first);
+    if (itv == vby.end()) continue;
+    const std::vector<int64_t>& X = itv->second;
+    const std::vector<int64_t>& E = kv.second;
+    const int64_t dk = envL.size(E[0])[0], Kp = up4(dk);
+    std::vector<int64_t> rowoff(X.size()), coloff(E.size());
+    int64_t Mf = 0, Nf = 0;
+    for (size_t i = 0; i < X.size(); ++i) {
+      const int64_t* z = v.size(X[i]);
+      rowoff[i] = Mf;
+      Mf += z[1] * z[2] * z[3];
     }
-    if (!same_struct(P4.C, H.out) || P4.C.sectors != H.out.sectors)
-      throw Error(ERR_SECTOR_MISMATCH, "heff: output structure differs from the contraction chain");
-  }
-
-  // ---- step 1: one fused GEMM per ket charge k (the reference's fused sizes),
-  // T1_k = [v blocks (k,.)]^T x [envL blocks (k,.)], stored column-major (ld M_k
-  // rounded up to 32).
-  // Operand rows (v block columns, envL block columns) are packed into
-  // K-contiguous rows of stride Kp = roundup(dk, 4): v per matvec, envL per bond.
-  struct T1B {
-    Key key;  // (b, w, s1, s2, r)
-    int64_t base, rowoff, coloff, Mf, db, Dw, d1, d2, dR;
-  };
-  auto up4 = [](int64_t x) { return (x + 3) / 4 * 4; };
-  std::vector<T1B> t1;
-  int64_t vp_elems = 0, el_elems = 0;
-  {
-    std::map<int64_t, std::vector<int64_t>> vby, eby;
-    for (int64_t x = 0; x < v.nblocks; ++x) vby[v.sector(x)[0]].push_back(x);
-    for (int64_t e = 0; e < envL.nblocks; ++e) eby[envL.sector(e)[0]].push_back(e);
-    int64_t off = 0;
-    for (auto& kv : eby) {
-      auto itv = vby.find(kv.first);
-      if (itv == vby.end()) continue;
-      const std::vector<int64_t>& X = itv->second;
-      const std::vector<int64_t>& E = kv.second;
-      const int64_t dk = envL.size(E[0])[0], Kp = up4(dk);
-      std::vector<int64_t> rowoff(X.size()), coloff(E.size());
-      int64_t Mf = 0, Nf = 0;
-      for (size_t i = 0; i < X.size(); ++i) {
-        const int64_t* z = v.size(X[i]);
-        rowoff[i] = Mf;
-        Mf += z[1] * z[2] * z[3];
-      }
+    for (size_t j = 0; j < E.size(); ++j) {
+      const int64_t* z = envL.size(E[j]);
+      coloff[j] = Nf;
+      Nf += z[1] * z[2];
+    }
+    const int64_t vbase = H.vp_elems, ebase = H.el_elems;
+    H.vp_elems += Mf * Kp;
+    H.el_elems += Nf * Kp;
+    // v block (dk, d1, d2, dR) -> packed rows ordered ir + dR*(i1 + d1*i2): the
+    // right-bond index runs fastest, so the MPO pass reads T1 with unit stride
+    for (size_t i = 0; i < X.size(); ++i) {
+      const int64_t* z = v.size(X[i]);
+      const int64_t d1 = z[1], d2 = z[2], dR = z[3];
+      const int64_t dims[4] = {dk, d1, d2, dR}, sst[4] = {1, dk, dk * d1, dk * d1 * d2},
+                    dst[4] = {1, dR * Kp, dR * d1 * Kp, Kp};
+      H.pack_v.add(4, dims, sst, dst, v.offsets[X[i]], vbase + rowoff[i] * Kp);
+    }
+    for (size_t j = 0; j < E.size(); ++j) {
+      const int64_t* z = envL.size(E[j]);
+      const int64_t dims[2] = {dk, z[1] * z[2]}, sst[2] = {1, dk}, dst[2] = {1, Kp};
+      H.pack_envL.add(2, dims, sst, dst, envL.offsets[E[j]], ebase + coloff[j] * Kp);
+    }
+    // columns of T1_k start on 128-byte boundaries (ld = Mf rounded up to 32)
+    const int64_t Mld = (Mf + 31) / 32 * 32;
+    H.g1.add_prob(Mf, Nf);
+    H.g1.add_crows_affine(off, Mld, Mf);
+    H.g1.add_seg(dk, vbase, Kp, ebase, Kp);
+    for (size_t i = 0; i < X.size(); ++i)
       for (size_t j = 0; j < E.size(); ++j) {
-        const int64_t* z = envL.size(E[j]);
-        coloff[j] = Nf;
-        Nf += z[1] * z[2];
+        const int64_t* xs = v.sector(X[i]);
+        const int64_t* es = envL.sector(E[j]);
+        T1B t;
+        t.key = {es[2], es[1], xs[1], xs[2], xs[3]};
+        t.base = off;
+        t.rowoff = rowoff[i];
+        t.coloff = coloff[j];
+        t.Mf = Mld;
+        t.db = envL.size(E[j])[2];
+        t.Dw = envL.size(E[j])[1];
+        t.d1 = v.size(X[i])[1];
+        t.d2 = v.size(X[i])[2];
+        t.dR = v.size(X[i])[3];
+        t1.push_back(t);
       }
-      const int64_t vbase = vp_elems, ebase = el_elems;
-      vp_elems += Mf * Kp;
-      el_elems += Nf * Kp;
-      // v block (dk, d1, d2, dR) -> packed rows ordered ir + dR*(i1 + d1*i2): the
-      // right-bond index runs fastest, so the MPO pass reads T1 with unit stride
-      for (size_t i = 0; i < X.size(); ++i) {
-        const int64_t* z = v.size(X[i]);
-        const int64_t d1 = z[1], d2 = z[2], dR = z[3];
-        const int64_t dims[4] = {dk, d1, d2, dR}, sst[4] = {1, dk, dk * d1, dk * d1 * d2},
-                      dst[4] = {1, dR * Kp, dR * d1 * Kp, Kp};
-        H.pack_v.add(4, dims, sst, dst, v.offsets[X[i]], vbase + rowoff[i] * Kp);
-      }
-      for (size_t j = 0; j < E.size(); ++j) {
-        const int64_t* z = envL.size(E[j]);
-        const int64_t dims[2] = {dk, z[1] * z[2]}, sst[2] = {1, dk}, dst[2] = {1, Kp};
-        H.pack_envL.add(2, dims, sst, dst, envL.offsets[E[j]], ebase + coloff[j] * Kp);
-      }
-      // columns of T1_k start on 128-byte boundaries (ld = Mf rounded up to 32)
-      const int64_t Mld = (Mf + 31) / 32 * 32;
-      H.g1.add_prob(Mf, Nf);
-      H.g1.add_crows_affine(off, Mld, Mf);
-      H.g1.add_seg(dk, vbase, Kp, ebase, Kp);
-      for (size_t i = 0; i < X.size(); ++i)
-        for (size_t j = 0; j < E.size(); ++j) {
-          const int64_t* xs = v.sector(X[i]);
-          const int64_t* es = envL.sector(E[j]);
-          T1B t;
-          t.key = {es[2], es[1], xs[1], xs[2], xs[3]};
-          t.base = off;
-          t.rowoff = rowoff[i];
-          t.coloff = coloff[j];
-          t.Mf = Mld;
-          t.db = envL.size(E[j])[2];
-          t.Dw = envL.size(E[j])[1];
-          t.d1 = v.size(X[i])[1];
-          t.d2 = v.size(X[i])[2];
-          t.dR = v.size(X[i])[3];
-          t1.push_back(t);
-        }
-      off += Mld * Nf;
-    }
-    H.t1_elems = off;
-    std::stable_sort(t1.begin(), t1.end(), [](const T1B& a, const T1B& b) {
-      return std::make_tuple(a.key[0], a.key[4], a.key[1], a.key[2], a.key[3]) <
-             std::make_tuple(b.key[0], b.key[4], b.key[1], b.key[2], b.key[3]);
-    });
+    off += Mld * Nf;
   }
+  H.t1_elems = off;
+  std::stable_sort(t1.begin(), t1.end(), [](const T1B& a, const T1B& b) {
+    return std::make_tuple(a.key[0], a.key[4], a.key[1], a.key[2], a.key[3]) <
+           std::make_tuple(b.key[0], b.key[4], b.key[1], b.key[2], b.key[3]);
+  });
+}
 
-  // ---- step 4 layout: one fused GEMM per output ket charge r'; rows = the out
-  // blocks (b, s1o, s2o, r'), K segments = envR blocks (r, x, r').  T3 region of
-  // (r', f) is K_f x M_f, K-contiguous; cell (o, f) is T3 block (b, s1o, s2o, x, r).
-  struct Cell {
-    int64_t base, rowoff, Kf, Kfp, db, d1o, d2o, Dx, dR;  // T3 rows of stride Kfp = roundup(Kf, 4)
-  };
-  int64_t er_elems = 0;
-  std::map<Key, Cell> cells;  // key (b, s1o, s2o, x, r)
-  {
-    std::map<int64_t, std::vector<int64_t>> oby, fby;
-    for (int64_t o = 0; o < H.out.nblocks; ++o) oby[H.out.sector(o)[3]].push_back(o);
-    for (int64_t f = 0; f < envR.nblocks; ++f) fby[envR.sector(f)[2]].push_back(f);
-    int64_t off = 0;
-    for (auto& kv : oby) {
-      const std::vector<int64_t>& O = kv.second;
-      std::vector<int64_t> rowoff(O.size());
-      int64_t Mf = 0;
-      for (size_t i = 0; i < O.size(); ++i) {
-        rowoff[i] = Mf;
-        const int64_t* z = H.out.size(O[i]);
-        Mf += z[0] * z[1] * z[2];
-      }
-      const int64_t N = H.out.size(O[0])[3];
-      H.g4.add_prob(Mf, N);
-      for (size_t i = 0; i < O.size(); ++i) {
-        const int64_t* z = H.out.size(O[i]);
-        const int64_t Mo = z[0] * z[1] * z[2];
-        for (int64_t m = 0; m < Mo; ++m) H.g4.add_crow(H.out.offsets[O[i]] + m, Mo);
-      }
-      auto itf = fby.find(kv.first);
-      if (itf == fby.end()) continue;
-      for (int64_t f : itf->second) {
-        const int64_t* fs = envR.sector(f);
-        const int64_t dR = envR.size(f)[0], Dx = envR.size(f)[1];
-        const int64_t Kf = dR * Dx, Kfp = up4(Kf);
-        const int64_t rbase = er_elems;
-        er_elems += Kfp * N;
-        {
-          const int64_t dims[2] = {Kf, N}, sst[2] = {1, Kf}, dst[2] = {1, Kfp};
-          H.pack_envR.add(2, dims, sst, dst, envR.offsets[f], rbase);
-        }
-        H.g4.add_seg(Kf, off, Kfp, rbase, Kfp);
-        for (size_t i = 0; i < O.size(); ++i) {
-          const int64_t* os = H.out.sector(O[i]);
-          const int64_t* z = H.out.size(O[i]);
-          cells[{os[0], os[1], os[2], fs[1], fs[0]}] = Cell{off, rowoff[i], Kf, Kfp, z[0], z[1], z[2], Dx, dR};
-        }
-        off += Kfp * Mf;
-      }
-    }
-    H.t3_elems = off;
-  }
-
-  // ---- steps 2+3: pixel groups (b, r); out types = the T3 cells of (b, r)
+// ---- steps 2+3: pixel groups (b, r); their out types are the T3 cells of (b, r).
+// Per group the combined two-site MPO W_group = WL (x) WR over the middle MPO bond
+// maps the C input channels (w, s1, s2) of T1 to the R output rows (s1o, s2o, x)
+// of the cells; groups with the same channel signature share one W_group.
+static void mpo_layout(Step13& H, const Struct& WL, const Struct& WR, const std::vector<T1B>& t1,
+                       const std::map<Key, Cell>& cells) {
   auto wl_by = [&]() {
     std::map<std::pair<int64_t, int64_t>, std::vector<int64_t>> m;  // (w, s1) -> WL blocks
     for (int64_t b = 0; b < WL.nblocks; ++b) m[{WL.sector(b)[0], WL.sector(b)[3]}].push_back(b);
@@ -1084,7 +1013,8 @@ HeffPlan heff_plan(const Struct& v, const Struct& envL, const Struct& envR, cons
       gi0 = itr->second.first;
       gi1 = itr->second.second;
     }
-    const Cell& c0 = cells[{b, outtypes[0][0], outtypes[0][1], outtypes[0][2], r}];
+    auto cell = [&](const Key& ot) -> const Cell& { return cells.at({b, ot[0], ot[1], ot[2], r}); };
+    const Cell& c0 = cell(outtypes[0]);
     const int64_t db = c0.db, dR = c0.dR;
     Key sigkey;
     for (size_t i = gi0; i < gi1; ++i) sigkey.insert(sigkey.end(), {t1[i].key[1], t1[i].key[2], t1[i].key[3]});
@@ -1098,7 +1028,7 @@ HeffPlan heff_plan(const Struct& v, const Struct& envL, const Struct& envR, cons
     }
     for (auto& ot : outtypes) {
       row0s.push_back(R);
-      const Cell& cc = cells[{b, ot[0], ot[1], ot[2], r}];
+      const Cell& cc = cell(ot);
       R += cc.Dx * cc.d1o * cc.d2o;
     }
     int32_t sig;
@@ -1112,7 +1042,7 @@ HeffPlan heff_plan(const Struct& v, const Struct& envL, const Struct& envR, cons
       H.maxC = std::max<int>(H.maxC, (int)C);
       for (size_t oi = 0; oi < outtypes.size(); ++oi) {
         const Key& ot = outtypes[oi];
-        const Cell& cc = cells[{b, ot[0], ot[1], ot[2], r}];
+        const Cell& cc = cell(ot);
         for (size_t i = gi0; i < gi1; ++i) {
           const Key& k = t1[i].key;
           MpoTerm tm{};
@@ -1157,7 +1087,7 @@ HeffPlan heff_plan(const Struct& v, const Struct& envL, const Struct& envR, cons
     g.npix = db * dR;
     for (size_t i = gi0; i < gi1; ++i) {
       const T1B& t = t1[i];
-      if (t.db != db || t.dR != dR) throw Error(ERR_SECTOR_MISMATCH, "heff: pixel group dims differ");
+      if (t.db != db || t.dR != dR) throw Error(ERR_SECTOR_MISMATCH, "mpo pass: pixel group dims differ");
       // column order c = jw + Dw*(i1 + d1*i2); T1 element (n + dR (i1 + d1 i2), jw + Dw m)
       for (int64_t c = 0; c < t.Dw * t.d1 * t.d2; ++c) {
         const int64_t jw = c % t.Dw, i1 = (c / t.Dw) % t.d1, i2 = c / (t.Dw * t.d1);
@@ -1166,12 +1096,12 @@ HeffPlan heff_plan(const Struct& v, const Struct& envL, const Struct& envR, cons
       }
     }
     for (auto& ot : outtypes) {
-      const Cell& cc = cells[{b, ot[0], ot[1], ot[2], r}];
-      // row order r = jx + Dx*(i1o + d1o*i2o); T3 element (n + dR jx, rowoff + mb + db (i1o + d1o i2o))
+      const Cell& cc = cell(ot);
+      if (cc.db != db || cc.dR != dR) throw Error(ERR_SECTOR_MISMATCH, "mpo pass: cell dims differ");
+      // row order rr = jx + Dx*(i1o + d1o*i2o)
       for (int64_t rr = 0; rr < cc.Dx * cc.d1o * cc.d2o; ++rr) {
-        const int64_t jx = rr % cc.Dx, i1 = (rr / cc.Dx) % cc.d1o, i2 = rr / (cc.Dx * cc.d1o);
-        H.outrows.push_back(
-            MpoOutRow{cc.base + dR * jx + cc.Kfp * (cc.rowoff + db * (i1 + cc.d1o * i2)), cc.Kfp});
+        const int64_t jx = rr % cc.Dx, io = rr / cc.Dx;
+        H.outrows.push_back(MpoOutRow{cc.base + cc.sx * jx + cc.so * io, cc.sn, cc.sm});
       }
     }
     const int32_t gidx = (int32_t)H.groups.size();
@@ -1180,6 +1110,107 @@ HeffPlan heff_plan(const Struct& v, const Struct& envL, const Struct& envR, cons
     H.mpo_flops += 2 * R * C * g.npix;
   }
   H.w_elems = w_off;
+}
+
+static void check_two_site(const Struct& v, const Struct& envL, const Struct& WL, const Struct& WR) {
+  if (v.rank() != 4 || envL.rank() != 3 || WL.rank() != 4 || WR.rank() != 4)
+    throw Error(ERR_INVALID, "ranks must be v 4, env 3, WL 4, WR 4");
+  need_pair(envL.legs[0], v.legs[0], "(envL ket, v left)");
+  need_pair(envL.legs[1], WL.legs[0], "(envL mpo, WL left)");
+  need_pair(v.legs[1], WL.legs[3], "(v p1, WL p_in)");
+  need_pair(WL.legs[2], WR.legs[0], "(WL right, WR left)");
+  need_pair(v.legs[2], WR.legs[3], "(v p2, WR p_in)");
+}
+
+HeffPlan heff_plan(const Struct& v, const Struct& envL, const Struct& envR, const Struct& WL, const Struct& WR,
+                   const HeffTiles& tiles) {
+  if (envR.rank() != 3) throw Error(ERR_INVALID, "heff: ranks must be v 4, envL 3, envR 3, WL 4, WR 4");
+  check_two_site(v, envL, WL, WR);
+  need_pair(WR.legs[2], envR.legs[1], "(WR right, envR mpo)");
+  need_pair(v.legs[3], envR.legs[0], "(v right, envR ket)");
+  HeffPlan H;
+  H.v = v;
+  H.envL = envL;
+  H.envR = envR;
+  H.WL = WL;
+  H.WR = WR;
+  const int64_t charge = envL.charge + v.charge + WL.charge + WR.charge + envR.charge;
+  H.out = make_struct(charge, {envL.legs[2], WL.legs[1], WR.legs[1], envR.legs[2]});
+
+  // reference-order cost from the reference's own four contracts
+  {
+    ContractCost P1 = contract_cost(envL, {-1, 2, 1}, v, {-1, 3, 4, 5});
+    ContractCost P2 = contract_cost(P1.C, {1, -1, -2, 4, 5}, WL, {-1, 2, 3, -2});
+    ContractCost P3 = contract_cost(P2.C, {1, 2, -1, -2, 5}, WR, {-1, 3, 4, -2});
+    ContractCost P4 = contract_cost(P3.C, {1, 2, 3, -1, -2}, envR, {-2, -1, 4});
+    const ContractCost* Ps[4] = {&P1, &P2, &P3, &P4};
+    const Struct* ins[4][2] = {{&envL, &v}, {&P1.C, &WL}, {&P2.C, &WR}, {&P3.C, &envR}};
+    for (int s = 0; s < 4; ++s) {
+      H.step_flops[s] = Ps[s]->flops;
+      H.step_elems[s] = ins[s][0]->numel() + ins[s][1]->numel() + Ps[s]->C.numel();
+      H.ref_flops += H.step_flops[s];
+      H.ref_elems += H.step_elems[s];
+    }
+    if (!same_struct(P4.C, H.out) || P4.C.sectors != H.out.sectors)
+      throw Error(ERR_SECTOR_MISMATCH, "heff: output structure differs from the contraction chain");
+  }
+
+  std::vector<T1B> t1;
+  step1_layout(H, v, envL, t1);
+
+  // ---- step 4 layout: one fused GEMM per output ket charge r'; rows = the out
+  // blocks (b, s1o, s2o, r'), K segments = envR blocks (r, x, r').  T3 region of
+  // (r', f) is K_f x M_f, K-contiguous (row stride Kfp = roundup(K_f, 4)); cell
+  // (o, f) is T3 block (b, s1o, s2o, x, r): element (n + dR jx, rowoff + m + db (i1o + d1o i2o)).
+  int64_t er_elems = 0;
+  std::map<Key, Cell> cells;  // key (b, s1o, s2o, x, r)
+  {
+    std::map<int64_t, std::vector<int64_t>> oby, fby;
+    for (int64_t o = 0; o < H.out.nblocks; ++o) oby[H.out.sector(o)[3]].push_back(o);
+    for (int64_t f = 0; f < envR.nblocks; ++f) fby[envR.sector(f)[2]].push_back(f);
+    int64_t off = 0;
+    for (auto& kv : oby) {
+      const std::vector<int64_t>& O = kv.second;
+      std::vector<int64_t> rowoff(O.size());
+      int64_t Mf = 0;
+      for (size_t i = 0; i < O.size(); ++i) {
+        rowoff[i] = Mf;
+        const int64_t* z = H.out.size(O[i]);
+        Mf += z[0] * z[1] * z[2];
+      }
+      const int64_t N = H.out.size(O[0])[3];
+      H.g4.add_prob(Mf, N);
+      for (size_t i = 0; i < O.size(); ++i) {
+        const int64_t* z = H.out.size(O[i]);
+        const int64_t Mo = z[0] * z[1] * z[2];
+        for (int64_t m = 0; m < Mo; ++m) H.g4.add_crow(H.out.offsets[O[i]] + m, Mo);
+      }
+      auto itf = fby.find(kv.first);
+      if (itf == fby.end()) continue;
+      for (int64_t f : itf->second) {
+        const int64_t* fs = envR.sector(f);
+        const int64_t dR = envR.size(f)[0], Dx = envR.size(f)[1];
+        const int64_t Kf = dR * Dx, Kfp = up4(Kf);
+        const int64_t rbase = er_elems;
+        er_elems += Kfp * N;
+        {
+          const int64_t dims[2] = {Kf, N}, sst[2] = {1, Kf}, dst[2] = {1, Kfp};
+          H.pack_envR.add(2, dims, sst, dst, envR.offsets[f], rbase);
+        }
+        H.g4.add_seg(Kf, off, Kfp, rbase, Kfp);
+        for (size_t i = 0; i < O.size(); ++i) {
+          const int64_t* os = H.out.sector(O[i]);
+          const int64_t* z = H.out.size(O[i]);
+          cells[{os[0], os[1], os[2], fs[1], fs[0]}] =
+              Cell{off + Kfp * rowoff[i], dR, Kfp * z[0], 1, Kfp, z[0], z[1], z[2], Dx, dR};
+        }
+        off += Kfp * Mf;
+      }
+    }
+    H.t3_elems = off;
+  }
+  mpo_layout(H, WL, WR, t1, cells);
+
   // kernel variants and split-K choices (step 1 is never split: K = one bond charge)
   auto cfg = [](GemmPlan& g, const TileCfg& c) {
     g.BM = c.BM;
@@ -1203,13 +1234,12 @@ HeffPlan heff_plan(const Struct& v, const Struct& envL, const Struct& envR, cons
   H.pack_envR.finalize();
   // prepared operator: [envL_p | envR_p | W]
   H.el_off = 0;
-  H.er_off = up4(el_elems);
+  H.er_off = up4(H.el_elems);
   H.pw_off = up4(H.er_off + er_elems);
   H.prep_elems = H.pw_off + H.w_elems;
   // per-matvec workspace: [v_p | T1 | T3 | partials]
   H.vp_off = 0;
-  auto up64 = [](int64_t x) { return (x + 63) / 64 * 64; };  // 256-byte (FP32) region starts
-  H.t1_off = up64(vp_elems);
+  H.t1_off = up64(H.vp_elems);
   H.t3_off = up64(H.t1_off + H.t1_elems);
   H.part_off = up64(H.t3_off + H.t3_elems);
   // partials sized for the largest split-K layout any tuning candidate may pick
@@ -1221,6 +1251,236 @@ HeffPlan heff_plan(const Struct& v, const Struct& envL, const Struct& envR, cons
     }
   H.ws_elems = H.part_off + part;
   return H;
+}
+
+// ---------------------------------------------------------------------------
+// Environment plans: out[x, y, x'] = sum env[a, m, b] K[a, p1, p2, x] WL WR K[b, q1, q2, x']
+// (the left update of src/apply.jl:49-58 over one or two sites), computed as the
+// H_eff's step 1 and MPO pass followed by one grouped GEMM over the bra side.
+// ---------------------------------------------------------------------------
+Struct dual_struct(const Struct& A) {
+  std::vector<Leg> l;
+  for (const Leg& x : A.legs) l.push_back(x.dual());
+  return make_struct(-A.charge, l);
+}
+
+static Leg trivial_leg(bool dual) { return make_leg(dual, {{0, 1}}); }
+
+// A rank-3 site tensor (l, p, r) as a rank-4 one (l, p, t, r) with a trivial leg t
+// (charge 0, dim 1): the sectors keep their order and the blocks their layout.
+static Struct with_trivial_leg(const Struct& A) {
+  Struct S = make_struct(A.charge, {A.legs[0], A.legs[1], trivial_leg(false), A.legs[2]});
+  if (S.nblocks != A.nblocks || S.offsets != A.offsets)
+    throw Error(ERR_SECTOR_MISMATCH, "trivial leg changed the block layout");
+  return S;
+}
+
+// identity MPO tensor (W.legs[2]^dual, t, W.legs[2], t^dual) continuing a one-site MPO W
+static Struct identity_mpo(const Struct& W, std::vector<double>& data) {
+  Struct I = make_struct(0, {W.legs[2].dual(), trivial_leg(false), W.legs[2], trivial_leg(true)});
+  data.assign(I.numel(), 0.0);
+  for (int64_t b = 0; b < I.nblocks; ++b) {
+    const int64_t* z = I.size(b);
+    if (I.sector(b)[0] != I.sector(b)[2] || z[0] != z[2]) throw Error(ERR_SECTOR_MISMATCH, "identity MPO block");
+    for (int64_t i = 0; i < z[0]; ++i) data[I.offsets[b] + i + z[0] * i] = 1.0;
+  }
+  return I;
+}
+
+static EnvCore env_core(const Struct& env, const Struct& ket, const Struct& WL, const Struct& WR) {
+  check_two_site(ket, env, WL, WR);
+  // the bra is the ket with dual legs: its left, p1 and p2 legs must carry the
+  // env's third leg and the MPO output legs (src/apply.jl:56, `dual(mat)`)
+  need_same(env.legs[2], ket.legs[0], "(env bra, bra left)");
+  need_same(WL.legs[1], ket.legs[1], "(WL p_out, bra p1)");
+  need_same(WR.legs[1], ket.legs[2], "(WR p_out, bra p2)");
+  EnvCore H;
+  H.env = env;
+  H.ket = ket;
+  H.WL = WL;
+  H.WR = WR;
+  H.out = make_struct(env.charge + WL.charge + WR.charge, {ket.legs[3], WR.legs[2], ket.legs[3].dual()});
+  std::vector<T1B> t1;
+  step1_layout(H, ket, env, t1);
+
+  // ---- final GEMM: one problem per output block (r, x, y): rows m = ir + dR*ix,
+  // columns the bra's right index iy, K = the bra-left side (b, q1, q2) of every
+  // bra block (b, q1, q2, y), concatenated in sector order.  T3 region of charge y
+  // is M_y x K_y with K contiguous (stride Kp_y = roundup(K_y, 4)); the bra is
+  // packed into rows of the same K order.
+  std::map<int64_t, std::vector<int64_t>> oby, kby;  // by y charge: out blocks, bra (= ket) blocks
+  for (int64_t o = 0; o < H.out.nblocks; ++o) oby[H.out.sector(o)[2]].push_back(o);
+  for (int64_t x = 0; x < ket.nblocks; ++x) kby[ket.sector(x)[3]].push_back(x);
+  std::map<Key, Cell> cells;
+  int64_t t3 = 0, bp = 0;
+  for (auto& kv : oby) {
+    const int64_t y = kv.first;
+    const std::vector<int64_t>& O = kv.second;
+    auto itk = kby.find(y);
+    std::vector<int64_t> K = itk == kby.end() ? std::vector<int64_t>() : itk->second;
+    std::vector<int64_t> koff(K.size());
+    int64_t Ky = 0;
+    for (size_t i = 0; i < K.size(); ++i) {
+      const int64_t* z = ket.size(K[i]);
+      koff[i] = Ky;
+      Ky += z[0] * z[1] * z[2];
+    }
+    const int64_t Kp = up4(Ky), dY = H.out.size(O[0])[2];
+    const int64_t breg = bp;
+    if (Ky > 0) {
+      for (size_t i = 0; i < K.size(); ++i) {
+        const int64_t* z = ket.size(K[i]);
+        const int64_t nk = z[0] * z[1] * z[2];
+        const int64_t dims[2] = {nk, z[3]}, sst[2] = {1, nk}, dst[2] = {1, Kp};
+        H.pack_bra.add(2, dims, sst, dst, ket.offsets[K[i]], breg + koff[i]);
+      }
+      bp += up4(Kp * dY);
+    }
+    const int64_t reg = t3;
+    int64_t rowoff = 0;
+    for (int64_t o : O) {
+      const int64_t* os = H.out.sector(o);
+      const int64_t* z = H.out.size(o);
+      const int64_t dR = z[0], Dx = z[1];
+      H.gF.add_prob(dR * Dx, dY);
+      H.gF.add_crows_affine(H.out.offsets[o], dR * Dx, dR * Dx);
+      if (Ky > 0) {
+        H.gF.add_seg(Ky, reg + Kp * rowoff, Kp, breg, Kp);
+        for (size_t i = 0; i < K.size(); ++i) {
+          const int64_t* ks = ket.sector(K[i]);
+          const int64_t* kz = ket.size(K[i]);
+          cells[{ks[0], ks[1], ks[2], os[1], os[0]}] =
+              Cell{reg + koff[i] + Kp * rowoff, Kp * dR, kz[0], Kp, 1, kz[0], kz[1], kz[2], Dx, dR};
+        }
+      }
+      rowoff += dR * Dx;
+    }
+    if (Ky > 0) t3 += up4(Kp * rowoff);
+  }
+  H.t3_elems = t3;
+  H.bp_elems = bp;
+  mpo_layout(H, WL, WR, t1, cells);
+  H.g1.BM = 64;
+  H.g1.BN = 128;
+  H.g1.KS = 0;
+  H.gF.BM = H.gF.BN = 64;
+  H.gF.KS = 0;
+  H.pack_v.finalize();
+  H.pack_envL.finalize();
+  H.pack_bra.finalize();
+  // workspace: [ket_p | env_p | bra_p | W | T1 | T3 | partials]
+  H.kp_off = 0;
+  H.ep_off = up64(H.vp_elems);
+  H.bp_off = up64(H.ep_off + H.el_elems);
+  H.w_off = up64(H.bp_off + H.bp_elems);
+  H.t1_off = up64(H.w_off + H.w_elems);
+  H.t3_off = up64(H.t1_off + H.t1_elems);
+  H.part_off = up64(H.t3_off + H.t3_elems);
+  return H;
+}
+
+static void env_core_splits(EnvCore& H, int mode) {
+  H.gF.choose_splits(mode == 0 ? 2048 : 1024, chunk_k(mode, 0), 32);
+  H.ws_elems = H.part_off + H.gF.part_elems;
+}
+
+static void finish_env_plan(EnvPlan& P) {
+  P.ws_elems = 0;
+  for (int c = 0; c < P.ncore; ++c) {
+    P.core_f64[c] = P.core[c];
+    env_core_splits(P.core[c], 1);
+    env_core_splits(P.core_f64[c], 0);
+    P.ws_elems = std::max({P.ws_elems, P.core[c].ws_elems, P.core_f64[c].ws_elems});
+  }
+  // pre-permuted operands live after the cores' workspace
+  int64_t off = up64(P.ws_elems);
+  for (int c = 0; c < P.ncore; ++c)
+    for (int k = 0; k < 3; ++k) {
+      P.perm_off[c][k] = off;
+      if (P.has_perm[c][k]) off = up64(off + P.perm[c][k].out.numel());
+    }
+  P.ws_elems = off;
+}
+
+EnvPlan envupdate_plan(int side, const Struct& env, const Struct& A, const Struct& W) {
+  if (env.rank() != 3 || A.rank() != 3 || W.rank() != 4) throw Error(ERR_INVALID, "ranks must be env 3, A 3, W 4");
+  EnvPlan P;
+  P.kind = side;
+  P.ncore = 1;
+  P.ins = {env, A, W};
+  const Struct dA = dual_struct(A);
+  ContractCost P1, P2, P3;
+  if (side == 0) {  // _mpsupdateleft (src/apply.jl:49-58)
+    P1 = contract_cost(env, {-1, 3, 4}, A, {-1, 2, 1});
+    P2 = contract_cost(P1.C, {1, -1, -2, 4}, W, {-2, 2, 3, -1});
+    P3 = contract_cost(P2.C, {1, -1, 2, -2}, dA, {-2, -1, 3});
+  } else if (side == 1) {  // _mpsupdateright (src/apply.jl:17-25)
+    P1 = contract_cost(env, {-1, 3, 4}, A, {1, 2, -1});
+    P2 = contract_cost(P1.C, {1, -1, -2, 4}, W, {3, 2, -2, -1});
+    P3 = contract_cost(P2.C, {1, -1, 2, -2}, dA, {3, -1, -2});
+  } else {
+    throw Error(ERR_INVALID, "side must be 0 (left) or 1 (right)");
+  }
+  P.ref_flops = P1.flops + P2.flops + P3.flops;
+  P.ref_elems = env.numel() + A.numel() + P1.C.numel() + P1.C.numel() + W.numel() + P2.C.numel() + P2.C.numel() +
+                dA.numel() + P3.C.numel();
+  Struct ket = A, WL = W;
+  if (side == 1) {  // the mirror image: A^M = A(r, p, l), W^M = W(w_r, p_out, w_l, p_in)
+    P.perm[0][0] = permutelegs_plan(A, {3, 2, 1});
+    P.perm[0][1] = permutelegs_plan(W, {3, 2, 1, 4});
+    P.has_perm[0][0] = P.has_perm[0][1] = true;
+    ket = P.perm[0][0].out;
+    WL = P.perm[0][1].out;
+  }
+  const Struct WR = identity_mpo(WL, P.wr_identity);
+  P.wr_is_identity = true;
+  P.core[0] = env_core(env, with_trivial_leg(ket), WL, WR);
+  if (!same_struct(P.core[0].out, P3.C) || P.core[0].out.sectors != P3.C.sectors)
+    throw Error(ERR_SECTOR_MISMATCH, "envupdate: output structure differs from the contraction chain");
+  P.outs[0] = P3.C;
+  finish_env_plan(P);
+  return P;
+}
+
+EnvPlan heff_adjoint_plan(const Struct& v, const Struct& envL, const Struct& envR, const Struct& WL,
+                          const Struct& WR) {
+  // validates the operands like the forward H_eff
+  HeffPlan fwd_check = heff_plan(v, envL, envR, WL, WR);
+  (void)fwd_check;
+  EnvPlan P;
+  P.kind = 2;
+  P.ncore = 2;
+  P.ins = {v, envL, envR, WL, WR};
+  const Struct bra = dual_struct(v);
+  // d<v|H v>/d envR[c,k,e] = sum envL[a,m,b] v[a,p,s,c] WL[m,q,n,p] WR[n,r,k,s] v[b,q,r,e]
+  ContractCost Y1 = contract_cost(envL, {-1, 2, 1}, v, {-1, 3, 4, 5});
+  ContractCost Y2 = contract_cost(Y1.C, {1, -1, -2, 4, 5}, WL, {-1, 2, 3, -2});
+  ContractCost Y3 = contract_cost(Y2.C, {1, 2, -1, -2, 5}, WR, {-1, 3, 4, -2});
+  ContractCost Y4 = contract_cost(Y3.C, {-1, -2, -3, 2, 1}, bra, {-1, -2, -3, 3});
+  // d<v|H v>/d envL[a,m,b] = sum v[a,p,s,c] WL[m,q,n,p] WR[n,r,k,s] envR[c,k,e] v[b,q,r,e]
+  ContractCost X1 = contract_cost(v, {1, 2, 3, -1}, envR, {-1, 4, 5});
+  ContractCost X2 = contract_cost(X1.C, {1, 2, -1, -2, 3}, WR, {4, 5, -2, -1});
+  ContractCost X3 = contract_cost(X2.C, {1, -1, 2, -2, 3}, WL, {4, 5, -2, -1});
+  ContractCost X4 = contract_cost(X3.C, {1, -1, -2, 2, -3}, bra, {3, -3, -2, -1});
+  P.ref_flops = Y1.flops + Y2.flops + Y3.flops + Y4.flops + X1.flops + X2.flops + X3.flops + X4.flops;
+  P.ref_elems = 0;
+  for (const ContractCost* c : {&Y1, &Y2, &Y3, &Y4, &X1, &X2, &X3, &X4}) P.ref_elems += 2 * c->C.numel();
+  // core 0: dE/denvR directly; core 1: dE/denvL as the mirror image (v^M = v(r, p2, p1, l),
+  // the MPO sites swapped and each mirrored: W^M = W(w_r, p_out, w_l, p_in))
+  P.core[0] = env_core(envL, v, WL, WR);
+  P.perm[1][0] = permutelegs_plan(v, {4, 3, 2, 1});
+  P.perm[1][1] = permutelegs_plan(WR, {3, 2, 1, 4});
+  P.perm[1][2] = permutelegs_plan(WL, {3, 2, 1, 4});
+  P.has_perm[1][0] = P.has_perm[1][1] = P.has_perm[1][2] = true;
+  P.core[1] = env_core(envR, P.perm[1][0].out, P.perm[1][1].out, P.perm[1][2].out);
+  if (!same_struct(P.core[0].out, Y4.C) || P.core[0].out.sectors != Y4.C.sectors)
+    throw Error(ERR_SECTOR_MISMATCH, "heff adjoint: dE/denvR structure differs from the contraction chain");
+  if (!same_struct(P.core[1].out, X4.C) || P.core[1].out.sectors != X4.C.sectors)
+    throw Error(ERR_SECTOR_MISMATCH, "heff adjoint: dE/denvL structure differs from the contraction chain");
+  P.outs[0] = X4.C;  // gL
+  P.outs[1] = Y4.C;  // gR
+  finish_env_plan(P);
+  return P;
 }
 
 }  // namespace mpsh

@@ -272,8 +272,8 @@ struct MpoGroup {
 struct MpoInRow {
   int64_t base, nstride, mstride;
 };
-struct MpoOutRow {  // T3 address = base + n + m*mstride
-  int64_t base, mstride;
+struct MpoOutRow {  // T3 address = base + n*nstride + m*mstride
+  int64_t base, nstride, mstride;
 };
 struct MpoWork {
   int32_t group, pix0;
@@ -308,11 +308,10 @@ inline int chunk_k(int mode, int KS) { return KS == 0 ? (mode == 0 ? 8 : 16) : (
 // the step-4 GEMM of a plan with another tile and split target
 GemmPlan gemm_variant(const GemmPlan& base, const TileCfg& c, int64_t split_target, int mode);
 
-struct HeffPlan {
-  Struct v, envL, envR, WL, WR, out;
-  GemmPlan g1;  // A = v, B = envL, C = T1 (ws); FP32/BF16 tile variant
-  GemmPlan g1h, g4h;  // the same problems with the BF16 tile variants
-  GemmPlan g1d, g4d;  // the same problems with the FP64 tile variants
+// Step 1 (T1 = v^T envL) and the combined-MPO pass (T3 = W_group T1): the first
+// stages of the fused H_eff, shared with the environment plans.
+struct Step13 {
+  GemmPlan g1;  // A = packed v, B = packed envL, C = T1
   int64_t t1_elems = 0;
   std::vector<MpoSig> sigs;
   std::vector<MpoTerm> terms;
@@ -324,22 +323,71 @@ struct HeffPlan {
   int64_t w_elems = 0;
   int64_t t3_elems = 0;
   int maxR = 0, maxC = 0;
+  int64_t mpo_flops = 0;  // dense W_group application flops of this engine
+  CopyPlan pack_v, pack_envL;  // v and envL into 16-byte aligned K-contiguous rows
+  int64_t vp_elems = 0, el_elems = 0;
+};
+
+struct HeffPlan : Step13 {
+  Struct v, envL, envR, WL, WR, out;
+  GemmPlan g1h, g4h;  // the same problems with the BF16 tile variants
+  GemmPlan g1d, g4d;  // the same problems with the FP64 tile variants
   GemmPlan g4;  // A = T3 (ws), B = envR_p (prepared), C = out
   // Prepared operator (per bond): envL and envR repacked into 16-byte aligned
   // K-contiguous rows (K padded to a multiple of 4) and the combined-MPO matrices.
-  CopyPlan pack_envL, pack_envR;
+  CopyPlan pack_envR;
   int64_t el_off = 0, er_off = 0, pw_off = 0, prep_elems = 0;
   // Per-matvec workspace: [v_p | T1 | T3 | split-K partials]; v is packed the same way.
-  CopyPlan pack_v;
   int64_t vp_off = 0, t1_off = 0, t3_off = 0, part_off = 0, ws_elems = 0;
   // reference-order algorithmic cost (src/apply.jl:99-103 via src/contract.jl:290-291)
   int64_t ref_flops = 0;
   int64_t ref_elems = 0;  // sum over the 4 steps of numel(A)+numel(B)+numel(C)
   int64_t step_flops[4] = {0, 0, 0, 0};
   int64_t step_elems[4] = {0, 0, 0, 0};
-  int64_t mpo_flops = 0;  // dense W_group application flops of this engine
 };
 HeffPlan heff_plan(const Struct& v, const Struct& envL, const Struct& envR, const Struct& WL,
                    const Struct& WR, const HeffTiles& tiles = HeffTiles());
+
+// ---------------------------------------------------------------------------
+// Environment plans.  One core computes, for a ket K (l, p1, p2, r) whose bra is
+// K with dual legs,
+//   out[x, y, x'] = sum env[a, m, b] K[a, p1, p2, x] WL[m, q1, n, p1] WR[n, q2, y, p2] K[b, q1, q2, x']
+// as the H_eff's step 1 and MPO pass (T3 written with the bra side (b, q1, q2)
+// K-contiguous) followed by one grouped GEMM out = T3^T bra per output block.
+// _mpsupdateleft (src/apply.jl:49-58) is one core over one site (K = A with a
+// trivial second physical leg, WR = identity); _mpsupdateright (:17-25) is the
+// left update of the mirror image A(r, p, l), W(w_r, p_out, w_l, p_in); the
+// adjoint of <v|H_eff v> (src/apply.jl:93-105) w.r.t. envR is one two-site
+// core and w.r.t. envL the two-site core of the mirror image.
+// ---------------------------------------------------------------------------
+struct EnvCore : Step13 {
+  Struct env, ket, WL, WR, out;
+  GemmPlan gF;        // A = T3 (ws), B = packed bra (ws), C = out
+  CopyPlan pack_bra;  // the bra into rows of its K order
+  int64_t bp_elems = 0;
+  // workspace: [ket_p | env_p | bra_p | W | T1 | T3 | split-K partials]
+  int64_t kp_off = 0, ep_off = 0, bp_off = 0, w_off = 0, t1_off = 0, t3_off = 0, part_off = 0, ws_elems = 0;
+};
+
+struct EnvPlan {
+  int kind = 0;  // 0 left update, 1 right update, 2 H_eff adjoint
+  int ncore = 1;
+  EnvCore core[2];      // FP32 split-K layout
+  EnvCore core_f64[2];  // FP64 split-K layout (same tables otherwise)
+  // operands permuted before a core runs: [core][ket, WL, WR]
+  RelegPlan perm[2][3];
+  bool has_perm[2][3] = {{false, false, false}, {false, false, false}};
+  int64_t perm_off[2][3] = {{0, 0, 0}, {0, 0, 0}};
+  bool wr_is_identity = false;  // one-site plans: core 0's WR is the identity below
+  std::vector<double> wr_identity;
+  std::vector<Struct> ins;  // the operand structures, in argument order
+  Struct outs[2];           // left/right update: outs[0]; adjoint: gL, gR
+  int64_t ws_elems = 0;
+  int64_t ref_flops = 0, ref_elems = 0;  // the reference contraction chain(s)
+};
+Struct dual_struct(const Struct& A);  // src/symtensor.jl:201-212
+EnvPlan envupdate_plan(int side, const Struct& env, const Struct& A, const Struct& W);
+EnvPlan heff_adjoint_plan(const Struct& v, const Struct& envL, const Struct& envR, const Struct& WL,
+                          const Struct& WR);
 
 }  // namespace mpsh

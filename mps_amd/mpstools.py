@@ -24,8 +24,8 @@ import torch
 
 from . import _lib as L
 from ._lib import check, lib
-from .symtensors import (SymTensor, U1Space, _ptr, _stream, contract, default_device, dual, fuselegs, get_struct,
-                         splitleg)
+from .symtensors import (PlanLRU, SymTensor, U1Space, _ptr, _stream, contract, default_device, dual, fuselegs,
+                         get_struct, splitleg)
 from .workloads import binom_capped_profile, gaussian
 
 
@@ -249,7 +249,8 @@ class _SvdPlan:
             pass
 
 
-_SVD_PLANS: Dict[object, _SvdPlan] = {}
+SVD_PLAN_CACHE = 512
+_SVD_PLANS: Dict[object, _SvdPlan] = PlanLRU(SVD_PLAN_CACHE)
 
 
 def _svd_blocks(M: SymTensor):

@@ -9,6 +9,7 @@ order, each block column-major); sector bookkeeping lives in libmpsh
 from __future__ import annotations
 
 import math
+from collections import OrderedDict
 from typing import Dict, Optional, Sequence, Tuple
 
 import numpy as np
@@ -86,6 +87,52 @@ def _leg_arrays(spaces: Sequence[U1Space]):
 
 
 # ---------------------------------------------------------------------------
+# bounded plan caches
+# ---------------------------------------------------------------------------
+class PlanLRU(OrderedDict):
+    """A dict that keeps at most ``maxsize`` entries, dropping the least recently
+    used (``get`` and assignment count as use).  Plan entries own their library
+    handles (:class:`PlanHandle`), so a dropped plan releases its tables; a sweep
+    meets new bond structures at every truncation, so unbounded caches would
+    grow with every sweep."""
+
+    def __init__(self, maxsize: int):
+        super().__init__()
+        self.maxsize = int(maxsize)
+
+    def get(self, key, default=None):
+        if key in self:
+            self.move_to_end(key)
+            return self[key]
+        return default
+
+    def __setitem__(self, key, value):
+        super().__setitem__(key, value)
+        self.move_to_end(key)
+        while len(self) > self.maxsize:
+            self.popitem(last=False)
+
+
+class PlanHandle:
+    """Owner of one plan handle of the library; ``destroy`` names its
+    ``mpsh_*_destroy`` entry point, called when the owner is collected."""
+
+    __slots__ = ("h", "destroy")
+
+    def __init__(self, h, destroy: str):
+        self.h = h
+        self.destroy = destroy
+
+    def __del__(self):
+        try:
+            if self.h:
+                getattr(lib(), self.destroy)(self.h)
+                self.h = None
+        except Exception:
+            pass
+
+
+# ---------------------------------------------------------------------------
 # structures (interned by value)
 # ---------------------------------------------------------------------------
 class Struct:
@@ -127,7 +174,8 @@ class Struct:
             pass
 
 
-_STRUCTS: Dict = {}
+# interned structures (host tables only; a dropped entry lives on in the tensors using it)
+_STRUCTS: Dict = PlanLRU(8192)
 
 
 def _struct_from_handle(handle) -> Struct:
@@ -409,7 +457,8 @@ def scale(A: SymTensor, alpha: float) -> SymTensor:
 # ---------------------------------------------------------------------------
 # re-legging   (src/symreleg.jl)
 # ---------------------------------------------------------------------------
-_RELEG: Dict = {}
+RELEG_PLAN_CACHE = 2048
+_RELEG: Dict = PlanLRU(RELEG_PLAN_CACHE)
 
 
 def _releg_run(key, make, A: SymTensor) -> SymTensor:
@@ -417,11 +466,11 @@ def _releg_run(key, make, A: SymTensor) -> SymTensor:
     if ent is None:
         plan, out = L._p(), L._p()
         check(make(plan, out))
-        ent = (plan, _struct_from_handle(out))
+        ent = (PlanHandle(plan, "mpsh_releg_destroy"), _struct_from_handle(out))
         _RELEG[key] = ent
     plan, st = ent
     data = torch.empty(st.numel, dtype=A.dtype, device=A.device)
-    check(lib().mpsh_releg_execute(plan, _DT[A.dtype], _ptr(A.data), _ptr(data), _stream()))
+    check(lib().mpsh_releg_execute(plan.h, _DT[A.dtype], _ptr(A.data), _ptr(data), _stream()))
     return SymTensor(st.charge, st.spaces, data, st)
 
 
@@ -468,16 +517,18 @@ def symmatrix(A: SymTensor, rowidxs: Sequence[int], colidxs: Sequence[int]) -> S
 # ---------------------------------------------------------------------------
 # contract   (src/contract.jl:11-47)
 # ---------------------------------------------------------------------------
-_CONTRACT: Dict = {}
+CONTRACT_PLAN_CACHE = 1024
+_CONTRACT: Dict = PlanLRU(CONTRACT_PLAN_CACHE)
 
 
 def contract_plan(sA: Struct, idxA, sB: Struct, idxB):
+    """(plan owner, output structure) of contract(A, idxA, B, idxB), LRU-cached by structure."""
     key = (sA.key, tuple(idxA), sB.key, tuple(idxB))
     ent = _CONTRACT.get(key)
     if ent is None:
         plan, out = L._p(), L._p()
         check(lib().mpsh_contract_plan(sA.handle, i32arr(idxA), sB.handle, i32arr(idxB), plan, out))
-        ent = (plan, _struct_from_handle(out))
+        ent = (PlanHandle(plan, "mpsh_contract_destroy"), _struct_from_handle(out))
         _CONTRACT[key] = ent
     return ent
 
@@ -512,11 +563,11 @@ def contract(A: SymTensor, idxA: Sequence[int], B: SymTensor, idxB: Sequence[int
         return float(res.item())
     if A.dtype != B.dtype:
         raise TypeError("contract of mixed dtypes is not supported on the device path")
-    plan, st = contract_plan(A.struct, idxA, B.struct, idxB)
+    owner, st = contract_plan(A.struct, idxA, B.struct, idxB)
     out = torch.empty(st.numel, dtype=A.dtype, device=A.device)
-    ws = torch.empty(max(1, int(lib().mpsh_contract_workspace(plan, _DT[A.dtype]))), dtype=torch.uint8,
+    ws = torch.empty(max(1, int(lib().mpsh_contract_workspace(owner.h, _DT[A.dtype]))), dtype=torch.uint8,
                      device=A.device)
-    check(lib().mpsh_contract_execute(plan, _DT[A.dtype], _ptr(A.data), _ptr(B.data), _ptr(out), _ptr(ws),
+    check(lib().mpsh_contract_execute(owner.h, _DT[A.dtype], _ptr(A.data), _ptr(B.data), _ptr(out), _ptr(ws),
                                       _stream()))
     return SymTensor(st.charge, st.spaces, out, st)
 

@@ -20,7 +20,8 @@ import torch
 
 from . import _lib as L
 from ._lib import check, lib
-from .symtensors import (SymTensor, Struct, _DT, _ptr, _stream, _struct_from_handle, contract, dual)
+from .symtensors import (PlanHandle, PlanLRU, SymTensor, Struct, _DT, _ptr, _stream, _struct_from_handle, contract,
+                         dual)
 
 _PREC = {"f64": L.MPSH_F64, "f32": L.MPSH_F32, "bf16": L.MPSH_BF16}
 _TORCH = {"f64": torch.float64, "f32": torch.float32, "bf16": torch.float32}
@@ -130,19 +131,14 @@ def _scratch(device, nbytes: int) -> torch.Tensor:
 # plans (host and device tables, a few hundred KB each at chi=2048), least
 # recently used dropped beyond HEFF_PLAN_CACHE
 HEFF_PLAN_CACHE = 512
-_HEFF: "OrderedDict" = OrderedDict()
+_HEFF: "OrderedDict" = PlanLRU(HEFF_PLAN_CACHE)
 
 
 def heff_plan(v: SymTensor, envL: SymTensor, envR: SymTensor, WL: SymTensor, WR: SymTensor) -> HeffPlan:
     key = (v.struct.key, envL.struct.key, envR.struct.key, WL.struct.key, WR.struct.key)
     p = _HEFF.get(key)
     if p is None:
-        p = HeffPlan(v.struct, envL.struct, envR.struct, WL.struct, WR.struct)
-        _HEFF[key] = p
-        while len(_HEFF) > HEFF_PLAN_CACHE:
-            _HEFF.popitem(last=False)
-    else:
-        _HEFF.move_to_end(key)
+        p = _HEFF[key] = HeffPlan(v.struct, envL.struct, envR.struct, WL.struct, WR.struct)
     return p
 
 
@@ -174,33 +170,126 @@ def _applymps2site(v: SymTensor, envL: SymTensor, envR: SymTensor, hmpoL: SymTen
     return SymTensor(p.out.charge, p.out.spaces, out, p.out)
 
 
-def _mpsupdateleft(lenv: SymTensor, mat: SymTensor, hmpo: SymTensor) -> SymTensor:
-    """``_mpsupdateleft(lenv, mat, hmpo)`` (apply.jl:49-58): three device contracts."""
-    return contract(contract(contract(lenv, (-1, 3, 4), mat, (-1, 2, 1)),
-                             (1, -1, -2, 4), hmpo, (-2, 2, 3, -1)),
-                    (1, -1, 2, -2), dual(mat), (-2, -1, 3))
+class EnvUpdatePlan:
+    """Fused environment update of one structure (``mpsh_envupdate_*``): operand
+    packs, the H_eff's step-1 grouped GEMM and combined-MPO pass over one site,
+    and one grouped GEMM against the bra dual(A)."""
+
+    def __init__(self, side: str, s_env: Struct, sA: Struct, sW: Struct):
+        plan, out = L._p(), L._p()
+        make = lib().mpsh_envupdate_left_plan if side == "left" else lib().mpsh_envupdate_right_plan
+        check(make(s_env.handle, sA.handle, sW.handle, plan, out))
+        self.owner = PlanHandle(plan, "mpsh_envupdate_destroy")
+        self.out = _struct_from_handle(out)
+        st = (L._i64 * 4)()
+        check(lib().mpsh_envupdate_stats(plan, st))
+        self.ref_flops, self.ref_elems, self.gemm_flops, self.mpo_flops = (int(x) for x in st)
+
+    def __call__(self, env: SymTensor, A: SymTensor, W: SymTensor) -> SymTensor:
+        dt = env.dtype
+        if A.dtype != dt or W.dtype != dt:
+            raise TypeError("environment update of mixed dtypes")
+        h = self.owner.h
+        ws = _scratch(env.device, max(16, int(lib().mpsh_envupdate_workspace(h, _DT[dt]))))
+        out = torch.empty(self.out.numel, dtype=dt, device=env.device)
+        check(lib().mpsh_envupdate_execute(h, _DT[dt], _ptr(env.data), _ptr(A.data), _ptr(W.data), _ptr(out),
+                                           _ptr(ws), _stream()))
+        return SymTensor(self.out.charge, self.out.spaces, out, self.out)
 
 
-def _mpsupdateright(renv: SymTensor, mat: SymTensor, hmpo: SymTensor) -> SymTensor:
-    """``_mpsupdateright(renv, mat, hmpo)`` (apply.jl:17-25): three device contracts."""
-    return contract(contract(contract(renv, (-1, 3, 4), mat, (1, 2, -1)),
-                             (1, -1, -2, 4), hmpo, (3, 2, -2, -1)),
-                    (1, -1, 2, -2), dual(mat), (3, -1, -2))
+ENV_PLAN_CACHE = 1024
+_ENV: "OrderedDict" = PlanLRU(ENV_PLAN_CACHE)
+
+
+def envupdate_plan(side: str, env: SymTensor, A: SymTensor, W: SymTensor) -> EnvUpdatePlan:
+    key = (side, env.struct.key, A.struct.key, W.struct.key)
+    p = _ENV.get(key)
+    if p is None:
+        p = _ENV[key] = EnvUpdatePlan(side, env.struct, A.struct, W.struct)
+    return p
+
+
+def _mpsupdateleft(lenv: SymTensor, mat: SymTensor, hmpo: SymTensor, path: str = "fused") -> SymTensor:
+    """``_mpsupdateleft(lenv, mat, hmpo)`` (apply.jl:49-58).  ``path="fused"``: one
+    fused device plan; ``path="plain"``: the reference's three device contracts."""
+    if path == "plain":
+        return contract(contract(contract(lenv, (-1, 3, 4), mat, (-1, 2, 1)),
+                                 (1, -1, -2, 4), hmpo, (-2, 2, 3, -1)),
+                        (1, -1, 2, -2), dual(mat), (-2, -1, 3))
+    if path != "fused":
+        raise ValueError(f"unknown path {path!r}")
+    return envupdate_plan("left", lenv, mat, hmpo)(lenv, mat, hmpo)
+
+
+def _mpsupdateright(renv: SymTensor, mat: SymTensor, hmpo: SymTensor, path: str = "fused") -> SymTensor:
+    """``_mpsupdateright(renv, mat, hmpo)`` (apply.jl:17-25).  ``path="fused"``: one
+    fused device plan (the left update of the mirror image); ``path="plain"``: the
+    reference's three device contracts."""
+    if path == "plain":
+        return contract(contract(contract(renv, (-1, 3, 4), mat, (1, 2, -1)),
+                                 (1, -1, -2, 4), hmpo, (3, 2, -2, -1)),
+                        (1, -1, 2, -2), dual(mat), (3, -1, -2))
+    if path != "fused":
+        raise ValueError(f"unknown path {path!r}")
+    return envupdate_plan("right", renv, mat, hmpo)(renv, mat, hmpo)
+
+
+class HeffAdjointPlan:
+    """Adjoint kernels of <v|H_eff v> w.r.t. both environments (``mpsh_heff_adjoint_*``)."""
+
+    def __init__(self, sv: Struct, sL: Struct, sR: Struct, sWL: Struct, sWR: Struct):
+        plan, gL, gR = L._p(), L._p(), L._p()
+        check(lib().mpsh_heff_adjoint_plan(sv.handle, sL.handle, sR.handle, sWL.handle, sWR.handle, plan, gL, gR))
+        self.owner = PlanHandle(plan, "mpsh_heff_adjoint_destroy")
+        self.gL, self.gR = _struct_from_handle(gL), _struct_from_handle(gR)
+        st = (L._i64 * 4)()
+        check(lib().mpsh_heff_adjoint_stats(plan, st))
+        self.ref_flops, self.ref_elems, self.gemm_flops, self.mpo_flops = (int(x) for x in st)
+
+    def __call__(self, v, envL, envR, WL, WR):
+        """(d<v|H v>/d envL, d<v|H v>/d envR) as packed buffers of the dtype of the operands."""
+        dt = v.dtype
+        for t in (envL, envR, WL, WR):
+            if t.dtype != dt:
+                raise TypeError("adjoint of mixed dtypes")
+        h = self.owner.h
+        ws = _scratch(v.device, max(16, int(lib().mpsh_heff_adjoint_workspace(h, _DT[dt]))))
+        gL = torch.empty(self.gL.numel, dtype=dt, device=v.device)
+        gR = torch.empty(self.gR.numel, dtype=dt, device=v.device)
+        check(lib().mpsh_heff_adjoint_execute(h, _DT[dt], _ptr(v.data), _ptr(envL.data), _ptr(envR.data),
+                                              _ptr(WL.data), _ptr(WR.data), _ptr(gL), _ptr(gR), _ptr(ws),
+                                              _stream()))
+        return gL, gR
+
+
+_ADJ: "OrderedDict" = PlanLRU(256)
+
+
+def heff_adjoint_plan(v, envL, envR, WL, WR) -> HeffAdjointPlan:
+    key = (v.struct.key, envL.struct.key, envR.struct.key, WL.struct.key, WR.struct.key)
+    p = _ADJ.get(key)
+    if p is None:
+        p = _ADJ[key] = HeffAdjointPlan(v.struct, envL.struct, envR.struct, WL.struct, WR.struct)
+    return p
 
 
 def rayleigh_gradient(v: SymTensor, envL: SymTensor, envR: SymTensor, WL: SymTensor, WR: SymTensor,
-                      prec: str = None):
+                      prec: str = None, path: str = "fused"):
     """Gradient of E = <v|H_eff v>/<v|v> (H_eff of ``_applymps2site``, apply.jl:93-105)
     with respect to the two-site tensor and both environments.
 
     Returns (E, dE/dv, dE/denvL, dE/denvR), each gradient with the structure of
     its variable.  E is linear in each environment, so dE/denvL is the adjoint
     contraction of (bra v, WL, WR, envR, ket v) with the envL legs left open,
-    divided by <v|v> (likewise for envR); both are chains of block-sparse device
-    contracts (the adjoints of the reference's four contracts).  dE/dv =
-    2 (H v - E v)/<v|v> holds for the Hermitian H_eff of DMRG (environments of
-    one MPS and a Hermitian MPO).  ``prec`` selects the H_eff arithmetic of the
-    v-gradient; the environment adjoints run in the tensors' dtype.
+    divided by <v|v> (likewise for envR): with ``path="fused"`` both come from
+    the adjoint kernels (``mpsh_heff_adjoint_*``, two environment cores: the
+    operands and their mirror image), with ``path="plain"`` from chains of
+    block-sparse device contracts (the adjoints of the reference's four
+    contracts).  dE/dv = 2 (H v - E v)/<v|v> holds for the Hermitian H_eff of
+    DMRG (environments of one MPS and a Hermitian MPO).  ``prec`` selects the
+    H_eff arithmetic of the v-gradient; the environment adjoints run in the
+    tensors' dtype.  Nothing is stored for a backward pass: the environments are
+    inputs, recomputed by the caller (``initialenv``) when they are needed.
     """
     from .symtensors import axpby_, dot, rmul_
     nn = dot(v, v)
@@ -208,6 +297,19 @@ def rayleigh_gradient(v: SymTensor, envL: SymTensor, envR: SymTensor, WL: SymTen
     E = dot(v, Hv) / nn
     gv = Hv.copy()
     axpby_(-2.0 * E / nn, v, 2.0 / nn, gv)
+
+    def as_grad(G, Gs, like):
+        # the adjoint's legs are the duals of the variable's: same sectors, same block order
+        if Gs.sectors() != like.struct.sectors() or Gs.numel != like.struct.numel:
+            raise ValueError("adjoint structure does not match the environment")
+        return rmul_(SymTensor(like.charge, like.space, G.to(like.dtype), like.struct), 1.0 / nn)
+
+    if path == "fused":
+        P = heff_adjoint_plan(v, envL, envR, WL, WR)
+        gL, gR = P(v, envL, envR, WL, WR)
+        return E, gv, as_grad(gL, P.gL, envL), as_grad(gR, P.gR, envR)
+    if path != "plain":
+        raise ValueError(f"unknown path {path!r}")
     bra = dual(v)
     # dE/denvL[a,m,b] = sum v[a,p,s,c] WL[m,q,n,p] WR[n,r,k,s] envR[c,k,e] v[b,q,r,e] / <v|v>
     t = contract(v, (1, 2, 3, -1), envR, (-1, 4, 5))
@@ -219,14 +321,7 @@ def rayleigh_gradient(v: SymTensor, envL: SymTensor, envR: SymTensor, WL: SymTen
     t = contract(t, (1, -1, -2, 4, 5), WL, (-1, 2, 3, -2))
     t = contract(t, (1, 2, -1, -2, 5), WR, (-1, 3, 4, -2))
     Y = contract(t, (-1, -2, -3, 2, 1), bra, (-1, -2, -3, 3))
-
-    def as_grad(G, like):
-        # the adjoint's legs are the duals of the variable's: same sectors, same block order
-        if G.struct.sectors() != like.struct.sectors() or G.struct.numel != like.struct.numel:
-            raise ValueError("adjoint structure does not match the environment")
-        return rmul_(SymTensor(like.charge, like.space, G.data.to(like.dtype).clone(), like.struct), 1.0 / nn)
-
-    return E, gv, as_grad(X, envL), as_grad(Y, envR)
+    return E, gv, as_grad(X.data.clone(), X.struct, envL), as_grad(Y.data.clone(), Y.struct, envR)
 
 
 def heff_workload(name: str, seed: int = 1911, dtype=torch.float64, device=None):

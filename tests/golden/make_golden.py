@@ -8,6 +8,9 @@ Fixtures (npz, no pickled objects; structures as a JSON string):
 * envupdate_xxz_L32_chi64.npz -- _mpsupdateleft / _mpsupdateright (src/apply.jl:17-58)
 * sweep_xxz_L8.npz        -- one dmrg2sitesweep! (src/dmrg.jl:58-179) energy trace from a
                               canonical random U(1) MPS, plus the U(1)-sector ED energy
+* sweep_xxz_L32_chi64.npz -- the same at the XXZ L=32 chi=64 configuration (Delta=1, N=16,
+                              13 charge sectors per bond, krylovdim 8): canonical start and
+                              the oracle sweep's energy list
 
 Inputs come from the SplitMix64 / Box-Muller generator (mps_amd.workloads.gaussian).
 """
@@ -114,5 +117,35 @@ def main():
     print("wrote sweep_xxz_L8.npz", len(energies), "energies, last", energies[-1], "ED", e_ed)
 
 
+def sweep_fixture(name, Lx, N, chi, nsec, kw, with_ed):
+    """Canonical random U(1) MPS (seeded Gaussian blocks, right-canonical at site 1,
+    normalised) and the oracle's dmrg2sitesweep! energy list from it."""
+    nsec_kw = {} if nsec is None else {"nsec": nsec}
+    bonds = [[(0, 1)]] + [binom_capped_profile(l, Lx, N, chi, **nsec_kw) for l in range(1, Lx)] + [[(N, 1)]]
+    flats = []
+    for l in range(1, Lx + 1):
+        sp = (ost.Space(bonds[l - 1]), VD, ost.Space(bonds[l]).dual())
+        n = sum(int(np.prod(z)) for z in ost.allsectorsandsizes(0, sp)[1])
+        flats.append(gaussian(1911, 1000 + l, n))
+    mps = ota.random_u1mps_oracle(Lx, N, bonds, flats)
+    ota.center_at(mps, 1)
+    ota.normalize_mps(mps)
+    init = {f"A{l + 1}": A for l, A in enumerate(mps.As)}
+    mpo = ota.xxz_u1mpo(Lx, 1.0)
+    env = ota.initialenv(mps, mpo)
+    energies = ota.dmrg2sitesweep(mps, mpo, env, **kw)
+    extra = {}
+    if with_ed:
+        extra["e_ed"] = np.asarray(ota.u1_ground_energy(Lx, [(i, i + 1) for i in range(1, Lx)], N))
+    meta = {k: space_json(t) for k, t in init.items()}
+    meta["settings"] = kw
+    np.savez_compressed(os.path.join(HERE, name), meta=np.array(json.dumps(meta)),
+                        energies=np.asarray(energies), **extra,
+                        **{f"flat_{k}": flat(t) for k, t in init.items()})
+    print("wrote", name, len(energies), "energies, last", energies[-1], extra.get("e_ed", ""))
+
+
 if __name__ == "__main__":
     main()
+    sweep_fixture("sweep_xxz_L32_chi64.npz", 32, 16, 64, 13,
+                  dict(maxdim=64, tol=1e-10, lanczostol=1e-10, krylovdim=8, krylovmaxiter=8), with_ed=False)

@@ -42,3 +42,21 @@ def test_golden_sweep_trace():
     np.testing.assert_allclose(es, extra["energies"], rtol=1e-10, atol=1e-10)
     # variational: every energy lies above the exact ground energy
     assert np.all(extra["energies"] >= float(extra["e_ed"]) - 1e-9)
+
+
+def test_golden_sweep_trace_xxz_L32_chi64():
+    """XXZ L=32, Delta=1, chi=64 (13 sectors per bond), krylovdim 8: the fixture's start
+    is right-canonical at site 1 and normalised, the oracle sweep from it reproduces
+    the stored energy list, and the list has the reference's layout (quirk Q6)."""
+    T, extra, meta = load_fixture("sweep_xxz_L32_chi64.npz")
+    L = 32
+    As = [T[f"A{l}"] for l in range(1, L + 1)]
+    for A in As[1:]:
+        I = ost.contract(A, (1, -1, -2), ost.dual(A), (2, -1, -2))
+        for b in I.blocks.values():
+            assert np.allclose(b, np.eye(b.shape[0]), atol=1e-12)
+    assert abs(ost.norm(As[0]) - 1.0) < 1e-12
+    mps, mpo = ota.MPState(list(As), 1), ota.xxz_u1mpo(L, 1.0)
+    es = ota.dmrg2sitesweep(mps, mpo, ota.initialenv(mps, mpo), **meta["settings"])
+    assert len(es) == len(extra["energies"]) == 2 * L - 3 + (L - 2)
+    np.testing.assert_allclose(es, extra["energies"], rtol=1e-10, atol=1e-10)

@@ -18,6 +18,7 @@ if not torch.cuda.is_available():
     pytest.skip("needs a GPU", allow_module_level=True)
 
 import mps_amd as M  # noqa: E402
+from mps_amd import tensornetalgs as ta  # noqa: E402
 from mps_amd.tensornetalgs import heff_workload  # noqa: E402
 from mps_amd.workloads import gaussian  # noqa: E402
 
@@ -205,3 +206,108 @@ def test_dma_gemm_f64_variants(name, tiles):
     assert torch.isfinite(outs[1]).all()
     err = float(torch.linalg.vector_norm(outs[1] - outs[0]) / torch.linalg.vector_norm(outs[0]))
     assert err < 1e-13, err
+
+
+def _site_tensor(v, seed=3, stream=22):
+    """A = (Vl, Vd, Vr†) with the workload's bonds: left update input of envL, right of envR."""
+    return M.randn((v.space[0], v.space[1], v.space[3]), 0, seed=seed, stream=stream, device=DEV)
+
+
+@pytest.mark.timeout(600)
+@pytest.mark.parametrize("name", ["xxz_L100_chi512", "refbench_chi1000", "cyl6x24_chi2048"])
+def test_env_updates_fused_vs_oracle_and_plain(name):
+    """Fused environment updates (one plan: packs, step-1 GEMM, MPO pass, GEMM with the
+    bra) against the oracle's three contracts (src/apply.jl:17-58) block by block
+    (FP64 1e-12) and against the device's three-contract path; FP32 to 1e-5."""
+    v, L, R, WL, WR = _workload(name)
+    A = _site_tensor(v)
+    for side, fn, ofn, args in (("left", M._mpsupdateleft, ota.mpsupdateleft, (L, A, WL)),
+                                ("right", M._mpsupdateright, ota.mpsupdateright, (R, A, WR))):
+        want = ofn(*[oracle_of(t) for t in args])
+        got = fn(*args)
+        _check_structure(got, want)
+        err = rel_block_err(oracle_of(got), want)
+        plain = fn(*args, path="plain")
+        d = (got.data - plain.data).norm().item() / plain.data.norm().item()
+        got32 = fn(*[t.to(torch.float32) for t in args])
+        e32 = global_rel_err(oracle_of(got32), want)
+        print(f"{name} {side}: {len(want.blocks)} blocks, f64 max block rel err {err:.2e}, fused vs plain {d:.2e}, "
+              f"f32 rel err {e32:.2e}", flush=True)
+        assert err <= 1e-12
+        assert d <= 1e-12
+        assert e32 <= 1e-5
+
+
+def test_env_updates_edge_bonds_and_charges():
+    """Edge environments (dimension-1 bond legs, as at the chain ends) and a site
+    tensor of nonzero charge: fused equals the oracle."""
+    Vd = M.U1Space((0, 1), (1, 1))
+    Vw = M.U1Space((-1, 1), (0, 3), (1, 1))
+    V1 = M.U1Space((0, 1))
+    V2 = M.U1Space((0, 1), (1, 1))
+    V3 = M.U1Space((0, 2), (1, 3), (2, 1))
+    for charge in (0, 1):
+        A = M.randn((V2, Vd, V3.dual()), charge, seed=5, stream=30 + charge, device=DEV)
+        W = M.randn((Vw, Vd, Vw.dual(), Vd.dual()), 0, seed=5, stream=40, device=DEV)
+        Le = M.fill(1.0, (V2.dual(), Vw.dual(), V2), device=DEV)
+        Re = M.randn((V3, Vw, V3.dual()), 0, seed=5, stream=41, device=DEV)
+        for fn, ofn, args in ((M._mpsupdateleft, ota.mpsupdateleft, (Le, A, W)),
+                              (M._mpsupdateright, ota.mpsupdateright, (Re, A, W))):
+            want = ofn(*[oracle_of(t) for t in args])
+            got = fn(*args)
+            _check_structure(got, want)
+            assert rel_block_err(oracle_of(got), want) <= 1e-12
+    # one-dimensional boundary legs on both sides
+    A = M.randn((V1, Vd, V2.dual()), 0, seed=6, stream=1, device=DEV)
+    Le = M.fill(1.0, (V1.dual(), M.U1Space((0, 1)).dual(), V1), device=DEV)
+    W1 = M.randn((M.U1Space((0, 1)), Vd, Vw.dual(), Vd.dual()), 0, seed=6, stream=2, device=DEV)
+    want = ota.mpsupdateleft(*[oracle_of(t) for t in (Le, A, W1)])
+    got = M._mpsupdateleft(Le, A, W1)
+    _check_structure(got, want)
+    assert rel_block_err(oracle_of(got), want) <= 1e-12
+
+
+def _oracle_adjoint(v, L, R, WL, WR):
+    """d<v|H v>/denvL and d<v|H v>/denvR from the oracle's contract (the reference's
+    contraction on the host), the same label chains as the device's plain path."""
+    from oracle import symtensors as ost
+    bra = ost.dual(v)
+    t = ost.contract(v, (1, 2, 3, -1), R, (-1, 4, 5))
+    t = ost.contract(t, (1, 2, -1, -2, 3), WR, (4, 5, -2, -1))
+    t = ost.contract(t, (1, -1, 2, -2, 3), WL, (4, 5, -2, -1))
+    X = ost.contract(t, (1, -1, -2, 2, -3), bra, (3, -3, -2, -1))
+    t = ost.contract(L, (-1, 2, 1), v, (-1, 3, 4, 5))
+    t = ost.contract(t, (1, -1, -2, 4, 5), WL, (-1, 2, 3, -2))
+    t = ost.contract(t, (1, 2, -1, -2, 5), WR, (-1, 3, 4, -2))
+    Y = ost.contract(t, (-1, -2, -3, 2, 1), bra, (-1, -2, -3, 3))
+    return X, Y
+
+
+@pytest.mark.timeout(600)
+@pytest.mark.parametrize("name", ["xxz_L32_chi64", "xxz_L100_chi512", "cyl6x24_chi2048"])
+def test_heff_adjoint_kernels_vs_oracle(name):
+    """Adjoint kernels of <v|H_eff v> (mpsh_heff_adjoint_*) against the oracle's
+    contraction chain (FP64 1e-12 per block) and the device's plain contract chain;
+    FP32 against FP64 to 1e-5."""
+    from mps_amd.tensornetalgs import heff_adjoint_plan
+    ts = _workload(name)
+    P = heff_adjoint_plan(*ts)
+    gL, gR = P(*ts)
+    X, Y = _oracle_adjoint(*[oracle_of(t) for t in ts])
+    v = ts[0]
+    for g, st, want in ((gL, P.gL, X), (gR, P.gR, Y)):
+        assert st.sectors() == list(want.blocks.keys())
+        assert tuple(o_space(V) for V in st.spaces) == want.space
+        got = o_from_flat(st.charge, tuple(o_space(V) for V in st.spaces), g.cpu().numpy())
+        err = rel_block_err(got, want)
+        print(f"{name} adjoint: {len(want.blocks)} blocks, max block rel err {err:.2e}", flush=True)
+        assert err <= 1e-12
+    # the structures are those of envL / envR (what the gradient is returned as)
+    assert P.gL.sectors() == ts[1].struct.sectors() and P.gR.sectors() == ts[2].struct.sectors()
+    E, gv, dL, dR = ta.rayleigh_gradient(*ts, path="plain")
+    E2, gv2, dL2, dR2 = ta.rayleigh_gradient(*ts)
+    for a, b in ((dL2, dL), (dR2, dR)):
+        assert (a.data - b.data).norm().item() <= 1e-12 * b.data.norm().item()
+    gL32, gR32 = P(*[t.to(torch.float32) for t in ts])
+    for a, b in ((gL32, gL), (gR32, gR)):
+        assert (a.double() - b).norm().item() <= 1e-5 * b.norm().item()
