@@ -140,6 +140,20 @@ mpsh_status mpsh_heff_apply_prepared(mpsh_heff plan, mpsh_dtype dt, const void* 
  * no reference counterpart. */
 mpsh_status mpsh_heff_apply_prepared_stages(mpsh_heff plan, mpsh_dtype dt, int32_t stages, const void* prepared,
                                             const void* v, void* out, void* workspace, void* stream);
+/* The same application without a workspace and without allocating: no
+ * hipMalloc / hipFree or allocating library call, so it may run under HIP graph
+ * capture and in loops that must not allocate.  No counterpart in the
+ * reference; for callers that cannot allocate.  The per-matvec scratch (packed
+ * v, T1, T3, split-K partials) is a fixed device array of
+ * MPSH_NOALLOC_SCRATCH_BYTES per GPU, held until the process ends; plans whose
+ * mpsh_heff_apply_workspace exceeds it are rejected (MPSH_ERR_INVALID), as is a
+ * plan whose tables are not yet on the current device (mpsh_heff_prepare for
+ * this dtype uploads them).  All calls on one GPU share the array: calls that
+ * overlap on one GPU race.  Results are bitwise those of
+ * mpsh_heff_apply_prepared. */
+#define MPSH_NOALLOC_SCRATCH_BYTES (64u << 20)
+mpsh_status mpsh_heff_apply_prepared_noalloc(mpsh_heff plan, mpsh_dtype dt, const void* prepared, const void* v,
+                                             void* out, void* stream);
 /* Autotuner over the plan's block-size histogram: times every candidate tile
  * variant of the step-1 GEMM and every (tile, split-K) variant of the step-4
  * GEMM on the plan's real problems (reps launches each, HIP events on `stream`,

@@ -16,6 +16,7 @@ DOT_PARTIALS = 1024
 LZ_PARTS = 256  # MPSH_LZ_PARTS
 GATHER_FIELDS = 8  # MPSH_GATHER_FIELDS
 GATHER_TILE = 4096  # elements per work group of mpsh_gather_blocks
+NOALLOC_SCRATCH_BYTES = 64 << 20  # MPSH_NOALLOC_SCRATCH_BYTES
 
 STATUS_NAMES = {
     0: "OK", 1: "INVALID", 2: "SPACE_MISMATCH", 3: "SECTOR_MISMATCH", 4: "NOT_SIMILAR",
@@ -79,6 +80,7 @@ _SIGS = {
     "mpsh_heff_apply_workspace": (_sz, [_p, C.c_int]),
     "mpsh_heff_prepare": (C.c_int, [_p, C.c_int, _p, _p, _p, _p, _p, _p]),
     "mpsh_heff_apply_prepared": (C.c_int, [_p, C.c_int, _p, _p, _p, _p, _p]),
+    "mpsh_heff_apply_prepared_noalloc": (C.c_int, [_p, C.c_int, _p, _p, _p, _p]),
     "mpsh_heff_apply_prepared_stages": (C.c_int, [_p, C.c_int, _i32, _p, _p, _p, _p, _p]),
     "mpsh_heff_tune": (C.c_int, [_p, C.c_int, _p, _p, _p, _p, _i32, _i32, _pi32, _p]),
     "mpsh_heff_destroy": (None, [_p]),

@@ -327,6 +327,17 @@ struct mpsh_heff_s {
     }
     return it->second.second;
   }
+  // the tables already on the current device, or nullptr (no upload)
+  const HeffDev* peek() {
+    std::lock_guard<std::mutex> g(mu);
+    auto it = dev.find(current_device());
+    return it == dev.end() ? nullptr : &it->second;
+  }
+  const DevGemm* peek_variant(const VariantKey& k) {
+    std::lock_guard<std::mutex> g(mu);
+    auto it = var.find({current_device(), k});
+    return it == var.end() ? nullptr : &it->second.second;
+  }
   const HeffDev& get() {
     std::lock_guard<std::mutex> g(mu);
     int d = current_device();
@@ -343,6 +354,49 @@ struct mpsh_heff_s {
     return it->second;
   }
 };
+
+// Scratch of the allocation-free H_eff application: one copy of this array per
+// GPU, held until the process ends.  Host code uses only the device address that
+// hipGetSymbolAddress returns for the current device (kept per device below),
+// never the array's name.  Every call on one GPU shares it: calls that overlap
+// on one GPU race.  Nothing in it needs to start at zero (the kernels write every
+// element they later read).
+static __device__ __align__(256) unsigned char scratch[MPSH_NOALLOC_SCRATCH_BYTES];
+static constexpr int kMaxDevices = 64;
+static std::mutex g_scratch_mu;
+static void* g_scratch_ptr[kMaxDevices];  // per device: address of `scratch`, null until looked up
+
+// address of the current device's `scratch` (looked up once per device, no allocation)
+static void* noalloc_scratch() {
+  const int d = current_device();
+  if (d < 0 || d >= kMaxDevices) throw Error(ERR_UNSUPPORTED, "device index beyond the scratch table");
+  std::lock_guard<std::mutex> g(g_scratch_mu);
+  if (!g_scratch_ptr[d]) {
+    void* ptr = nullptr;
+    check(hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch)), "hipGetSymbolAddress(scratch)");
+    if (!ptr || ((uintptr_t)ptr & 255)) throw Error(ERR_DEVICE, "scratch symbol address is not 256-byte aligned");
+    g_scratch_ptr[d] = ptr;
+  }
+  return g_scratch_ptr[d];
+}
+
+// The kernel stages of one prepared H_eff application on a workspace of
+// mpsh_heff_apply_workspace bytes (mask: 1 pack v, 2 step-1 GEMM, 4 MPO pass,
+// 8 step-4 GEMM).
+static void heff_stages(const HeffPlan& p, const HeffDev& d, const DevGemm& g1, const DevGemm& g4, mpsh_dtype dt,
+                        int32_t stages, const void* prepared, const void* v, void* out, char* ws, hipStream_t s) {
+  const int es = elem_size(dt);
+  const char* pr = (const char*)prepared;
+  void* vp = ws + p.vp_off * es;
+  void* T1 = ws + p.t1_off * es;
+  void* T3 = ws + p.t3_off * es;
+  void* Pt = ws + p.part_off * es;
+  const int mode = gemm_mode(dt);
+  if (stages & 1) check(launch_copy(d.pack_v, es, v, vp, s), "pack v");
+  if (stages & 2) check(launch_gemm(g1, mode, vp, pr + p.el_off * es, T1, Pt, s), "step-1 gemm");
+  if (stages & 4) check(launch_mpo_apply(d.mpo, es, pr + p.pw_off * es, T1, T3, s), "mpo");
+  if (stages & 8) check(launch_gemm(g4, mode, T3, pr + p.er_off * es, out, Pt, s), "step-4 gemm");
+}
 
 // environment updates and the H_eff adjoint share one plan object
 struct mpsh_env_s {
@@ -647,6 +701,10 @@ mpsh_status mpsh_heff_prepare(mpsh_heff plan, mpsh_dtype dt, const void* envL, c
     throw Error(ERR_INVALID, "null argument");
   need_aligned(prepared, "prepared");
   const HeffDev& d = plan->get();
+  // the GEMM tables of the selected variants are uploaded here as well, so that
+  // applications on this device need no allocation
+  plan->variant(plan->sel1[gemm_mode(dt)]);
+  plan->variant(plan->sel4[gemm_mode(dt)]);
   hipStream_t s = (hipStream_t)stream;
   const int es = elem_size(dt);
   char* pr = (char*)prepared;
@@ -667,22 +725,9 @@ mpsh_status mpsh_heff_apply_prepared_stages(mpsh_heff plan, mpsh_dtype dt, int32
     throw Error(ERR_INVALID, "null argument");
   need_aligned(prepared, "prepared");
   need_aligned(workspace, "workspace");
-  const HeffDev& d = plan->get();
-  hipStream_t s = (hipStream_t)stream;
-  const int es = elem_size(dt);
-  const char* pr = (const char*)prepared;
-  char* ws = (char*)workspace;
-  void* vp = ws + p.vp_off * es;
-  void* T1 = ws + p.t1_off * es;
-  void* T3 = ws + p.t3_off * es;
-  void* Pt = ws + p.part_off * es;
   const int mode = gemm_mode(dt);
-  if (stages & 1) check(launch_copy(d.pack_v, es, v, vp, s), "pack v");
-  const DevGemm& g1 = plan->variant(plan->sel1[mode]);
-  const DevGemm& g4 = plan->variant(plan->sel4[mode]);
-  if (stages & 2) check(launch_gemm(g1, gemm_mode(dt), vp, pr + p.el_off * es, T1, Pt, s), "step-1 gemm");
-  if (stages & 4) check(launch_mpo_apply(d.mpo, es, pr + p.pw_off * es, T1, T3, s), "mpo");
-  if (stages & 8) check(launch_gemm(g4, gemm_mode(dt), T3, pr + p.er_off * es, out, Pt, s), "step-4 gemm");
+  heff_stages(p, plan->get(), plan->variant(plan->sel1[mode]), plan->variant(plan->sel4[mode]), dt, stages,
+              prepared, v, out, (char*)workspace, (hipStream_t)stream);
   return MPSH_OK;
   API_CATCH
 }
@@ -690,6 +735,33 @@ mpsh_status mpsh_heff_apply_prepared_stages(mpsh_heff plan, mpsh_dtype dt, int32
 mpsh_status mpsh_heff_apply_prepared(mpsh_heff plan, mpsh_dtype dt, const void* prepared, const void* v,
                                      void* out, void* workspace, void* stream) {
   return mpsh_heff_apply_prepared_stages(plan, dt, 15, prepared, v, out, workspace, stream);
+}
+
+mpsh_status mpsh_heff_apply_prepared_noalloc(mpsh_heff plan, mpsh_dtype dt, const void* prepared, const void* v,
+                                             void* out, void* stream) {
+  API_TRY
+  check_dtype(dt);
+  if (!plan) throw Error(ERR_INVALID, "null plan");
+  const HeffPlan& p = plan->p;
+  if ((!v && p.v.numel()) || (!out && p.out.numel()) || (!prepared && p.prep_elems))
+    throw Error(ERR_INVALID, "null argument");
+  need_aligned(prepared, "prepared");
+  const size_t need = mpsh_heff_apply_workspace(plan, dt);
+  if (need > (size_t)MPSH_NOALLOC_SCRATCH_BYTES)
+    throw Error(ERR_INVALID, "mpsh_heff_apply_prepared_noalloc: the plan needs " + std::to_string(need) +
+                                 " bytes of scratch, more than the fixed " +
+                                 std::to_string((size_t)MPSH_NOALLOC_SCRATCH_BYTES) +
+                                 " (use mpsh_heff_apply_prepared with a workspace)");
+  const int mode = gemm_mode(dt);
+  const HeffDev* d = plan->peek();
+  const DevGemm* g1 = plan->peek_variant(plan->sel1[mode]);
+  const DevGemm* g4 = plan->peek_variant(plan->sel4[mode]);
+  if (!d || !g1 || !g4)
+    throw Error(ERR_INVALID, "mpsh_heff_apply_prepared_noalloc: the plan's tables are not on this device "
+                             "(call mpsh_heff_prepare for this dtype on this device first)");
+  heff_stages(p, *d, *g1, *g4, dt, 15, prepared, v, out, (char*)noalloc_scratch(), (hipStream_t)stream);
+  return MPSH_OK;
+  API_CATCH
 }
 
 // Block-size histogram signature of a plan's two GEMMs (totals and averages in
