@@ -29,8 +29,7 @@ METRIC = "H_eff matvecs/sec + DMRG sweep time at χ=2048; |ΔE| vs FP64 CPU orac
 # HBM bytes per GEMM launch: rocprofv3 FETCH_SIZE x 2.000 + WRITE_SIZE x 1.000
 # (scales from a 256 MiB device copy), profiles/heff_cyl6x24_chi2048_f32_hbm_traffic.txt
 TRAFFIC = {("cyl6x24_chi2048", "f32", "step1"): 228.8e6}  # step-1 variant gemm_dma<1, 64, 128, 4>
-# algorithmic bytes of the combined-MPO pass (T1 read + T3 written), FP32 / FP64
-MPO_BYTES = {}
+
 
 
 def parse():
@@ -153,9 +152,10 @@ def env_update_rates(ts64, steps):
     envL / envR and MPO tensors; FP64 and FP32, ms per update (inputs in HBM)."""
     import torch
     import mps_amd as M
+    from mps_amd.tensornetalgs import envupdate_plan
     v, L, R, WL, WR = ts64
     A = M.randn((v.space[0], v.space[1], v.space[3]), 0, seed=1911, stream=21, device=v.device)
-    out = {}
+    out = {"path": "fused plan (mpsh_envupdate_*)", "note": "GFLOP/s at the reference chain's FLOPs"}
     for dt, name in ((torch.float64, "f64"), (torch.float32, "f32")):
         a, l, r, wl, wr = (t.to(dt) for t in (A, L, R, WL, WR))
         for side, fn, args in (("left", M._mpsupdateleft, (l, a, wl)), ("right", M._mpsupdateright, (r, a, wr))):
@@ -166,7 +166,20 @@ def env_update_rates(ts64, steps):
             for _ in range(steps):
                 fn(*args)
             torch.cuda.synchronize()
-            out[f"{side}_{name}_ms"] = round(1e3 * (time.perf_counter() - t0) / steps, 4)
+            ms = 1e3 * (time.perf_counter() - t0) / steps
+            fl = envupdate_plan(side, *args).ref_flops
+            out[f"{side}_{name}_ms"] = round(ms, 4)
+            out[f"{side}_{name}_gflops"] = round(fl / (ms * 1e-3) / 1e9, 1)
+            out["ref_gflop_per_update"] = round(fl / 1e9, 4)
+            if name == "f64":
+                for _ in range(1):
+                    fn(*args, path="plain")
+                torch.cuda.synchronize()
+                t0 = time.perf_counter()
+                for _ in range(max(2, steps // 4)):
+                    fn(*args, path="plain")
+                torch.cuda.synchronize()
+                out[f"{side}_plain_f64_ms"] = round(1e3 * (time.perf_counter() - t0) / max(2, steps // 4), 4)
     return out
 
 
@@ -364,7 +377,7 @@ def precision_tv(dev):
             "runs": runs}
 
 
-def dmrg_sweep(dev, prec=None, refine_tol="default"):
+def dmrg_sweep(dev, prec=None, refine_tol="default", **overrides):
     """One full two-site DMRG sweep (L->R->L, dmrg2sitesweep!) at chi=2048 on the
     6x24 Heisenberg cylinder from a seeded random U(1) MPS canonical at site 1:
     wall time of the sweep with its stage totals; the final energy is checked
@@ -378,6 +391,7 @@ def dmrg_sweep(dev, prec=None, refine_tol="default"):
         c["prec"] = prec
     if refine_tol != "default":
         c["refine_tol"] = refine_tol
+    c.update(overrides)
     L, N = c["W"] * c["Lx"], c["W"] * c["Lx"] // 2
     t0 = time.perf_counter()
     mpo = mt.heisenberg_cylinder_mpo(c["W"], c["Lx"], device=dev)
@@ -385,6 +399,9 @@ def dmrg_sweep(dev, prec=None, refine_tol="default"):
     env = ta.initialenv(mps, mpo)
     torch.cuda.synchronize()
     setup_s = time.perf_counter() - t0
+    env_bytes = sum(t.data.numel() * t.data.element_size() for t in env)
+    mps_bytes = sum(t.data.numel() * t.data.element_size() for t in mps.As)
+    torch.cuda.reset_peak_memory_stats()
     print(f"dmrg sweep ({c['prec']} matvecs): random chi={c['chi']} MPS and environments ready ({setup_s:.1f} s)",
           file=sys.stderr, flush=True)
     kw = dict(maxdim=c["maxdim"], tol=c["tol"], lanczostol=c["lanczostol"], krylovdim=c["krylovdim"],
@@ -407,6 +424,8 @@ def dmrg_sweep(dev, prec=None, refine_tol="default"):
            "max_bond": max(mps.bonddims()), "energy_final": energies[-1], "energy_first": energies[0],
            "energy_measure": e_measure, "energy_rel_diff_vs_measure": abs(energies[-1] - e_measure) / abs(e_measure),
            "max_discarded_weight": max(x["discarded_weight"] for x in log if "discarded_weight" in x),
+           "memory_gb": {"environments_initial": round(env_bytes / 1e9, 3), "mps_initial": round(mps_bytes / 1e9, 3),
+                         "device_peak_allocated_during_sweep": round(torch.cuda.max_memory_allocated() / 1e9, 3)},
            "autotune_in_sweep": {k: ta.TUNE_COUNTS[k] - tc0[k] for k in tc0},
            "settings": {k: c[k] for k in ("maxdim", "tol", "lanczostol", "krylovdim", "krylovmaxiter", "prec",
                                            "refine_tol")}}
@@ -538,6 +557,7 @@ def main():
                          "algorithmic_bytes_per_launch": plan.step_elems[si] * ebytes,
                          "traffic": TRAFFIC.get((args.workload, prec, key))}
     dom = max(launches, key=lambda k: launches[k]["avg_launch_ms"])
+    mpo_bytes = (plan.t1_elems + plan.t3_elems) * ebytes
     D = launches[dom]
     result = {
         "metric": METRIC,
@@ -568,10 +588,10 @@ def main():
                      "flops_per_launch": D["flops_per_launch"], "avg_launch_ms": D["avg_launch_ms"],
                      "algorithmic_bytes_per_launch": D["algorithmic_bytes_per_launch"],
                      "dominant_of": "the longer of the two GEMM launches per matvec", "launches": launches},
-        "mpo_pass_roofline": {"bound": "hbm", "achieved": round(MPO_BYTES.get((args.workload, prec), 0.0) /
-                                                                 (stages[2] * 1e-3) / 1e9, 1) if stages[2] > 0 else 0.0,
-                              "peak": HBM_PEAK / 1e9, "unit": "GB/s",
-                              "bytes": MPO_BYTES.get((args.workload, prec)), "avg_launch_ms": round(stages[2], 5)},
+        "mpo_pass_roofline": {"bound": "hbm", "achieved": round(mpo_bytes / (stages[2] * 1e-3) / 1e9, 1)
+                              if stages[2] > 0 else 0.0, "peak": HBM_PEAK / 1e9, "unit": "GB/s",
+                              "algorithmic_bytes": mpo_bytes, "avg_launch_ms": round(stages[2], 5),
+                              "note": "T1 read once + T3 written once (the combined-MPO matrices stay in LDS)"},
     }
     # E = <v|H v>/<v|v> of this matvec in the benchmark precision, FP64 device path as reference
     v64 = ts64[0]
@@ -612,9 +632,10 @@ def main():
         result["dmrg_sweep"], mps, mpo = dmrg_sweep(dev)
         swept = (mps, mpo)
         # the same sweep from the same start with FP64 matvecs throughout
-        r64, mps64, _ = dmrg_sweep(dev, prec="f64", refine_tol=None)
+        r64, mps64, _ = dmrg_sweep(dev, prec="f64", refine_tol=None, lanczostol=DMRG_CFG["refine_tol"],
+                                   krylovmaxiter=DMRG_CFG["krylovmaxiter"] + 1)
         result["dmrg_sweep_f64"] = {k: r64[k] for k in ("sweep_s", "stage_s", "matvecs", "energy_final",
-                                                        "energy_measure", "max_discarded_weight")}
+                                                        "energy_measure", "max_discarded_weight", "settings")}
         e32, e64s = result["dmrg_sweep"]["energy_final"], r64["energy_final"]
         result["dmrg_sweep_f64"]["energy_rel_diff_vs_refined_f32_sweep"] = abs(e32 - e64s) / abs(e64s)
         result["dmrg_sweep_f64"]["note"] = ("truncation to chi=2048 cuts through near-degenerate singular "

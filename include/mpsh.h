@@ -116,6 +116,10 @@ mpsh_status mpsh_heff_plan(mpsh_struct v, mpsh_struct envL, mpsh_struct envR, mp
 /* stats[0..9]: reference-order FLOPs, reference-order elements moved, step FLOPs x4,
  * step elements x4 (the reference's own four contracts, src/apply.jl:99-103). */
 mpsh_status mpsh_heff_stats(mpsh_heff plan, int64_t* stats);
+/* stats[0..7] of this engine's own stages (instrumentation, no reference counterpart):
+ * T1 elements, T3 elements, combined-MPO elements, MPO-pass FLOPs, step-1 GEMM FLOPs,
+ * step-4 GEMM FLOPs, packed-v elements, per-matvec workspace elements. */
+mpsh_status mpsh_heff_engine_stats(mpsh_heff plan, int64_t* stats);
 
 /* One-shot application, the drop-in for _applymps2site: workspace of
  * mpsh_heff_workspace() bytes, 16-byte aligned. */

@@ -75,6 +75,7 @@ _SIGS = {
     "mpsh_heff_plan": (C.c_int, [_p, _p, _p, _p, _p, C.POINTER(_p), C.POINTER(_p)]),
     "mpsh_heff_workspace": (_sz, [_p, C.c_int]),
     "mpsh_heff_stats": (C.c_int, [_p, _pi64]),
+    "mpsh_heff_engine_stats": (C.c_int, [_p, _pi64]),
     "mpsh_heff_apply": (C.c_int, [_p, C.c_int, _p, _p, _p, _p, _p, _p, _p, _p]),
     "mpsh_heff_prepared_size": (_sz, [_p, C.c_int]),
     "mpsh_heff_apply_workspace": (_sz, [_p, C.c_int]),

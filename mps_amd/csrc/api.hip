@@ -80,17 +80,38 @@ struct DevBuf {
   }
 };
 
+// Every table of one plan on one device in a single allocation and a single
+// upload: fill(f) runs the builder f twice, first only sizing the tables, then
+// staging them at their final offsets, and copies the staged bytes once.
 struct DevTables {
-  std::vector<std::unique_ptr<DevBuf>> bufs;
+  DevBuf buf;
+  std::vector<char> host;
+  size_t off = 0;
+  bool sizing = false;
   template <class T>
   const T* up(const std::vector<T>& v) {
     if (v.empty()) return nullptr;
-    auto b = std::make_unique<DevBuf>();
-    check(hipMalloc(&b->p, v.size() * sizeof(T)), "hipMalloc(plan tables)");
-    check(upload_tables(b->p, v.data(), v.size() * sizeof(T)), "upload(plan tables)");
-    const T* r = (const T*)b->p;
-    bufs.push_back(std::move(b));
-    return r;
+    const size_t bytes = v.size() * sizeof(T);
+    off = (off + 255) / 256 * 256;
+    const size_t o = off;
+    off += bytes;
+    if (sizing) return nullptr;
+    std::memcpy(host.data() + o, v.data(), bytes);
+    return (const T*)((char*)buf.p + o);
+  }
+  template <class F>
+  void fill(F&& f) {
+    sizing = true;
+    off = 0;
+    f(*this);
+    const size_t total = std::max<size_t>(off, 256);
+    check(hipMalloc(&buf.p, total), "hipMalloc(plan tables)");
+    host.assign(total, 0);
+    sizing = false;
+    off = 0;
+    f(*this);
+    check(upload_tables(buf.p, host.data(), std::max<size_t>(off, 1)), "upload(plan tables)");
+    std::vector<char>().swap(host);
   }
   DevCopy copy(const CopyPlan& c) {
     DevCopy d;
@@ -223,7 +244,8 @@ struct mpsh_releg_s {
     auto it = dev.find(d);
     if (it == dev.end()) {
       auto t = std::make_unique<DevTables>();
-      DevCopy c = t->copy(p.copy);
+      DevCopy c;
+      t->fill([&](DevTables& T) { c = T.copy(p.copy); });
       it = dev.emplace(d, std::make_pair(std::move(t), c)).first;
     }
     return it->second.second;
@@ -249,10 +271,12 @@ struct mpsh_contract_s {
     if (it == dev.end()) {
       ContractDev c;
       c.t = std::make_unique<DevTables>();
-      c.ga = c.t->copy(p.FA.gather);
-      c.gb = c.t->copy(p.FB.gather);
-      c.sc = c.t->copy(p.scatter);
-      c.gm = c.t->gemm(p.gemm);
+      c.t->fill([&](DevTables& T) {
+        c.ga = T.copy(p.FA.gather);
+        c.gb = T.copy(p.FB.gather);
+        c.sc = T.copy(p.scatter);
+        c.gm = T.gemm(p.gemm);
+      });
       c.gmd = c.gm;
       c.gmd.KS = 0;
       it = dev.emplace(d, std::move(c)).first;
@@ -322,7 +346,8 @@ struct mpsh_heff_s {
       const TileCfg c{std::get<2>(k), std::get<3>(k), std::get<4>(k)};
       GemmPlan gp = gemm_variant(stage == 1 ? p.g1 : p.g4, c, std::get<5>(k), mode);
       auto t = std::make_unique<DevTables>();
-      DevGemm dg = t->gemm(gp);
+      DevGemm dg;
+      t->fill([&](DevTables& T) { dg = T.gemm(gp); });
       it = var.emplace(std::make_pair(d, k), std::make_pair(std::move(t), dg)).first;
     }
     return it->second.second;
@@ -345,10 +370,12 @@ struct mpsh_heff_s {
     if (it == dev.end()) {
       HeffDev h;
       h.t = std::make_unique<DevTables>();
-      h.mpo = upload_mpo(*h.t, p);
-      h.pack_v = h.t->copy(p.pack_v);
-      h.pack_envL = h.t->copy(p.pack_envL);
-      h.pack_envR = h.t->copy(p.pack_envR);
+      h.t->fill([&](DevTables& T) {
+        h.mpo = upload_mpo(T, p);
+        h.pack_v = T.copy(p.pack_v);
+        h.pack_envL = T.copy(p.pack_envL);
+        h.pack_envR = T.copy(p.pack_envR);
+      });
       it = dev.emplace(d, std::move(h)).first;
     }
     return it->second;
@@ -410,23 +437,26 @@ struct mpsh_env_s {
     if (it == dev.end()) {
       EnvDev e;
       e.t = std::make_unique<DevTables>();
-      for (int c = 0; c < p.ncore; ++c) {
-        const EnvCore& C = p.core[c];
-        EnvCoreDev& D = e.core[c];
-        D.pack_ket = e.t->copy(C.pack_v);
-        D.pack_env = e.t->copy(C.pack_envL);
-        D.pack_bra = e.t->copy(C.pack_bra);
-        D.mpo = upload_mpo(*e.t, C);
-        D.g1 = e.t->gemm(C.g1);
-        D.gF = e.t->gemm(C.gF);
-        D.gF64 = e.t->gemm(p.core_f64[c].gF);
-        for (int k = 0; k < 3; ++k)
-          if (p.has_perm[c][k]) e.perm[c][k] = e.t->copy(p.perm[c][k].copy);
-      }
-      if (p.wr_is_identity) {
-        e.wr_f64 = e.t->up(p.wr_identity);
-        e.wr_f32 = e.t->up(std::vector<float>(p.wr_identity.begin(), p.wr_identity.end()));
-      }
+      const std::vector<float> wr32(p.wr_identity.begin(), p.wr_identity.end());
+      e.t->fill([&](DevTables& T) {
+        for (int c = 0; c < p.ncore; ++c) {
+          const EnvCore& C = p.core[c];
+          EnvCoreDev& D = e.core[c];
+          D.pack_ket = T.copy(C.pack_v);
+          D.pack_env = T.copy(C.pack_envL);
+          D.pack_bra = T.copy(C.pack_bra);
+          D.mpo = upload_mpo(T, C);
+          D.g1 = T.gemm(C.g1);
+          D.gF = T.gemm(C.gF);
+          D.gF64 = T.gemm(p.core_f64[c].gF);
+          for (int k = 0; k < 3; ++k)
+            if (p.has_perm[c][k]) e.perm[c][k] = T.copy(p.perm[c][k].copy);
+        }
+        if (p.wr_is_identity) {
+          e.wr_f64 = T.up(p.wr_identity);
+          e.wr_f32 = T.up(wr32);
+        }
+      });
       it = dev.emplace(d, std::move(e)).first;
     }
     return it->second;
@@ -663,6 +693,20 @@ mpsh_status mpsh_heff_plan(mpsh_struct v, mpsh_struct envL, mpsh_struct envR, mp
 }
 
 static size_t align16(size_t b) { return (b + 15) / 16 * 16; }
+
+mpsh_status mpsh_heff_engine_stats(mpsh_heff plan, int64_t* stats) {
+  if (!plan || !stats) return fail(ERR_INVALID, "null argument");
+  const HeffPlan& p = plan->p;
+  stats[0] = p.t1_elems;
+  stats[1] = p.t3_elems;
+  stats[2] = p.w_elems;
+  stats[3] = p.mpo_flops;
+  stats[4] = p.g1.flops;
+  stats[5] = p.g4.flops;
+  stats[6] = p.vp_elems;
+  stats[7] = p.ws_elems;
+  return MPSH_OK;
+}
 
 size_t mpsh_heff_prepared_size(mpsh_heff plan, mpsh_dtype dt) {
   if (!plan) return 0;

@@ -41,6 +41,10 @@ class HeffPlan:
         self.ref_elems = int(st[1])
         self.step_flops = [int(st[2 + i]) for i in range(4)]
         self.step_elems = [int(st[6 + i]) for i in range(4)]
+        es = (L._i64 * 8)()
+        check(lib().mpsh_heff_engine_stats(plan, es))
+        self.t1_elems, self.t3_elems, self.w_elems, self.mpo_flops = (int(x) for x in es[:4])
+        self.vp_elems = int(es[6])
 
     def workspace_bytes(self, prec: str) -> int:
         """Workspace of the one-shot apply (prepared operator included)."""
@@ -555,22 +559,142 @@ def eigsolve(H: HeffOperator, x0: SymTensor, tol=1e-7, krylovdim=5, maxiter=8, r
 # ---------------------------------------------------------------------------
 # environments and the sweep   (src/dmrg.jl)
 # ---------------------------------------------------------------------------
-def initialenv(mps, mpo, at: int = 1):
-    """``initialenv(mps, mpo; at)`` (dmrg.jl:9-36); returns env[1..L+2] as a 0-based list."""
+class EnvStore:
+    """The environments of a sweep (env[0..L+1], as ``initialenv`` returns them)
+    under a device-memory budget.
+
+    The reference keeps environments in files and prefetches them
+    (``_initialenv_tempfiles`` / ``dmrg2sitesweep_envio!``, src/dmrg_envio.jl:5-39,
+    151-261).  Here they stay in HBM up to ``budget`` bytes; past it, the resident
+    environment farthest from the last one accessed is dropped (never the two
+    boundary environments, nor the ``protect`` most recently accessed), and a
+    dropped environment is recomputed when it is read, from its neighbour towards
+    its chain end with the fused update: a left environment env[i] =
+    _mpsupdateleft(env[i-1], A_i, W_i), a right one env[i] =
+    _mpsupdateright(env[i+1], A_i, W_i), recursively.  An environment update at
+    chi=2048 costs well under a millisecond on the GPU, less than moving the
+    ~50 MB environment over PCIe and back, so dropping beats offloading.
+
+    The two boundary environments and the ``protect`` most recently accessed
+    ones always stay resident, so a budget below that many environments is
+    exceeded by them.  Recomputation reads the MPS tensors as they are now: it is exact for the
+    sweep's access pattern, in which an environment is read only while the site
+    tensors it was built from are unchanged (a left environment env[i] from
+    A_1..A_i, a right one from A_i..A_L).  ``set(i, T, side)`` records which
+    side an environment belongs to (``initialenv`` and the sweeps call it);
+    ``env[i] = T`` works it out from the legs: a left environment env[i]
+    carries A_i's right bond leg first, a right one A_i's left bond leg.
+    Supports ``len``, ``env[i]`` and assignment, so ``dmrg2sitesweep`` and
+    ``dmrg2site_window`` take it in place of the list."""
+
+    def __init__(self, n: int, mps, mpo, budget: int, protect: int = 4, update_left=None, update_right=None):
+        self.t = [None] * n
+        self.kind = [None] * n
+        self.mps, self.mpo = mps, mpo
+        self.budget = int(budget)
+        self.protect = int(protect)
+        self.recent = []
+        self.last = 0
+        self.recomputed = 0
+        self.dropped = 0
+        self.peak = 0
+        self._left = update_left or _mpsupdateleft
+        self._right = update_right or _mpsupdateright
+
+    def __len__(self):
+        return len(self.t)
+
+    @staticmethod
+    def _bytes(T) -> int:
+        return T.data.numel() * T.data.element_size()
+
+    def resident_bytes(self) -> int:
+        return sum(self._bytes(T) for T in self.t if T is not None)
+
+    def _touch(self, i):
+        self.last = i
+        if i in self.recent:
+            self.recent.remove(i)
+        self.recent.append(i)
+        del self.recent[:-self.protect]
+
+    def _enforce(self):
+        n = len(self.t)
+        used = self.resident_bytes()
+        while used > self.budget:
+            cands = [i for i in range(1, n - 1) if self.t[i] is not None and i not in self.recent]
+            if not cands:
+                break
+            v = max(cands, key=lambda i: (abs(i - self.last), i))
+            used -= self._bytes(self.t[v])
+            self.t[v] = None
+            self.dropped += 1
+        self.peak = max(self.peak, used)
+
+    def set(self, i, T, side: str):
+        if side not in ("L", "R"):
+            raise ValueError("side must be 'L' or 'R'")
+        self.t[i] = T
+        self.kind[i] = side
+        self._touch(i)
+        self._enforce()
+
+    def __setitem__(self, i, T):
+        n = len(self.t)
+        if i == 0 or i == n - 1:
+            side = "L" if i == 0 else "R"
+        else:
+            A = self.mps.As[i - 1]
+            is_left, is_right = T.space[0] == A.space[2], T.space[0] == A.space[0]
+            if is_left == is_right:
+                raise ValueError(f"environment {i}: side not determined by its legs; use set(i, T, side)")
+            side = "L" if is_left else "R"
+        self.set(i, T, side)
+
+    def __getitem__(self, i):
+        T = self.t[i]
+        if T is None:
+            if self.kind[i] is None:
+                raise KeyError(f"environment {i} was never set")
+            if self.kind[i] == "L":
+                T = self._left(self[i - 1], self.mps.As[i - 1], self.mpo.Ws[i - 1])
+            else:
+                T = self._right(self[i + 1], self.mps.As[i - 1], self.mpo.Ws[i - 1])
+            self.recomputed += 1
+            self.t[i] = T
+            self._touch(i)
+            self._enforce()
+        else:
+            self._touch(i)
+        return T
+
+
+def _set_env(env, i, T, side):
+    """env[i] = T for a list; for an :class:`EnvStore` also records the side."""
+    if isinstance(env, EnvStore):
+        env.set(i, T, side)
+    else:
+        env[i] = T
+
+
+def initialenv(mps, mpo, at: int = 1, budget: int = None):
+    """``initialenv(mps, mpo; at)`` (dmrg.jl:9-36); returns env[1..L+2] as a 0-based
+    list, or with ``budget`` (bytes) as an :class:`EnvStore` that keeps at most
+    about that much of them in device memory and recomputes the rest on demand."""
     from .symtensors import fill
     if mps.center != at:
         raise ValueError(f"MPS center has to be {at}")
     lx = len(mps)
-    env = [None] * (lx + 2)
+    env = [None] * (lx + 2) if budget is None else EnvStore(lx + 2, mps, mpo, budget)
     lVa, rVa = mps.leftspace(), mps.rightspace()
     lVw, rVw = mpo.leftspace(), mpo.rightspace()
     dt, dev = mps.As[0].dtype, mps.As[0].device
-    env[0] = fill(1.0, (lVa.dual(), lVw.dual(), lVa), dtype=dt, device=dev)
-    env[lx + 1] = fill(1.0, (rVa, rVw, rVa.dual()), dtype=dt, device=dev)
+    _set_env(env, 0, fill(1.0, (lVa.dual(), lVw.dual(), lVa), dtype=dt, device=dev), "L")
+    _set_env(env, lx + 1, fill(1.0, (rVa, rVw, rVa.dual()), dtype=dt, device=dev), "R")
     for l in range(lx, at - 1, -1):
-        env[l] = _mpsupdateright(env[l + 1], mps.As[l - 1], mpo.Ws[l - 1])
+        _set_env(env, l, _mpsupdateright(env[l + 1], mps.As[l - 1], mpo.Ws[l - 1]), "R")
     for l in range(1, at):
-        env[l] = _mpsupdateleft(env[l - 1], mps.As[l - 1], mpo.Ws[l - 1])
+        _set_env(env, l, _mpsupdateleft(env[l - 1], mps.As[l - 1], mpo.Ws[l - 1]), "L")
     return env
 
 
@@ -613,7 +737,7 @@ def dmrg2site_window(mps, mpo, env, nbonds: int, maxdim: int = 200, tol: float =
         sync()
         t4 = time.perf_counter()
         mps.As[l - 1] = splitleg(r.u(), 1, A.space[0:2])
-        env[l] = _mpsupdateleft(env[l - 1], mps.As[l - 1], Ws[l - 1])
+        _set_env(env, l, _mpsupdateleft(env[l - 1], mps.As[l - 1], Ws[l - 1]), "L")
         A = splitleg(r.sv(), 2, AA2.space[2:4])
         sync()
         t5 = time.perf_counter()
@@ -699,7 +823,7 @@ def dmrg2sitesweep(mps, mpo, env, maxdim: int = 200, tol: float = 1e-9, lanczost
         r, AA2 = trunc(v, AA)
         energies.append(energy(AA2, l))
         mps.As[l - 1] = splitleg(r.u(), 1, A.space[0:2])
-        env[l] = _mpsupdateleft(env[l - 1], mps.As[l - 1], Ws[l - 1])
+        _set_env(env, l, _mpsupdateleft(env[l - 1], mps.As[l - 1], Ws[l - 1]), "L")
         A = splitleg(r.sv(), 2, AA2.space[2:4])
     l = lx - 1
     AA = contract(A, (1, 2, -1), mps.As[l], (-1, 3, 4))
@@ -710,7 +834,7 @@ def dmrg2sitesweep(mps, mpo, env, maxdim: int = 200, tol: float = 1e-9, lanczost
         e = energy(AA2, l)
         energies.append(e)
         mps.As[l] = splitleg(r.v(), 2, AA2.space[2:4])
-        env[l + 1] = _mpsupdateright(env[l + 2], mps.As[l], Ws[l])
+        _set_env(env, l + 1, _mpsupdateright(env[l + 2], mps.As[l], Ws[l]), "R")
         A = splitleg(r.us(), 1, AA2.space[0:2])
         AA = contract(mps.As[l - 2], (1, 2, -1), A, (-1, 3, 4))
         v = solve(AA, l - 1)
@@ -719,6 +843,6 @@ def dmrg2sitesweep(mps, mpo, env, maxdim: int = 200, tol: float = 1e-9, lanczost
     r, AA2 = trunc(v, AA)
     energies.append(energy(AA2, l))
     mps.As[l] = splitleg(r.v(), 2, AA2.space[2:4])
-    env[l + 1] = _mpsupdateright(env[l + 2], mps.As[l], Ws[l])
+    _set_env(env, l + 1, _mpsupdateright(env[l + 2], mps.As[l], Ws[l]), "R")
     mps.As[l - 1] = splitleg(r.us(), 1, AA2.space[0:2])
     return energies

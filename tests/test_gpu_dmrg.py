@@ -246,3 +246,32 @@ def test_dmrg_sweep_with_preconditioned_svd():
             os.environ["MPSH_SVD_PRECOND"] = old
     assert widest[0] >= 96, widest[0]  # the preconditioned class was exercised
     np.testing.assert_allclose(runs[0], runs[1], rtol=1e-10, atol=0)
+
+
+def test_sweeps_under_an_environment_budget_equal_unbounded_sweeps():
+    """Two sweeps (XXZ L=24, chi=128) with all environments resident and with an
+    EnvStore holding about a quarter of them (the rest dropped and recomputed by
+    the fused updates when read): the energies are bitwise equal, so recomputed
+    environments are the ones dropped; the store stayed near its budget."""
+    L, N, chi = 24, 12, 128
+    kw = dict(maxdim=chi, tol=1e-10, lanczostol=1e-9, krylovdim=6, krylovmaxiter=4)
+    runs, store = [], None
+    full = None
+    for budgeted in (False, True):
+        mps = mt.random_mps(L, N, chi, seed=41, device=DEV)
+        mpo = mt.xxz_u1mpo(L, 1.0, device=DEV)
+        if not budgeted:
+            env = ta.initialenv(mps, mpo)
+            full = sum(t.data.numel() * t.data.element_size() for t in env)
+            largest = max(t.data.numel() * t.data.element_size() for t in env)
+        else:
+            env = store = ta.initialenv(mps, mpo, budget=full // 4)
+        es = []
+        for _ in range(2):
+            es += ta.dmrg2sitesweep(mps, mpo, env, **kw)
+        runs.append(es)
+    print(f"env budget {full // 4} of {full} bytes: {store.dropped} dropped, {store.recomputed} recomputed, "
+          f"peak {store.peak}", flush=True)
+    assert runs[0] == runs[1]
+    assert store.dropped > 0 and store.recomputed > 0
+    assert store.peak <= full // 4 + (store.protect + 2) * largest
