@@ -125,14 +125,14 @@ def time_fused(plan, prec, ts, steps, warmup, stage_events=True, tune=True):
 
 def gemm_kernel(prec, tuned, step):
     """Name of the step-1 / step-4 GEMM kernel variant: the autotuned one, else the
-    one MPSH_HEFF_TILES or the defaults select (FP32 and FP64 DMA-streamed 64x128 /
-    64x64, BF16 128x64 / 64x64x2); DMA kernels carry their LDS ring depth (FP32 4,
-    FP64 3)."""
+    one MPSH_HEFF_TILES or the defaults select (DMA-streamed 64x128 / 64x64 in every
+    precision; BF16 runs the BF16-storage kernels, mode 3 writing BF16 T1, mode 4
+    the FP32 output); DMA kernels carry their LDS ring depth (FP64 3, else 4)."""
     key = {"f32": "s1", "bf16": "s1h", "f64": "s1d"}[prec]
-    cfg = {"f32": "64x128x0", "bf16": "128x64x1", "f64": "64x128x0"}[prec]
+    cfg = "64x128x0"
     if step == "step4":
         key = key.replace("s1", "s4")
-        cfg = {"f32": "64x64x0", "bf16": "64x64x2", "f64": "64x64x0"}[prec]
+        cfg = "64x64x0"
     if tuned:
         cfg = tuned[step]
     else:
@@ -141,7 +141,7 @@ def gemm_kernel(prec, tuned, step):
                 cfg = item.split("=", 1)[1]
     parts = (cfg.split("x") + ["1"])[:3]
     bm, bn, ks = parts
-    mode = {"f64": 0, "f32": 1, "bf16": 2}[prec]
+    mode = {"f64": 0, "f32": 1, "bf16": 3 if step == "step1" else 4}[prec]
     return (f"gemm_dma<{mode}, {bm}, {bn}, {3 if mode == 0 else 4}>" if ks == "0"
             else f"gemm_grouped<{mode}, {bm}, {bn}, {ks}>")
 
@@ -575,7 +575,7 @@ def main():
         "config": {"workload": args.workload, "chi": sum(V for V in ts[0].space[0].dims),
                    "sectors_per_bond": len(ts[0].space[0].charges), "mpo_D": ts[1].space[1].dim(),
                    "path": "fused", "precision": {"f32": "FP32 MFMA", "f64": "FP64 MFMA",
-                                                  "bf16": "BF16 MFMA operands, FP32 accumulate"}[prec]},
+                                                  "bf16": "BF16 storage and MFMA operands, FP32 accumulate"}[prec]},
         "gflops_per_matvec": round(F / 1e9, 4),
         "achieved_tflops": round(F / (ms * 1e-3) / 1e12, 3),
         "matvec_frac_of_peak": round(F / (ms * 1e-3) / PEAK[prec], 4),

@@ -121,10 +121,18 @@ struct DevTables {
     d.ntiles = c.ntiles;
     return d;
   }
-  DevGemm gemm(const GemmPlan& g) {
+  // mode >= 0: the balanced per-work-group schedule for this kernel variant's
+  // resident work groups (mode as in launch_gemm); mode < 0: shared queues
+  DevGemm gemm(const GemmPlan& g, int mode = -1) {
     DevGemm d;
-    std::vector<int32_t> qoff;
-    std::vector<GemmTile> t = g.tiles(&qoff);
+    std::vector<int32_t> qoff, woff;
+    const int slots = mode >= 0 ? gemm_slots_per_xcd(mode, g.BM, g.BN, g.KS) : 0;
+    std::vector<GemmTile> t = slots > 0 ? g.schedule(slots, chunk_k(mode, g.KS), &woff) : g.tiles(&qoff);
+    if (slots > 0) {
+      (void)g.tiles(&qoff);
+      d.q.slots = slots;
+      d.q.woff = up(woff);
+    }
     for (int x = 0; x <= NXCD; ++x) d.q.off[x] = qoff[x];
     std::vector<RedDesc> r = g.reductions();
     d.probs = up(g.probs);
@@ -203,6 +211,18 @@ HeffTiles tiles_from_env() {
   return t;
 }
 int gemm_mode(mpsh_dtype dt) { return dt == MPSH_F64 ? 0 : (dt == MPSH_F32 ? 1 : 2); }
+
+// GEMM tile schedule, read once from MPSH_GEMM_SCHEDULE: "balanced" gives each
+// work group its own cost-balanced tile list (GemmPlan::schedule), anything else
+// keeps the shared per-XCD queues (measured equal or faster on the H_eff GEMMs).
+// Returns the mode to build balanced tables for, or -1 for the shared queues.
+int schedule_mode(int mode) {
+  static const int on = [] {
+    const char* e = std::getenv("MPSH_GEMM_SCHEDULE");
+    return (e && std::string(e) == "balanced") ? 1 : 0;
+  }();
+  return on ? mode : -1;
+}
 
 void check_dtype(int dt) {
   if (dt != MPSH_F64 && dt != MPSH_F32 && dt != MPSH_BF16) throw Error(ERR_INVALID, "unknown dtype");
@@ -299,11 +319,14 @@ DevMpo upload_mpo(DevTables& t, const Step13& p) {
   m.groups = t.up(p.groups);
   m.inrows = t.up(p.inrows);
   m.outrows = t.up(p.outrows);
-  m.work = t.up(p.work);
+  for (int i = 0; i < 3; ++i) m.work[i] = t.up(p.work[i]);
   m.nterms = (int)p.terms.size();
-  m.nwork = (int)p.work.size();
+  for (int i = 0; i < 3; ++i) m.nwork[i] = (int)p.work[i].size();
   m.cmax = (int)p.maxC;
   m.rmax = (int)p.maxR;
+  m.t1_elems = p.t1_elems;
+  m.t3_elems = p.t3_elems;
+  mpo_options(m);
   return m;
 }
 
@@ -311,7 +334,7 @@ DevMpo upload_mpo(DevTables& t, const Step13& p) {
 struct EnvCoreDev {
   DevCopy pack_ket, pack_env, pack_bra;
   DevMpo mpo;
-  DevGemm g1, gF, gF64;
+  DevGemm g1, g1d, gF, gF64;  // step 1 and the final GEMM, FP32 and FP64 schedules
 };
 struct EnvDev {
   std::unique_ptr<DevTables> t;
@@ -347,7 +370,7 @@ struct mpsh_heff_s {
       GemmPlan gp = gemm_variant(stage == 1 ? p.g1 : p.g4, c, std::get<5>(k), mode);
       auto t = std::make_unique<DevTables>();
       DevGemm dg;
-      t->fill([&](DevTables& T) { dg = T.gemm(gp); });
+      t->fill([&](DevTables& T) { dg = T.gemm(gp, schedule_mode(mode)); });
       it = var.emplace(std::make_pair(d, k), std::make_pair(std::move(t), dg)).first;
     }
     return it->second.second;
@@ -407,22 +430,59 @@ static void* noalloc_scratch() {
   return g_scratch_ptr[d];
 }
 
+static size_t align16(size_t b) { return (b + 15) / 16 * 16; }
+
+// Byte layout of an H_eff plan's prepared operator [envL_p | envR_p | W] and
+// per-matvec workspace [v_p | T1 | T3 | split-K partials] for one dtype.  FP64 /
+// FP32 keep everything in that type; BF16 keeps v_p, envL_p, envR_p, T1 and T3
+// in BF16 (the packs convert) with the combined MPO W and the split-K partials
+// in FP32, and runs the BF16-storage GEMM kernels (modes 3 and 4).
+struct HeffBytes {
+  size_t es;                          // bytes per packed / T1 / T3 element
+  size_t vp, t1, t3, part;            // workspace offsets
+  size_t el, er, w;                   // prepared-operator offsets
+  size_t prep, ws;                    // total sizes (16-byte multiples)
+  int wes;                            // bytes per W element
+  int mode1, mode4;                   // kernel modes of the step-1 / step-4 GEMMs
+};
+static HeffBytes heff_bytes(const HeffPlan& p, mpsh_dtype dt) {
+  HeffBytes b;
+  const bool h = dt == MPSH_BF16;
+  b.es = h ? 2 : (size_t)elem_size(dt);
+  const size_t ps = h ? 4 : b.es;  // partials
+  b.wes = h ? 4 : (int)b.es;
+  b.vp = p.vp_off * b.es;
+  b.t1 = p.t1_off * b.es;
+  b.t3 = p.t3_off * b.es;
+  b.part = p.part_off * b.es;
+  b.ws = align16(b.part + (size_t)std::max<int64_t>(p.ws_elems - p.part_off, 1) * ps);
+  b.el = p.el_off * b.es;
+  b.er = p.er_off * b.es;
+  b.w = h ? (p.pw_off * b.es + 255) / 256 * 256 : p.pw_off * b.es;
+  b.prep = align16(b.w + (size_t)p.w_elems * b.wes);
+  b.mode1 = h ? 3 : gemm_mode(dt);
+  b.mode4 = h ? 4 : gemm_mode(dt);
+  return b;
+}
+
 // The kernel stages of one prepared H_eff application on a workspace of
 // mpsh_heff_apply_workspace bytes (mask: 1 pack v, 2 step-1 GEMM, 4 MPO pass,
 // 8 step-4 GEMM).
 static void heff_stages(const HeffPlan& p, const HeffDev& d, const DevGemm& g1, const DevGemm& g4, mpsh_dtype dt,
                         int32_t stages, const void* prepared, const void* v, void* out, char* ws, hipStream_t s) {
-  const int es = elem_size(dt);
+  const HeffBytes b = heff_bytes(p, dt);
   const char* pr = (const char*)prepared;
-  void* vp = ws + p.vp_off * es;
-  void* T1 = ws + p.t1_off * es;
-  void* T3 = ws + p.t3_off * es;
-  void* Pt = ws + p.part_off * es;
-  const int mode = gemm_mode(dt);
-  if (stages & 1) check(launch_copy(d.pack_v, es, v, vp, s), "pack v");
-  if (stages & 2) check(launch_gemm(g1, mode, vp, pr + p.el_off * es, T1, Pt, s), "step-1 gemm");
-  if (stages & 4) check(launch_mpo_apply(d.mpo, es, pr + p.pw_off * es, T1, T3, s), "mpo");
-  if (stages & 8) check(launch_gemm(g4, mode, T3, pr + p.er_off * es, out, Pt, s), "step-4 gemm");
+  void* vp = ws + b.vp;
+  void* T1 = ws + b.t1;
+  void* T3 = ws + b.t3;
+  void* Pt = ws + b.part;
+  if (stages & 1)
+    check(dt == MPSH_BF16 ? launch_copy_to_bf16(d.pack_v, (const float*)v, vp, s)
+                          : launch_copy(d.pack_v, (int)b.es, v, vp, s),
+          "pack v");
+  if (stages & 2) check(launch_gemm(g1, b.mode1, vp, pr + b.el, T1, Pt, s), "step-1 gemm");
+  if (stages & 4) check(launch_mpo_apply(d.mpo, (int)b.es, pr + b.w, T1, T3, s), "mpo");
+  if (stages & 8) check(launch_gemm(g4, b.mode4, T3, pr + b.er, out, Pt, s), "step-4 gemm");
 }
 
 // environment updates and the H_eff adjoint share one plan object
@@ -446,9 +506,10 @@ struct mpsh_env_s {
           D.pack_env = T.copy(C.pack_envL);
           D.pack_bra = T.copy(C.pack_bra);
           D.mpo = upload_mpo(T, C);
-          D.g1 = T.gemm(C.g1);
-          D.gF = T.gemm(C.gF);
-          D.gF64 = T.gemm(p.core_f64[c].gF);
+          D.g1 = T.gemm(C.g1, schedule_mode(1));
+          D.g1d = T.gemm(C.g1, schedule_mode(0));
+          D.gF = T.gemm(C.gF, schedule_mode(1));
+          D.gF64 = T.gemm(p.core_f64[c].gF, schedule_mode(0));
           for (int k = 0; k < 3; ++k)
             if (p.has_perm[c][k]) e.perm[c][k] = T.copy(p.perm[c][k].copy);
         }
@@ -476,7 +537,8 @@ static void run_env_core(const mpsh_env_s& h, const EnvDev& d, int c, mpsh_dtype
   check(launch_copy(D.pack_env, es, env, ws + C.ep_off * es, s), "env: pack env");
   check(launch_copy(D.pack_bra, es, ket, ws + C.bp_off * es, s), "env: pack bra");
   check(launch_mpo_combine(D.mpo, es, WL, WR, ws + C.w_off * es, s), "env: mpo combine");
-  check(launch_gemm(D.g1, mode, ws + C.kp_off * es, ws + C.ep_off * es, ws + C.t1_off * es, ws + C.part_off * es, s),
+  check(launch_gemm(dt == MPSH_F64 ? D.g1d : D.g1, mode, ws + C.kp_off * es, ws + C.ep_off * es, ws + C.t1_off * es,
+                    ws + C.part_off * es, s),
         "env: step-1 gemm");
   check(launch_mpo_apply(D.mpo, es, ws + C.w_off * es, ws + C.t1_off * es, ws + C.t3_off * es, s), "env: mpo");
   check(launch_gemm(dt == MPSH_F64 ? D.gF64 : D.gF, mode, ws + C.t3_off * es, ws + C.bp_off * es, out,
@@ -678,9 +740,13 @@ mpsh_status mpsh_heff_plan(mpsh_struct v, mpsh_struct envL, mpsh_struct envR, mp
       throw Error(ERR_UNSUPPORTED, "combined MPO group wider than 64 channels");
     const TileCfg* c1[3] = {&t.s1d, &t.s1, &t.s1h};
     const TileCfg* c4[3] = {&t.s4d, &t.s4, &t.s4h};
+    // index m = gemm_mode(dt); BF16 (m = 2) runs the BF16-storage kernels (modes 3 and 4)
     for (int m = 0; m < 3; ++m) {
-      h->sel1[m] = VariantKey{1, m, c1[m]->BM, c1[m]->BN, c1[m]->KS, 0};
-      h->sel4[m] = VariantKey{4, m, c4[m]->BM, c4[m]->BN, c4[m]->KS, m == 0 ? t.split_target_d : t.split_target};
+      h->sel1[m] = VariantKey{1, m == 2 ? 3 : m, c1[m]->BM, c1[m]->BN, c1[m]->KS, 0};
+      h->sel4[m] = VariantKey{4, m == 2 ? 4 : m, c4[m]->BM, c4[m]->BN, c4[m]->KS,
+                              m == 0 ? t.split_target_d : t.split_target};
+      if (!tune_tile_ok(std::get<1>(h->sel1[m]), *c1[m]) || !tune_tile_ok(std::get<1>(h->sel4[m]), *c4[m]))
+        throw Error(ERR_INVALID, "MPSH_HEFF_TILES: no such kernel variant for this precision");
     }
   } catch (...) {
     delete h;
@@ -691,8 +757,6 @@ mpsh_status mpsh_heff_plan(mpsh_struct v, mpsh_struct envL, mpsh_struct envR, mp
   return MPSH_OK;
   API_CATCH
 }
-
-static size_t align16(size_t b) { return (b + 15) / 16 * 16; }
 
 mpsh_status mpsh_heff_engine_stats(mpsh_heff plan, int64_t* stats) {
   if (!plan || !stats) return fail(ERR_INVALID, "null argument");
@@ -709,13 +773,13 @@ mpsh_status mpsh_heff_engine_stats(mpsh_heff plan, int64_t* stats) {
 }
 
 size_t mpsh_heff_prepared_size(mpsh_heff plan, mpsh_dtype dt) {
-  if (!plan) return 0;
-  return align16((size_t)plan->p.prep_elems * (size_t)elem_size(dt));
+  if (!plan || (dt != MPSH_F64 && dt != MPSH_F32 && dt != MPSH_BF16)) return 0;
+  return heff_bytes(plan->p, dt).prep;
 }
 
 size_t mpsh_heff_apply_workspace(mpsh_heff plan, mpsh_dtype dt) {
-  if (!plan) return 0;
-  return align16((size_t)std::max<int64_t>(plan->p.ws_elems, 1) * (size_t)elem_size(dt));
+  if (!plan || (dt != MPSH_F64 && dt != MPSH_F32 && dt != MPSH_BF16)) return 0;
+  return heff_bytes(plan->p, dt).ws;
 }
 
 size_t mpsh_heff_workspace(mpsh_heff plan, mpsh_dtype dt) {
@@ -750,11 +814,16 @@ mpsh_status mpsh_heff_prepare(mpsh_heff plan, mpsh_dtype dt, const void* envL, c
   plan->variant(plan->sel1[gemm_mode(dt)]);
   plan->variant(plan->sel4[gemm_mode(dt)]);
   hipStream_t s = (hipStream_t)stream;
-  const int es = elem_size(dt);
+  const HeffBytes b = heff_bytes(p, dt);
   char* pr = (char*)prepared;
-  check(launch_copy(d.pack_envL, es, envL, pr + p.el_off * es, s), "pack envL");
-  check(launch_copy(d.pack_envR, es, envR, pr + p.er_off * es, s), "pack envR");
-  check(launch_mpo_combine(d.mpo, es, WL, WR, pr + p.pw_off * es, s), "mpo combine");
+  if (dt == MPSH_BF16) {  // FP32 operands packed into BF16, the combined MPO in FP32
+    check(launch_copy_to_bf16(d.pack_envL, (const float*)envL, pr + b.el, s), "pack envL");
+    check(launch_copy_to_bf16(d.pack_envR, (const float*)envR, pr + b.er, s), "pack envR");
+  } else {
+    check(launch_copy(d.pack_envL, (int)b.es, envL, pr + b.el, s), "pack envL");
+    check(launch_copy(d.pack_envR, (int)b.es, envR, pr + b.er, s), "pack envR");
+  }
+  check(launch_mpo_combine(d.mpo, b.wes, WL, WR, pr + b.w, s), "mpo combine");
   return MPSH_OK;
   API_CATCH
 }
@@ -854,45 +923,46 @@ mpsh_status mpsh_heff_tune(mpsh_heff plan, mpsh_dtype dt, const void* prepared, 
   if (!cached && p.g1.probs.size() + p.g4.probs.size() > 0) {
     hipStream_t s = (hipStream_t)stream;
     const HeffDev& d = plan->get();
-    const int es = elem_size(dt);
+    const HeffBytes b = heff_bytes(p, dt);
     const char* pr = (const char*)prepared;
     char* ws = (char*)workspace;
-    void* vp = ws + p.vp_off * es;
-    void* T1 = ws + p.t1_off * es;
-    void* T3 = ws + p.t3_off * es;
-    void* Pt = ws + p.part_off * es;
+    void* vp = ws + b.vp;
+    void* T1 = ws + b.t1;
+    void* T3 = ws + b.t3;
+    void* Pt = ws + b.part;
     reps = std::max(1, std::min(reps, 50));
     hipEvent_t e0, e1;
     check(hipEventCreate(&e0), "hipEventCreate");
     check(hipEventCreate(&e1), "hipEventCreate");
-    auto timeit = [&](const DevGemm& g, const void* A, const void* B, void* C) {
-      check(launch_gemm(g, mode, A, B, C, Pt, s), "tune gemm");  // warm
+    auto timeit = [&](const DevGemm& g, int kmode, const void* A, const void* B, void* C) {
+      check(launch_gemm(g, kmode, A, B, C, Pt, s), "tune gemm");  // warm
       check(hipEventRecord(e0, s), "hipEventRecord");
-      for (int r = 0; r < reps; ++r) check(launch_gemm(g, mode, A, B, C, Pt, s), "tune gemm");
+      for (int r = 0; r < reps; ++r) check(launch_gemm(g, kmode, A, B, C, Pt, s), "tune gemm");
       check(hipEventRecord(e1, s), "hipEventRecord");
       check(hipEventSynchronize(e1), "hipEventSynchronize");
       float ms = 0;
       check(hipEventElapsedTime(&ms, e0, e1), "hipEventElapsedTime");
       return ms;
     };
-    check(launch_copy(d.pack_v, es, v, vp, s), "pack v");
+    heff_stages(p, d, plan->variant(plan->sel1[mode]), plan->variant(plan->sel4[mode]), dt, 1, prepared, v, out, ws,
+                s);
     float best = 1e30f;
     for (const TileCfg& c : kTuneTiles) {
-      if (!tune_tile_ok(mode, c)) continue;
-      const VariantKey k{1, mode, c.BM, c.BN, c.KS, 0};
-      const float t = timeit(plan->variant(k), vp, pr + p.el_off * es, T1);
+      if (!tune_tile_ok(b.mode1, c)) continue;
+      const VariantKey k{1, b.mode1, c.BM, c.BN, c.KS, 0};
+      const float t = timeit(plan->variant(k), b.mode1, vp, pr + b.el, T1);
       if (t < best) {
         best = t;
         plan->sel1[mode] = k;
       }
     }
-    check(launch_mpo_apply(d.mpo, es, pr + p.pw_off * es, T1, T3, s), "mpo");
+    check(launch_mpo_apply(d.mpo, (int)b.es, pr + b.w, T1, T3, s), "mpo");
     best = 1e30f;
     for (const TileCfg& c : kTuneTiles) {
-      if (!tune_tile_ok(mode, c)) continue;
+      if (!tune_tile_ok(b.mode4, c)) continue;
       for (int64_t sp : kTuneSplits) {
-        const VariantKey k{4, mode, c.BM, c.BN, c.KS, sp};
-        const float t = timeit(plan->variant(k), T3, pr + p.er_off * es, out);
+        const VariantKey k{4, b.mode4, c.BM, c.BN, c.KS, sp};
+        const float t = timeit(plan->variant(k), b.mode4, T3, pr + b.er, out);
         if (t < best) {
           best = t;
           plan->sel4[mode] = k;

@@ -12,6 +12,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <cstdlib>
 #include <cstdint>
 #include <type_traits>
 
@@ -28,10 +29,11 @@ typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
 // ---------------------------------------------------------------------------
 // One work group per tile; the plan's tile map names the tile's descriptor and
 // its index within it, so a work group starts with one table load.
-template <typename T>
+// TD: destination type (a conversion on the way, e.g. FP32 -> BF16 packs)
+template <typename T, typename TD = T>
 __global__ __launch_bounds__(256) void copy_kernel(const CopyDesc* __restrict__ descs,
                                                    const int32_t* __restrict__ tmap,
-                                                   const T* __restrict__ src, T* __restrict__ dst) {
+                                                   const T* __restrict__ src, TD* __restrict__ dst) {
   const int2 tm = ((const int2*)tmap)[blockIdx.x];
   const CopyDesc* d = descs + tm.x;
   int64_t t = tm.y;
@@ -61,7 +63,7 @@ __global__ __launch_bounds__(256) void copy_kernel(const CopyDesc* __restrict__ 
     __syncthreads();
     for (int j = ly; j < 64; j += 4) {
       const int64_t x = tx * 64 + lx, y = ty * 64 + j;
-      if (x < d0 && y < d1) dst[db + x * w0 + y * w1] = tile[lx][j];
+      if (x < d0 && y < d1) dst[db + x * w0 + y * w1] = (TD)tile[lx][j];
     }
   } else {
     const int64_t n01 = d0 * d1;
@@ -92,8 +94,15 @@ __global__ __launch_bounds__(256) void copy_kernel(const CopyDesc* __restrict__ 
     }
 #pragma unroll
     for (int u = 0; u < 16; ++u)
-      if (e0 + 256 * u < n01) dst[to[u]] = val[u];
+      if (e0 + 256 * u < n01) dst[to[u]] = (TD)val[u];
   }
+}
+
+hipError_t launch_copy_to_bf16(const DevCopy& c, const float* src, void* dst, hipStream_t s) {
+  if (c.ntiles == 0) return hipSuccess;
+  hipLaunchKernelGGL((copy_kernel<float, __bf16>), dim3((unsigned)c.ntiles), dim3(256), 0, s, c.descs, c.tile_map,
+                     src, (__bf16*)dst);
+  return hipGetLastError();
 }
 
 hipError_t launch_copy(const DevCopy& c, int elem_size, const void* src, void* dst, hipStream_t s) {
@@ -263,6 +272,23 @@ struct Frag<2> {
   }
 };
 
+// This work group's tiles: its own list (balanced schedule) or every
+// (gridDim.x / NXCD)-th tile of its XCD's queue.  False: nothing to do.
+__device__ __forceinline__ bool tile_range(const XcdQueues& xq, int& t0, int& tend, int& tstep) {
+  const int xcd = blockIdx.x % NXCD, j = blockIdx.x / NXCD;
+  if (xq.woff) {
+    if (j >= xq.slots) return false;
+    t0 = xq.woff[xcd * xq.slots + j];
+    tend = xq.woff[xcd * xq.slots + j + 1];
+    tstep = 1;
+  } else {
+    t0 = xq.off[xcd] + j;
+    tend = xq.off[xcd + 1];
+    tstep = gridDim.x / NXCD;
+  }
+  return true;
+}
+
 // Tile BM x BN, 4 waves in 2x2, each wave (BM/2) x (BN/2) = SM x SN MFMA 16x16
 // sub-tiles; K chunk = KS k-slices of one MFMA group each.  Operand rows come
 // from base-address tables and are 16-byte aligned and K-contiguous, so each
@@ -298,9 +324,9 @@ __global__ __launch_bounds__(256) void gemm_grouped(
 
   // work group b runs on XCD b % NXCD (round-robin dispatch) and walks that
   // XCD's tile queue; gridDim.x is a multiple of NXCD
-  const int xcd = blockIdx.x % NXCD, gq = gridDim.x / NXCD;
-  const int tend = xq.off[xcd + 1];
-  for (int t = xq.off[xcd] + blockIdx.x / NXCD; t < tend; t += gq) {
+  int t0, tend, tstep;
+  if (!tile_range(xq, t0, tend, tstep)) return;
+  for (int t = t0; t < tend; t += tstep) {
     const GemmTile tile = tiles[t];
     const GemmProb p = probs[tile.prob];
     Acc acc[SM][SN];
@@ -525,9 +551,11 @@ struct DmaT;
 template <>
 struct DmaT<1> {  // FP32: 4 floats per K group, MFMA 16x16x4 f32 (output column 4g + r)
   typedef float T;
+  typedef float TC;  // output and split-K partial types
+  typedef float TP;
   typedef floatx4 F;
   typedef floatx4 Acc;
-  static constexpr int EG = 4;
+  static constexpr int EG = 4, NK = 4;  // elements per 16-byte K group, MFMAs per group
   static __device__ __forceinline__ void mfma(Acc& c, const F& b, const F& a, int kk) {
     c = __builtin_amdgcn_mfma_f32_16x16x4f32(b[kk], a[kk], c, 0, 0, 0);
   }
@@ -536,13 +564,35 @@ struct DmaT<1> {  // FP32: 4 floats per K group, MFMA 16x16x4 f32 (output column
 template <>
 struct DmaT<0> {  // FP64: 2 doubles per K group, MFMA 16x16x4 f64 (output column g + 4r)
   typedef double T;
+  typedef double TC;
+  typedef double TP;
   typedef double2 F;
   typedef doublex4 Acc;
-  static constexpr int EG = 2;
+  static constexpr int EG = 2, NK = 2;
   static __device__ __forceinline__ void mfma(Acc& c, const F& b, const F& a, int kk) {
     c = __builtin_amdgcn_mfma_f64_16x16x4f64(kk == 0 ? b.x : b.y, kk == 0 ? a.x : a.y, c, 0, 0, 0);
   }
   static __device__ __forceinline__ int col(int g, int r) { return g + 4 * r; }
+};
+
+// BF16 storage: 8 bf16 per K group, one MFMA 16x16x32 bf16 per group (FP32
+// accumulation, output column 4g + r); mode 3 writes BF16, mode 4 FP32
+template <>
+struct DmaT<3> {
+  typedef __bf16 T;
+  typedef __bf16 TC;
+  typedef float TP;
+  typedef bf16x8 F;
+  typedef floatx4 Acc;
+  static constexpr int EG = 8, NK = 1;
+  static __device__ __forceinline__ void mfma(Acc& c, const F& b, const F& a, int) {
+    c = __builtin_amdgcn_mfma_f32_16x16x32_bf16(b, a, c, 0, 0, 0);
+  }
+  static __device__ __forceinline__ int col(int g, int r) { return 4 * g + r; }
+};
+template <>
+struct DmaT<4> : DmaT<3> {
+  typedef float TC;
 };
 
 // NS: LDS ring depth (loads run NS - 1 chunks ahead).  Measured on the H_eff
@@ -554,9 +604,11 @@ __global__ __launch_bounds__(256) void gemm_dma(
     const GemmProb* __restrict__ probs, const GemmSeg* __restrict__ segs, const int64_t* __restrict__ atab,
     const int64_t* __restrict__ btab, const int64_t* __restrict__ crow, const int64_t* __restrict__ cld,
     const GemmTile* __restrict__ tiles, XcdQueues xq, const typename DmaT<MODE>::T* __restrict__ A,
-    const typename DmaT<MODE>::T* __restrict__ B, typename DmaT<MODE>::T* __restrict__ C,
-    typename DmaT<MODE>::T* __restrict__ P) {
+    const typename DmaT<MODE>::T* __restrict__ B, typename DmaT<MODE>::TC* __restrict__ C,
+    typename DmaT<MODE>::TP* __restrict__ P) {
   typedef typename DmaT<MODE>::T T;
+  typedef typename DmaT<MODE>::TC TC;
+  typedef typename DmaT<MODE>::TP TP;
   typedef typename DmaT<MODE>::F F;
   typedef typename DmaT<MODE>::Acc Acc;
   constexpr int EG = DmaT<MODE>::EG;  // elements per 16-byte K group
@@ -591,9 +643,9 @@ __global__ __launch_bounds__(256) void gemm_dma(
     return st == 0 ? S0 : st == 1 ? S1 : st == 2 ? S2 : S3;
   };
 
-  const int xcd = blockIdx.x % NXCD, gq = gridDim.x / NXCD;
-  const int tend = xq.off[xcd + 1];
-  for (int t = xq.off[xcd] + blockIdx.x / NXCD; t < tend; t += gq) {
+  int t0, tend, tstep;
+  if (!tile_range(xq, t0, tend, tstep)) return;
+  for (int t = t0; t < tend; t += tstep) {
     // tile and problem fields moved to scalar registers here: the loop below then
     // uses no register written by a pending vector load, and the compiler places
     // no full vmcnt wait (which would also drain the DMA of later chunks) in it
@@ -711,7 +763,7 @@ __global__ __launch_bounds__(256) void gemm_dma(
           T* slot = wp + (i < NA ? 0 : BM * BK) + ((i < NA ? i : i - NA) * 256 + wave * 64 + lane) * EG;
 #pragma unroll
           for (int e = 0; e < EG; ++e)
-            if (EG * lkl + e >= kr) slot[e] = (T)0;
+            if (EG * lkl + e >= kr) slot[e] = T(0.0f);
         }
         wait_lgkmcnt0();
         barrier_raw();
@@ -722,7 +774,7 @@ __global__ __launch_bounds__(256) void gemm_dma(
 #pragma unroll
       for (int i = 0; i < SM; ++i) fa[i] = *(const F*)(sp + offa[i]);
 #pragma unroll
-      for (int kk = 0; kk < EG; ++kk)
+      for (int kk = 0; kk < DmaT<MODE>::NK; ++kk)
 #pragma unroll
         for (int i = 0; i < SM; ++i)
 #pragma unroll
@@ -759,9 +811,9 @@ __global__ __launch_bounds__(256) void gemm_dma(
       if (it + 2 < nch) iter(I2{}, it + 2);
       if (NS == 4 && it + 3 < nch) iter(I3{}, it + 3);
     }
-    // epilogue as in gemm_grouped
+    // epilogue as in gemm_grouped (split-K partials in TP, outputs in TC)
     const bool split = p.nsplit > 1;
-    T* Pp = P + (split ? p.part_off + (int64_t)tile.split * p.M * p.N : 0);
+    TP* Pp = P + (split ? p.part_off + (int64_t)tile.split * p.M * p.N : 0);
 #pragma unroll
     for (int i = 0; i < SM; ++i) {
       const int m = tile.m0 + wm * (BM / 2) + i * 16 + l16;
@@ -776,13 +828,17 @@ __global__ __launch_bounds__(256) void gemm_dma(
           cl_ = p.ldc;
         }
       }
-      T* Cb = split ? Pp : C;
 #pragma unroll
       for (int j = 0; j < SN; ++j)
 #pragma unroll
         for (int r = 0; r < 4; ++r) {
           const int n = tile.n0 + wn * (BN / 2) + j * 16 + DmaT<MODE>::col(g, r);
-          if (n < p.N) Cb[cb_ + (int64_t)n * cl_] = acc[i][j][r];
+          if (n < p.N) {
+            if (split)
+              Pp[cb_ + (int64_t)n * cl_] = acc[i][j][r];
+            else
+              C[cb_ + (int64_t)n * cl_] = (TC)acc[i][j][r];
+          }
         }
     }
     barrier_raw();  // the next tile refills the stages
@@ -790,7 +846,7 @@ __global__ __launch_bounds__(256) void gemm_dma(
 }
 
 // C(m, n) = sum_s P[part_off + s*M*N + m + M*n] for every split problem
-template <typename T>
+template <typename T, typename TC = T>
 // One work group per 1024 outputs of a split problem, named by the plan's
 // reduction map.  Each output sums its partials in split order; a thread's four
 // outputs are summed side by side, so their loads of one split are in flight
@@ -799,7 +855,7 @@ __global__ __launch_bounds__(256) void splitk_reduce(const int32_t* __restrict__
                                                      const GemmProb* __restrict__ probs,
                                                      const int64_t* __restrict__ crow,
                                                      const int64_t* __restrict__ cld, const T* __restrict__ P,
-                                                     T* __restrict__ C) {
+                                                     TC* __restrict__ C) {
   const int2 rm = ((const int2*)rmap)[blockIdx.x];
   const GemmProb p = probs[rm.x];
   const int64_t MN = (int64_t)p.M * p.N;
@@ -815,11 +871,14 @@ __global__ __launch_bounds__(256) void splitk_reduce(const int32_t* __restrict__
   for (int u = 0; u < 4; ++u) {
     const int64_t e = e0 + 256 * u;
     if (e < MN) {
-      const int64_t n = e / p.M, m = e - n * p.M;
+      // M * N < 2^31 for every split problem (checked by the planner): 32-bit division
+      const uint32_t ue = (uint32_t)e, uM = (uint32_t)p.M;
+      const uint32_t un = ue / uM;
+      const int64_t n = un, m = ue - un * uM;
       if (p.crow0 >= 0)
-        C[crow[p.crow0 + m] + n * cld[p.crow0 + m]] = acc[u];
+        C[crow[p.crow0 + m] + n * cld[p.crow0 + m]] = (TC)acc[u];
       else
-        C[p.c_off + m + n * p.ldc] = acc[u];
+        C[p.c_off + m + n * p.ldc] = (TC)acc[u];
     }
   }
 }
@@ -837,11 +896,9 @@ static int num_cu() {
   return g_num_cu;
 }
 
+// work groups of a variant one CU holds at once (queried once per variant)
 template <int MODE, int BM, int BN, int KS>
-static void launch_variant(const DevGemm& gm, const void* A, const void* B, void* C, void* P, hipStream_t s) {
-  typedef typename GT<MODE>::T T;
-  // persistent grid: per XCD as many work groups as its CUs hold at once
-  // (occupancy of this variant), no more than its queue holds
+static int occ_grouped() {
   static int occ = 0;
   if (occ == 0) {
     int nb = 0;
@@ -850,9 +907,29 @@ static void launch_variant(const DevGemm& gm, const void* A, const void* B, void
       nb = 2;
     occ = nb;
   }
+  return occ;
+}
+template <int MODE, int BM, int BN>
+static int occ_dma() {
+  static int occ = 0;
+  if (occ == 0) {
+    int nb = 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, gemm_dma<MODE, BM, BN>, 256, 0) != hipSuccess || nb <= 0)
+      nb = 2;
+    occ = nb;
+  }
+  return occ;
+}
+
+template <int MODE, int BM, int BN, int KS>
+static void launch_variant(const DevGemm& gm, const void* A, const void* B, void* C, void* P, hipStream_t s) {
+  typedef typename GT<MODE>::T T;
+  // persistent grid: per XCD as many work groups as its CUs hold at once
+  // (occupancy of this variant), no more than its queue holds
+  const int occ = occ_grouped<MODE, BM, BN, KS>();
   int qmax = 0;
   for (int x = 0; x < NXCD; ++x) qmax = std::max(qmax, gm.q.off[x + 1] - gm.q.off[x]);
-  const int per = std::min(std::max(1, num_cu() / NXCD) * occ, qmax);
+  const int per = gm.q.woff ? gm.q.slots : std::min(std::max(1, num_cu() / NXCD) * occ, qmax);
   dim3 grid((unsigned)(per * NXCD)), block(256);
   hipLaunchKernelGGL((gemm_grouped<MODE, BM, BN, KS>), grid, block, 0, s, gm.probs, gm.segs, gm.atab, gm.btab,
                      gm.crow, gm.cld, gm.tiles, gm.q, (const T*)A, (const T*)B, (T*)C, (T*)P);
@@ -861,18 +938,14 @@ static void launch_variant(const DevGemm& gm, const void* A, const void* B, void
 template <int MODE, int BM, int BN>
 static void launch_variant_dma(const DevGemm& gm, const void* A, const void* B, void* C, void* P, hipStream_t s) {
   typedef typename DmaT<MODE>::T T;
-  static int occ = 0;
-  if (occ == 0) {
-    int nb = 0;
-    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, gemm_dma<MODE, BM, BN>, 256, 0) != hipSuccess || nb <= 0)
-      nb = 2;
-    occ = nb;
-  }
+  typedef typename DmaT<MODE>::TC TC;
+  typedef typename DmaT<MODE>::TP TP;
+  const int occ = occ_dma<MODE, BM, BN>();
   int qmax = 0;
   for (int x = 0; x < NXCD; ++x) qmax = std::max(qmax, gm.q.off[x + 1] - gm.q.off[x]);
-  const int per = std::min(std::max(1, num_cu() / NXCD) * occ, qmax);
+  const int per = gm.q.woff ? gm.q.slots : std::min(std::max(1, num_cu() / NXCD) * occ, qmax);
   hipLaunchKernelGGL((gemm_dma<MODE, BM, BN>), dim3((unsigned)(per * NXCD)), dim3(256), 0, s, gm.probs, gm.segs,
-                     gm.atab, gm.btab, gm.crow, gm.cld, gm.tiles, gm.q, (const T*)A, (const T*)B, (T*)C, (T*)P);
+                     gm.atab, gm.btab, gm.crow, gm.cld, gm.tiles, gm.q, (const T*)A, (const T*)B, (TC*)C, (TP*)P);
 }
 
 // Compiled tile variants: (BM, BN) in {64x64, 128x64, 64x128, 128x128} with K
@@ -886,7 +959,21 @@ static hipError_t launch_mode(const DevGemm& gm, const void* A, const void* B, v
   case bm * 10000 + bn * 10 + ks:                          \
     launch_variant<MODE, bm, bn, ks>(gm, A, B, C, P, s); \
     break;
-  if constexpr (MODE == 0) {
+  if constexpr (MODE >= 3) {  // BF16 storage: DMA-streamed operands only
+    switch (key) {
+      case 64 * 10000 + 64 * 10:
+        launch_variant_dma<MODE, 64, 64>(gm, A, B, C, P, s);
+        break;
+      case 128 * 10000 + 64 * 10:
+        launch_variant_dma<MODE, 128, 64>(gm, A, B, C, P, s);
+        break;
+      case 64 * 10000 + 128 * 10:
+        launch_variant_dma<MODE, 64, 128>(gm, A, B, C, P, s);
+        break;
+      default:
+        return hipErrorInvalidValue;
+    }
+  } else if constexpr (MODE == 0) {
     switch (key) {
       MPSH_V(64, 64, 1)
       MPSH_V(128, 64, 1)
@@ -933,6 +1020,28 @@ static hipError_t launch_mode(const DevGemm& gm, const void* A, const void* B, v
   return hipGetLastError();
 }
 
+int gemm_slots_per_xcd(int mode, int BM, int BN, int KS) {
+  const int key = BM * 10000 + BN * 10 + KS;
+  int occ = 0;
+#define MPSH_O(m, bm, bn, ks) \
+  if (mode == m && key == bm * 10000 + bn * 10 + ks) occ = occ_grouped<m, bm, bn, ks>();
+#define MPSH_OD(m, bm, bn) \
+  if (mode == m && key == bm * 10000 + bn * 10) occ = occ_dma<m, bm, bn>();
+  MPSH_O(0, 64, 64, 1) MPSH_O(0, 128, 64, 1) MPSH_O(0, 64, 128, 1)
+  MPSH_OD(0, 64, 64) MPSH_OD(0, 128, 64) MPSH_OD(0, 64, 128)
+  MPSH_OD(1, 64, 64) MPSH_OD(1, 128, 64) MPSH_OD(1, 64, 128)
+  MPSH_O(1, 64, 64, 1) MPSH_O(1, 128, 64, 1) MPSH_O(1, 64, 128, 1) MPSH_O(1, 128, 128, 1)
+  MPSH_O(1, 64, 64, 2) MPSH_O(1, 128, 64, 2) MPSH_O(1, 64, 128, 2) MPSH_O(1, 128, 128, 2)
+  MPSH_O(2, 64, 64, 1) MPSH_O(2, 128, 64, 1) MPSH_O(2, 64, 128, 1) MPSH_O(2, 128, 128, 1)
+  MPSH_O(2, 64, 64, 2) MPSH_O(2, 128, 64, 2) MPSH_O(2, 64, 128, 2) MPSH_O(2, 128, 128, 2)
+  MPSH_OD(3, 64, 64) MPSH_OD(3, 128, 64) MPSH_OD(3, 64, 128)
+  MPSH_OD(4, 64, 64) MPSH_OD(4, 128, 64) MPSH_OD(4, 64, 128)
+#undef MPSH_O
+#undef MPSH_OD
+  if (occ <= 0) return 0;  // not a compiled variant
+  return std::max(1, num_cu() / NXCD) * occ;
+}
+
 hipError_t launch_gemm(const DevGemm& gm, int mode, const void* A, const void* B, void* C, void* P,
                        hipStream_t s) {
   if (gm.ntiles == 0) return hipSuccess;
@@ -942,15 +1051,25 @@ hipError_t launch_gemm(const DevGemm& gm, int mode, const void* A, const void* B
     e = launch_mode<0>(gm, A, B, C, P, s);
   else if (mode == 1)
     e = launch_mode<1>(gm, A, B, C, P, s);
-  else
+  else if (mode == 2)
     e = launch_mode<2>(gm, A, B, C, P, s);
-  if (e != hipSuccess || gm.nred == 0) return e;
-  if (mode == 0)
-    hipLaunchKernelGGL(splitk_reduce<double>, dim3((unsigned)gm.red_groups), dim3(256), 0, s, gm.red_map,
-                       gm.probs, gm.crow, gm.cld, (const double*)P, (double*)C);
+  else if (mode == 3)
+    e = launch_mode<3>(gm, A, B, C, P, s);
+  else if (mode == 4)
+    e = launch_mode<4>(gm, A, B, C, P, s);
   else
-    hipLaunchKernelGGL(splitk_reduce<float>, dim3((unsigned)gm.red_groups), dim3(256), 0, s, gm.red_map,
-                       gm.probs, gm.crow, gm.cld, (const float*)P, (float*)C);
+    return hipErrorInvalidValue;
+  if (e != hipSuccess || gm.nred == 0) return e;
+  const dim3 rg((unsigned)gm.red_groups), rb(256);
+  if (mode == 0)
+    hipLaunchKernelGGL((splitk_reduce<double, double>), rg, rb, 0, s, gm.red_map, gm.probs, gm.crow, gm.cld,
+                       (const double*)P, (double*)C);
+  else if (mode == 3)
+    hipLaunchKernelGGL((splitk_reduce<float, __bf16>), rg, rb, 0, s, gm.red_map, gm.probs, gm.crow, gm.cld,
+                       (const float*)P, (__bf16*)C);
+  else
+    hipLaunchKernelGGL((splitk_reduce<float, float>), rg, rb, 0, s, gm.red_map, gm.probs, gm.crow, gm.cld,
+                       (const float*)P, (float*)C);
   return hipGetLastError();
 }
 
@@ -985,41 +1104,76 @@ __global__ __launch_bounds__(256) void mpo_combine(const MpoSig* __restrict__ si
   }
 }
 
-// One pixel's outputs: out[r] = sum_c W[r][c] in[c] for r < R, with the C inputs
-// in registers.  CM >= C is the group's channel class; channels past C read
-// row C-1 (a real row) and meet zero W columns, so the loads are unconditional.
-template <typename T, int CM, int CMAX>
-__device__ __forceinline__ void mpo_pixel(const T* Ws, const int64_t* ib, const int64_t* ins, const int64_t* im,
-                                          const int64_t* ob, const int64_t* on, const int64_t* om, int R, int64_t m,
-                                          int64_t n, bool live, const T* __restrict__ T1, T* __restrict__ T3) {
-  T v[CM];
+// Address of element offset `off` of a T1 / T3 buffer.  IX = uint32_t: `off` is
+// a byte offset (zero-extended onto the buffer's base, the global-load form with
+// a scalar base and a 32-bit vector offset); IX = int64_t: an element offset.
+template <typename IX, typename E>
+__device__ __forceinline__ E* mpo_at(E* base, IX off) {
+  if constexpr (sizeof(IX) == 4)
+    return reinterpret_cast<E*>(reinterpret_cast<typename std::conditional<std::is_const<E>::value, const char,
+                                                                              char>::type*>(base) + off);
+  else
+    return base + off;
+}
+
+// One run of P pixels (m, n0 .. n0+P-1): out[u][r] = sum_c W[r][c] in[u][c], the
+// C x P inputs in registers.  Each channel's address is formed once per run and
+// each W value read once for the P pixels.  CM >= C is the group's channel
+// class; channels past C read row C-1 (a real row) and meet zero W columns, and
+// pixels past the row end re-read its last pixel, so every load is
+// unconditional; only the stores are masked.  ib/ins/im and ob/on/om are in
+// the units of IX (bytes for uint32_t, elements for int64_t).
+template <typename T, typename TI, typename TO, typename IX, int CM, int CMAX, int P>
+__device__ __forceinline__ void mpo_run(const T* Ws, const IX* ib, const IX* ins, const IX* im, const IX* ob,
+                                        const IX* on, const IX* om, int R, IX m, IX n0, IX dR, bool live,
+                                        const TI* __restrict__ T1, TO* __restrict__ T3) {
+  IX nu[P];
 #pragma unroll
-  for (int c = 0; c < CM; ++c) v[c] = T1[ib[c] + n * ins[c] + m * im[c]];
+  for (int u = 0; u < P; ++u) nu[u] = n0 + u < dR ? n0 + u : dR - 1;
+  T v[P][CM];
+#pragma unroll
+  for (int c = 0; c < CM; ++c) {
+    const IX o = ib[c] + m * im[c], st = ins[c];
+#pragma unroll
+    for (int u = 0; u < P; ++u) v[u][c] = (T)*mpo_at(T1, (IX)(o + nu[u] * st));
+  }
   if (!live) return;
   for (int r = 0; r < R; ++r) {
     const T* wr = Ws + r * CMAX;
-    T acc = 0;
+    T acc[P];
 #pragma unroll
-    for (int c = 0; c < CM; ++c) acc += wr[c] * v[c];
-    T3[ob[r] + n * on[r] + m * om[r]] = acc;
+    for (int u = 0; u < P; ++u) acc[u] = 0;
+#pragma unroll
+    for (int c = 0; c < CM; ++c) {
+      const T w = wr[c];
+#pragma unroll
+      for (int u = 0; u < P; ++u) acc[u] += w * v[u][c];
+    }
+    const IX o = ob[r] + m * om[r], st = on[r];
+#pragma unroll
+    for (int u = 0; u < P; ++u)
+      if (n0 + u < dR) *mpo_at(T3, (IX)(o + (n0 + u) * st)) = (TO)acc[u];
   }
 }
 
-// out[r][pix] = sum_c W[r][c] * in[c][pix], pix = n + dR*m (output row r of pixel
-// (m, n) at ob[r] + n*on[r] + m*om[r]); one thread per pixel,
-// 256 pixels of one (bra_l, ket_r) group per work group.  CMAX bounds the
-// plan's channel count (LDS layout); each group runs the smallest channel
-// class (multiple of 4) that holds its own C.
-template <typename T, int CMAX>
+// out[r][pix] = sum_c W[r][c] * in[c][pix] over the pixels (m, n) of one
+// (bra_l, ket_r) group (output row r of pixel (m, n) at ob[r] + n*on[r] + m*om[r]);
+// each thread takes a run of P consecutive n of one row m, a work group 256
+// runs.  CMAX bounds the plan's channel count (LDS layout); each group runs the
+// smallest channel class (multiple of 4) that holds its own C.
+// T: arithmetic and W type; TI / TO: T1 / T3 storage (BF16 storage computes in FP32);
+// IX: offset arithmetic (uint32_t byte offsets when T1 and T3 are below 4 GiB)
+template <typename T, int CMAX, int P, typename TI = T, typename TO = T, typename IX = int64_t>
 __global__ __launch_bounds__(256) void mpo_apply(const MpoSig* __restrict__ sigs,
                                                  const MpoGroup* __restrict__ groups,
                                                  const MpoInRow* __restrict__ inrows,
                                                  const MpoOutRow* __restrict__ outrows,
                                                  const MpoWork* __restrict__ work, const T* __restrict__ W,
-                                                 const T* __restrict__ T1, T* __restrict__ T3) {
+                                                 const TI* __restrict__ T1, TO* __restrict__ T3) {
+  constexpr IX si = sizeof(IX) == 4 ? (IX)sizeof(TI) : 1, so = sizeof(IX) == 4 ? (IX)sizeof(TO) : 1;
   __shared__ __attribute__((aligned(16))) T Ws[64 * CMAX];
-  __shared__ int64_t ib[CMAX], ins[CMAX], im[CMAX];
-  __shared__ int64_t ob[64], on[64], om[64];
+  __shared__ IX ib[CMAX], ins[CMAX], im[CMAX];
+  __shared__ IX ob[64], on[64], om[64];
   const MpoWork wk = work[blockIdx.x];
   const MpoGroup g = groups[wk.group];
   const MpoSig sg = sigs[g.sig];
@@ -1032,30 +1186,37 @@ __global__ __launch_bounds__(256) void mpo_apply(const MpoSig* __restrict__ sigs
   if (C > 0)
     for (int i = threadIdx.x; i < CMAX; i += blockDim.x) {
       const MpoInRow q = inrows[g.in0 + (i < C ? i : C - 1)];
-      ib[i] = q.base;
-      ins[i] = q.nstride;
-      im[i] = q.mstride;
+      ib[i] = (IX)q.base * si;
+      ins[i] = (IX)q.nstride * si;
+      im[i] = (IX)q.mstride * si;
     }
   for (int i = threadIdx.x; i < R; i += blockDim.x) {
     const MpoOutRow q = outrows[g.out0 + i];
-    ob[i] = q.base;
-    on[i] = q.nstride;
-    om[i] = q.mstride;
+    ob[i] = (IX)q.base * so;
+    on[i] = (IX)q.nstride * so;
+    om[i] = (IX)q.mstride * so;
   }
   __syncthreads();
-  const int64_t p0 = (int64_t)wk.pix0 + threadIdx.x;
-  const bool live = p0 < g.npix;
-  const int64_t p = live ? p0 : 0;
-  const int64_t m = p / g.dR, n = p - m * g.dR;
+  // run q of the group: row m = q / nr, first pixel n0 = P * (q % nr), nr runs per row
+  const uint32_t dR = (uint32_t)g.dR, nr = (dR + P - 1) / P;
+  const int64_t nruns = (int64_t)g.db * nr;
+  const int64_t q0 = (int64_t)wk.run0 + threadIdx.x;
+  const bool live = q0 < nruns;
+  const uint32_t q = live ? (uint32_t)q0 : 0u;
+  const uint32_t mq = q / nr;
+  const IX m = (IX)mq, n0 = (IX)((q - mq * nr) * P);
   if (C <= 0) {  // a group no input type reaches: its T3 cells are zero
     if (live)
-      for (int r = 0; r < R; ++r) T3[ob[r] + n * on[r] + m * om[r]] = T(0);
+      for (int r = 0; r < R; ++r)
+        for (int u = 0; u < P; ++u)
+          if (n0 + u < (IX)dR) *mpo_at(T3, (IX)(ob[r] + (n0 + u) * on[r] + m * om[r])) = TO(0.0f);
     return;
   }
   const int cls = (C + 3) / 4;  // uniform over the block
-#define MPSH_PIX(K)                                                                             \
-  case K:                                                                                       \
-    if constexpr (4 * K <= CMAX) mpo_pixel<T, 4 * K, CMAX>(Ws, ib, ins, im, ob, on, om, R, m, n, live, T1, T3); \
+#define MPSH_PIX(K)                                                                                             \
+  case K:                                                                                                       \
+    if constexpr (4 * K <= CMAX)                                                                                \
+      mpo_run<T, TI, TO, IX, 4 * K, CMAX, P>(Ws, ib, ins, im, ob, on, om, R, m, n0, (IX)dR, live, T1, T3);     \
     break;
   switch (cls) {
     MPSH_PIX(1)
@@ -1067,7 +1228,8 @@ __global__ __launch_bounds__(256) void mpo_apply(const MpoSig* __restrict__ sigs
     MPSH_PIX(7)
     MPSH_PIX(8)
     default:
-      if constexpr (CMAX == 64) mpo_pixel<T, 64, CMAX>(Ws, ib, ins, im, ob, on, om, R, m, n, live, T1, T3);
+      if constexpr (CMAX == 64)
+        mpo_run<T, TI, TO, IX, 64, CMAX, P>(Ws, ib, ins, im, ob, on, om, R, m, n0, (IX)dR, live, T1, T3);
       break;
   }
 #undef MPSH_PIX
@@ -1085,19 +1247,56 @@ hipError_t launch_mpo_combine(const DevMpo& d, int elem_size, const void* WL, co
   return hipGetLastError();
 }
 
+void mpo_options(DevMpo& d) {
+  const char* e = std::getenv("MPSH_MPO_RUN");
+  const int v = e ? std::atoi(e) : 1;
+  d.run = (v == 2 || v == 4) ? v : 1;
+  const char* f = std::getenv("MPSH_MPO_OFF64");
+  d.off64 = f && std::atoi(f) != 0;
+}
+
 hipError_t launch_mpo_apply(const DevMpo& d, int elem_size, const void* W, const void* T1, void* T3, hipStream_t s) {
-  if (d.nwork == 0) return hipSuccess;
-  dim3 grid(d.nwork), block(256);
-#define MPSH_MPO(T, CM)                                                                                  \
-  hipLaunchKernelGGL((mpo_apply<T, CM>), grid, block, 0, s, d.sigs, d.groups, d.inrows, d.outrows, d.work, \
-                     (const T*)W, (const T*)T1, (T*)T3)
-  if (elem_size == 8) {
-    if (d.cmax <= 32) MPSH_MPO(double, 32);
-    else MPSH_MPO(double, 64);
-  } else {
-    if (d.cmax <= 32) MPSH_MPO(float, 32);
-    else MPSH_MPO(float, 64);
+  // runs of d.run pixels per thread; wide channel classes (CMAX 64) and FP64 at most P = 2
+  const int pw = elem_size == 8 ? std::min(2, d.run) : d.run;
+  const int P = d.cmax > 32 ? std::max(1, pw / 2) : pw;
+  const int pi = P == 4 ? 2 : (P == 2 ? 1 : 0);  // index into kMpoRuns
+  if (d.nwork[pi] == 0) return hipSuccess;
+  // 32-bit byte offsets when every T1 / T3 byte offset fits
+  const int64_t bytes = std::max(d.t1_elems, d.t3_elems) * elem_size;
+  const bool off32 = !d.off64 && bytes < ((int64_t)1 << 32);
+  dim3 grid(d.nwork[pi]), block(256);
+#define MPSH_MPO(T, CM, PP, TI, IX)                                                                                \
+  hipLaunchKernelGGL((mpo_apply<T, CM, PP, TI, TI, IX>), grid, block, 0, s, d.sigs, d.groups, d.inrows, d.outrows, \
+                     d.work[pi], (const T*)W, (const TI*)T1, (TI*)T3)
+#define MPSH_MPO_P(T, TI, IX)                   \
+  if (d.cmax <= 32) {                           \
+    if (P == 4) MPSH_MPO(T, 32, 4, TI, IX);     \
+    else if (P == 2) MPSH_MPO(T, 32, 2, TI, IX);\
+    else MPSH_MPO(T, 32, 1, TI, IX);            \
+  } else {                                      \
+    if (P == 2) MPSH_MPO(T, 64, 2, TI, IX);     \
+    else MPSH_MPO(T, 64, 1, TI, IX);            \
   }
+#define MPSH_MPO_T(IX)                                            \
+  if (elem_size == 8) {                                           \
+    if (d.cmax <= 32) {                                           \
+      if (P == 2) MPSH_MPO(double, 32, 2, double, IX);            \
+      else MPSH_MPO(double, 32, 1, double, IX);                   \
+    } else {                                                      \
+      MPSH_MPO(double, 64, 1, double, IX);                        \
+    }                                                             \
+  } else if (elem_size == 4) {                                    \
+    MPSH_MPO_P(float, float, IX)                                  \
+  } else { /* BF16 T1 / T3, FP32 W and arithmetic */              \
+    MPSH_MPO_P(float, __bf16, IX)                                 \
+  }
+  if (off32) {
+    MPSH_MPO_T(uint32_t)
+  } else {
+    MPSH_MPO_T(int64_t)
+  }
+#undef MPSH_MPO_T
+#undef MPSH_MPO_P
 #undef MPSH_MPO
   return hipGetLastError();
 }

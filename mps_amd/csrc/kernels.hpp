@@ -21,8 +21,14 @@ void set_last_error(const std::string& m);
 // synchronous host-to-device copy of plan tables that does not wait for other streams' work (api.hip)
 hipError_t upload_tables(void* dst, const void* src, size_t bytes);
 
+// Tile queues of a persistent GEMM launch.  With woff == nullptr, work group b
+// walks queue b % NXCD ([off[x], off[x+1])) with stride gridDim.x / NXCD; with
+// woff, the grid has NXCD * slots work groups and work group b runs the list
+// [woff[x*slots + j], woff[x*slots + j + 1]), x = b % NXCD, j = b / NXCD.
 struct XcdQueues {
   int32_t off[NXCD + 1];
+  int32_t slots;
+  const int32_t* woff;
 };
 
 struct DevGemm {
@@ -45,16 +51,28 @@ struct DevMpo {
   const MpoGroup* groups = nullptr;
   const MpoInRow* inrows = nullptr;
   const MpoOutRow* outrows = nullptr;
-  const MpoWork* work = nullptr;
-  int nterms = 0, nwork = 0, cmax = 32, rmax = 32;
+  const MpoWork* work[3] = {nullptr, nullptr, nullptr};  // runs of kMpoRuns[i] pixels
+  int nwork[3] = {0, 0, 0};
+  int nterms = 0, cmax = 32, rmax = 32;
+  int64_t t1_elems = 0, t3_elems = 0;  // T1 / T3 sizes (32-bit byte offsets when both fit)
+  int run = 1;                          // pixels per thread (1, 2, 4; FP64 and CMAX 64 at most 2)
+  bool off64 = false;                   // force 64-bit offsets
 };
+// the MPO pass options of newly uploaded plans: MPSH_MPO_RUN (1, 2, 4) and MPSH_MPO_OFF64 (0 / 1)
+void mpo_options(DevMpo& d);
 
 hipError_t launch_copy(const DevCopy& c, int elem_size, const void* src, void* dst, hipStream_t s);
-// mode 0 = f64, 1 = f32, 2 = bf16 operands from f32 storage
+// the same copy converting FP32 to BF16 (packs of the BF16-storage H_eff)
+hipError_t launch_copy_to_bf16(const DevCopy& c, const float* src, void* dst, hipStream_t s);
+// resident work groups per XCD of a GEMM kernel variant (CUs per XCD x occupancy)
+int gemm_slots_per_xcd(int mode, int BM, int BN, int KS);
+// mode 0 = f64, 1 = f32, 2 = bf16 operands from f32 storage, 3 / 4 = bf16 storage
+// (DMA-streamed; 3 writes bf16, 4 fp32; split-K partials fp32)
 // P: split-K partial buffer (unused when no problem is split)
 hipError_t launch_gemm(const DevGemm& g, int mode, const void* A, const void* B, void* C, void* P, hipStream_t s);
 hipError_t launch_mpo_combine(const DevMpo& d, int elem_size, const void* WL, const void* WR, void* W,
                               hipStream_t s);
+// elem_size 8 / 4: T1, T3, W of that type; 2: BF16 T1 and T3, FP32 W and arithmetic
 hipError_t launch_mpo_apply(const DevMpo& d, int elem_size, const void* W, const void* T1, void* T3, hipStream_t s);
 hipError_t launch_dot(int elem_size, int64_t n, const void* x, const void* y, double* out, double* partials,
                       int max_partials, hipStream_t s);

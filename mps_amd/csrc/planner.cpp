@@ -272,6 +272,7 @@ void GemmPlan::choose_splits(int64_t target, int BK, int64_t min_chunks_per_spli
   for (auto& p : probs) {
     const int64_t ch = prob_chunks(*this, p, BK);
     int64_t s = std::min<int64_t>(want, ch / std::max<int64_t>(1, min_chunks_per_split));
+    if (s > 1 && (int64_t)p.M * p.N >= ((int64_t)1 << 31)) s = 1;  // the reduction indexes M*N in 32 bits
     if (s > 1) {
       p.nsplit = (int32_t)s;
       p.part_off = part_elems;
@@ -346,6 +347,44 @@ std::vector<GemmTile> GemmPlan::tiles(std::vector<int32_t>* qoff) const {
   for (int x = 0; x < NXCD; ++x) {
     out.insert(out.end(), queue[x].begin(), queue[x].end());
     if (qoff) (*qoff)[x + 1] = (int32_t)out.size();
+  }
+  return out;
+}
+
+std::vector<GemmTile> GemmPlan::schedule(int slots, int BK, std::vector<int32_t>* woff) const {
+  if (slots < 1) throw Error(ERR_INVALID, "schedule: slots < 1");
+  std::vector<int32_t> qoff;
+  const std::vector<GemmTile> q = tiles(&qoff);
+  // K chunks of each problem (the split ranges follow from them)
+  std::vector<int64_t> ctot(probs.size(), 0);
+  for (size_t pi = 0; pi < probs.size(); ++pi)
+    for (int s2 = 0; s2 < probs[pi].nseg; ++s2) ctot[pi] += (segs[probs[pi].seg0 + s2].K + BK - 1) / BK;
+  // a tile's cost: its K chunks plus a fixed cost for its set-up and its stores
+  constexpr int64_t kTileCost = 3;
+  auto cost = [&](const GemmTile& t) {
+    const int64_t c = ctot[t.prob], ns = probs[t.prob].nsplit;
+    return c * (t.split + 1) / ns - c * t.split / ns + kTileCost;
+  };
+  std::vector<GemmTile> out;
+  out.reserve(q.size());
+  woff->assign((size_t)NXCD * slots + 1, 0);
+  for (int x = 0; x < NXCD; ++x) {
+    const int32_t b = qoff[x], e = qoff[x + 1];
+    std::vector<int32_t> order(e - b);
+    std::iota(order.begin(), order.end(), b);
+    std::stable_sort(order.begin(), order.end(), [&](int32_t i, int32_t j) { return cost(q[i]) > cost(q[j]); });
+    std::vector<std::vector<int32_t>> lists(slots);
+    std::vector<int64_t> load(slots, 0);
+    for (int32_t i : order) {
+      const int w = (int)(std::min_element(load.begin(), load.end()) - load.begin());
+      load[w] += cost(q[i]);
+      lists[w].push_back(i);
+    }
+    for (int w = 0; w < slots; ++w) {
+      std::sort(lists[w].begin(), lists[w].end());  // queue order: the panel walk of tiles()
+      for (int32_t i : lists[w]) out.push_back(q[i]);
+      (*woff)[(size_t)x * slots + w + 1] = (int32_t)out.size();
+    }
   }
   return out;
 }
@@ -882,7 +921,9 @@ static void need_same(const Leg& a, const Leg& b, const char* what) {
   if (a != b) throw Error(ERR_SPACE_MISMATCH, std::string("not contractible! ") + what);
 }
 
-static int64_t up4(int64_t x) { return (x + 3) / 4 * 4; }
+// K-contiguous operand rows are padded to multiples of 8 elements: 16-byte
+// aligned rows for the DMA-streamed GEMMs in every storage type (BF16 included)
+static int64_t up8(int64_t x) { return (x + 7) / 8 * 8; }
 static int64_t up64(int64_t x) { return (x + 63) / 64 * 64; }  // 256-byte (FP32) region starts
 
 namespace {
@@ -916,7 +957,7 @@ static void step1_layout(Step13& H, const Struct& v, const Struct& envL, std::ve
     if (itv == vby.end()) continue;
     const std::vector<int64_t>& X = itv->second;
     const std::vector<int64_t>& E = kv.second;
-    const int64_t dk = envL.size(E[0])[0], Kp = up4(dk);
+    const int64_t dk = envL.size(E[0])[0], Kp = up8(dk);
     std::vector<int64_t> rowoff(X.size()), coloff(E.size());
     int64_t Mf = 0, Nf = 0;
     for (size_t i = 0; i < X.size(); ++i) {
@@ -1106,7 +1147,10 @@ static void mpo_layout(Step13& H, const Struct& WL, const Struct& WR, const std:
     }
     const int32_t gidx = (int32_t)H.groups.size();
     H.groups.push_back(g);
-    for (int64_t p0 = 0; p0 < g.npix; p0 += 256) H.work.push_back(MpoWork{gidx, (int32_t)p0});
+    for (int i = 0; i < 3; ++i) {
+      const int64_t runs = db * ((dR + kMpoRuns[i] - 1) / kMpoRuns[i]);
+      for (int64_t r0 = 0; r0 < runs; r0 += 256) H.work[i].push_back(MpoWork{gidx, (int32_t)r0});
+    }
     H.mpo_flops += 2 * R * C * g.npix;
   }
   H.w_elems = w_off;
@@ -1190,7 +1234,7 @@ HeffPlan heff_plan(const Struct& v, const Struct& envL, const Struct& envR, cons
       for (int64_t f : itf->second) {
         const int64_t* fs = envR.sector(f);
         const int64_t dR = envR.size(f)[0], Dx = envR.size(f)[1];
-        const int64_t Kf = dR * Dx, Kfp = up4(Kf);
+        const int64_t Kf = dR * Dx, Kfp = up8(Kf);
         const int64_t rbase = er_elems;
         er_elems += Kfp * N;
         {
@@ -1225,17 +1269,17 @@ HeffPlan heff_plan(const Struct& v, const Struct& envL, const Struct& envR, cons
   cfg(H.g4h, tiles.s4h);
   cfg(H.g1d, tiles.s1d);
   cfg(H.g4d, tiles.s4d);
-  // K chunk: 16 * KS for FP32 / FP64, 32 * KS for BF16 (chunk_k)
+  // K chunk of each kernel (chunk_k); BF16 runs the BF16-storage kernels (modes 3, 4)
   H.g4.choose_splits(tiles.split_target, chunk_k(1, tiles.s4.KS), 32);
-  H.g4h.choose_splits(tiles.split_target, chunk_k(2, tiles.s4h.KS), 32);
+  H.g4h.choose_splits(tiles.split_target, chunk_k(4, tiles.s4h.KS), 32);
   H.g4d.choose_splits(tiles.split_target_d, chunk_k(0, tiles.s4d.KS), 32);
   H.pack_v.finalize();
   H.pack_envL.finalize();
   H.pack_envR.finalize();
   // prepared operator: [envL_p | envR_p | W]
   H.el_off = 0;
-  H.er_off = up4(H.el_elems);
-  H.pw_off = up4(H.er_off + er_elems);
+  H.er_off = up8(H.el_elems);
+  H.pw_off = up8(H.er_off + er_elems);
   H.prep_elems = H.pw_off + H.w_elems;
   // per-matvec workspace: [v_p | T1 | T3 | partials]
   H.vp_off = 0;
@@ -1244,7 +1288,7 @@ HeffPlan heff_plan(const Struct& v, const Struct& envL, const Struct& envR, cons
   H.part_off = up64(H.t3_off + H.t3_elems);
   // partials sized for the largest split-K layout any tuning candidate may pick
   int64_t part = std::max({H.g4.part_elems, H.g4h.part_elems, H.g4d.part_elems});
-  for (int mode = 0; mode < 3; ++mode)
+  for (int mode : {0, 1, 4})  // the step-4 kernel modes of FP64, FP32, BF16
     for (const TileCfg& c : kTuneTiles) {
       if (!tune_tile_ok(mode, c)) continue;
       for (int64_t sp : kTuneSplits) part = std::max(part, gemm_variant(H.g4, c, sp, mode).part_elems);
@@ -1325,7 +1369,7 @@ static EnvCore env_core(const Struct& env, const Struct& ket, const Struct& WL, 
       koff[i] = Ky;
       Ky += z[0] * z[1] * z[2];
     }
-    const int64_t Kp = up4(Ky), dY = H.out.size(O[0])[2];
+    const int64_t Kp = up8(Ky), dY = H.out.size(O[0])[2];
     const int64_t breg = bp;
     if (Ky > 0) {
       for (size_t i = 0; i < K.size(); ++i) {
@@ -1334,7 +1378,7 @@ static EnvCore env_core(const Struct& env, const Struct& ket, const Struct& WL, 
         const int64_t dims[2] = {nk, z[3]}, sst[2] = {1, nk}, dst[2] = {1, Kp};
         H.pack_bra.add(2, dims, sst, dst, ket.offsets[K[i]], breg + koff[i]);
       }
-      bp += up4(Kp * dY);
+      bp += up8(Kp * dY);
     }
     const int64_t reg = t3;
     int64_t rowoff = 0;
@@ -1355,7 +1399,7 @@ static EnvCore env_core(const Struct& env, const Struct& ket, const Struct& WL, 
       }
       rowoff += dR * Dx;
     }
-    if (Ky > 0) t3 += up4(Kp * rowoff);
+    if (Ky > 0) t3 += up8(Kp * rowoff);
   }
   H.t3_elems = t3;
   H.bp_elems = bp;

@@ -184,6 +184,12 @@ struct GemmPlan {
   // (LPT over panel groups) and the tiles that share operand panels stay in
   // one queue, so each XCD's L2 fetches its panels once.
   std::vector<GemmTile> tiles(std::vector<int32_t>* qoff = nullptr) const;
+  // Per-work-group tile lists for `slots` resident work groups per XCD: each
+  // XCD's queue (tiles()) is dealt to its work groups by LPT on the tiles' K
+  // chunks (chunk BK elements) plus a fixed per-tile cost, each list kept in
+  // queue order.  Returns the lists concatenated; woff (NXCD*slots + 1 entries)
+  // bounds list j of XCD x at [woff[x*slots + j], woff[x*slots + j + 1]).
+  std::vector<GemmTile> schedule(int slots, int BK, std::vector<int32_t>* woff) const;
   std::vector<RedDesc> reductions() const;
 };
 
@@ -275,9 +281,12 @@ struct MpoInRow {
 struct MpoOutRow {  // T3 address = base + n*nstride + m*mstride
   int64_t base, nstride, mstride;
 };
-struct MpoWork {
-  int32_t group, pix0;
+struct MpoWork {  // one work group of the MPO pass: runs [run0, run0 + 256) of a pixel group
+  int32_t group, run0;
 };
+// The MPO pass gives each thread a run of P consecutive pixels along the ket_r
+// index n of one bra_l row m; P = 1, 2 or 4 (by storage type and channel count)
+constexpr int kMpoRuns[3] = {1, 2, 4};
 
 // Kernel variant of one grouped GEMM: tile BM x BN, K chunk multiplier KS.
 struct TileCfg {
@@ -285,7 +294,7 @@ struct TileCfg {
 };
 struct HeffTiles {
   TileCfg s1{64, 128, 0}, s4{64, 64, 0};     // FP32 (DMA-streamed operands)
-  TileCfg s1h{128, 64, 1}, s4h{64, 64, 2};   // BF16 operands
+  TileCfg s1h{64, 128, 0}, s4h{64, 64, 0};   // BF16 storage (DMA-streamed BF16 operands)
   TileCfg s1d{64, 128, 0}, s4d{64, 64, 0};   // FP64 (DMA-streamed operands)
   int64_t split_target = 1024;               // minimum work items for the step-4 GEMM (FP32, BF16)
   int64_t split_target_d = 2048;             // ... FP64 (not autotuned; measured best on the H_eff shapes)
@@ -297,14 +306,20 @@ struct HeffTiles {
 constexpr TileCfg kTuneTiles[] = {{64, 64, 1}, {64, 128, 1}, {128, 64, 1}, {64, 64, 2}, {128, 128, 1},
                                   {128, 64, 2}, {64, 64, 0},  {128, 64, 0}, {64, 128, 0}};
 constexpr int64_t kTuneSplits[] = {512, 1024, 2048};
-// KS = 0: the kernel with DMA-streamed operands (FP32 / FP64), tiles up to 128 x 64.
+// KS = 0: the kernel with DMA-streamed operands (FP32 / FP64 / BF16 storage), tiles
+// up to 128 x 64.  Modes: 0 FP64, 1 FP32, 2 BF16 operands from FP32 storage
+// (register-staged only), 3 / 4 BF16 storage (DMA only; 3 writes BF16, 4 FP32).
 inline bool tune_tile_ok(int mode, const TileCfg& c) {
+  if (mode >= 3) return c.KS == 0 && c.BM * c.BN <= 128 * 64;
   if (c.KS == 0) return mode != 2 && c.BM * c.BN <= 128 * 64;
   return mode != 0 || (c.KS == 1 && c.BM * c.BN <= 128 * 64);
 }
-// K chunk of a GEMM mode (elements): 16 * KS for FP64 / FP32, 32 * KS for BF16;
-// KS = 0: 16 floats / 8 doubles
-inline int chunk_k(int mode, int KS) { return KS == 0 ? (mode == 0 ? 8 : 16) : (mode == 2 ? 32 : 16) * KS; }
+// K chunk of a GEMM mode (elements): 16 * KS for FP64 / FP32, 32 * KS for BF16
+// operands; KS = 0 (one 64-byte row per operand row): 8 doubles, 16 floats, 32 bf16
+inline int chunk_k(int mode, int KS) {
+  if (KS == 0) return mode == 0 ? 8 : (mode >= 3 ? 32 : 16);
+  return (mode == 2 ? 32 : 16) * KS;
+}
 // the step-4 GEMM of a plan with another tile and split target
 GemmPlan gemm_variant(const GemmPlan& base, const TileCfg& c, int64_t split_target, int mode);
 
@@ -319,7 +334,7 @@ struct Step13 {
   std::vector<MpoGroup> groups;
   std::vector<MpoInRow> inrows;
   std::vector<MpoOutRow> outrows;
-  std::vector<MpoWork> work;
+  std::vector<MpoWork> work[3];  // work lists for runs of kMpoRuns[i] pixels
   int64_t w_elems = 0;
   int64_t t3_elems = 0;
   int maxR = 0, maxC = 0;

@@ -178,6 +178,43 @@ def test_dma_gemm_variants_bitwise(name, tiles):
 
 
 @pytest.mark.parametrize("name", ["xxz_L32_chi64", "xxz_L100_chi512", "cyl6x24_chi2048"])
+@pytest.mark.parametrize("prec", ["f64", "f32", "bf16"])
+def test_mpo_pass_variants_bitwise(name, prec):
+    """The MPO pass sums each output's channels in the same order whatever the
+    run length (1, 2 or 4 pixels per thread; FP64 at most 2) and offset width
+    (32-bit byte offsets or 64-bit element offsets), so every variant's H_eff
+    output is bitwise equal to the default's."""
+    import os
+    from mps_amd.tensornetalgs import HeffPlan
+    dt = torch.float64 if prec == "f64" else torch.float32
+    ts = [t.to(dt) for t in _workload(name)]
+    keys = ("MPSH_MPO_RUN", "MPSH_MPO_OFF64")
+    old = {k: os.environ.get(k) for k in keys}
+    outs = {}
+    try:
+        for run in (1, 2, 4):
+            for off64 in (0, 1):
+                os.environ["MPSH_MPO_RUN"] = str(run)
+                os.environ["MPSH_MPO_OFF64"] = str(off64)
+                plan = HeffPlan(*[t.struct for t in ts])  # options are read when the plan is uploaded
+                prepared = plan.prepare(prec, *(t.data for t in ts[1:]))
+                out = torch.full((plan.out.numel,), float("nan"), dtype=dt, device=DEV)
+                plan.apply_prepared(prec, prepared, ts[0].data, out)
+                torch.cuda.synchronize()
+                outs[(run, off64)] = out
+    finally:
+        for k, v in old.items():
+            if v is None:
+                os.environ.pop(k, None)
+            else:
+                os.environ[k] = v
+    ref = outs[(1, 0)]
+    assert torch.isfinite(ref).all()
+    for key, out in outs.items():
+        assert torch.equal(out, ref), key
+
+
+@pytest.mark.parametrize("name", ["xxz_L32_chi64", "xxz_L100_chi512", "cyl6x24_chi2048"])
 @pytest.mark.parametrize("tiles", [(64, 64), (128, 64), (64, 128)])
 def test_dma_gemm_f64_variants(name, tiles):
     """The FP64 GEMM with DMA-streamed operands (K chunk 8 doubles) against the
