@@ -477,12 +477,12 @@ __global__ __launch_bounds__(256) void gemm_grouped(
     // n = g + 4r (FP64 16x16x4 form) or 4g + r (FP32/BF16 forms).  Each store
     // instruction writes 16 consecutive rows of 4 columns.
     const bool split = p.nsplit > 1;
-    T* Pp = P + (split ? p.part_off + (int64_t)tile.split * p.M * p.N : 0);
+    T* Pp = P + (split ? p.part_off + (int64_t)tile.split * p.pld * p.N : 0);
 #pragma unroll
     for (int i = 0; i < SM; ++i) {
       const int m = tile.m0 + wm * (BM / 2) + i * 16 + l16;
       if (m >= p.M) continue;
-      int64_t cb_ = m, cl_ = p.M;
+      int64_t cb_ = m, cl_ = p.pld;
       if (!split) {
         if (p.crow0 >= 0) {
           cb_ = crow[p.crow0 + m];
@@ -595,6 +595,48 @@ struct DmaT<4> : DmaT<3> {
   typedef float TC;
 };
 
+// Write-back of one wave's staged (MS x NW) FP32 accumulator sub-tile E (column
+// n at E + n * (MS + 4); the pad of 4 spreads the four MFMA lane groups' columns
+// over distinct banks) to out[m + n * ld], m = m0 + 0 .. MS-1, n = n0 + 0 .. NW-1:
+// 16-byte vectors of VW outputs along m (8 lanes cover 32 FP32 rows of one
+// column), element stores where a vector crosses M or is not 16-byte aligned.
+template <typename O, int MS, int NW>
+__device__ __forceinline__ void staged_store_impl(const float* E, O* out, int64_t ld, int m0, int n0, int M, int N,
+                                                  int lane) {
+  constexpr int LDE = MS + 4, VW = 16 / (int)sizeof(O), MQ = MS / VW, NV = MS * NW / VW / 64;
+  static_assert(NV * VW * 64 == MS * NW, "whole vectors per lane");
+#pragma unroll
+  for (int q = 0; q < NV; ++q) {
+    const int v = lane + 64 * q, ml = (v % MQ) * VW, nl = v / MQ;
+    const int m = m0 + ml, n = n0 + nl;
+    if (n >= N || m >= M) continue;
+    float x[VW];
+#pragma unroll
+    for (int e = 0; e < VW; e += 4) {
+      const floatx4 f = *(const floatx4*)(E + nl * LDE + ml + e);
+      x[e] = f[0];
+      x[e + 1] = f[1];
+      x[e + 2] = f[2];
+      x[e + 3] = f[3];
+    }
+    O* dst = out + m + (int64_t)n * ld;
+    if (m + VW <= M && ((uintptr_t)dst & 15) == 0) {
+      if constexpr (sizeof(O) == 4) {
+        *(floatx4*)dst = floatx4{x[0], x[1], x[2], x[3]};
+      } else {
+        bf16x8 b;
+#pragma unroll
+        for (int e = 0; e < 8; ++e) b[e] = (__bf16)x[e];
+        *(bf16x8*)dst = b;
+      }
+    } else {
+#pragma unroll
+      for (int e = 0; e < VW; ++e)
+        if (m + e < M) dst[e] = (O)x[e];
+    }
+  }
+}
+
 // NS: LDS ring depth (loads run NS - 1 chunks ahead).  Measured on the H_eff
 // GEMMs (autotuned tiles): FP32 is faster with 4 stages (3547 vs 3491
 // matvecs/s), FP64 with 3 (2120 vs 2050: a smaller ring leaves LDS for one
@@ -641,6 +683,15 @@ __global__ __launch_bounds__(256) void gemm_dma(
   auto stage = [&](auto ST) -> T* {
     constexpr int st = decltype(ST)::value;
     return st == 0 ? S0 : st == 1 ? S1 : st == 2 ? S2 : S3;
+  };
+  // LDS-staged epilogue: FP32 accumulators of FP32 / BF16 modes, one wave's
+  // sub-tile per ring stage (FP64 keeps the element-wise epilogue: its 3-stage
+  // ring has no stage per wave)
+  constexpr int LDE = BM / 2 + 4;
+  constexpr bool kStaged = MODE != 0 && NS == 4 && (BN / 2) * LDE * 4 <= STG * (int)sizeof(T);
+  auto staged_store = [&](const float* E, auto* out, int64_t ld, int m0, int n0, int M, int N, int ln) {
+    typedef typename std::remove_pointer<decltype(out)>::type O;
+    staged_store_impl<O, BM / 2, BN / 2>(E, out, ld, m0, n0, M, N, ln);
   };
 
   int t0, tend, tstep;
@@ -811,14 +862,40 @@ __global__ __launch_bounds__(256) void gemm_dma(
       if (it + 2 < nch) iter(I2{}, it + 2);
       if (NS == 4 && it + 3 < nch) iter(I3{}, it + 3);
     }
-    // epilogue as in gemm_grouped (split-K partials in TP, outputs in TC)
+    // epilogue (split-K partials in TP, outputs in TC)
     const bool split = p.nsplit > 1;
-    TP* Pp = P + (split ? p.part_off + (int64_t)tile.split * p.M * p.N : 0);
+    TP* Pp = P + (split ? p.part_off + (int64_t)tile.split * p.pld * p.N : 0);
+    if constexpr (kStaged) {
+      if (split || p.crow0 < 0) {
+        // outputs along m contiguous (split-K partials, or affine rows): each wave
+        // stages its (BM/2) x (BN/2) accumulators in its own ring stage as FP32,
+        // column n at E + n * LDE, and writes them back in 16-byte vectors along m
+        barrier_raw();  // every wave is done reading the ring
+        float* E = reinterpret_cast<float*>(wave == 0 ? S0 : wave == 1 ? S1 : wave == 2 ? S2 : S3);
+#pragma unroll
+        for (int i = 0; i < SM; ++i)
+#pragma unroll
+          for (int j = 0; j < SN; ++j)
+#pragma unroll
+            for (int r = 0; r < 4; ++r) E[(j * 16 + DmaT<MODE>::col(g, r)) * LDE + i * 16 + l16] = acc[i][j][r];
+        const int mw = tile.m0 + wm * (BM / 2), nw = tile.n0 + wn * (BN / 2);
+        // the write-back's lane offsets are formed here, not hoisted out of the
+        // tile loop into registers that the K loop would hold
+        int ln = lane;
+        asm volatile("" : "+v"(ln));
+        if (split)
+          staged_store(E, Pp, (int64_t)p.pld, mw, nw, p.M, p.N, ln);
+        else
+          staged_store(E, C + p.c_off, p.ldc, mw, nw, p.M, p.N, ln);
+        barrier_raw();  // the next tile refills the stages
+        continue;
+      }
+    }
 #pragma unroll
     for (int i = 0; i < SM; ++i) {
       const int m = tile.m0 + wm * (BM / 2) + i * 16 + l16;
       if (m >= p.M) continue;
-      int64_t cb_ = m, cl_ = p.M;
+      int64_t cb_ = m, cl_ = p.pld;
       if (!split) {
         if (p.crow0 >= 0) {
           cb_ = crow[p.crow0 + m];
@@ -845,7 +922,7 @@ __global__ __launch_bounds__(256) void gemm_dma(
   }
 }
 
-// C(m, n) = sum_s P[part_off + s*M*N + m + M*n] for every split problem
+// C(m, n) = sum_s P[part_off + s*pld*N + m + pld*n] for every split problem
 template <typename T, typename TC = T>
 // One work group per 1024 outputs of a split problem, named by the plan's
 // reduction map.  Each output sums its partials in split order; a thread's four
@@ -858,23 +935,31 @@ __global__ __launch_bounds__(256) void splitk_reduce(const int32_t* __restrict__
                                                      TC* __restrict__ C) {
   const int2 rm = ((const int2*)rmap)[blockIdx.x];
   const GemmProb p = probs[rm.x];
-  const int64_t MN = (int64_t)p.M * p.N;
+  const int64_t MN = (int64_t)p.M * p.N, PN = (int64_t)p.pld * p.N;
   const int64_t e0 = (int64_t)rm.y * 1024 + threadIdx.x;
+  // output e = m + M * n; pld * N < 2^31 for every split problem (checked by the
+  // planner): 32-bit division
+  int64_t mm[4], nn[4], off[4];
+#pragma unroll
+  for (int u = 0; u < 4; ++u) {
+    const int64_t e = e0 + 256 * u < MN ? e0 + 256 * u : 0;
+    const uint32_t ue = (uint32_t)e, uM = (uint32_t)p.M;
+    const uint32_t un = ue / uM;
+    nn[u] = un;
+    mm[u] = ue - un * uM;
+    off[u] = mm[u] + nn[u] * p.pld;
+  }
   T acc[4] = {0, 0, 0, 0};
   for (int s = 0; s < p.nsplit; ++s) {
-    const T* ps = P + p.part_off + s * MN;
+    const T* ps = P + p.part_off + s * PN;
 #pragma unroll
     for (int u = 0; u < 4; ++u)
-      if (e0 + 256 * u < MN) acc[u] += ps[e0 + 256 * u];
+      if (e0 + 256 * u < MN) acc[u] += ps[off[u]];
   }
 #pragma unroll
   for (int u = 0; u < 4; ++u) {
-    const int64_t e = e0 + 256 * u;
-    if (e < MN) {
-      // M * N < 2^31 for every split problem (checked by the planner): 32-bit division
-      const uint32_t ue = (uint32_t)e, uM = (uint32_t)p.M;
-      const uint32_t un = ue / uM;
-      const int64_t n = un, m = ue - un * uM;
+    if (e0 + 256 * u < MN) {
+      const int64_t m = mm[u], n = nn[u];
       if (p.crow0 >= 0)
         C[crow[p.crow0 + m] + n * cld[p.crow0 + m]] = (TC)acc[u];
       else

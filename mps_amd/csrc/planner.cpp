@@ -265,6 +265,7 @@ void GemmPlan::choose_splits(int64_t target, int BK, int64_t min_chunks_per_spli
   part_elems = 0;
   for (auto& p : probs) {
     p.nsplit = 1;
+    p.pld = (p.M + 3) & ~3;
     p.part_off = 0;
   }
   if (tiles == 0 || tiles >= target) return;
@@ -272,11 +273,11 @@ void GemmPlan::choose_splits(int64_t target, int BK, int64_t min_chunks_per_spli
   for (auto& p : probs) {
     const int64_t ch = prob_chunks(*this, p, BK);
     int64_t s = std::min<int64_t>(want, ch / std::max<int64_t>(1, min_chunks_per_split));
-    if (s > 1 && (int64_t)p.M * p.N >= ((int64_t)1 << 31)) s = 1;  // the reduction indexes M*N in 32 bits
+    if (s > 1 && (int64_t)p.pld * p.N >= ((int64_t)1 << 31)) s = 1;  // the reduction indexes M*N in 32 bits
     if (s > 1) {
       p.nsplit = (int32_t)s;
       p.part_off = part_elems;
-      part_elems += s * (int64_t)p.M * p.N;
+      part_elems += s * (int64_t)p.pld * p.N;
     }
   }
 }

@@ -128,11 +128,13 @@ struct CopyPlan {
 // same for B), crow0 < 0 means C(m, n) = C[c_off + m + n * ldc].
 // nseg == 0 means "write zeros" (reference src/contract.jl:292-294).  With
 // nsplit > 1 the K chunks are split over work items that write dense
-// partials at part_off (+ split * M * N, column-major), summed by a reduction.
+// partials at part_off (+ split * pld * N, column-major with leading dimension
+// pld = M rounded up to 4, so every partial column starts 16-byte aligned),
+// summed by a reduction.
 struct GemmProb {
   int32_t M, N, seg0, nseg;
   int64_t crow0;
-  int32_t nsplit, pad;
+  int32_t nsplit, pld;
   int64_t part_off;
   int64_t c_off, ldc;
 };
