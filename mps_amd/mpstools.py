@@ -501,6 +501,26 @@ def bond_spaces(L: int, N: int, chi: int, nsec: int = None, dmax: int = None) ->
     return out
 
 
+def state_footprint(L: int, N: int, chi: int, mpo_leg: U1Space, nsec: int = None, dmax: int = None,
+                    itemsize: int = 8) -> Dict[str, int]:
+    """Bytes of an MPS on bond_spaces(L, N, chi, nsec, dmax) and of its full set of
+    environments (legs (bond dual, MPO bond dual, bond), src/dmrg.jl:9-36) with
+    the bulk MPO bond leg ``mpo_leg`` (the boundary environments have a trivial
+    MPO leg), from the structures alone (host-only)."""
+    bs = bond_spaces(L, N, chi, nsec, dmax)
+    Vd = U1Space((0, 1), (1, 1))
+    triv = U1Space((0, 1))
+    env = mps = largest = 0
+    for l in range(L + 1):
+        Vw = triv if l in (0, L) else mpo_leg
+        n = get_struct(0, (bs[l].dual(), Vw.dual(), bs[l])).numel
+        env += n
+        largest = max(largest, n)
+    for l in range(1, L + 1):
+        mps += get_struct(0, (bs[l - 1], Vd, bs[l].dual())).numel
+    return {"mps": mps * itemsize, "environments": env * itemsize, "largest_environment": largest * itemsize}
+
+
 def random_mps(L: int, N: int, chi: int, seed: int = 1911, nsec: int = None, dmax: int = None,
                dtype=torch.float64, device=None) -> MPState:
     """Seeded Gaussian U(1) MPS (legs (l, p, r̄), charge 0 per site), right-canonical at 1, normalised."""
