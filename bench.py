@@ -28,7 +28,20 @@ HBM_PEAK = 8.0e12
 METRIC = "H_eff matvecs/sec + DMRG sweep time at χ=2048; |ΔE| vs FP64 CPU oracle"
 # HBM bytes per GEMM launch: rocprofv3 FETCH_SIZE x 2.000 + WRITE_SIZE x 1.000
 # (scales from a 256 MiB device copy), profiles/heff_cyl6x24_chi2048_f32_hbm_traffic.txt
-TRAFFIC = {("cyl6x24_chi2048", "f32", "step1"): 228.8e6}  # step-1 variant gemm_dma<1, 64, 128, 4>
+# HBM bytes per launch from rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE (scaled by a plain
+# copy's readings: x2.000 / x1.000) for the autotuned variants of this workload;
+# step 4 = its GEMM plus the split-K reduction
+TRAFFIC = {
+    ("cyl6x24_chi2048", "f32", "step1"): 211.5e6,   # gemm_dma<1, 64, 128, 4>
+    ("cyl6x24_chi2048", "f32", "step4"): 184.3e6,   # gemm_dma<1, 64, 64, 4> 153.3 + splitk_reduce 31.0
+    ("cyl6x24_chi2048", "f32", "mpo"): 211.4e6,     # mpo_apply<float, 32, 1>
+    ("cyl6x24_chi2048", "bf16", "step1"): 102.0e6,  # gemm_dma<3, 64, 128, 4>
+    ("cyl6x24_chi2048", "bf16", "step4"): 98.6e6,   # gemm_dma<4, 64, 128, 4> 81.4 + splitk_reduce 17.2
+    ("cyl6x24_chi2048", "bf16", "mpo"): 120.3e6,    # mpo_apply<float, 32, 1, __bf16>
+    ("cyl6x24_chi2048", "f64", "step1"): 433.4e6,   # gemm_dma<0, 64, 128, 3>
+    ("cyl6x24_chi2048", "f64", "step4"): 369.4e6,   # gemm_dma<0, 64, 64, 3> 304.8 + splitk_reduce 64.6
+    ("cyl6x24_chi2048", "f64", "mpo"): 390.2e6,     # mpo_apply<double, 32, 1>
+}
 
 
 
@@ -541,7 +554,7 @@ def main():
     print(f"bench: {args.workload} {prec} fused H_eff {1e3 / ms:.1f} matvecs/s", file=sys.stderr, flush=True)
     value = 1e3 / ms
     F = plan.ref_flops
-    ebytes = 8 if prec == "f64" else 4
+    ebytes = {"f64": 8, "f32": 4, "bf16": 2}[prec]  # storage bytes (BF16 mode: the H_eff output is FP32)
     # the two grouped-GEMM launches of a matvec (steps 1 and 4 of the reference
     # order, src/apply.jl:99,102); algorithmic FLOPs = 2*M*N*K of those contracts,
     # durations from HIP events on the launch stream (step 4 includes its split-K
@@ -591,6 +604,7 @@ def main():
         "mpo_pass_roofline": {"bound": "hbm", "achieved": round(mpo_bytes / (stages[2] * 1e-3) / 1e9, 1)
                               if stages[2] > 0 else 0.0, "peak": HBM_PEAK / 1e9, "unit": "GB/s",
                               "algorithmic_bytes": mpo_bytes, "avg_launch_ms": round(stages[2], 5),
+                              "traffic": TRAFFIC.get((args.workload, prec, "mpo")),
                               "note": "T1 read once + T3 written once (the combined-MPO matrices stay in LDS)"},
     }
     # E = <v|H v>/<v|v> of this matvec in the benchmark precision, FP64 device path as reference
