@@ -319,9 +319,9 @@ DevMpo upload_mpo(DevTables& t, const Step13& p) {
   m.groups = t.up(p.groups);
   m.inrows = t.up(p.inrows);
   m.outrows = t.up(p.outrows);
-  for (int i = 0; i < 3; ++i) m.work[i] = t.up(p.work[i]);
+  for (int i = 0; i < 4; ++i) m.work[i] = t.up(p.work[i]);
   m.nterms = (int)p.terms.size();
-  for (int i = 0; i < 3; ++i) m.nwork[i] = (int)p.work[i].size();
+  for (int i = 0; i < 4; ++i) m.nwork[i] = (int)p.work[i].size();
   m.cmax = (int)p.maxC;
   m.rmax = (int)p.maxR;
   m.t1_elems = p.t1_elems;

@@ -1282,26 +1282,30 @@ __global__ __launch_bounds__(256) void mpo_apply(const MpoSig* __restrict__ sigs
     om[i] = (IX)q.mstride * so;
   }
   __syncthreads();
-  // run q of the group: row m = q / nr, first pixel n0 = P * (q % nr), nr runs per row
-  const uint32_t dR = (uint32_t)g.dR, nr = (dR + P - 1) / P;
+  // run q of the group: row m = q / nr, first pixel n0 = Pg * (q % nr), nr runs per
+  // row; P = 0: the mixed list, runs of mpo_mixed_run(C) pixels
+  const int Pg = P > 0 ? P : mpo_mixed_run(C);
+  const uint32_t dR = (uint32_t)g.dR, nr = (dR + Pg - 1) / Pg;
   const int64_t nruns = (int64_t)g.db * nr;
   const int64_t q0 = (int64_t)wk.run0 + threadIdx.x;
   const bool live = q0 < nruns;
   const uint32_t q = live ? (uint32_t)q0 : 0u;
   const uint32_t mq = q / nr;
-  const IX m = (IX)mq, n0 = (IX)((q - mq * nr) * P);
+  const IX m = (IX)mq, n0 = (IX)((q - mq * nr) * Pg);
   if (C <= 0) {  // a group no input type reaches: its T3 cells are zero
     if (live)
       for (int r = 0; r < R; ++r)
-        for (int u = 0; u < P; ++u)
+        for (int u = 0; u < Pg; ++u)
           if (n0 + u < (IX)dR) *mpo_at(T3, (IX)(ob[r] + (n0 + u) * on[r] + m * om[r])) = TO(0.0f);
     return;
   }
   const int cls = (C + 3) / 4;  // uniform over the block
+  // P = 0: classes 1 and 2 (C <= 8) run 4 pixels per thread, the others 1
 #define MPSH_PIX(K)                                                                                             \
   case K:                                                                                                       \
     if constexpr (4 * K <= CMAX)                                                                                \
-      mpo_run<T, TI, TO, IX, 4 * K, CMAX, P>(Ws, ib, ins, im, ob, on, om, R, m, n0, (IX)dR, live, T1, T3);     \
+      mpo_run<T, TI, TO, IX, 4 * K, CMAX, (P > 0 ? P : (K <= 2 ? 4 : 1))>(Ws, ib, ins, im, ob, on, om, R, m, n0, \
+                                                                        (IX)dR, live, T1, T3);                  \
     break;
   switch (cls) {
     MPSH_PIX(1)
@@ -1314,7 +1318,8 @@ __global__ __launch_bounds__(256) void mpo_apply(const MpoSig* __restrict__ sigs
     MPSH_PIX(8)
     default:
       if constexpr (CMAX == 64)
-        mpo_run<T, TI, TO, IX, 64, CMAX, P>(Ws, ib, ins, im, ob, on, om, R, m, n0, (IX)dR, live, T1, T3);
+        mpo_run<T, TI, TO, IX, 64, CMAX, (P > 0 ? P : 1)>(Ws, ib, ins, im, ob, on, om, R, m, n0, (IX)dR, live, T1,
+                                                          T3);
       break;
   }
 #undef MPSH_PIX
@@ -1334,17 +1339,19 @@ hipError_t launch_mpo_combine(const DevMpo& d, int elem_size, const void* WL, co
 
 void mpo_options(DevMpo& d) {
   const char* e = std::getenv("MPSH_MPO_RUN");
-  const int v = e ? std::atoi(e) : 1;
-  d.run = (v == 2 || v == 4) ? v : 1;
+  const int v = e ? std::atoi(e) : 0;
+  d.run = (v == 1 || v == 2 || v == 4) ? v : 0;
   const char* f = std::getenv("MPSH_MPO_OFF64");
   d.off64 = f && std::atoi(f) != 0;
 }
 
 hipError_t launch_mpo_apply(const DevMpo& d, int elem_size, const void* W, const void* T1, void* T3, hipStream_t s) {
-  // runs of d.run pixels per thread; wide channel classes (CMAX 64) and FP64 at most P = 2
-  const int pw = elem_size == 8 ? std::min(2, d.run) : d.run;
-  const int P = d.cmax > 32 ? std::max(1, pw / 2) : pw;
-  const int pi = P == 4 ? 2 : (P == 2 ? 1 : 0);  // index into kMpoRuns
+  // d.run pixels per thread (0: the mixed list; FP64 then one pixel per thread,
+  // measured 2 % faster than mixed there, while FP32 / BF16 gain 0-4 % from it);
+  // wide channel classes (CMAX 64) and FP64 at most 2
+  const int pw = d.run == 0 ? (elem_size == 8 ? 1 : 0) : (elem_size == 8 ? std::min(2, d.run) : d.run);
+  const int P = pw == 0 ? 0 : (d.cmax > 32 ? std::max(1, pw / 2) : pw);
+  const int pi = P == 0 ? 3 : (P == 4 ? 2 : (P == 2 ? 1 : 0));  // work list
   if (d.nwork[pi] == 0) return hipSuccess;
   // 32-bit byte offsets when every T1 / T3 byte offset fits
   const int64_t bytes = std::max(d.t1_elems, d.t3_elems) * elem_size;
@@ -1355,20 +1362,24 @@ hipError_t launch_mpo_apply(const DevMpo& d, int elem_size, const void* W, const
                      d.work[pi], (const T*)W, (const TI*)T1, (TI*)T3)
 #define MPSH_MPO_P(T, TI, IX)                   \
   if (d.cmax <= 32) {                           \
-    if (P == 4) MPSH_MPO(T, 32, 4, TI, IX);     \
+    if (P == 0) MPSH_MPO(T, 32, 0, TI, IX);     \
+    else if (P == 4) MPSH_MPO(T, 32, 4, TI, IX);\
     else if (P == 2) MPSH_MPO(T, 32, 2, TI, IX);\
     else MPSH_MPO(T, 32, 1, TI, IX);            \
   } else {                                      \
-    if (P == 2) MPSH_MPO(T, 64, 2, TI, IX);     \
+    if (P == 0) MPSH_MPO(T, 64, 0, TI, IX);     \
+    else if (P == 2) MPSH_MPO(T, 64, 2, TI, IX);\
     else MPSH_MPO(T, 64, 1, TI, IX);            \
   }
 #define MPSH_MPO_T(IX)                                            \
   if (elem_size == 8) {                                           \
     if (d.cmax <= 32) {                                           \
-      if (P == 2) MPSH_MPO(double, 32, 2, double, IX);            \
+      if (P == 0) MPSH_MPO(double, 32, 0, double, IX);            \
+      else if (P == 2) MPSH_MPO(double, 32, 2, double, IX);       \
       else MPSH_MPO(double, 32, 1, double, IX);                   \
     } else {                                                      \
-      MPSH_MPO(double, 64, 1, double, IX);                        \
+      if (P == 0) MPSH_MPO(double, 64, 0, double, IX);            \
+      else MPSH_MPO(double, 64, 1, double, IX);                   \
     }                                                             \
   } else if (elem_size == 4) {                                    \
     MPSH_MPO_P(float, float, IX)                                  \

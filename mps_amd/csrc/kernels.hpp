@@ -51,11 +51,11 @@ struct DevMpo {
   const MpoGroup* groups = nullptr;
   const MpoInRow* inrows = nullptr;
   const MpoOutRow* outrows = nullptr;
-  const MpoWork* work[3] = {nullptr, nullptr, nullptr};  // runs of kMpoRuns[i] pixels
-  int nwork[3] = {0, 0, 0};
+  const MpoWork* work[4] = {nullptr, nullptr, nullptr, nullptr};  // runs of kMpoRuns[i] pixels; [3] mixed
+  int nwork[4] = {0, 0, 0, 0};
   int nterms = 0, cmax = 32, rmax = 32;
   int64_t t1_elems = 0, t3_elems = 0;  // T1 / T3 sizes (32-bit byte offsets when both fit)
-  int run = 1;                          // pixels per thread (1, 2, 4; FP64 and CMAX 64 at most 2)
+  int run = 0;                          // pixels per thread (1, 2, 4; FP64 and CMAX 64 at most 2), 0: mixed
   bool off64 = false;                   // force 64-bit offsets
 };
 // the MPO pass options of newly uploaded plans: MPSH_MPO_RUN (1, 2, 4) and MPSH_MPO_OFF64 (0 / 1)

@@ -1148,8 +1148,9 @@ static void mpo_layout(Step13& H, const Struct& WL, const Struct& WR, const std:
     }
     const int32_t gidx = (int32_t)H.groups.size();
     H.groups.push_back(g);
-    for (int i = 0; i < 3; ++i) {
-      const int64_t runs = db * ((dR + kMpoRuns[i] - 1) / kMpoRuns[i]);
+    for (int i = 0; i < 4; ++i) {
+      const int P = i < 3 ? kMpoRuns[i] : mpo_mixed_run((int)C);
+      const int64_t runs = db * ((dR + P - 1) / P);
       for (int64_t r0 = 0; r0 < runs; r0 += 256) H.work[i].push_back(MpoWork{gidx, (int32_t)r0});
     }
     H.mpo_flops += 2 * R * C * g.npix;

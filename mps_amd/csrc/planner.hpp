@@ -289,6 +289,9 @@ struct MpoWork {  // one work group of the MPO pass: runs [run0, run0 + 256) of 
 // The MPO pass gives each thread a run of P consecutive pixels along the ket_r
 // index n of one bra_l row m; P = 1, 2 or 4 (by storage type and channel count)
 constexpr int kMpoRuns[3] = {1, 2, 4};
+// the mixed work list: runs of 4 pixels in groups of at most 8 channels (little
+// work per pixel: fewer work groups), one pixel per thread otherwise
+constexpr int mpo_mixed_run(int C) { return C <= 8 ? 4 : 1; }
 
 // Kernel variant of one grouped GEMM: tile BM x BN, K chunk multiplier KS.
 struct TileCfg {
@@ -336,7 +339,7 @@ struct Step13 {
   std::vector<MpoGroup> groups;
   std::vector<MpoInRow> inrows;
   std::vector<MpoOutRow> outrows;
-  std::vector<MpoWork> work[3];  // work lists for runs of kMpoRuns[i] pixels
+  std::vector<MpoWork> work[4];  // work lists for runs of kMpoRuns[i] pixels; [3]: kMpoMixed runs
   int64_t w_elems = 0;
   int64_t t3_elems = 0;
   int maxR = 0, maxC = 0;

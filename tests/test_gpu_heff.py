@@ -181,9 +181,10 @@ def test_dma_gemm_variants_bitwise(name, tiles):
 @pytest.mark.parametrize("prec", ["f64", "f32", "bf16"])
 def test_mpo_pass_variants_bitwise(name, prec):
     """The MPO pass sums each output's channels in the same order whatever the
-    run length (1, 2 or 4 pixels per thread; FP64 at most 2) and offset width
-    (32-bit byte offsets or 64-bit element offsets), so every variant's H_eff
-    output is bitwise equal to the default's."""
+    run length (the default mixed list: 4 pixels per thread in groups of at
+    most 8 channels, 1 otherwise; or 1, 2 or 4 everywhere, FP64 at most 2) and
+    offset width (32-bit byte offsets or 64-bit element offsets), so every
+    variant's H_eff output is bitwise equal to the default's."""
     import os
     from mps_amd.tensornetalgs import HeffPlan
     dt = torch.float64 if prec == "f64" else torch.float32
@@ -192,7 +193,7 @@ def test_mpo_pass_variants_bitwise(name, prec):
     old = {k: os.environ.get(k) for k in keys}
     outs = {}
     try:
-        for run in (1, 2, 4):
+        for run in (0, 1, 2, 4):
             for off64 in (0, 1):
                 os.environ["MPSH_MPO_RUN"] = str(run)
                 os.environ["MPSH_MPO_OFF64"] = str(off64)
@@ -208,7 +209,7 @@ def test_mpo_pass_variants_bitwise(name, prec):
                 os.environ.pop(k, None)
             else:
                 os.environ[k] = v
-    ref = outs[(1, 0)]
+    ref = outs[(0, 0)]
     assert torch.isfinite(ref).all()
     for key, out in outs.items():
         assert torch.equal(out, ref), key
