@@ -558,9 +558,10 @@ def main():
     # the two grouped-GEMM launches of a matvec (steps 1 and 4 of the reference
     # order, src/apply.jl:99,102); algorithmic FLOPs = 2*M*N*K of those contracts,
     # durations from HIP events on the launch stream (step 4 includes its split-K
-    # reduction launch)
+    # or stream-K reduction launch)
     launches = {}
-    for key, si, stage, what in (("step1", 0, 1, "v^T envL"), ("step4", 3, 3, "T3^T envR, with splitk_reduce")):
+    red = "tile_reduce" if tuned and tuned.get("split", 0) < 0 else "splitk_reduce"
+    for key, si, stage, what in (("step1", 0, 1, "v^T envL"), ("step4", 3, 3, f"T3^T envR, with {red}")):
         fl = plan.step_flops[si]
         t = stages[stage]
         launches[key] = {"kernel": f"{gemm_kernel(prec, tuned, key)} ({what})", "flops_per_launch": fl,

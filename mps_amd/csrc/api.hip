@@ -123,9 +123,37 @@ struct DevTables {
   }
   // mode >= 0: the balanced per-work-group schedule for this kernel variant's
   // resident work groups (mode as in launch_gemm); mode < 0: shared queues
+  // A stream-K plan (GemmPlan::use_streamk) always gets its per-work-group piece
+  // lists, for min(resident work groups, the plan's bound) work groups per XCD.
   DevGemm gemm(const GemmPlan& g, int mode = -1) {
     DevGemm d;
     std::vector<int32_t> qoff, woff;
+    if (g.streamk) {
+      if (g.KS != 0 || g.sk_mode < 0) throw Error(ERR_INVALID, "stream-K needs a DMA-streamed kernel variant");
+      const int cap = sk_max_slots(g.sk_mode, g.BM, g.BN);
+      int slots = gemm_slots_per_xcd(g.sk_mode, g.BM, g.BN, g.KS);
+      if (slots <= 0 || slots > cap) slots = cap;
+      std::vector<TileRed> tr;
+      std::vector<GemmTile> t = g.streamk_schedule(slots, chunk_k(g.sk_mode, g.KS), &woff, &tr);
+      (void)g.tiles(&qoff);
+      d.q.slots = slots;
+      d.q.woff = up(woff);
+      for (int x = 0; x <= NXCD; ++x) d.q.off[x] = qoff[x];
+      d.probs = up(g.probs);
+      d.segs = up(g.segs);
+      d.atab = up(g.atab);
+      d.btab = up(g.btab);
+      d.crow = up(g.crow);
+      d.cld = up(g.cld);
+      d.tiles = up(t);
+      d.ntiles = (int)t.size();
+      d.treds = up(tr);
+      d.ntred = (int)tr.size();
+      d.BM = g.BM;
+      d.BN = g.BN;
+      d.KS = g.KS;
+      return d;
+    }
     const int slots = mode >= 0 ? gemm_slots_per_xcd(mode, g.BM, g.BN, g.KS) : 0;
     std::vector<GemmTile> t = slots > 0 ? g.schedule(slots, chunk_k(mode, g.KS), &woff) : g.tiles(&qoff);
     if (slots > 0) {
@@ -175,7 +203,8 @@ void need_aligned(const void* p, const char* what) {
 }
 
 // Optional override of the H_eff kernel variants, read at plan creation:
-// MPSH_HEFF_TILES="s1=64x128x1,s4=64x64x2,s1h=128x64x1,s4h=64x64x2,s1d=64x64x1,s4d=64x64x1,split=1024,splitd=2048"
+// MPSH_HEFF_TILES="s1=64x128x1,s4=64x64x2,s1h=128x64x1,s4h=64x64x2,s1d=64x64x1,s4d=64x64x1,split=1024,splith=2048,splitd=-1"
+// (split targets < 0: stream-K)
 HeffTiles tiles_from_env() {
   HeffTiles t;
   const char* e = std::getenv("MPSH_HEFF_TILES");
@@ -195,6 +224,10 @@ HeffTiles tiles_from_env() {
     }
     if (k == "splitd") {
       t.split_target_d = std::atoll(v.c_str());
+      continue;
+    }
+    if (k == "splith") {
+      t.split_target_h = std::atoll(v.c_str());
       continue;
     }
     int bm = 0, bn = 0, ks = 1;
@@ -744,7 +777,7 @@ mpsh_status mpsh_heff_plan(mpsh_struct v, mpsh_struct envL, mpsh_struct envR, mp
     for (int m = 0; m < 3; ++m) {
       h->sel1[m] = VariantKey{1, m == 2 ? 3 : m, c1[m]->BM, c1[m]->BN, c1[m]->KS, 0};
       h->sel4[m] = VariantKey{4, m == 2 ? 4 : m, c4[m]->BM, c4[m]->BN, c4[m]->KS,
-                              m == 0 ? t.split_target_d : t.split_target};
+                              m == 0 ? t.split_target_d : (m == 2 ? t.split_target_h : t.split_target)};
       if (!tune_tile_ok(std::get<1>(h->sel1[m]), *c1[m]) || !tune_tile_ok(std::get<1>(h->sel4[m]), *c4[m]))
         throw Error(ERR_INVALID, "MPSH_HEFF_TILES: no such kernel variant for this precision");
     }

@@ -705,6 +705,10 @@ __global__ __launch_bounds__(256) void gemm_dma(
     tile.m0 = uniform_i32(tile.m0);
     tile.n0 = uniform_i32(tile.n0);
     tile.split = uniform_i32(tile.split);
+    tile.kc0 = uniform_i32(tile.kc0);
+    tile.kc1 = uniform_i32(tile.kc1);
+    tile.pld = uniform_i32(tile.pld);
+    tile.poff = uniform_i64(tile.poff);
     GemmProb p = probs[tile.prob];
     p.M = uniform_i32(p.M);
     p.N = uniform_i32(p.N);
@@ -722,9 +726,13 @@ __global__ __launch_bounds__(256) void gemm_dma(
       for (int j = 0; j < SN; ++j)
 #pragma unroll
         for (int r = 0; r < 4; ++r) acc[i][j][r] = 0;
-    int64_t ctot = 0;
-    for (int s = 0; s < p.nseg; ++s) ctot += (uniform_i64(segs[p.seg0 + s].K) + BK - 1) / BK;
-    const int64_t cb = ctot * tile.split / p.nsplit, ce = ctot * (tile.split + 1) / p.nsplit;
+    int64_t cb = tile.kc0, ce = tile.kc1;  // an explicit chunk range (stream-K pieces)
+    if (ce < 0) {                          // else this split's share of the problem's chunks
+      int64_t ctot = 0;
+      for (int s = 0; s < p.nseg; ++s) ctot += (uniform_i64(segs[p.seg0 + s].K) + BK - 1) / BK;
+      cb = ctot * tile.split / p.nsplit;
+      ce = ctot * (tile.split + 1) / p.nsplit;
+    }
     const int nch = (int)(ce - cb);
     // issue side: segment, k offset, this lane's row bases
     int seg = 0;
@@ -862,9 +870,14 @@ __global__ __launch_bounds__(256) void gemm_dma(
       if (it + 2 < nch) iter(I2{}, it + 2);
       if (NS == 4 && it + 3 < nch) iter(I3{}, it + 3);
     }
-    // epilogue (split-K partials in TP, outputs in TC)
-    const bool split = p.nsplit > 1;
-    TP* Pp = P + (split ? p.part_off + (int64_t)tile.split * p.pld * p.N : 0);
+    // epilogue (split-K partials and stream-K partial tiles in TP, outputs in TC).
+    // Partial element (m, n) at Pp[(m - pm0) + (n - pn0) * pl]: the split's dense
+    // M x N partial (pm0 = pn0 = 0, pl = pld), or the piece's BM x BN partial tile
+    const bool piece = tile.pld > 0;
+    const bool split = p.nsplit > 1 || piece;
+    TP* Pp = P + (piece ? tile.poff : (p.nsplit > 1 ? p.part_off + (int64_t)tile.split * p.pld * p.N : 0));
+    const int pm0 = piece ? tile.m0 : 0, pn0 = piece ? tile.n0 : 0;
+    const int64_t pl = piece ? (int64_t)tile.pld : (int64_t)p.pld;
     if constexpr (kStaged) {
       if (split || p.crow0 < 0) {
         // outputs along m contiguous (split-K partials, or affine rows): each wave
@@ -884,7 +897,7 @@ __global__ __launch_bounds__(256) void gemm_dma(
         int ln = lane;
         asm volatile("" : "+v"(ln));
         if (split)
-          staged_store(E, Pp, (int64_t)p.pld, mw, nw, p.M, p.N, ln);
+          staged_store(E, Pp, pl, mw - pm0, nw - pn0, p.M - pm0, p.N - pn0, ln);
         else
           staged_store(E, C + p.c_off, p.ldc, mw, nw, p.M, p.N, ln);
         barrier_raw();  // the next tile refills the stages
@@ -895,7 +908,7 @@ __global__ __launch_bounds__(256) void gemm_dma(
     for (int i = 0; i < SM; ++i) {
       const int m = tile.m0 + wm * (BM / 2) + i * 16 + l16;
       if (m >= p.M) continue;
-      int64_t cb_ = m, cl_ = p.pld;
+      int64_t cb_ = m - pm0 - (int64_t)pn0 * pl, cl_ = pl;
       if (!split) {
         if (p.crow0 >= 0) {
           cb_ = crow[p.crow0 + m];
@@ -968,6 +981,40 @@ __global__ __launch_bounds__(256) void splitk_reduce(const int32_t* __restrict__
   }
 }
 
+// Stream-K reduction: one work group per cut tile r sums the tile's np partial
+// tiles (BM x BN, column-major, leading dimension BM) in piece order and writes
+// the tile's part of C; each thread sums four consecutive rows of one column.
+template <typename T, typename TC>
+__global__ __launch_bounds__(256) void tile_reduce(const TileRed* __restrict__ reds,
+                                                   const GemmProb* __restrict__ probs,
+                                                   const int64_t* __restrict__ crow,
+                                                   const int64_t* __restrict__ cld, const T* __restrict__ P,
+                                                   TC* __restrict__ C, int bm, int bn) {
+  const TileRed r = reds[blockIdx.x];
+  const GemmProb p = probs[r.prob];
+  const int64_t bs = (int64_t)bm * bn;
+  const T* src = P + r.blk0 * bs;
+  for (int e = 4 * (int)threadIdx.x; e < bm * bn; e += 1024) {
+    const int ml = e % bm, nl = e / bm;  // bm is a multiple of 4: the four rows share a column
+    const int m = r.m0 + ml, n = r.n0 + nl;
+    if (m >= p.M || n >= p.N) continue;
+    T acc[4] = {0, 0, 0, 0};
+    for (int i = 0; i < r.np; ++i) {
+      const T* q = src + i * bs + e;
+#pragma unroll
+      for (int u = 0; u < 4; ++u) acc[u] += q[u];
+    }
+#pragma unroll
+    for (int u = 0; u < 4; ++u) {
+      if (m + u >= p.M) break;
+      if (p.crow0 >= 0)
+        C[crow[p.crow0 + m + u] + (int64_t)n * cld[p.crow0 + m + u]] = (TC)acc[u];
+      else
+        C[p.c_off + m + u + (int64_t)n * p.ldc] = (TC)acc[u];
+    }
+  }
+}
+
 static int g_num_cu = 0;
 static int num_cu() {
   if (g_num_cu == 0) {
@@ -981,6 +1028,21 @@ static int num_cu() {
   return g_num_cu;
 }
 
+// The occupancy query's count, lowered so the work groups' static LDS stays below
+// the CU's LDS: on MI355X five work groups of 32 KiB (the CU's whole 160 KiB)
+// were measured to run four at a time, the fifth work group of each CU as a
+// second wave of the persistent grid.
+static int lds_fit(const void* fn, int occ) {
+  hipFuncAttributes a;
+  int dev = 0, lds = 0;
+  if (hipFuncGetAttributes(&a, fn) != hipSuccess || a.sharedSizeBytes == 0) return occ;
+  if (hipGetDevice(&dev) != hipSuccess ||
+      hipDeviceGetAttribute(&lds, hipDeviceAttributeMaxSharedMemoryPerMultiprocessor, dev) != hipSuccess || lds <= 0)
+    return occ;
+  const int fit = (int)((size_t)(lds - 1) / a.sharedSizeBytes);
+  return std::max(1, std::min(occ, fit));
+}
+
 // work groups of a variant one CU holds at once (queried once per variant)
 template <int MODE, int BM, int BN, int KS>
 static int occ_grouped() {
@@ -990,7 +1052,7 @@ static int occ_grouped() {
     if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, gemm_grouped<MODE, BM, BN, KS>, 256, 0) != hipSuccess ||
         nb <= 0)
       nb = 2;
-    occ = nb;
+    occ = lds_fit((const void*)gemm_grouped<MODE, BM, BN, KS>, nb);
   }
   return occ;
 }
@@ -1001,7 +1063,7 @@ static int occ_dma() {
     int nb = 0;
     if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, gemm_dma<MODE, BM, BN>, 256, 0) != hipSuccess || nb <= 0)
       nb = 2;
-    occ = nb;
+    occ = lds_fit((const void*)gemm_dma<MODE, BM, BN>, nb);
   }
   return occ;
 }
@@ -1130,7 +1192,7 @@ int gemm_slots_per_xcd(int mode, int BM, int BN, int KS) {
 hipError_t launch_gemm(const DevGemm& gm, int mode, const void* A, const void* B, void* C, void* P,
                        hipStream_t s) {
   if (gm.ntiles == 0) return hipSuccess;
-  if (gm.nred > 0 && P == nullptr) return hipErrorInvalidValue;
+  if ((gm.nred > 0 || gm.ntred > 0) && P == nullptr) return hipErrorInvalidValue;
   hipError_t e;
   if (mode == 0)
     e = launch_mode<0>(gm, A, B, C, P, s);
@@ -1144,7 +1206,22 @@ hipError_t launch_gemm(const DevGemm& gm, int mode, const void* A, const void* B
     e = launch_mode<4>(gm, A, B, C, P, s);
   else
     return hipErrorInvalidValue;
-  if (e != hipSuccess || gm.nred == 0) return e;
+  if (e != hipSuccess) return e;
+  if (gm.ntred > 0) {
+    const dim3 tg((unsigned)gm.ntred), tb(256);
+    if (mode == 0)
+      hipLaunchKernelGGL((tile_reduce<double, double>), tg, tb, 0, s, gm.treds, gm.probs, gm.crow, gm.cld,
+                         (const double*)P, (double*)C, gm.BM, gm.BN);
+    else if (mode == 3)
+      hipLaunchKernelGGL((tile_reduce<float, __bf16>), tg, tb, 0, s, gm.treds, gm.probs, gm.crow, gm.cld,
+                         (const float*)P, (__bf16*)C, gm.BM, gm.BN);
+    else
+      hipLaunchKernelGGL((tile_reduce<float, float>), tg, tb, 0, s, gm.treds, gm.probs, gm.crow, gm.cld,
+                         (const float*)P, (float*)C, gm.BM, gm.BN);
+    e = hipGetLastError();
+    if (e != hipSuccess) return e;
+  }
+  if (gm.nred == 0) return hipSuccess;
   const dim3 rg((unsigned)gm.red_groups), rb(256);
   if (mode == 0)
     hipLaunchKernelGGL((splitk_reduce<double, double>), rg, rb, 0, s, gm.red_map, gm.probs, gm.crow, gm.cld,

@@ -40,6 +40,8 @@ struct DevGemm {
   const int32_t* red_map = nullptr;  // per reduction work group: (problem, group within it)
   int ntiles = 0, nred = 0;
   int64_t red_groups = 0;
+  const TileRed* treds = nullptr;  // stream-K: cut tiles, one reduction work group each
+  int ntred = 0;
   int BM = 64, BN = 64, KS = 1;  // tile and K-chunk multiplier
   XcdQueues q{};                 // tiles [q.off[x], q.off[x+1]) run on XCD x
 };

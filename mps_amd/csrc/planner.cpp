@@ -4,6 +4,7 @@
 
 #include <algorithm>
 #include <array>
+#include <cmath>
 #include <map>
 #include <numeric>
 #include <set>
@@ -260,6 +261,8 @@ static int64_t prob_chunks(const GemmPlan& g, const GemmProb& p, int BK) {
 }
 
 void GemmPlan::choose_splits(int64_t target, int BK, int64_t min_chunks_per_split) {
+  streamk = false;
+  sk_mode = -1;
   int64_t tiles = 0;
   for (auto& p : probs) tiles += ((p.M + BM - 1) / BM) * ((p.N + BN - 1) / BN);
   part_elems = 0;
@@ -282,12 +285,36 @@ void GemmPlan::choose_splits(int64_t target, int BK, int64_t min_chunks_per_spli
   }
 }
 
+void GemmPlan::use_streamk(int mode) {
+  choose_splits(0, 16, 32);  // no per-problem split
+  streamk = true;
+  sk_mode = mode;
+  // partial tiles: at most two per work group, and a tile of c chunks is cut
+  // into at most c / kStreamKMinPiece pieces (only tiles of >= 2 minimum pieces)
+  const int BK = chunk_k(mode, 0);
+  int64_t cut = 0;
+  for (const GemmProb& p : probs) {
+    const int64_t c = prob_chunks(*this, p, BK);
+    if (c >= 2 * kStreamKMinPiece)
+      cut += (int64_t)((p.M + BM - 1) / BM) * ((p.N + BN - 1) / BN) * (c / kStreamKMinPiece);
+  }
+  part_elems = std::min<int64_t>(2 * (int64_t)NXCD * sk_max_slots(mode, BM, BN), cut) * BM * BN;
+}
+
+// split target < 0: stream-K (DMA-streamed variants; others keep split-K at 1024 items)
+void set_splits(GemmPlan& g, int64_t split_target, int mode) {
+  if (split_target < 0 && g.KS == 0)
+    g.use_streamk(mode);
+  else
+    g.choose_splits(split_target < 0 ? 1024 : split_target, chunk_k(mode, g.KS), 32);  // target 0: no split-K
+}
+
 GemmPlan gemm_variant(const GemmPlan& base, const TileCfg& c, int64_t split_target, int mode) {
   GemmPlan g = base;
   g.BM = c.BM;
   g.BN = c.BN;
   g.KS = c.KS;
-  g.choose_splits(split_target, chunk_k(mode, c.KS), 32);  // target 0: no split-K
+  set_splits(g, split_target, mode);
   return g;
 }
 
@@ -387,6 +414,97 @@ std::vector<GemmTile> GemmPlan::schedule(int slots, int BK, std::vector<int32_t>
       (*woff)[(size_t)x * slots + w + 1] = (int32_t)out.size();
     }
   }
+  return out;
+}
+
+std::vector<GemmTile> GemmPlan::streamk_schedule(int slots, int BK, std::vector<int32_t>* woff,
+                                                 std::vector<TileRed>* reds) const {
+  if (slots < 1) throw Error(ERR_INVALID, "streamk_schedule: slots < 1");
+  std::vector<int32_t> qoff;
+  const std::vector<GemmTile> q = tiles(&qoff);  // every problem unsplit (use_streamk)
+  std::vector<int64_t> ctot(probs.size(), 0);
+  for (size_t pi = 0; pi < probs.size(); ++pi)
+    for (int s2 = 0; s2 < probs[pi].nseg; ++s2) ctot[pi] += (segs[probs[pi].seg0 + s2].K + BK - 1) / BK;
+  struct Piece {
+    int32_t tile;
+    int64_t k0, k1;
+  };
+  std::vector<GemmTile> out;
+  woff->assign((size_t)NXCD * slots + 1, 0);
+  reds->clear();
+  int64_t blk = 0;
+  const int64_t ovh = kStreamKOverhead, minp = kStreamKMinPiece;
+  for (int x = 0; x < NXCD; ++x) {
+    const int32_t b = qoff[x], e = qoff[x + 1];
+    // work of the queue: chunks plus one overhead per piece (each work-group
+    // boundary adds at most one piece)
+    int64_t total = ovh * (int64_t)(slots - 1);
+    for (int32_t t = b; t < e; ++t) total += ctot[q[t].prob] + ovh;
+    std::vector<std::vector<Piece>> lists(slots);
+    int j = 0;
+    double cum = 0;  // work placed so far on this XCD
+    // work group j's share ends at `end`: the work left spread evenly over the
+    // work groups left, so rounding and short-piece rules do not pile up
+    double end = (double)total / slots;
+    auto next = [&]() {
+      ++j;
+      end = cum + ((double)total - cum) / (slots - j);
+    };
+    for (int32_t t = b; t < e; ++t) {
+      const int64_t c = ctot[q[t].prob];
+      int64_t k = 0;
+      for (;;) {
+        int64_t take = c - k;
+        if (j < slots - 1) {
+          const int64_t room = (int64_t)std::floor(end - cum + 0.5) - ovh;
+          if (room < take) {
+            if (room < minp && !lists[j].empty()) {  // too little room left: the next work group starts it
+              next();
+              continue;
+            }
+            take = std::max(room, std::min(take, minp));
+            if (c - k - take < minp) take = c - k;  // no short remainder
+          }
+        }
+        lists[j].push_back(Piece{t, k, k + take});
+        cum += (double)(ovh + take);
+        k += take;
+        if (j < slots - 1 && cum >= end - 0.5) next();
+        if (k >= c) break;
+      }
+    }
+    // pieces of each tile, in K order (= work-group order)
+    std::vector<std::vector<std::pair<int, int>>> where(e - b);  // (list, index)
+    for (int w = 0; w < slots; ++w)
+      for (int i = 0; i < (int)lists[w].size(); ++i) where[lists[w][i].tile - b].push_back({w, i});
+    std::vector<std::vector<GemmTile>> lt(slots);
+    for (int w = 0; w < slots; ++w) lt[w].resize(lists[w].size());
+    for (int32_t t = b; t < e; ++t) {
+      const auto& ps = where[t - b];
+      const GemmTile& tl = q[t];
+      const bool cut = ps.size() > 1;
+      if (cut) reds->push_back(TileRed{tl.prob, tl.m0, tl.n0, (int32_t)ps.size(), blk});
+      for (size_t i = 0; i < ps.size(); ++i) {
+        const Piece& pc = lists[ps[i].first][ps[i].second];
+        GemmTile g = tl;
+        g.split = 0;
+        g.kc0 = (int32_t)pc.k0;
+        g.kc1 = (int32_t)pc.k1;
+        if (cut) {
+          g.pld = BM;
+          g.poff = (blk + (int64_t)i) * BM * BN;
+        }
+        lt[ps[i].first][ps[i].second] = g;
+      }
+      if (cut) blk += (int64_t)ps.size();
+    }
+    for (int w = 0; w < slots; ++w) {
+      out.insert(out.end(), lt[w].begin(), lt[w].end());
+      (*woff)[(size_t)x * slots + w + 1] = (int32_t)out.size();
+    }
+  }
+  if (blk * BM * BN > part_elems)
+    throw Error(ERR_INVALID, "stream-K schedule needs more partial tiles than its plan holds");
   return out;
 }
 
@@ -1272,9 +1390,9 @@ HeffPlan heff_plan(const Struct& v, const Struct& envL, const Struct& envR, cons
   cfg(H.g1d, tiles.s1d);
   cfg(H.g4d, tiles.s4d);
   // K chunk of each kernel (chunk_k); BF16 runs the BF16-storage kernels (modes 3, 4)
-  H.g4.choose_splits(tiles.split_target, chunk_k(1, tiles.s4.KS), 32);
-  H.g4h.choose_splits(tiles.split_target, chunk_k(4, tiles.s4h.KS), 32);
-  H.g4d.choose_splits(tiles.split_target_d, chunk_k(0, tiles.s4d.KS), 32);
+  set_splits(H.g4, tiles.split_target, 1);
+  set_splits(H.g4h, tiles.split_target_h, 4);
+  set_splits(H.g4d, tiles.split_target_d, 0);
   H.pack_v.finalize();
   H.pack_envL.finalize();
   H.pack_envR.finalize();

@@ -146,6 +146,18 @@ constexpr int NXCD = 8;  // MI355X: 8 accelerator complex dies, one L2 each
 
 struct GemmTile {
   int32_t prob, m0, n0, split;
+  // explicit K-chunk range [kc0, kc1) of the problem's flattened segment chunks
+  // (kc1 < 0: the split's share, chunks * split / nsplit .. chunks * (split+1) / nsplit);
+  // pld > 0: a piece of a stream-K tile, writing its partial tile (m, n) to
+  // P[poff + (m - m0) + (n - n0) * pld] (poff = block * BM * BN, pld = BM)
+  int32_t kc0 = 0, kc1 = -1, pld = 0, pad = 0;
+  int64_t poff = 0;
+};
+// One stream-K tile cut into np > 1 pieces: its partial tiles are the BM x BN
+// blocks blk0 .. blk0 + np - 1 of P, summed in that order by the reduction.
+struct TileRed {
+  int32_t prob, m0, n0, np;
+  int64_t blk0;
 };
 struct RedDesc {  // one split problem of the reduction pass
   int32_t prob, pad;
@@ -159,6 +171,11 @@ struct GemmPlan {
   int64_t flops = 0;  // 2*M*N*K summed (algorithmic)
   int BM = 64, BN = 64, KS = 1;  // tile shape and K-chunk multiplier of the kernel variant
   int64_t part_elems = 0;
+  // stream-K (use_streamk): no per-problem split; every work group of the launch
+  // gets a contiguous range of equal K-chunk work across the tiles of its XCD's
+  // queue (streamk_schedule), tiles cut between work groups write partial tiles
+  bool streamk = false;
+  int sk_mode = -1;  // kernel mode the stream-K bound was sized for
   // open a problem; its C rows follow with add_crow()
   void add_prob(int64_t M, int64_t N) {
     probs.push_back(GemmProb{(int32_t)M, (int32_t)N, (int32_t)segs.size(), 0, (int64_t)crow.size(), 1, 0, 0, 0, 0});
@@ -193,7 +210,34 @@ struct GemmPlan {
   // bounds list j of XCD x at [woff[x*slots + j], woff[x*slots + j + 1]).
   std::vector<GemmTile> schedule(int slots, int BK, std::vector<int32_t>* woff) const;
   std::vector<RedDesc> reductions() const;
+  // Stream-K instead of split-K for a DMA-streamed kernel variant (KS = 0) of
+  // `mode`: resets every split and sizes part_elems for the largest partial
+  // layout streamk_schedule can produce (two partial tiles per work group, at
+  // most sk_max_slots(mode) work groups per XCD).
+  void use_streamk(int mode);
+  // Per-work-group piece lists for `slots` work groups per XCD (slots <=
+  // sk_max_slots(mode)): XCD x's queue (tiles()) is cut into `slots` contiguous
+  // ranges of equal work (K chunks of BK elements plus kStreamKOverhead chunks
+  // per piece); a tile that straddles ranges becomes one piece per range (no
+  // piece shorter than kStreamKMinPiece chunks unless its tile is).  woff as in
+  // schedule(); reds lists the cut tiles.
+  std::vector<GemmTile> streamk_schedule(int slots, int BK, std::vector<int32_t>* woff,
+                                         std::vector<TileRed>* reds) const;
 };
+constexpr int kStreamKOverhead = 4;  // chunks: a piece's set-up, pipeline fill and stores
+constexpr int kStreamKMinPiece = 8;  // chunks
+// upper bound of the resident work groups per XCD of a DMA-streamed GEMM variant
+// (32 CUs per XCD, at most 8 work groups of 256 threads per CU, 160 KiB of LDS
+// per CU over the variant's LDS ring: 3 stages for FP64, 4 otherwise, of
+// (BM + BN) rows of 64 bytes)
+inline int sk_max_slots(int mode, int BM, int BN) {
+  const int ns = mode == 0 ? 3 : 4;
+  const int lds = ns * (BM + BN) * 64;
+  int occ = (160 * 1024) / lds;
+  if (occ > 8) occ = 8;
+  if (occ < 1) occ = 1;
+  return 32 * occ;
+}
 
 // ---------------------------------------------------------------------------
 // contract(A, idxA, B, idxB): the reference's fuse -> GEMM -> split -> permute
@@ -301,8 +345,13 @@ struct HeffTiles {
   TileCfg s1{64, 128, 0}, s4{64, 64, 0};     // FP32 (DMA-streamed operands)
   TileCfg s1h{64, 128, 0}, s4h{64, 64, 0};   // BF16 storage (DMA-streamed BF16 operands)
   TileCfg s1d{64, 128, 0}, s4d{64, 64, 0};   // FP64 (DMA-streamed operands)
-  int64_t split_target = 1024;               // minimum work items for the step-4 GEMM (FP32, BF16)
-  int64_t split_target_d = 2048;             // ... FP64 (not autotuned; measured best on the H_eff shapes)
+  // step-4 GEMM work division: split-K to at least this many work items, or
+  // stream-K (< 0); measured best on the H_eff shapes (cyl6x24_chi2048, MI355X):
+  // stream-K for FP32 (3923 vs 3911 matvecs/s at split 2048) and FP64 (2296 vs
+  // 2253), split-K at 2048 items for BF16 storage (8697 vs 8051 with stream-K)
+  int64_t split_target = -1;                 // FP32
+  int64_t split_target_h = 2048;             // BF16 storage
+  int64_t split_target_d = -1;               // FP64
 };
 
 // Autotuner candidates for the two H_eff GEMMs (tile BM x BN x KS; step-4
@@ -310,7 +359,7 @@ struct HeffTiles {
 // KS = 0; KS = 0 (DMA-streamed operands) is FP32 / FP64 only.
 constexpr TileCfg kTuneTiles[] = {{64, 64, 1}, {64, 128, 1}, {128, 64, 1}, {64, 64, 2}, {128, 128, 1},
                                   {128, 64, 2}, {64, 64, 0},  {128, 64, 0}, {64, 128, 0}};
-constexpr int64_t kTuneSplits[] = {512, 1024, 2048};
+constexpr int64_t kTuneSplits[] = {512, 1024, 2048, -1};  // -1: stream-K
 // KS = 0: the kernel with DMA-streamed operands (FP32 / FP64 / BF16 storage), tiles
 // up to 128 x 64.  Modes: 0 FP64, 1 FP32, 2 BF16 operands from FP32 storage
 // (register-staged only), 3 / 4 BF16 storage (DMA only; 3 writes BF16, 4 FP32).
@@ -327,6 +376,9 @@ inline int chunk_k(int mode, int KS) {
 }
 // the step-4 GEMM of a plan with another tile and split target
 GemmPlan gemm_variant(const GemmPlan& base, const TileCfg& c, int64_t split_target, int mode);
+// split-K (target >= 0 work items, 0: none) or stream-K (target < 0, DMA variants;
+// register-staged variants fall back to split-K at 1024 items) for a kernel mode
+void set_splits(GemmPlan& g, int64_t split_target, int mode);
 
 // Step 1 (T1 = v^T envL) and the combined-MPO pass (T3 = W_group T1): the first
 // stages of the fused H_eff, shared with the environment plans.

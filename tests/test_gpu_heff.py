@@ -246,6 +246,45 @@ def test_dma_gemm_f64_variants(name, tiles):
     assert err < 1e-13, err
 
 
+@pytest.mark.parametrize("name", ["xxz_L32_chi64", "xxz_L100_chi512", "cyl6x24_chi2048"])
+@pytest.mark.parametrize("prec", ["f64", "f32", "bf16"])
+@pytest.mark.parametrize("tiles", [(64, 64), (64, 128)])
+def test_streamk_step4_vs_unsplit(name, prec, tiles):
+    """Stream-K step 4 (tiles cut between work groups into K pieces whose partial
+    tiles tile_reduce sums in piece order) against the unsplit GEMM and split-K:
+    the same products summed in another order, so the outputs agree to rounding
+    (FP64 1e-13, FP32 and BF16 storage with FP32 accumulation 1e-5 relative)."""
+    import os
+    from mps_amd.tensornetalgs import HeffPlan
+    dt = torch.float64 if prec == "f64" else torch.float32
+    ts = [t.to(dt) for t in _workload(name)]
+    key, skey = {"f64": ("s4d", "splitd"), "f32": ("s4", "split"), "bf16": ("s4h", "splith")}[prec]
+    bm, bn = tiles
+    outs = {}
+    old = os.environ.get("MPSH_HEFF_TILES")
+    try:
+        for split in (0, 1024, -1):
+            os.environ["MPSH_HEFF_TILES"] = f"{key}={bm}x{bn}x0,{skey}={split}"
+            plan = HeffPlan(*[t.struct for t in ts])
+            prepared = plan.prepare(prec, *(t.data for t in ts[1:]))
+            out = torch.full((plan.out.numel,), float("nan"), dtype=dt, device=DEV)
+            plan.apply_prepared(prec, prepared, ts[0].data, out)
+            plan.apply_prepared(prec, prepared, ts[0].data, out)  # a second call: partial tiles rewritten
+            torch.cuda.synchronize()
+            outs[split] = out
+    finally:
+        if old is None:
+            os.environ.pop("MPSH_HEFF_TILES", None)
+        else:
+            os.environ["MPSH_HEFF_TILES"] = old
+    tol = 1e-13 if prec == "f64" else 1e-5
+    ref = outs[0]
+    assert torch.isfinite(ref).all() and torch.isfinite(outs[-1]).all()
+    for split in (1024, -1):
+        err = float(torch.linalg.vector_norm(outs[split] - ref) / torch.linalg.vector_norm(ref))
+        assert err < tol, (split, err)
+
+
 def _site_tensor(v, seed=3, stream=22):
     """A = (Vl, Vd, Vr†) with the workload's bonds: left update input of envL, right of envR."""
     return M.randn((v.space[0], v.space[1], v.space[3]), 0, seed=seed, stream=stream, device=DEV)
