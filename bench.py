@@ -33,13 +33,13 @@ METRIC = "H_eff matvecs/sec + DMRG sweep time at χ=2048; |ΔE| vs FP64 CPU orac
 # step 4 = its GEMM plus the split-K reduction
 TRAFFIC = {
     ("cyl6x24_chi2048", "f32", "step1"): 211.5e6,   # gemm_dma<1, 64, 128, 4>
-    ("cyl6x24_chi2048", "f32", "step4"): 184.3e6,   # gemm_dma<1, 64, 64, 4> 153.3 + splitk_reduce 31.0
-    ("cyl6x24_chi2048", "f32", "mpo"): 211.4e6,     # mpo_apply<float, 32, 1>
-    ("cyl6x24_chi2048", "bf16", "step1"): 102.0e6,  # gemm_dma<3, 64, 128, 4>
+    ("cyl6x24_chi2048", "f32", "step4"): 181.0e6,   # gemm_dma<1, 64, 64, 4> 150.0 + splitk_reduce 31.0
+    ("cyl6x24_chi2048", "f32", "mpo"): 210.5e6,     # mpo_apply<float, 32, 0>
+    ("cyl6x24_chi2048", "bf16", "step1"): 102.1e6,  # gemm_dma<3, 64, 128, 4>
     ("cyl6x24_chi2048", "bf16", "step4"): 98.6e6,   # gemm_dma<4, 64, 128, 4> 81.4 + splitk_reduce 17.2
-    ("cyl6x24_chi2048", "bf16", "mpo"): 120.3e6,    # mpo_apply<float, 32, 1, __bf16>
-    ("cyl6x24_chi2048", "f64", "step1"): 433.4e6,   # gemm_dma<0, 64, 128, 3>
-    ("cyl6x24_chi2048", "f64", "step4"): 369.4e6,   # gemm_dma<0, 64, 64, 3> 304.8 + splitk_reduce 64.6
+    ("cyl6x24_chi2048", "bf16", "mpo"): 118.8e6,    # mpo_apply<float, 32, 0, __bf16>
+    ("cyl6x24_chi2048", "f64", "step1"): 432.5e6,   # gemm_dma<0, 64, 128, 3>
+    ("cyl6x24_chi2048", "f64", "step4"): 370.6e6,   # gemm_dma<0, 64, 64, 3> 306.0 + splitk_reduce 64.6
     ("cyl6x24_chi2048", "f64", "mpo"): 390.2e6,     # mpo_apply<double, 32, 1>
 }
 
@@ -560,7 +560,12 @@ def main():
     # durations from HIP events on the launch stream (step 4 includes its split-K
     # or stream-K reduction launch)
     launches = {}
-    red = "tile_reduce" if tuned and tuned.get("split", 0) < 0 else "splitk_reduce"
+    skey = {"f32": "split", "bf16": "splith", "f64": "splitd"}[prec]
+    split = tuned.get("split", 2048) if tuned else 2048
+    for item in ("" if tuned else os.environ.get("MPSH_HEFF_TILES", "")).split(","):
+        if item.startswith(skey + "="):
+            split = int(item.split("=", 1)[1])
+    red = "tile_reduce" if split < 0 else "splitk_reduce"
     for key, si, stage, what in (("step1", 0, 1, "v^T envL"), ("step4", 3, 3, f"T3^T envR, with {red}")):
         fl = plan.step_flops[si]
         t = stages[stage]

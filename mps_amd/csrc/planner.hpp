@@ -346,12 +346,14 @@ struct HeffTiles {
   TileCfg s1h{64, 128, 0}, s4h{64, 64, 0};   // BF16 storage (DMA-streamed BF16 operands)
   TileCfg s1d{64, 128, 0}, s4d{64, 64, 0};   // FP64 (DMA-streamed operands)
   // step-4 GEMM work division: split-K to at least this many work items, or
-  // stream-K (< 0); measured best on the H_eff shapes (cyl6x24_chi2048, MI355X):
-  // stream-K for FP32 (3923 vs 3911 matvecs/s at split 2048) and FP64 (2296 vs
-  // 2253), split-K at 2048 items for BF16 storage (8697 vs 8051 with stream-K)
-  int64_t split_target = -1;                 // FP32
+  // stream-K (< 0).  Measured on the H_eff shapes (cyl6x24_chi2048, MI355X):
+  // split-K at 2048 items beats 1024 (FP32 3911 vs 3696 matvecs/s); stream-K runs
+  // within 1 % of it in FP32 and FP64 (8 % slower in BF16) but fetches 3x the HBM
+  // bytes per launch (FP32 474 vs 153 MB: the pieces of one K range of an A panel's
+  // tiles run at different times, so the panel is fetched once per tile)
+  int64_t split_target = 2048;               // FP32
   int64_t split_target_h = 2048;             // BF16 storage
-  int64_t split_target_d = -1;               // FP64
+  int64_t split_target_d = 2048;             // FP64
 };
 
 // Autotuner candidates for the two H_eff GEMMs (tile BM x BN x KS; step-4
@@ -359,7 +361,7 @@ struct HeffTiles {
 // KS = 0; KS = 0 (DMA-streamed operands) is FP32 / FP64 only.
 constexpr TileCfg kTuneTiles[] = {{64, 64, 1}, {64, 128, 1}, {128, 64, 1}, {64, 64, 2}, {128, 128, 1},
                                   {128, 64, 2}, {64, 64, 0},  {128, 64, 0}, {64, 128, 0}};
-constexpr int64_t kTuneSplits[] = {512, 1024, 2048, -1};  // -1: stream-K
+constexpr int64_t kTuneSplits[] = {512, 1024, 2048};
 // KS = 0: the kernel with DMA-streamed operands (FP32 / FP64 / BF16 storage), tiles
 // up to 128 x 64.  Modes: 0 FP64, 1 FP32, 2 BF16 operands from FP32 storage
 // (register-staged only), 3 / 4 BF16 storage (DMA only; 3 writes BF16, 4 FP32).
