@@ -125,7 +125,9 @@ struct DevTables {
   // resident work groups (mode as in launch_gemm); mode < 0: shared queues
   // A stream-K plan (GemmPlan::use_streamk) always gets its per-work-group piece
   // lists, for min(resident work groups, the plan's bound) work groups per XCD.
-  DevGemm gemm(const GemmPlan& g, int mode = -1) {
+  // kmode >= 0: the kernel mode the launch runs in; DMA variants then get the
+  // per-tile records of that mode (GemmPlan::tile_records)
+  DevGemm gemm(const GemmPlan& g, int mode = -1, int kmode = -1) {
     DevGemm d;
     std::vector<int32_t> qoff, woff;
     if (g.streamk) {
@@ -149,6 +151,8 @@ struct DevTables {
       d.ntiles = (int)t.size();
       d.treds = up(tr);
       d.ntred = (int)tr.size();
+      d.recs = up(g.tile_records(t, chunk_k(g.sk_mode, g.KS)));
+      d.rec_mode = g.sk_mode;
       d.BM = g.BM;
       d.BN = g.BN;
       d.KS = g.KS;
@@ -171,6 +175,10 @@ struct DevTables {
     d.cld = up(g.cld);
     d.tiles = up(t);
     d.ntiles = (int)t.size();
+    if (kmode >= 0 && g.KS == 0) {
+      d.recs = up(g.tile_records(t, chunk_k(kmode, g.KS)));
+      d.rec_mode = kmode;
+    }
     d.nred = (int)r.size() - 1;
     d.red_groups = r.back().start;
     if (d.nred > 0) {
@@ -403,7 +411,7 @@ struct mpsh_heff_s {
       GemmPlan gp = gemm_variant(stage == 1 ? p.g1 : p.g4, c, std::get<5>(k), mode);
       auto t = std::make_unique<DevTables>();
       DevGemm dg;
-      t->fill([&](DevTables& T) { dg = T.gemm(gp, schedule_mode(mode)); });
+      t->fill([&](DevTables& T) { dg = T.gemm(gp, schedule_mode(mode), mode); });
       it = var.emplace(std::make_pair(d, k), std::make_pair(std::move(t), dg)).first;
     }
     return it->second.second;
@@ -539,10 +547,10 @@ struct mpsh_env_s {
           D.pack_env = T.copy(C.pack_envL);
           D.pack_bra = T.copy(C.pack_bra);
           D.mpo = upload_mpo(T, C);
-          D.g1 = T.gemm(C.g1, schedule_mode(1));
-          D.g1d = T.gemm(C.g1, schedule_mode(0));
-          D.gF = T.gemm(C.gF, schedule_mode(1));
-          D.gF64 = T.gemm(p.core_f64[c].gF, schedule_mode(0));
+          D.g1 = T.gemm(C.g1, schedule_mode(1), 1);
+          D.g1d = T.gemm(C.g1, schedule_mode(0), 0);
+          D.gF = T.gemm(C.gF, schedule_mode(1), 1);
+          D.gF64 = T.gemm(p.core_f64[c].gF, schedule_mode(0), 0);
           for (int k = 0; k < 3; ++k)
             if (p.has_perm[c][k]) e.perm[c][k] = T.copy(p.perm[c][k].copy);
         }

@@ -647,7 +647,7 @@ __global__ __launch_bounds__(256) void gemm_dma(
     const int64_t* __restrict__ btab, const int64_t* __restrict__ crow, const int64_t* __restrict__ cld,
     const GemmTile* __restrict__ tiles, XcdQueues xq, const typename DmaT<MODE>::T* __restrict__ A,
     const typename DmaT<MODE>::T* __restrict__ B, typename DmaT<MODE>::TC* __restrict__ C,
-    typename DmaT<MODE>::TP* __restrict__ P) {
+    typename DmaT<MODE>::TP* __restrict__ P, const int32_t* __restrict__ recs) {
   typedef typename DmaT<MODE>::T T;
   typedef typename DmaT<MODE>::TC TC;
   typedef typename DmaT<MODE>::TP TP;
@@ -696,11 +696,46 @@ __global__ __launch_bounds__(256) void gemm_dma(
 
   int t0, tend, tstep;
   if (!tile_range(xq, t0, tend, tstep)) return;
+  // with per-tile records (recs): lane i < kTileRecWords holds word i of the next
+  // tile's record, loaded one tile ahead; its words move to scalar registers at
+  // the tile's start (readlane)
+  int32_t rnext = 0;
+  if (recs && t0 < tend && lane < kTileRecWords) rnext = recs[(int64_t)t0 * kTileRecWords + lane];
   for (int t = t0; t < tend; t += tstep) {
+    GemmTile tile;
+    GemmProb p;
+    int nch = 0, seg = 0;
+    int64_t k0 = 0, K = 0;
+    GemmSeg sg0;
+    if (recs) {
+      const int32_t rc = rnext;
+      if (t + tstep < tend && lane < kTileRecWords) rnext = recs[(int64_t)(t + tstep) * kTileRecWords + lane];
+      auto w = [&](int i) { return (int32_t)__builtin_amdgcn_readlane(rc, i); };
+      auto w64 = [&](int i) { return (int64_t)(((uint64_t)(uint32_t)w(i + 1) << 32) | (uint64_t)(uint32_t)w(i)); };
+      tile.m0 = w(0);
+      tile.n0 = w(1);
+      p.M = w(2);
+      p.N = w(3);
+      p.seg0 = w(4);
+      p.nseg = w(5);
+      nch = w(6);
+      seg = w(7);
+      k0 = w(8);
+      tile.split = w(9);
+      p.nsplit = w(10);
+      p.pld = w(11);
+      tile.pld = w(12);
+      p.crow0 = w64(14);
+      p.c_off = w64(16);
+      p.ldc = w64(18);
+      p.part_off = w64(20);
+      tile.poff = w64(22);
+      sg0 = GemmSeg{w64(24), w64(26), w64(28), w64(30), w64(32), w64(34), w64(36)};
+    } else {
     // tile and problem fields moved to scalar registers here: the loop below then
     // uses no register written by a pending vector load, and the compiler places
     // no full vmcnt wait (which would also drain the DMA of later chunks) in it
-    GemmTile tile = tiles[t];
+    tile = tiles[t];
     tile.prob = uniform_i32(tile.prob);
     tile.m0 = uniform_i32(tile.m0);
     tile.n0 = uniform_i32(tile.n0);
@@ -709,7 +744,7 @@ __global__ __launch_bounds__(256) void gemm_dma(
     tile.kc1 = uniform_i32(tile.kc1);
     tile.pld = uniform_i32(tile.pld);
     tile.poff = uniform_i64(tile.poff);
-    GemmProb p = probs[tile.prob];
+    p = probs[tile.prob];
     p.M = uniform_i32(p.M);
     p.N = uniform_i32(p.N);
     p.seg0 = uniform_i32(p.seg0);
@@ -719,13 +754,6 @@ __global__ __launch_bounds__(256) void gemm_dma(
     p.part_off = uniform_i64(p.part_off);
     p.c_off = uniform_i64(p.c_off);
     p.ldc = uniform_i64(p.ldc);
-    Acc acc[SM][SN];
-#pragma unroll
-    for (int i = 0; i < SM; ++i)
-#pragma unroll
-      for (int j = 0; j < SN; ++j)
-#pragma unroll
-        for (int r = 0; r < 4; ++r) acc[i][j][r] = 0;
     int64_t cb = tile.kc0, ce = tile.kc1;  // an explicit chunk range (stream-K pieces)
     if (ce < 0) {                          // else this split's share of the problem's chunks
       int64_t ctot = 0;
@@ -733,10 +761,9 @@ __global__ __launch_bounds__(256) void gemm_dma(
       cb = ctot * tile.split / p.nsplit;
       ce = ctot * (tile.split + 1) / p.nsplit;
     }
-    const int nch = (int)(ce - cb);
+    nch = (int)(ce - cb);
     // issue side: segment, k offset, this lane's row bases
-    int seg = 0;
-    int64_t k0 = 0, K = 0, cpos = 0;
+    int64_t cpos = 0;
     for (; seg < p.nseg; ++seg) {
       const int64_t nc = (uniform_i64(segs[p.seg0 + seg].K) + BK - 1) / BK;
       if (cpos + nc > cb) {
@@ -745,9 +772,17 @@ __global__ __launch_bounds__(256) void gemm_dma(
       }
       cpos += nc;
     }
+    if (nch > 0) sg0 = segs[p.seg0 + seg];
+    }
+    Acc acc[SM][SN];
+#pragma unroll
+    for (int i = 0; i < SM; ++i)
+#pragma unroll
+      for (int j = 0; j < SN; ++j)
+#pragma unroll
+        for (int r = 0; r < 4; ++r) acc[i][j][r] = 0;
     int64_t ab[NA], bb[NB];
-    auto bases = [&]() {
-      GemmSeg sg = segs[p.seg0 + seg];
+    auto bases_from = [&](GemmSeg sg) {
       sg.K = uniform_i64(sg.K);
       sg.atab0 = uniform_i64(sg.atab0);
       sg.btab0 = uniform_i64(sg.btab0);
@@ -787,7 +822,8 @@ __global__ __launch_bounds__(256) void gemm_dma(
         }
       }
     };
-    if (nch > 0) bases();
+    auto bases = [&]() { bases_from(segs[p.seg0 + seg]); };
+    if (nch > 0) bases_from(sg0);
     int krem[4];  // K entries left at the start of the chunk held by each stage (>= BK: no tail)
     auto issue = [&](auto ST) {
       constexpr int st = decltype(ST)::value;
@@ -1092,7 +1128,8 @@ static void launch_variant_dma(const DevGemm& gm, const void* A, const void* B, 
   for (int x = 0; x < NXCD; ++x) qmax = std::max(qmax, gm.q.off[x + 1] - gm.q.off[x]);
   const int per = gm.q.woff ? gm.q.slots : std::min(std::max(1, num_cu() / NXCD) * occ, qmax);
   hipLaunchKernelGGL((gemm_dma<MODE, BM, BN>), dim3((unsigned)(per * NXCD)), dim3(256), 0, s, gm.probs, gm.segs,
-                     gm.atab, gm.btab, gm.crow, gm.cld, gm.tiles, gm.q, (const T*)A, (const T*)B, (TC*)C, (TP*)P);
+                     gm.atab, gm.btab, gm.crow, gm.cld, gm.tiles, gm.q, (const T*)A, (const T*)B, (TC*)C, (TP*)P,
+                     gm.rec_mode == MODE ? gm.recs : nullptr);
 }
 
 // Compiled tile variants: (BM, BN) in {64x64, 128x64, 64x128, 128x128} with K

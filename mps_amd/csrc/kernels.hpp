@@ -42,6 +42,8 @@ struct DevGemm {
   int64_t red_groups = 0;
   const TileRed* treds = nullptr;  // stream-K: cut tiles, one reduction work group each
   int ntred = 0;
+  const int32_t* recs = nullptr;   // per-tile records (DMA variants) for kernel mode rec_mode, or none
+  int rec_mode = -1;
   int BM = 64, BN = 64, KS = 1;  // tile and K-chunk multiplier
   XcdQueues q{};                 // tiles [q.off[x], q.off[x+1]) run on XCD x
 };

@@ -508,6 +508,65 @@ std::vector<GemmTile> GemmPlan::streamk_schedule(int slots, int BK, std::vector<
   return out;
 }
 
+std::vector<int32_t> GemmPlan::tile_records(const std::vector<GemmTile>& tl, int BK) const {
+  std::vector<int32_t> r(tl.size() * kTileRecWords, 0);
+  auto put64 = [](int32_t* w, int i, int64_t v) {
+    w[i] = (int32_t)(uint32_t)((uint64_t)v & 0xffffffffu);
+    w[i + 1] = (int32_t)(uint32_t)((uint64_t)v >> 32);
+  };
+  for (size_t t = 0; t < tl.size(); ++t) {
+    const GemmTile& x = tl[t];
+    const GemmProb& p = probs[x.prob];
+    int64_t ctot = 0;
+    for (int s = 0; s < p.nseg; ++s) ctot += (segs[p.seg0 + s].K + BK - 1) / BK;
+    int64_t cb = x.kc0, ce = x.kc1;
+    if (ce < 0) {
+      cb = ctot * x.split / p.nsplit;
+      ce = ctot * (x.split + 1) / p.nsplit;
+    }
+    int seg = 0;
+    int64_t k0 = 0, cpos = 0;
+    for (; seg < p.nseg; ++seg) {
+      const int64_t nc = (segs[p.seg0 + seg].K + BK - 1) / BK;
+      if (cpos + nc > cb) {
+        k0 = (cb - cpos) * BK;
+        break;
+      }
+      cpos += nc;
+    }
+    int32_t* w = r.data() + t * kTileRecWords;
+    w[0] = x.m0;
+    w[1] = x.n0;
+    w[2] = p.M;
+    w[3] = p.N;
+    w[4] = p.seg0;
+    w[5] = p.nseg;
+    w[6] = (int32_t)(ce - cb);
+    w[7] = seg;
+    w[8] = (int32_t)k0;
+    w[9] = x.split;
+    w[10] = p.nsplit;
+    w[11] = p.pld;
+    w[12] = x.pld;
+    put64(w, 14, p.crow0);
+    put64(w, 16, p.c_off);
+    put64(w, 18, p.ldc);
+    put64(w, 20, p.part_off);
+    put64(w, 22, x.poff);
+    if (ce > cb && seg < p.nseg) {
+      const GemmSeg& g = segs[p.seg0 + seg];
+      put64(w, 24, g.K);
+      put64(w, 26, g.atab0);
+      put64(w, 28, g.btab0);
+      put64(w, 30, g.a_off);
+      put64(w, 32, g.lda);
+      put64(w, 34, g.b_off);
+      put64(w, 36, g.ldb);
+    }
+  }
+  return r;
+}
+
 std::vector<RedDesc> GemmPlan::reductions() const {
   std::vector<RedDesc> r;
   int64_t start = 0;

@@ -153,6 +153,16 @@ struct GemmTile {
   int32_t kc0 = 0, kc1 = -1, pld = 0, pad = 0;
   int64_t poff = 0;
 };
+// Per-tile record of a DMA-streamed GEMM launch in one kernel mode (K chunk
+// fixed): everything the kernel reads at a tile's start, so a work group loads
+// one record (prefetched a tile ahead) instead of the tile -> problem ->
+// segment chain.  Words (int32; int64 fields low word first):
+//   0 m0, 1 n0, 2 M, 3 N, 4 seg0, 5 nseg, 6 chunks of this tile, 7 start
+//   segment (relative to seg0), 8 start k (elements), 9 split, 10 nsplit,
+//   11 pld, 12 piece pld (stream-K, else 0), 13 unused, 14 crow0, 16 c_off,
+//   18 ldc, 20 part_off, 22 piece poff, 24.. the start segment: 24 K, 26 atab0,
+//   28 btab0, 30 a_off, 32 lda, 34 b_off, 36 ldb
+constexpr int kTileRecWords = 40;
 // One stream-K tile cut into np > 1 pieces: its partial tiles are the BM x BN
 // blocks blk0 .. blk0 + np - 1 of P, summed in that order by the reduction.
 struct TileRed {
@@ -223,6 +233,8 @@ struct GemmPlan {
   // schedule(); reds lists the cut tiles.
   std::vector<GemmTile> streamk_schedule(int slots, int BK, std::vector<int32_t>* woff,
                                          std::vector<TileRed>* reds) const;
+  // the kTileRecWords-word records of `tiles` (in that order) for K chunks of BK
+  std::vector<int32_t> tile_records(const std::vector<GemmTile>& tiles, int BK) const;
 };
 constexpr int kStreamKOverhead = 4;  // chunks: a piece's set-up, pipeline fill and stores
 constexpr int kStreamKMinPiece = 8;  // chunks
