@@ -641,8 +641,17 @@ __device__ __forceinline__ void staged_store_impl(const float* E, O* out, int64_
 // GEMMs (autotuned tiles): FP32 is faster with 4 stages (3547 vs 3491
 // matvecs/s), FP64 with 3 (2120 vs 2050: a smaller ring leaves LDS for one
 // more resident work group per CU).
+// Waves per SIMD the register allocation must allow: 64x64 FP32/BF16 tiles with
+// a 4-stage ring take 32 KiB of LDS, five work groups per CU, which needs at
+// most 96 registers per lane (unconstrained the variant took 82 VGPRs + 16
+// AGPRs, allocated as 104: four waves per SIMD, the fifth work group of each
+// CU then ran as a second wave).  Measured 1-2 % faster matvecs.
+template <int MODE, int BM, int BN, int NS>
+constexpr int dma_waves_per_eu() {
+  return MODE != 0 && BM == 64 && BN == 64 && NS == 4 ? 5 : 1;
+}
 template <int MODE, int BM, int BN, int NS = (MODE == 0 ? 3 : 4)>
-__global__ __launch_bounds__(256) void gemm_dma(
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(dma_waves_per_eu<MODE, BM, BN, NS>()))) void gemm_dma(
     const GemmProb* __restrict__ probs, const GemmSeg* __restrict__ segs, const int64_t* __restrict__ atab,
     const int64_t* __restrict__ btab, const int64_t* __restrict__ crow, const int64_t* __restrict__ cld,
     const GemmTile* __restrict__ tiles, XcdQueues xq, const typename DmaT<MODE>::T* __restrict__ A,
@@ -1064,10 +1073,8 @@ static int num_cu() {
   return g_num_cu;
 }
 
-// The occupancy query's count, lowered so the work groups' static LDS stays below
-// the CU's LDS: on MI355X five work groups of 32 KiB (the CU's whole 160 KiB)
-// were measured to run four at a time, the fifth work group of each CU as a
-// second wave of the persistent grid.
+// The occupancy query's count, lowered to what the CU's LDS holds (a persistent
+// grid sized past it would run its last work groups as a second wave).
 static int lds_fit(const void* fn, int occ) {
   hipFuncAttributes a;
   int dev = 0, lds = 0;
@@ -1075,7 +1082,7 @@ static int lds_fit(const void* fn, int occ) {
   if (hipGetDevice(&dev) != hipSuccess ||
       hipDeviceGetAttribute(&lds, hipDeviceAttributeMaxSharedMemoryPerMultiprocessor, dev) != hipSuccess || lds <= 0)
     return occ;
-  const int fit = (int)((size_t)(lds - 1) / a.sharedSizeBytes);
+  const int fit = (int)((size_t)lds / a.sharedSizeBytes);
   return std::max(1, std::min(occ, fit));
 }
 
