@@ -26,20 +26,18 @@ if ROOT not in sys.path:
 PEAK = {"f32": 157.3e12, "f64": 78.6e12, "bf16": 2.5e15}
 HBM_PEAK = 8.0e12
 METRIC = "H_eff matvecs/sec + DMRG sweep time at χ=2048; |ΔE| vs FP64 CPU oracle"
-# HBM bytes per GEMM launch: rocprofv3 FETCH_SIZE x 2.000 + WRITE_SIZE x 1.000
-# (scales from a 256 MiB device copy), profiles/heff_cyl6x24_chi2048_f32_hbm_traffic.txt
 # HBM bytes per launch from rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE (scaled by a plain
 # copy's readings: x2.000 / x1.000) for the autotuned variants of this workload;
 # step 4 = its GEMM plus the split-K reduction
 TRAFFIC = {
-    ("cyl6x24_chi2048", "f32", "step1"): 211.5e6,   # gemm_dma<1, 64, 128, 4>
-    ("cyl6x24_chi2048", "f32", "step4"): 181.0e6,   # gemm_dma<1, 64, 64, 4> 150.0 + splitk_reduce 31.0
+    ("cyl6x24_chi2048", "f32", "step1"): 211.7e6,   # gemm_dma<1, 64, 128, 4>
+    ("cyl6x24_chi2048", "f32", "step4"): 185.3e6,   # gemm_dma<1, 64, 64, 4> 154.3 + splitk_reduce 31.0
     ("cyl6x24_chi2048", "f32", "mpo"): 210.5e6,     # mpo_apply<float, 32, 0>
-    ("cyl6x24_chi2048", "bf16", "step1"): 102.1e6,  # gemm_dma<3, 64, 128, 4>
-    ("cyl6x24_chi2048", "bf16", "step4"): 98.6e6,   # gemm_dma<4, 64, 128, 4> 81.4 + splitk_reduce 17.2
+    ("cyl6x24_chi2048", "bf16", "step1"): 102.3e6,  # gemm_dma<3, 64, 128, 4>
+    ("cyl6x24_chi2048", "bf16", "step4"): 99.1e6,   # gemm_dma<4, 64, 128, 4> 81.9 + splitk_reduce 17.2
     ("cyl6x24_chi2048", "bf16", "mpo"): 118.8e6,    # mpo_apply<float, 32, 0, __bf16>
-    ("cyl6x24_chi2048", "f64", "step1"): 432.5e6,   # gemm_dma<0, 64, 128, 3>
-    ("cyl6x24_chi2048", "f64", "step4"): 370.6e6,   # gemm_dma<0, 64, 64, 3> 306.0 + splitk_reduce 64.6
+    ("cyl6x24_chi2048", "f64", "step1"): 432.0e6,   # gemm_dma<0, 64, 128, 3>
+    ("cyl6x24_chi2048", "f64", "step4"): 384.0e6,   # gemm_dma<0, 64, 64, 3> 319.4 + splitk_reduce 64.6
     ("cyl6x24_chi2048", "f64", "mpo"): 390.2e6,     # mpo_apply<double, 32, 1>
 }
 
