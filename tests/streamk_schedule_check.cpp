@@ -10,6 +10,10 @@
 //     disjoint and inside the plan's partial buffer (part_elems);
 //   * every work group's work (chunks + kStreamKOverhead per piece) is within
 //     kStreamKMinPiece + 2 * kStreamKOverhead of its XCD's mean.
+// For the same plans it checks the per-tile records the DMA kernel reads
+// (GemmPlan::tile_records) of the stream-K pieces and of split-K tile lists
+// (tiles() and schedule()) against the plan, the first chunk's segment and k
+// offset found by enumerating the problem's chunks one by one.
 // Prints one line per case and exits non-zero on the first violation.
 #include <algorithm>
 #include <cstdint>
@@ -29,6 +33,57 @@ static int fail(const char* what, int c) {
   return 1;
 }
 
+static int64_t rec64(const int32_t* w, int i) {
+  return (int64_t)(((uint64_t)(uint32_t)w[i + 1] << 32) | (uint64_t)(uint32_t)w[i]);
+}
+
+static int check_records(const GemmPlan& g, const std::vector<GemmTile>& t, int BK, int c) {
+  const std::vector<int32_t> r = g.tile_records(t, BK);
+  if (r.size() != t.size() * (size_t)kTileRecWords) return fail("record count", c);
+  for (size_t i = 0; i < t.size(); ++i) {
+    const GemmTile& x = t[i];
+    const GemmProb& p = g.probs[x.prob];
+    const int32_t* w = r.data() + i * kTileRecWords;
+    // (segment, k offset) of every K chunk of the problem, in K order
+    std::vector<std::pair<int, int64_t>> ch;
+    for (int s = 0; s < p.nseg; ++s)
+      for (int64_t k = 0; k < g.segs[p.seg0 + s].K; k += BK) ch.push_back({s, k});
+    const int64_t n = (int64_t)ch.size();
+    int64_t cb = x.kc0, ce = x.kc1;
+    if (ce < 0) {
+      cb = n * x.split / p.nsplit;
+      ce = n * (x.split + 1) / p.nsplit;
+    }
+    if (w[0] != x.m0 || w[1] != x.n0 || w[2] != p.M || w[3] != p.N || w[4] != p.seg0 || w[5] != p.nseg)
+      return fail("record tile / problem fields", c);
+    if (w[6] != ce - cb) return fail("record chunk count", c);
+    if (w[9] != x.split || w[10] != p.nsplit || w[11] != p.pld || w[12] != x.pld)
+      return fail("record split fields", c);
+    if (rec64(w, 14) != p.crow0 || rec64(w, 16) != p.c_off || rec64(w, 18) != p.ldc || rec64(w, 20) != p.part_off ||
+        rec64(w, 22) != x.poff)
+      return fail("record output fields", c);
+    if (ce > cb) {
+      const int s = ch[cb].first;
+      const GemmSeg& sg = g.segs[p.seg0 + s];
+      if (w[7] != s || w[8] != ch[cb].second) return fail("record first chunk", c);
+      if (rec64(w, 24) != sg.K || rec64(w, 26) != sg.atab0 || rec64(w, 28) != sg.btab0 || rec64(w, 30) != sg.a_off ||
+          rec64(w, 32) != sg.lda || rec64(w, 34) != sg.b_off || rec64(w, 36) != sg.ldb)
+        return fail("record segment descriptor", c);
+    }
+  }
+  return 0;
+}
+
+// records of the split-K tile lists of a plan (splits for `mode` at 2048 work items)
+static int check_split_records(const GemmPlan& g0, int mode, int slots, int c) {
+  GemmPlan g = g0;
+  set_splits(g, 2048, mode);
+  const int BK = chunk_k(mode, 0);
+  std::vector<int32_t> qoff, woff;
+  if (check_records(g, g.tiles(&qoff), BK, c)) return 1;
+  return check_records(g, g.schedule(slots, BK, &woff), BK, c);
+}
+
 static int check(const GemmPlan& g0, int mode, int slots, int c) {
   GemmPlan g = g0;
   g.use_streamk(mode);
@@ -37,6 +92,7 @@ static int check(const GemmPlan& g0, int mode, int slots, int c) {
   const int BK = chunk_k(mode, 0);
   std::vector<GemmTile> t = g.streamk_schedule(slots, BK, &woff, &reds);
   if ((int)woff.size() != NXCD * slots + 1 || woff.back() != (int)t.size()) return fail("woff size", c);
+  if (check_records(g, t, BK, c) || check_split_records(g0, mode, slots, c)) return 1;
   std::vector<int64_t> ctot(g.probs.size(), 0);
   for (size_t p = 0; p < g.probs.size(); ++p)
     for (int s = 0; s < g.probs[p].nseg; ++s) ctot[p] += (g.segs[g.probs[p].seg0 + s].K + BK - 1) / BK;

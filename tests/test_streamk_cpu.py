@@ -1,7 +1,9 @@
 """CPU check of the stream-K step-4 work division (planner.cpp: use_streamk,
-streamk_schedule): tests/streamk_schedule_check.cpp is compiled with the
-planner's source by g++ and run; it asserts chunk coverage, minimum piece
-length, the partial-tile bound and the per-work-group balance."""
+streamk_schedule) and of the DMA kernel's per-tile records (tile_records):
+tests/streamk_schedule_check.cpp is compiled with the planner's source by g++
+and run; it asserts chunk coverage, minimum piece length, the partial-tile
+bound, the per-work-group balance, and every record field of stream-K and
+split-K tile lists against the plan (first chunk found by enumeration)."""
 import os
 import shutil
 import subprocess
