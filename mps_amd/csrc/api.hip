@@ -360,9 +360,10 @@ DevMpo upload_mpo(DevTables& t, const Step13& p) {
   m.groups = t.up(p.groups);
   m.inrows = t.up(p.inrows);
   m.outrows = t.up(p.outrows);
-  for (int i = 0; i < 4; ++i) m.work[i] = t.up(p.work[i]);
+  for (int i = 0; i < 5; ++i) m.work[i] = t.up(p.work[i]);
   m.nterms = (int)p.terms.size();
-  for (int i = 0; i < 4; ++i) m.nwork[i] = (int)p.work[i].size();
+  for (int i = 0; i < 5; ++i) m.nwork[i] = (int)p.work[i].size();
+  m.vec = p.padr > 1;
   m.cmax = (int)p.maxC;
   m.rmax = (int)p.maxR;
   m.t1_elems = p.t1_elems;
@@ -857,6 +858,9 @@ mpsh_status mpsh_heff_prepare(mpsh_heff plan, mpsh_dtype dt, const void* envL, c
   hipStream_t s = (hipStream_t)stream;
   const HeffBytes b = heff_bytes(p, dt);
   char* pr = (char*)prepared;
+  // padded plans: the K padding inside envR_p's rows is read by the step-4 GEMM
+  // (against zeros in T3) and must be finite: the region is zeroed first
+  if (p.padr > 1) check(launch_zero(pr + b.er, (size_t)(p.pw_off - p.er_off) * b.es, s), "zero envR");
   if (dt == MPSH_BF16) {  // FP32 operands packed into BF16, the combined MPO in FP32
     check(launch_copy_to_bf16(d.pack_envL, (const float*)envL, pr + b.el, s), "pack envL");
     check(launch_copy_to_bf16(d.pack_envR, (const float*)envR, pr + b.er, s), "pack envR");

@@ -98,6 +98,19 @@ __global__ __launch_bounds__(256) void copy_kernel(const CopyDesc* __restrict__ 
   }
 }
 
+__global__ __launch_bounds__(256) void zero_kernel(floatx4* __restrict__ dst, int64_t n) {
+  for (int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x; i < n; i += (int64_t)gridDim.x * 256)
+    dst[i] = floatx4{0.0f, 0.0f, 0.0f, 0.0f};
+}
+
+hipError_t launch_zero(void* dst, size_t bytes, hipStream_t s) {
+  const int64_t n = (int64_t)(bytes / 16);
+  if (n == 0) return hipSuccess;
+  const unsigned grid = (unsigned)std::min<int64_t>((n + 255) / 256, 4096);
+  hipLaunchKernelGGL(zero_kernel, dim3(grid), dim3(256), 0, s, (floatx4*)dst, n);
+  return hipGetLastError();
+}
+
 hipError_t launch_copy_to_bf16(const DevCopy& c, const float* src, void* dst, hipStream_t s) {
   if (c.ntiles == 0) return hipSuccess;
   hipLaunchKernelGGL((copy_kernel<float, __bf16>), dim3((unsigned)c.ntiles), dim3(256), 0, s, c.descs, c.tile_map,
@@ -1329,19 +1342,39 @@ __device__ __forceinline__ E* mpo_at(E* base, IX off) {
 // pixels past the row end re-read its last pixel, so every load is
 // unconditional; only the stores are masked.  ib/ins/im and ob/on/om are in
 // the units of IX (bytes for uint32_t, elements for int64_t).
-template <typename T, typename TI, typename TO, typename IX, int CM, int CMAX, int P>
+// VEC (padded plans): channel rows and T3 rows have unit pixel stride and start
+// on multiples of 4 elements, and the run lies inside the group's padded range,
+// so a run's P pixels of one row are one aligned vector load / store; pixels
+// past dR compute on the padding and store zeros (T3's padding enters the
+// step-4 GEMM).
+template <typename T, typename TI, typename TO, typename IX, int CM, int CMAX, int P, bool VEC>
 __device__ __forceinline__ void mpo_run(const T* Ws, const IX* ib, const IX* ins, const IX* im, const IX* ob,
                                         const IX* on, const IX* om, int R, IX m, IX n0, IX dR, bool live,
                                         const TI* __restrict__ T1, TO* __restrict__ T3) {
-  IX nu[P];
-#pragma unroll
-  for (int u = 0; u < P; ++u) nu[u] = n0 + u < dR ? n0 + u : dR - 1;
   T v[P][CM];
+  if constexpr (VEC) {
 #pragma unroll
-  for (int c = 0; c < CM; ++c) {
-    const IX o = ib[c] + m * im[c], st = ins[c];
+    for (int c = 0; c < CM; ++c) {
+      const TI* a = mpo_at(T1, (IX)(ib[c] + m * im[c] + n0 * ins[c]));
+      if constexpr (P == 1) {
+        v[0][c] = (T)*a;
+      } else {
+        typedef TI VI __attribute__((ext_vector_type(P)));
+        const VI x = *(const VI*)a;
 #pragma unroll
-    for (int u = 0; u < P; ++u) v[u][c] = (T)*mpo_at(T1, (IX)(o + nu[u] * st));
+        for (int u = 0; u < P; ++u) v[u][c] = (T)x[u];
+      }
+    }
+  } else {
+    IX nu[P];
+#pragma unroll
+    for (int u = 0; u < P; ++u) nu[u] = n0 + u < dR ? n0 + u : dR - 1;
+#pragma unroll
+    for (int c = 0; c < CM; ++c) {
+      const IX o = ib[c] + m * im[c], st = ins[c];
+#pragma unroll
+      for (int u = 0; u < P; ++u) v[u][c] = (T)*mpo_at(T1, (IX)(o + nu[u] * st));
+    }
   }
   if (!live) return;
   for (int r = 0; r < R; ++r) {
@@ -1356,9 +1389,22 @@ __device__ __forceinline__ void mpo_run(const T* Ws, const IX* ib, const IX* ins
       for (int u = 0; u < P; ++u) acc[u] += w * v[u][c];
     }
     const IX o = ob[r] + m * om[r], st = on[r];
+    if constexpr (VEC) {
+      TO* d = mpo_at(T3, (IX)(o + n0 * st));
+      if constexpr (P == 1) {
+        *d = n0 < dR ? (TO)acc[0] : TO(0.0f);
+      } else {
+        typedef TO VO __attribute__((ext_vector_type(P)));
+        VO y;
 #pragma unroll
-    for (int u = 0; u < P; ++u)
-      if (n0 + u < dR) *mpo_at(T3, (IX)(o + (n0 + u) * st)) = (TO)acc[u];
+        for (int u = 0; u < P; ++u) y[u] = n0 + u < dR ? (TO)acc[u] : TO(0.0f);
+        *(VO*)d = y;
+      }
+    } else {
+#pragma unroll
+      for (int u = 0; u < P; ++u)
+        if (n0 + u < dR) *mpo_at(T3, (IX)(o + (n0 + u) * st)) = (TO)acc[u];
+    }
   }
 }
 
@@ -1369,7 +1415,7 @@ __device__ __forceinline__ void mpo_run(const T* Ws, const IX* ib, const IX* ins
 // smallest channel class (multiple of 4) that holds its own C.
 // T: arithmetic and W type; TI / TO: T1 / T3 storage (BF16 storage computes in FP32);
 // IX: offset arithmetic (uint32_t byte offsets when T1 and T3 are below 4 GiB)
-template <typename T, int CMAX, int P, typename TI = T, typename TO = T, typename IX = int64_t>
+template <typename T, int CMAX, int P, typename TI = T, typename TO = T, typename IX = int64_t, bool VEC = false>
 __global__ __launch_bounds__(256) void mpo_apply(const MpoSig* __restrict__ sigs,
                                                  const MpoGroup* __restrict__ groups,
                                                  const MpoInRow* __restrict__ inrows,
@@ -1404,29 +1450,32 @@ __global__ __launch_bounds__(256) void mpo_apply(const MpoSig* __restrict__ sigs
   }
   __syncthreads();
   // run q of the group: row m = q / nr, first pixel n0 = Pg * (q % nr), nr runs per
-  // row; P = 0: the mixed list, runs of mpo_mixed_run(C) pixels
-  const int Pg = P > 0 ? P : mpo_mixed_run(C);
-  const uint32_t dR = (uint32_t)g.dR, nr = (dR + Pg - 1) / Pg;
+  // row over the padded range dRp (== dR unless padded); P = 0: the mixed list,
+  // runs of mpo_mixed_run(C) pixels (mpo_vec_run(C, sizeof(T)) in padded plans)
+  const int Pg = P > 0 ? P : (VEC ? mpo_vec_run(C, (int)sizeof(T)) : mpo_mixed_run(C));
+  const uint32_t dR = (uint32_t)g.dR, dRr = (uint32_t)g.dRp, nr = (dRr + Pg - 1) / Pg;
   const int64_t nruns = (int64_t)g.db * nr;
   const int64_t q0 = (int64_t)wk.run0 + threadIdx.x;
   const bool live = q0 < nruns;
   const uint32_t q = live ? (uint32_t)q0 : 0u;
   const uint32_t mq = q / nr;
   const IX m = (IX)mq, n0 = (IX)((q - mq * nr) * Pg);
-  if (C <= 0) {  // a group no input type reaches: its T3 cells are zero
+  if (C <= 0) {  // a group no input type reaches: its T3 cells (and their padding) are zero
     if (live)
       for (int r = 0; r < R; ++r)
         for (int u = 0; u < Pg; ++u)
-          if (n0 + u < (IX)dR) *mpo_at(T3, (IX)(ob[r] + (n0 + u) * on[r] + m * om[r])) = TO(0.0f);
+          if (n0 + u < (IX)dRr) *mpo_at(T3, (IX)(ob[r] + (n0 + u) * on[r] + m * om[r])) = TO(0.0f);
     return;
   }
   const int cls = (C + 3) / 4;  // uniform over the block
-  // P = 0: classes 1 and 2 (C <= 8) run 4 pixels per thread, the others 1
-#define MPSH_PIX(K)                                                                                             \
-  case K:                                                                                                       \
-    if constexpr (4 * K <= CMAX)                                                                                \
-      mpo_run<T, TI, TO, IX, 4 * K, CMAX, (P > 0 ? P : (K <= 2 ? 4 : 1))>(Ws, ib, ins, im, ob, on, om, R, m, n0, \
-                                                                        (IX)dR, live, T1, T3);                  \
+  // P = 0: classes 1 and 2 (C <= 8) run 4 pixels per thread, the others 1; padded
+  // plans mpo_vec_run(C, sizeof(T)) pixels (C <= 4 K, the same thresholds)
+#define MPSH_PIX(K)                                                                                       \
+  case K:                                                                                                 \
+    if constexpr (4 * K <= CMAX)                                                                          \
+      mpo_run<T, TI, TO, IX, 4 * K, CMAX,                                                                 \
+              (P > 0 ? P : (VEC ? mpo_vec_run(4 * K, (int)sizeof(T)) : (K <= 2 ? 4 : 1))), VEC>(         \
+          Ws, ib, ins, im, ob, on, om, R, m, n0, (IX)dR, live, T1, T3);                                   \
     break;
   switch (cls) {
     MPSH_PIX(1)
@@ -1439,8 +1488,8 @@ __global__ __launch_bounds__(256) void mpo_apply(const MpoSig* __restrict__ sigs
     MPSH_PIX(8)
     default:
       if constexpr (CMAX == 64)
-        mpo_run<T, TI, TO, IX, 64, CMAX, (P > 0 ? P : 1)>(Ws, ib, ins, im, ob, on, om, R, m, n0, (IX)dR, live, T1,
-                                                          T3);
+        mpo_run<T, TI, TO, IX, 64, CMAX, (P > 0 ? P : (VEC ? mpo_vec_run(64, (int)sizeof(T)) : 1)), VEC>(
+            Ws, ib, ins, im, ob, on, om, R, m, n0, (IX)dR, live, T1, T3);
       break;
   }
 #undef MPSH_PIX
@@ -1467,6 +1516,35 @@ void mpo_options(DevMpo& d) {
 }
 
 hipError_t launch_mpo_apply(const DevMpo& d, int elem_size, const void* W, const void* T1, void* T3, hipStream_t s) {
+  // 32-bit byte offsets when every T1 / T3 byte offset fits
+  const int64_t bytes = std::max(d.t1_elems, d.t3_elems) * elem_size;
+  const bool off32 = !d.off64 && bytes < ((int64_t)1 << 32);
+  if (d.vec) {
+    // padded plans: the vector-access runs of mpo_vec_run (the list of the
+    // arithmetic type's size; MPSH_MPO_RUN does not apply)
+    const int pi = elem_size == 8 ? 4 : 3;
+    if (d.nwork[pi] == 0) return hipSuccess;
+    dim3 grid(d.nwork[pi]), block(256);
+#define MPSH_MPOV(T, CM, TI, IX)                                                                           \
+  hipLaunchKernelGGL((mpo_apply<T, CM, 0, TI, TI, IX, true>), grid, block, 0, s, d.sigs, d.groups, d.inrows, \
+                     d.outrows, d.work[pi], (const T*)W, (const TI*)T1, (TI*)T3)
+#define MPSH_MPOV_T(IX)                                                                       \
+  if (elem_size == 8) {                                                                       \
+    if (d.cmax <= 32) MPSH_MPOV(double, 32, double, IX); else MPSH_MPOV(double, 64, double, IX); \
+  } else if (elem_size == 4) {                                                                \
+    if (d.cmax <= 32) MPSH_MPOV(float, 32, float, IX); else MPSH_MPOV(float, 64, float, IX);    \
+  } else {                                                                                    \
+    if (d.cmax <= 32) MPSH_MPOV(float, 32, __bf16, IX); else MPSH_MPOV(float, 64, __bf16, IX);  \
+  }
+    if (off32) {
+      MPSH_MPOV_T(uint32_t)
+    } else {
+      MPSH_MPOV_T(int64_t)
+    }
+#undef MPSH_MPOV_T
+#undef MPSH_MPOV
+    return hipGetLastError();
+  }
   // d.run pixels per thread (0: the mixed list; FP64 then one pixel per thread,
   // measured 2 % faster than mixed there, while FP32 / BF16 gain 0-4 % from it);
   // wide channel classes (CMAX 64) and FP64 at most 2
@@ -1474,9 +1552,6 @@ hipError_t launch_mpo_apply(const DevMpo& d, int elem_size, const void* W, const
   const int P = pw == 0 ? 0 : (d.cmax > 32 ? std::max(1, pw / 2) : pw);
   const int pi = P == 0 ? 3 : (P == 4 ? 2 : (P == 2 ? 1 : 0));  // work list
   if (d.nwork[pi] == 0) return hipSuccess;
-  // 32-bit byte offsets when every T1 / T3 byte offset fits
-  const int64_t bytes = std::max(d.t1_elems, d.t3_elems) * elem_size;
-  const bool off32 = !d.off64 && bytes < ((int64_t)1 << 32);
   dim3 grid(d.nwork[pi]), block(256);
 #define MPSH_MPO(T, CM, PP, TI, IX)                                                                                \
   hipLaunchKernelGGL((mpo_apply<T, CM, PP, TI, TI, IX>), grid, block, 0, s, d.sigs, d.groups, d.inrows, d.outrows, \

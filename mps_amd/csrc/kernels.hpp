@@ -55,8 +55,10 @@ struct DevMpo {
   const MpoGroup* groups = nullptr;
   const MpoInRow* inrows = nullptr;
   const MpoOutRow* outrows = nullptr;
-  const MpoWork* work[4] = {nullptr, nullptr, nullptr, nullptr};  // runs of kMpoRuns[i] pixels; [3] mixed
-  int nwork[4] = {0, 0, 0, 0};
+  // runs of kMpoRuns[i] pixels; [3] mixed; [4] mixed of 8-byte types in padded plans
+  const MpoWork* work[5] = {nullptr, nullptr, nullptr, nullptr, nullptr};
+  int nwork[5] = {0, 0, 0, 0, 0};
+  bool vec = false;  // padded plan: T1 / T3 row segments aligned, pixel runs by vector access
   int nterms = 0, cmax = 32, rmax = 32;
   int64_t t1_elems = 0, t3_elems = 0;  // T1 / T3 sizes (32-bit byte offsets when both fit)
   int run = 0;                          // pixels per thread (1, 2, 4; FP64 and CMAX 64 at most 2), 0: mixed
@@ -66,6 +68,9 @@ struct DevMpo {
 void mpo_options(DevMpo& d);
 
 hipError_t launch_copy(const DevCopy& c, int elem_size, const void* src, void* dst, hipStream_t s);
+// dst[0 .. bytes) = 0; dst 16-byte aligned, bytes a multiple of 16 (a kernel, not a
+// memset node, so the call may be captured in a HIP graph)
+hipError_t launch_zero(void* dst, size_t bytes, hipStream_t s);
 // the same copy converting FP32 to BF16 (packs of the BF16-storage H_eff)
 hipError_t launch_copy_to_bf16(const DevCopy& c, const float* src, void* dst, hipStream_t s);
 // resident work groups per XCD of a GEMM kernel variant (CUs per XCD x occupancy)

@@ -5,6 +5,7 @@
 #include <algorithm>
 #include <array>
 #include <cmath>
+#include <cstdlib>
 #include <map>
 #include <numeric>
 #include <set>
@@ -1108,7 +1109,7 @@ namespace {
 // One stored block pair of the step-1 GEMM output T1_k = v_k^T envL_k: key (b, w, s1, s2, r).
 struct T1B {
   Key key;
-  int64_t base, rowoff, coloff, Mf, db, Dw, d1, d2, dR;
+  int64_t base, rowoff, coloff, Mf, db, Dw, d1, d2, dR, dRp;  // dRp: dR rounded up to the plan's row alignment
 };
 // One T3 cell (b, s1o, s2o, x, r): the MPO pass writes output row rr = jx + Dx*(i1o + d1o*i2o)
 // of pixel (m, n) (m the bra_l index, n the ket_r index) at
@@ -1122,9 +1123,14 @@ struct Cell {
 // T1_k = [v blocks (k,.)]^T x [envL blocks (k,.)], stored column-major (ld M_k
 // rounded up to 32).  Operand rows (v block columns, envL block columns) are
 // packed into K-contiguous rows of stride Kp = roundup(dk, 4): v per matvec,
-// envL per bond.  Fills H.g1, H.pack_v, H.pack_envL, H.t1_elems, H.vp_elems,
-// H.el_elems and the T1 block list (sorted by (b, r, w, s1, s2)).
-static void step1_layout(Step13& H, const Struct& v, const Struct& envL, std::vector<T1B>& t1) {
+// envL per bond.  padr > 1: each v block's rows hold the right-bond index in
+// segments of dRp = roundup(dR, padr) rows (rows past dR are not written and
+// their T1 rows never read), so every MPO-pass channel row starts on a
+// multiple of padr.  Fills H.g1, H.pack_v, H.pack_envL, H.t1_elems, H.vp_elems,
+// H.el_elems, H.padr and the T1 block list (sorted by (b, r, w, s1, s2)).
+static void step1_layout(Step13& H, const Struct& v, const Struct& envL, std::vector<T1B>& t1, int padr) {
+  H.padr = padr;
+  auto padded = [&](int64_t dR) { return (dR + padr - 1) / padr * padr; };
   std::map<int64_t, std::vector<int64_t>> vby, eby;
   for (int64_t x = 0; x < v.nblocks; ++x) vby[v.sector(x)[0]].push_back(x);
   for (int64_t e = 0; e < envL.nblocks; ++e) eby[envL.sector(e)[0]].push_back(e);
@@ -1141,7 +1147,7 @@ static void step1_layout(Step13& H, const Struct& v, const Struct& envL, std::ve
     for (size_t i = 0; i < X.size(); ++i) {
       const int64_t* z = v.size(X[i]);
       rowoff[i] = Mf;
-      Mf += z[1] * z[2] * z[3];
+      Mf += z[1] * z[2] * padded(z[3]);
     }
     for (size_t j = 0; j < E.size(); ++j) {
       const int64_t* z = envL.size(E[j]);
@@ -1155,9 +1161,9 @@ static void step1_layout(Step13& H, const Struct& v, const Struct& envL, std::ve
     // right-bond index runs fastest, so the MPO pass reads T1 with unit stride
     for (size_t i = 0; i < X.size(); ++i) {
       const int64_t* z = v.size(X[i]);
-      const int64_t d1 = z[1], d2 = z[2], dR = z[3];
+      const int64_t d1 = z[1], d2 = z[2], dR = z[3], dRp = padded(dR);
       const int64_t dims[4] = {dk, d1, d2, dR}, sst[4] = {1, dk, dk * d1, dk * d1 * d2},
-                    dst[4] = {1, dR * Kp, dR * d1 * Kp, Kp};
+                    dst[4] = {1, dRp * Kp, dRp * d1 * Kp, Kp};
       H.pack_v.add(4, dims, sst, dst, v.offsets[X[i]], vbase + rowoff[i] * Kp);
     }
     for (size_t j = 0; j < E.size(); ++j) {
@@ -1185,6 +1191,7 @@ static void step1_layout(Step13& H, const Struct& v, const Struct& envL, std::ve
         t.d1 = v.size(X[i])[1];
         t.d2 = v.size(X[i])[2];
         t.dR = v.size(X[i])[3];
+        t.dRp = padded(t.dR);
         t1.push_back(t);
       }
     off += Mld * Nf;
@@ -1297,20 +1304,23 @@ static void mpo_layout(Step13& H, const Struct& WL, const Struct& WR, const std:
     } else {
       sig = its->second;
     }
+    const int64_t dRp = (dR + H.padr - 1) / H.padr * H.padr;
     MpoGroup g{};
     g.sig = sig;
     g.dR = (int32_t)dR;
     g.db = (int32_t)db;
+    g.dRp = (int32_t)dRp;
     g.in0 = (int32_t)H.inrows.size();
     g.out0 = (int32_t)H.outrows.size();
     g.npix = db * dR;
     for (size_t i = gi0; i < gi1; ++i) {
       const T1B& t = t1[i];
-      if (t.db != db || t.dR != dR) throw Error(ERR_SECTOR_MISMATCH, "mpo pass: pixel group dims differ");
-      // column order c = jw + Dw*(i1 + d1*i2); T1 element (n + dR (i1 + d1 i2), jw + Dw m)
+      if (t.db != db || t.dR != dR || t.dRp != dRp)
+        throw Error(ERR_SECTOR_MISMATCH, "mpo pass: pixel group dims differ");
+      // column order c = jw + Dw*(i1 + d1*i2); T1 element (n + dRp (i1 + d1 i2), jw + Dw m)
       for (int64_t c = 0; c < t.Dw * t.d1 * t.d2; ++c) {
         const int64_t jw = c % t.Dw, i1 = (c / t.Dw) % t.d1, i2 = c / (t.Dw * t.d1);
-        H.inrows.push_back(MpoInRow{t.base + t.rowoff + t.dR * (i1 + t.d1 * i2) + t.Mf * (t.coloff + jw), 1,
+        H.inrows.push_back(MpoInRow{t.base + t.rowoff + t.dRp * (i1 + t.d1 * i2) + t.Mf * (t.coloff + jw), 1,
                                     t.Mf * t.Dw});
       }
     }
@@ -1325,9 +1335,13 @@ static void mpo_layout(Step13& H, const Struct& WL, const Struct& WR, const std:
     }
     const int32_t gidx = (int32_t)H.groups.size();
     H.groups.push_back(g);
-    for (int i = 0; i < 4; ++i) {
-      const int P = i < 3 ? kMpoRuns[i] : mpo_mixed_run((int)C);
-      const int64_t runs = db * ((dR + P - 1) / P);
+    // runs cover the group's padded range dRp (== dR in unpadded plans)
+    const bool pad = H.padr > 1;
+    for (int i = 0; i < 5; ++i) {
+      if (i == 4 && !pad) break;
+      const int P = i < 3 ? kMpoRuns[i]
+                          : (pad ? mpo_vec_run((int)C, i == 3 ? 4 : 8) : mpo_mixed_run((int)C));
+      const int64_t runs = db * ((dRp + P - 1) / P);
       for (int64_t r0 = 0; r0 < runs; r0 += 256) H.work[i].push_back(MpoWork{gidx, (int32_t)r0});
     }
     H.mpo_flops += 2 * R * C * g.npix;
@@ -1378,13 +1392,17 @@ HeffPlan heff_plan(const Struct& v, const Struct& envL, const Struct& envR, cons
       throw Error(ERR_SECTOR_MISMATCH, "heff: output structure differs from the contraction chain");
   }
 
+  // T1 / T3 rows aligned to kMpoPadR (MPSH_HEFF_PADR=0: unpadded), see step1_layout
+  const char* pad_env = std::getenv("MPSH_HEFF_PADR");
+  const int padr = pad_env && std::atoi(pad_env) == 0 ? 1 : kMpoPadR;
   std::vector<T1B> t1;
-  step1_layout(H, v, envL, t1);
+  step1_layout(H, v, envL, t1, padr);
 
   // ---- step 4 layout: one fused GEMM per output ket charge r'; rows = the out
   // blocks (b, s1o, s2o, r'), K segments = envR blocks (r, x, r').  T3 region of
-  // (r', f) is K_f x M_f, K-contiguous (row stride Kfp = roundup(K_f, 4)); cell
-  // (o, f) is T3 block (b, s1o, s2o, x, r): element (n + dR jx, rowoff + m + db (i1o + d1o i2o)).
+  // (r', f) is K_f x M_f, K-contiguous (row stride Kfp = roundup(K_f, 8), K_f =
+  // dRp Dx); cell (o, f) is T3 block (b, s1o, s2o, x, r): element
+  // (n + dRp jx, rowoff + m + db (i1o + d1o i2o)).
   int64_t er_elems = 0;
   std::map<Key, Cell> cells;  // key (b, s1o, s2o, x, r)
   {
@@ -1412,20 +1430,22 @@ HeffPlan heff_plan(const Struct& v, const Struct& envL, const Struct& envR, cons
       if (itf == fby.end()) continue;
       for (int64_t f : itf->second) {
         const int64_t* fs = envR.sector(f);
-        const int64_t dR = envR.size(f)[0], Dx = envR.size(f)[1];
-        const int64_t Kf = dR * Dx, Kfp = up8(Kf);
+        // K index n + dRp jx: segments of dRp per MPO index, zeros past dR (the
+        // MPO pass writes them into T3, the prepare zero-fills envR_p first)
+        const int64_t dR = envR.size(f)[0], Dx = envR.size(f)[1], dRp = (dR + padr - 1) / padr * padr;
+        const int64_t Kf = dRp * Dx, Kfp = up8(Kf);
         const int64_t rbase = er_elems;
         er_elems += Kfp * N;
         {
-          const int64_t dims[2] = {Kf, N}, sst[2] = {1, Kf}, dst[2] = {1, Kfp};
-          H.pack_envR.add(2, dims, sst, dst, envR.offsets[f], rbase);
+          const int64_t dims[3] = {dR, Dx, N}, sst[3] = {1, dR, dR * Dx}, dst[3] = {1, dRp, Kfp};
+          H.pack_envR.add(3, dims, sst, dst, envR.offsets[f], rbase);
         }
         H.g4.add_seg(Kf, off, Kfp, rbase, Kfp);
         for (size_t i = 0; i < O.size(); ++i) {
           const int64_t* os = H.out.sector(O[i]);
           const int64_t* z = H.out.size(O[i]);
           cells[{os[0], os[1], os[2], fs[1], fs[0]}] =
-              Cell{off + Kfp * rowoff[i], dR, Kfp * z[0], 1, Kfp, z[0], z[1], z[2], Dx, dR};
+              Cell{off + Kfp * rowoff[i], dRp, Kfp * z[0], 1, Kfp, z[0], z[1], z[2], Dx, dR};
         }
         off += Kfp * Mf;
       }
@@ -1524,7 +1544,7 @@ static EnvCore env_core(const Struct& env, const Struct& ket, const Struct& WL, 
   H.WR = WR;
   H.out = make_struct(env.charge + WL.charge + WR.charge, {ket.legs[3], WR.legs[2], ket.legs[3].dual()});
   std::vector<T1B> t1;
-  step1_layout(H, ket, env, t1);
+  step1_layout(H, ket, env, t1, 1);  // T3 rows run along the bra side here: no padding
 
   // ---- final GEMM: one problem per output block (r, x, y): rows m = ir + dR*ix,
   // columns the bra's right index iy, K = the bra-left side (b, q1, q2) of every

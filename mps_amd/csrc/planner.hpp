@@ -329,7 +329,9 @@ struct MpoPair {   // WL block (w, s1o, m, s1) x WR block (m, s2o, x, s2)
   int32_t Dm, pad;
 };
 struct MpoGroup {
-  int32_t sig, dR, db, pad;
+  // dRp: the ket_r range the pixel runs cover, dR rounded up to the plan's
+  // row alignment (T1 / T3 rows of padded plans hold zeros past dR)
+  int32_t sig, dR, db, dRp;
   int32_t in0, out0;  // first row in the in/out row tables
   int64_t npix;
 };
@@ -348,6 +350,14 @@ constexpr int kMpoRuns[3] = {1, 2, 4};
 // the mixed work list: runs of 4 pixels in groups of at most 8 channels (little
 // work per pixel: fewer work groups), one pixel per thread otherwise
 constexpr int mpo_mixed_run(int C) { return C <= 8 ? 4 : 1; }
+// the work lists of padded plans (every channel row and T3 row segment starts on a
+// multiple of 4 elements): runs of P pixels moved by one aligned vector access
+// per row, P as large as the P x C inputs a thread holds allow (esize: bytes of
+// the arithmetic type)
+constexpr int mpo_vec_run(int C, int esize) {
+  return esize == 8 ? (C <= 16 ? 2 : 1) : (C <= 16 ? 4 : (C <= 32 ? 2 : 1));
+}
+constexpr int kMpoPadR = 4;  // row alignment (elements) of padded plans
 
 // Kernel variant of one grouped GEMM: tile BM x BN, K chunk multiplier KS.
 struct TileCfg {
@@ -405,7 +415,11 @@ struct Step13 {
   std::vector<MpoGroup> groups;
   std::vector<MpoInRow> inrows;
   std::vector<MpoOutRow> outrows;
-  std::vector<MpoWork> work[4];  // work lists for runs of kMpoRuns[i] pixels; [3]: kMpoMixed runs
+  // work lists for runs of kMpoRuns[i] pixels; [3]: mixed runs (mpo_mixed_run, or
+  // mpo_vec_run for 4- and 2-byte types in padded plans); [4]: mpo_vec_run of
+  // 8-byte types (padded plans only)
+  std::vector<MpoWork> work[5];
+  int32_t padr = 1;  // ket_r row alignment of T1 and T3 (kMpoPadR: padded plan)
   int64_t w_elems = 0;
   int64_t t3_elems = 0;
   int maxR = 0, maxC = 0;

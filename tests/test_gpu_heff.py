@@ -181,38 +181,44 @@ def test_dma_gemm_variants_bitwise(name, tiles):
 @pytest.mark.parametrize("prec", ["f64", "f32", "bf16"])
 def test_mpo_pass_variants_bitwise(name, prec):
     """The MPO pass sums each output's channels in the same order whatever the
-    run length (the default mixed list: 4 pixels per thread in groups of at
-    most 8 channels, 1 otherwise; or 1, 2 or 4 everywhere, FP64 at most 2) and
-    offset width (32-bit byte offsets or 64-bit element offsets), so every
-    variant's H_eff output is bitwise equal to the default's."""
+    run length (unpadded plans, MPSH_HEFF_PADR=0: the default mixed list, 4
+    pixels per thread in groups of at most 8 channels, 1 otherwise; or 1, 2 or 4
+    everywhere, FP64 at most 2) and offset width (32-bit byte offsets or 64-bit
+    element offsets), so every variant's H_eff output is bitwise equal to the
+    default's; padded plans (the default layout, vector-access runs) likewise
+    over the offset width."""
     import os
     from mps_amd.tensornetalgs import HeffPlan
     dt = torch.float64 if prec == "f64" else torch.float32
     ts = [t.to(dt) for t in _workload(name)]
-    keys = ("MPSH_MPO_RUN", "MPSH_MPO_OFF64")
+    keys = ("MPSH_MPO_RUN", "MPSH_MPO_OFF64", "MPSH_HEFF_PADR")
     old = {k: os.environ.get(k) for k in keys}
     outs = {}
     try:
-        for run in (0, 1, 2, 4):
-            for off64 in (0, 1):
-                os.environ["MPSH_MPO_RUN"] = str(run)
-                os.environ["MPSH_MPO_OFF64"] = str(off64)
-                plan = HeffPlan(*[t.struct for t in ts])  # options are read when the plan is uploaded
-                prepared = plan.prepare(prec, *(t.data for t in ts[1:]))
-                out = torch.full((plan.out.numel,), float("nan"), dtype=dt, device=DEV)
-                plan.apply_prepared(prec, prepared, ts[0].data, out)
-                torch.cuda.synchronize()
-                outs[(run, off64)] = out
+        for padr, runs in (("0", (0, 1, 2, 4)), ("1", (0,))):
+            for run in runs:
+                for off64 in (0, 1):
+                    os.environ["MPSH_HEFF_PADR"] = padr
+                    os.environ["MPSH_MPO_RUN"] = str(run)
+                    os.environ["MPSH_MPO_OFF64"] = str(off64)
+                    plan = HeffPlan(*[t.struct for t in ts])  # options are read when the plan is made / uploaded
+                    prepared = plan.prepare(prec, *(t.data for t in ts[1:]))
+                    out = torch.full((plan.out.numel,), float("nan"), dtype=dt, device=DEV)
+                    plan.apply_prepared(prec, prepared, ts[0].data, out)
+                    torch.cuda.synchronize()
+                    outs[(padr, run, off64)] = out
     finally:
         for k, v in old.items():
             if v is None:
                 os.environ.pop(k, None)
             else:
                 os.environ[k] = v
-    ref = outs[(0, 0)]
-    assert torch.isfinite(ref).all()
-    for key, out in outs.items():
-        assert torch.equal(out, ref), key
+    for padr in ("0", "1"):
+        ref = outs[(padr, 0, 0)]
+        assert torch.isfinite(ref).all()
+        for key, out in outs.items():
+            if key[0] == padr:
+                assert torch.equal(out, ref), key
 
 
 @pytest.mark.parametrize("name", ["xxz_L32_chi64", "xxz_L100_chi512", "cyl6x24_chi2048"])
@@ -388,3 +394,42 @@ def test_heff_adjoint_kernels_vs_oracle(name):
     gL32, gR32 = P(*[t.to(torch.float32) for t in ts])
     for a, b in ((gL32, gL), (gR32, gR)):
         assert (a.double() - b).norm().item() <= 1e-5 * b.norm().item()
+
+
+@pytest.mark.parametrize("name", ["xxz_L32_chi64", "xxz_L100_chi512", "cyl6x24_chi2048"])
+@pytest.mark.parametrize("prec", ["f64", "f32", "bf16"])
+def test_padded_and_unpadded_layouts_agree(name, prec):
+    """Padded H_eff plans (the default: every ket_r row segment of T1, T3 and the
+    prepared envR rounded up to 4 elements, the MPO pass moving its pixel runs by
+    aligned vector access and writing T3's padding as zeros, the prepare
+    zero-filling envR's) against unpadded plans (MPSH_HEFF_PADR=0): the prepared
+    operator and the workspace start filled with NaN bytes, so padding that
+    reached the output through the step-4 GEMM would show; both outputs are
+    finite and agree to rounding (FP64 1e-13; FP32 / BF16, whose step-4 K chunks
+    fall differently, 1e-5 relative)."""
+    import os
+    from mps_amd.tensornetalgs import HeffPlan
+    dt = torch.float64 if prec == "f64" else torch.float32
+    ts = [t.to(dt) for t in _workload(name)]
+    old = os.environ.get("MPSH_HEFF_PADR")
+    outs = {}
+    try:
+        for padr in ("0", "1"):
+            os.environ["MPSH_HEFF_PADR"] = padr
+            plan = HeffPlan(*[t.struct for t in ts])
+            prepared = torch.full((max(16, plan.prepared_bytes(prec)),), 255, dtype=torch.uint8, device=DEV)
+            plan.prepare(prec, *(t.data for t in ts[1:]), prepared=prepared)
+            ws = torch.full((max(16, plan.apply_workspace_bytes(prec)),), 255, dtype=torch.uint8, device=DEV)
+            out = torch.full((plan.out.numel,), float("nan"), dtype=dt, device=DEV)
+            plan.apply_prepared(prec, prepared, ts[0].data, out, ws=ws)
+            torch.cuda.synchronize()
+            outs[padr] = out.double()
+    finally:
+        if old is None:
+            os.environ.pop("MPSH_HEFF_PADR", None)
+        else:
+            os.environ["MPSH_HEFF_PADR"] = old
+    assert torch.isfinite(outs["1"]).all() and torch.isfinite(outs["0"]).all()
+    d = float((outs["1"] - outs["0"]).norm() / outs["0"].norm())
+    print(f"{name} {prec}: padded vs unpadded rel diff {d:.2e}", flush=True)
+    assert d <= (1e-13 if prec == "f64" else 1e-5)
