@@ -30,15 +30,15 @@ METRIC = "H_eff matvecs/sec + DMRG sweep time at χ=2048; |ΔE| vs FP64 CPU orac
 # copy's readings: x2.000 / x1.000) for the autotuned variants of this workload;
 # step 4 = its GEMM plus the split-K reduction
 TRAFFIC = {
-    ("cyl6x24_chi2048", "f32", "step1"): 211.7e6,   # gemm_dma<1, 64, 128, 4>
-    ("cyl6x24_chi2048", "f32", "step4"): 185.3e6,   # gemm_dma<1, 64, 64, 4> 154.3 + splitk_reduce 31.0
-    ("cyl6x24_chi2048", "f32", "mpo"): 210.5e6,     # mpo_apply<float, 32, 0>
-    ("cyl6x24_chi2048", "bf16", "step1"): 102.3e6,  # gemm_dma<3, 64, 128, 4>
-    ("cyl6x24_chi2048", "bf16", "step4"): 99.1e6,   # gemm_dma<4, 64, 128, 4> 81.9 + splitk_reduce 17.2
-    ("cyl6x24_chi2048", "bf16", "mpo"): 118.8e6,    # mpo_apply<float, 32, 0, __bf16>
-    ("cyl6x24_chi2048", "f64", "step1"): 432.0e6,   # gemm_dma<0, 64, 128, 3>
-    ("cyl6x24_chi2048", "f64", "step4"): 384.0e6,   # gemm_dma<0, 64, 64, 3> 319.4 + splitk_reduce 64.6
-    ("cyl6x24_chi2048", "f64", "mpo"): 390.2e6,     # mpo_apply<double, 32, 1>
+    ("cyl6x24_chi2048", "f32", "step1"): 211.8e6,   # gemm_dma<1, 64, 128, 4>
+    ("cyl6x24_chi2048", "f32", "step4"): 187.8e6,   # gemm_dma<1, 64, 64, 4> 156.8 + splitk_reduce 31.0
+    ("cyl6x24_chi2048", "f32", "mpo"): 201.6e6,     # mpo_apply<float, 32, 0, float, float, unsigned int, true>
+    ("cyl6x24_chi2048", "bf16", "step1"): 103.5e6,  # gemm_dma<3, 64, 128, 4>
+    ("cyl6x24_chi2048", "bf16", "step4"): 97.8e6,   # gemm_dma<4, 64, 128, 4> 80.6 + splitk_reduce 17.2
+    ("cyl6x24_chi2048", "bf16", "mpo"): 110.5e6,    # mpo_apply<float, 32, 0, __bf16, __bf16, unsigned int, true>
+    ("cyl6x24_chi2048", "f64", "step1"): 433.7e6,   # gemm_dma<0, 64, 128, 3>
+    ("cyl6x24_chi2048", "f64", "step4"): 390.0e6,   # gemm_dma<0, 64, 64, 3> 325.3 + splitk_reduce 64.7
+    ("cyl6x24_chi2048", "f64", "mpo"): 387.0e6,     # mpo_apply<double, 32, 0, double, double, unsigned int, true>
 }
 
 
