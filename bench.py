@@ -213,6 +213,35 @@ def refbench(dev, steps):
     return out
 
 
+def largest_chi(dev, steps):
+    """The largest-chi workload: one two-site bond of the spinless t-V chain L=128
+    at chi=4096 (~70 sectors of dims <= 400, Jordan-Wigner MPO D=6; seeded
+    Gaussian blocks): fused H_eff rates with BF16 storage (FP32 accumulation),
+    FP32 and FP64, the plain FP64 path, and each reduced precision's Rayleigh
+    quotient <v|Hv>/<v|v> against the FP64 one."""
+    import torch
+    from mps_amd.tensornetalgs import heff_plan, heff_workload
+    ts64 = heff_workload("tv_L128_chi4096", dtype=torch.float64, device=dev)
+    v64 = ts64[0].data
+    vv = float(torch.dot(v64, v64))
+    out = {"workload": "tv_L128_chi4096", "chi": 4096, "mpo_D": 6}
+    e = {}
+    for prec in ("bf16", "f32", "f64"):
+        ts = ts64 if prec == "f64" else [t.to(torch.float32) for t in ts64]
+        pl = heff_plan(*ts)
+        ms, _, o, _, _ = time_fused(pl, prec, ts, steps, 5, stage_events=False)
+        out["gflop_per_matvec"] = round(pl.ref_flops / 1e9, 4)
+        out[f"fused_{prec}_matvecs_per_s"] = round(1e3 / ms, 3)
+        e[prec] = float(torch.dot(v64, o.double())) / vv
+    for prec in ("bf16", "f32"):
+        out[f"{prec}_rayleigh_rel_err_vs_f64"] = abs(e[prec] - e["f64"]) / abs(e["f64"])
+    ms_plain = time_plain(ts64, max(3, steps // 4), 2)
+    out["plain_f64_matvecs_per_s"] = round(1e3 / ms_plain, 3)
+    out["bf16_speedup_vs_plain_f64"] = round(out["fused_bf16_matvecs_per_s"] * ms_plain / 1e3, 3)
+    out["f32_speedup_vs_plain_f64"] = round(out["fused_f32_matvecs_per_s"] * ms_plain / 1e3, 3)
+    return out
+
+
 def time_plain(ts, steps, warmup):
     import torch
     import mps_amd as M
@@ -636,6 +665,7 @@ def main():
         result["other_paths"] = extra
         result["env_update"] = env_update_rates(ts64, max(5, args.steps // 10))
         result["refbench_chi1000"] = refbench(dev, max(10, args.steps // 4))
+        result["largest_chi"] = largest_chi(dev, max(10, args.steps // 4))
         print("bench: other paths done", file=sys.stderr, flush=True)
     dmrg_state = None
     if args.dmrg_bonds > 0:
