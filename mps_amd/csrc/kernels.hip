@@ -996,47 +996,54 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(dma_waves_p
 // C(m, n) = sum_s P[part_off + s*pld*N + m + pld*n] for every split problem
 template <typename T, typename TC = T>
 // One work group per 1024 outputs of a split problem, named by the plan's
-// reduction map.  Each output sums its partials in split order; a thread's four
-// outputs are summed side by side, so their loads of one split are in flight
-// together.
+// reduction map.  Each output sums its partials in split order.  A thread's four
+// outputs are summed side by side, up to eight splits' partials of them loaded
+// before they are added (one memory round trip per eight splits), and the
+// output row table is read before the partials, so its latency overlaps theirs.
 __global__ __launch_bounds__(256) void splitk_reduce(const int32_t* __restrict__ rmap,
                                                      const GemmProb* __restrict__ probs,
                                                      const int64_t* __restrict__ crow,
                                                      const int64_t* __restrict__ cld, const T* __restrict__ P,
                                                      TC* __restrict__ C) {
+  constexpr int SB = 8;  // splits loaded per round trip
   const int2 rm = ((const int2*)rmap)[blockIdx.x];
   const GemmProb p = probs[rm.x];
   const int64_t MN = (int64_t)p.M * p.N, PN = (int64_t)p.pld * p.N;
   const int64_t e0 = (int64_t)rm.y * 1024 + threadIdx.x;
   // output e = m + M * n; pld * N < 2^31 for every split problem (checked by the
   // planner): 32-bit division
-  int64_t mm[4], nn[4], off[4];
+  int64_t off[4], dst[4];  // partial offset and output offset of the thread's four outputs
+  bool ok[4];
 #pragma unroll
   for (int u = 0; u < 4; ++u) {
-    const int64_t e = e0 + 256 * u < MN ? e0 + 256 * u : 0;
+    ok[u] = e0 + 256 * u < MN;
+    const int64_t e = ok[u] ? e0 + 256 * u : 0;  // past MN: output 0 is read, nothing stored
     const uint32_t ue = (uint32_t)e, uM = (uint32_t)p.M;
     const uint32_t un = ue / uM;
-    nn[u] = un;
-    mm[u] = ue - un * uM;
-    off[u] = mm[u] + nn[u] * p.pld;
+    const int64_t m = ue - un * uM, n = un;
+    off[u] = m + n * p.pld;
+    if (p.crow0 >= 0)
+      dst[u] = crow[p.crow0 + m] + n * cld[p.crow0 + m];
+    else
+      dst[u] = p.c_off + m + n * p.ldc;
   }
   T acc[4] = {0, 0, 0, 0};
-  for (int s = 0; s < p.nsplit; ++s) {
-    const T* ps = P + p.part_off + s * PN;
+  const T* pp = P + p.part_off;
+  for (int s0 = 0; s0 < p.nsplit; s0 += SB) {
+    T v[SB][4];
 #pragma unroll
-    for (int u = 0; u < 4; ++u)
-      if (e0 + 256 * u < MN) acc[u] += ps[off[u]];
+    for (int j = 0; j < SB; ++j)
+#pragma unroll
+      for (int u = 0; u < 4; ++u) v[j][u] = s0 + j < p.nsplit ? pp[(int64_t)(s0 + j) * PN + off[u]] : T(0);
+#pragma unroll
+    for (int j = 0; j < SB; ++j)
+      if (s0 + j < p.nsplit)
+#pragma unroll
+        for (int u = 0; u < 4; ++u) acc[u] += v[j][u];
   }
 #pragma unroll
-  for (int u = 0; u < 4; ++u) {
-    if (e0 + 256 * u < MN) {
-      const int64_t m = mm[u], n = nn[u];
-      if (p.crow0 >= 0)
-        C[crow[p.crow0 + m] + n * cld[p.crow0 + m]] = (TC)acc[u];
-      else
-        C[p.c_off + m + n * p.ldc] = (TC)acc[u];
-    }
-  }
+  for (int u = 0; u < 4; ++u)
+    if (ok[u]) C[dst[u]] = (TC)acc[u];
 }
 
 // Stream-K reduction: one work group per cut tile r sums the tile's np partial
