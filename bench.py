@@ -242,6 +242,22 @@ def largest_chi(dev, steps):
     return out
 
 
+def unpadded_t1_t3(ts):
+    """Elements of T1 plus T3 of the H_eff on these operands without the plan's
+    ket_r row padding (a host-side plan made with MPSH_HEFF_PADR=0)."""
+    from mps_amd.tensornetalgs import HeffPlan
+    old = os.environ.get("MPSH_HEFF_PADR")
+    os.environ["MPSH_HEFF_PADR"] = "0"
+    try:
+        p = HeffPlan(*[t.struct for t in ts])
+    finally:
+        if old is None:
+            os.environ.pop("MPSH_HEFF_PADR", None)
+        else:
+            os.environ["MPSH_HEFF_PADR"] = old
+    return p.t1_elems + p.t3_elems
+
+
 def time_plain(ts, steps, warmup):
     import torch
     import mps_amd as M
@@ -603,7 +619,7 @@ def main():
                          "algorithmic_bytes_per_launch": plan.step_elems[si] * ebytes,
                          "traffic": TRAFFIC.get((args.workload, prec, key))}
     dom = max(launches, key=lambda k: launches[k]["avg_launch_ms"])
-    mpo_bytes = (plan.t1_elems + plan.t3_elems) * ebytes
+    mpo_bytes = unpadded_t1_t3(ts) * ebytes  # algorithmic: the logical T1 and T3, no row padding
     D = launches[dom]
     result = {
         "metric": METRIC,
