@@ -14,6 +14,8 @@ events recorded on the launch stream around each stage.
 from __future__ import annotations
 
 import argparse
+import contextlib
+import gc
 import json
 import os
 import sys
@@ -82,6 +84,22 @@ def calibrate_copy(mib):
     print(json.dumps({"calibrate_copy_mib": mib, "bytes_each_way": n * 4}), flush=True)
 
 
+@contextlib.contextmanager
+def gc_paused():
+    """Python's cyclic collector collected just before a timed region and paused
+    inside it (as timeit does): a full collection in a process holding torch takes
+    tens of ms of host time and would otherwise land in whichever short timed loop
+    it happens to hit."""
+    gc.collect()
+    was = gc.isenabled()
+    gc.disable()
+    try:
+        yield
+    finally:
+        if was:
+            gc.enable()
+
+
 def time_fused(plan, prec, ts, steps, warmup, stage_events=True, tune=True):
     """Prepared-operator matvecs (GEMM variants autotuned first): returns
     (ms_per_step, per-stage ms, out, prepare_ms, tuned variants)."""
@@ -104,26 +122,27 @@ def time_fused(plan, prec, ts, steps, warmup, stage_events=True, tune=True):
     evs = [[torch.cuda.Event(enable_timing=True) for _ in range(n_ev)] for _ in range(steps)]
     t_start = torch.cuda.Event(enable_timing=True)
     t_end = torch.cuda.Event(enable_timing=True)
-    torch.cuda.synchronize()
-    wall0 = time.perf_counter()
-    t_start.record()
-    for i in range(steps):
-        if stage_events:
-            e = evs[i]
-            e[0].record()
-            plan.apply_stages(prec, 1, prepared, v.data, out, ws)
-            e[1].record()
-            plan.apply_stages(prec, 2, prepared, v.data, out, ws)
-            e[2].record()
-            plan.apply_stages(prec, 4, prepared, v.data, out, ws)
-            e[3].record()
-            plan.apply_stages(prec, 8, prepared, v.data, out, ws)
-            e[4].record()
-        else:
-            plan.apply_prepared(prec, prepared, v.data, out, ws)
-    t_end.record()
-    torch.cuda.synchronize()
-    wall = time.perf_counter() - wall0
+    with gc_paused():
+        torch.cuda.synchronize()
+        wall0 = time.perf_counter()
+        t_start.record()
+        for i in range(steps):
+            if stage_events:
+                e = evs[i]
+                e[0].record()
+                plan.apply_stages(prec, 1, prepared, v.data, out, ws)
+                e[1].record()
+                plan.apply_stages(prec, 2, prepared, v.data, out, ws)
+                e[2].record()
+                plan.apply_stages(prec, 4, prepared, v.data, out, ws)
+                e[3].record()
+                plan.apply_stages(prec, 8, prepared, v.data, out, ws)
+                e[4].record()
+            else:
+                plan.apply_prepared(prec, prepared, v.data, out, ws)
+        t_end.record()
+        torch.cuda.synchronize()
+        wall = time.perf_counter() - wall0
     total_ms = t_start.elapsed_time(t_end)
     stages = [0.0, 0.0, 0.0, 0.0]
     if stage_events:
@@ -172,12 +191,13 @@ def env_update_rates(ts64, steps):
         for side, fn, args in (("left", M._mpsupdateleft, (l, a, wl)), ("right", M._mpsupdateright, (r, a, wr))):
             for _ in range(2):
                 fn(*args)
-            torch.cuda.synchronize()
-            t0 = time.perf_counter()
-            for _ in range(steps):
-                fn(*args)
-            torch.cuda.synchronize()
-            ms = 1e3 * (time.perf_counter() - t0) / steps
+            with gc_paused():
+                torch.cuda.synchronize()
+                t0 = time.perf_counter()
+                for _ in range(steps):
+                    fn(*args)
+                torch.cuda.synchronize()
+                ms = 1e3 * (time.perf_counter() - t0) / steps
             fl = envupdate_plan(side, *args).ref_flops
             out[f"{side}_{name}_ms"] = round(ms, 4)
             out[f"{side}_{name}_gflops"] = round(fl / (ms * 1e-3) / 1e9, 1)
@@ -185,12 +205,13 @@ def env_update_rates(ts64, steps):
             if name == "f64":
                 for _ in range(1):
                     fn(*args, path="plain")
-                torch.cuda.synchronize()
-                t0 = time.perf_counter()
-                for _ in range(max(2, steps // 4)):
-                    fn(*args, path="plain")
-                torch.cuda.synchronize()
-                out[f"{side}_plain_f64_ms"] = round(1e3 * (time.perf_counter() - t0) / max(2, steps // 4), 4)
+                with gc_paused():
+                    torch.cuda.synchronize()
+                    t0 = time.perf_counter()
+                    for _ in range(max(2, steps // 4)):
+                        fn(*args, path="plain")
+                    torch.cuda.synchronize()
+                    out[f"{side}_plain_f64_ms"] = round(1e3 * (time.perf_counter() - t0) / max(2, steps // 4), 4)
     return out
 
 
@@ -263,12 +284,13 @@ def time_plain(ts, steps, warmup):
     import mps_amd as M
     for _ in range(warmup):
         M._applymps2site(*ts, path="plain")
-    torch.cuda.synchronize()
-    t0 = time.perf_counter()
-    for _ in range(steps):
-        M._applymps2site(*ts, path="plain")
-    torch.cuda.synchronize()
-    return (time.perf_counter() - t0) * 1e3 / steps
+    with gc_paused():
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(steps):
+            M._applymps2site(*ts, path="plain")
+        torch.cuda.synchronize()
+        return (time.perf_counter() - t0) * 1e3 / steps
 
 
 DMRG_CFG = dict(W=6, Lx=24, chi=2048, nsec=60, dmax=200, seed=1911, maxdim=2048, tol=1e-10, lanczostol=1e-6,
