@@ -38,8 +38,8 @@ TRAFFIC = {
     ("cyl6x24_chi2048", "bf16", "step1"): 103.5e6,  # gemm_dma<3, 64, 128, 4>
     ("cyl6x24_chi2048", "bf16", "step4"): 97.8e6,   # gemm_dma<4, 64, 128, 4> 80.6 + splitk_reduce 17.2
     ("cyl6x24_chi2048", "bf16", "mpo"): 110.5e6,    # mpo_apply<float, 32, 0, __bf16, __bf16, unsigned int, true>
-    ("cyl6x24_chi2048", "f64", "step1"): 433.7e6,   # gemm_dma<0, 64, 128, 3>
-    ("cyl6x24_chi2048", "f64", "step4"): 390.0e6,   # gemm_dma<0, 64, 64, 3> 325.3 + splitk_reduce 64.7
+    ("cyl6x24_chi2048", "f64", "step1"): 434.9e6,   # gemm_dma<0, 64, 128, 3>
+    ("cyl6x24_chi2048", "f64", "step4"): 390.5e6,   # gemm_dma<0, 64, 64, 3> 325.7 + splitk_reduce 64.8
     ("cyl6x24_chi2048", "f64", "mpo"): 387.0e6,     # mpo_apply<double, 32, 0, double, double, unsigned int, true>
 }
 

@@ -868,14 +868,16 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(dma_waves_p
         bases();
       }
     };
-    // FP64: a wave whose whole (BM/2) x (BN/2) sub-tile lies past M or N skips
-    // its MFMAs (one scalar test per tile; it still issues its DMA slots and meets
-    // every barrier), leaving its SIMD's MFMA pipe to the other resident waves.
-    // Measured 2.3 % faster FP64 matvecs; no change in FP32 or BF16, which keep
-    // the unconditional loop.
+    // FP64 64x128 tiles: a wave whose whole (BM/2) x (BN/2) sub-tile lies past M
+    // or N skips its MFMAs (one scalar test per tile; it still issues its DMA
+    // slots and meets every barrier), leaving its SIMD's MFMA pipe to the other
+    // resident waves.  Measured on the step-1 GEMM: 3.5 % faster; on the 64x64
+    // step-4 tiles it raised the launch's HBM fetch by 35 % for no gain, and in
+    // FP32 / BF16 it gained nothing, so those keep the unconditional loop.
+    constexpr bool kSkipIdle = MODE == 0 && BN == 128;
     const int wave_s = __builtin_amdgcn_readfirstlane(wave);
-    const bool wact = MODE != 0 || (tile.m0 + (wave_s >> 1) * (BM / 2) < p.M &&
-                                    tile.n0 + (wave_s & 1) * (BN / 2) < p.N);
+    const bool wact = !kSkipIdle || (tile.m0 + (wave_s >> 1) * (BM / 2) < p.M &&
+                                     tile.n0 + (wave_s & 1) * (BN / 2) < p.N);
     auto compute = [&](auto ST) {
       constexpr int st = decltype(ST)::value;
       const T* sp = stage(ST);
