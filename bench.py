@@ -33,13 +33,13 @@ METRIC = "H_eff matvecs/sec + DMRG sweep time at χ=2048; |ΔE| vs FP64 CPU orac
 # step 4 = its GEMM plus the split-K reduction
 TRAFFIC = {
     ("cyl6x24_chi2048", "f32", "step1"): 211.8e6,   # gemm_dma<1, 64, 128, 4>
-    ("cyl6x24_chi2048", "f32", "step4"): 187.8e6,   # gemm_dma<1, 64, 64, 4> 156.8 + splitk_reduce 31.0
-    ("cyl6x24_chi2048", "f32", "mpo"): 201.6e6,     # mpo_apply<float, 32, 0, float, float, unsigned int, true>
+    ("cyl6x24_chi2048", "f32", "step4"): 187.7e6,   # gemm_dma<1, 64, 64, 4> 156.6 + splitk_reduce 31.1
+    ("cyl6x24_chi2048", "f32", "mpo"): 201.5e6,     # mpo_apply<float, 32, 0, float, float, unsigned int, true>
     ("cyl6x24_chi2048", "bf16", "step1"): 103.5e6,  # gemm_dma<3, 64, 128, 4>
-    ("cyl6x24_chi2048", "bf16", "step4"): 97.8e6,   # gemm_dma<4, 64, 128, 4> 80.6 + splitk_reduce 17.2
-    ("cyl6x24_chi2048", "bf16", "mpo"): 110.5e6,    # mpo_apply<float, 32, 0, __bf16, __bf16, unsigned int, true>
+    ("cyl6x24_chi2048", "bf16", "step4"): 97.7e6,   # gemm_dma<4, 64, 128, 4> 80.5 + splitk_reduce 17.2
+    ("cyl6x24_chi2048", "bf16", "mpo"): 110.6e6,    # mpo_apply<float, 32, 0, __bf16, __bf16, unsigned int, true>
     ("cyl6x24_chi2048", "f64", "step1"): 434.9e6,   # gemm_dma<0, 64, 128, 3>
-    ("cyl6x24_chi2048", "f64", "step4"): 390.5e6,   # gemm_dma<0, 64, 64, 3> 325.7 + splitk_reduce 64.8
+    ("cyl6x24_chi2048", "f64", "step4"): 390.2e6,   # gemm_dma<0, 64, 64, 3> 325.4 + splitk_reduce 64.8
     ("cyl6x24_chi2048", "f64", "mpo"): 387.0e6,     # mpo_apply<double, 32, 0, double, double, unsigned int, true>
 }
 
@@ -102,7 +102,11 @@ def gc_paused():
 
 def time_fused(plan, prec, ts, steps, warmup, stage_events=True, tune=True):
     """Prepared-operator matvecs (GEMM variants autotuned first): returns
-    (ms_per_step, per-stage ms, out, prepare_ms, tuned variants)."""
+    (ms_per_step, per-stage ms, out, prepare_ms, tuned variants).  ms_per_step
+    times ``steps`` plain matvecs (one mpsh_heff_apply_prepared call each,
+    nothing else on the stream); with ``stage_events`` a separate run of as
+    many matvecs, launched stage by stage with HIP events between the stages,
+    gives each stage's average duration (the roofline's per-launch times)."""
     import torch
     v, L, R, WL, WR = ts
     out = torch.empty(plan.out.numel, dtype=v.dtype, device=v.device)
@@ -118,17 +122,12 @@ def time_fused(plan, prec, ts, steps, warmup, stage_events=True, tune=True):
         plan.apply_prepared(prec, prepared, v.data, out, ws)
     torch.cuda.synchronize()
     prep_ms = p0.elapsed_time(p1)
-    n_ev = 5 if stage_events else 0
-    evs = [[torch.cuda.Event(enable_timing=True) for _ in range(n_ev)] for _ in range(steps)]
-    t_start = torch.cuda.Event(enable_timing=True)
-    t_end = torch.cuda.Event(enable_timing=True)
-    with gc_paused():
-        torch.cuda.synchronize()
-        wall0 = time.perf_counter()
-        t_start.record()
-        for i in range(steps):
-            if stage_events:
-                e = evs[i]
+    stages = [0.0, 0.0, 0.0, 0.0]
+    if stage_events:
+        evs = [[torch.cuda.Event(enable_timing=True) for _ in range(5)] for _ in range(steps)]
+        with gc_paused():
+            torch.cuda.synchronize()
+            for e in evs:
                 e[0].record()
                 plan.apply_stages(prec, 1, prepared, v.data, out, ws)
                 e[1].record()
@@ -138,18 +137,23 @@ def time_fused(plan, prec, ts, steps, warmup, stage_events=True, tune=True):
                 e[3].record()
                 plan.apply_stages(prec, 8, prepared, v.data, out, ws)
                 e[4].record()
-            else:
-                plan.apply_prepared(prec, prepared, v.data, out, ws)
-        t_end.record()
-        torch.cuda.synchronize()
-        wall = time.perf_counter() - wall0
-    total_ms = t_start.elapsed_time(t_end)
-    stages = [0.0, 0.0, 0.0, 0.0]
-    if stage_events:
+            torch.cuda.synchronize()
         for e in evs:
             for k in range(4):
                 stages[k] += e[k].elapsed_time(e[k + 1])
         stages = [s / steps for s in stages]
+    t_start = torch.cuda.Event(enable_timing=True)
+    t_end = torch.cuda.Event(enable_timing=True)
+    with gc_paused():
+        torch.cuda.synchronize()
+        wall0 = time.perf_counter()
+        t_start.record()
+        for _ in range(steps):
+            plan.apply_prepared(prec, prepared, v.data, out, ws)
+        t_end.record()
+        torch.cuda.synchronize()
+        wall = time.perf_counter() - wall0
+    total_ms = t_start.elapsed_time(t_end)
     return max(total_ms, wall * 1e3) / steps, stages, out, prep_ms, tuned
 
 

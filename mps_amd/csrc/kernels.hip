@@ -1624,7 +1624,7 @@ __global__ __launch_bounds__(256) void dot_partial(int64_t n, const T* __restric
   sh[threadIdx.x] = acc;
   __syncthreads();
   for (int w = 128; w > 0; w >>= 1) {
-    if (threadIdx.x < w) sh[threadIdx.x] += sh[threadIdx.x + w];
+    if ((int)threadIdx.x < w) sh[threadIdx.x] += sh[threadIdx.x + w];
     __syncthreads();
   }
   if (threadIdx.x == 0) partials[blockIdx.x] = sh[0];
@@ -1638,7 +1638,7 @@ __global__ __launch_bounds__(256) void dot_final(int np, const double* __restric
   sh[threadIdx.x] = acc;
   __syncthreads();
   for (int w = 128; w > 0; w >>= 1) {
-    if (threadIdx.x < w) sh[threadIdx.x] += sh[threadIdx.x + w];
+    if ((int)threadIdx.x < w) sh[threadIdx.x] += sh[threadIdx.x + w];
     __syncthreads();
   }
   if (threadIdx.x == 0) out[0] = sh[0];
@@ -1699,12 +1699,12 @@ __global__ __launch_bounds__(256) void lz_dots_partial(int k, int64_t n, const d
     for (int t = 0; t < 8; ++t) sh[t][threadIdx.x] = acc[t];
     __syncthreads();
     for (int h = 128; h > 0; h >>= 1) {
-      if (threadIdx.x < h)
+      if ((int)threadIdx.x < h)
 #pragma unroll
         for (int t = 0; t < 8; ++t) sh[t][threadIdx.x] += sh[t][threadIdx.x + h];
       __syncthreads();
     }
-    if (threadIdx.x < ni) part[(int64_t)(i0 + threadIdx.x) * LZ_PARTS + blockIdx.x] = sh[threadIdx.x][0];
+    if ((int)threadIdx.x < ni) part[(int64_t)(i0 + threadIdx.x) * LZ_PARTS + blockIdx.x] = sh[threadIdx.x][0];
     __syncthreads();
   }
 }
@@ -1716,7 +1716,7 @@ __global__ __launch_bounds__(256) void lz_dots_final(const double* __restrict__ 
   sh[threadIdx.x] = part[(int64_t)blockIdx.x * LZ_PARTS + threadIdx.x];
   __syncthreads();
   for (int h = 128; h > 0; h >>= 1) {
-    if (threadIdx.x < h) sh[threadIdx.x] += sh[threadIdx.x + h];
+    if ((int)threadIdx.x < h) sh[threadIdx.x] += sh[threadIdx.x + h];
     __syncthreads();
   }
   if (threadIdx.x == 0) c[blockIdx.x] = sqrt_out ? sqrt(sh[0]) : sh[0];
