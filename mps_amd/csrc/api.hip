@@ -655,7 +655,11 @@ mpsh_status mpsh_struct_blocks(mpsh_struct s, int64_t* sectors, int64_t* sizes, 
 
 int64_t mpsh_struct_find(mpsh_struct s, const int64_t* sector) {
   if (!s || !sector) return -1;
-  return s->s.find(std::vector<int64_t>(sector, sector + s->s.rank()));
+  try {
+    return s->s.find(std::vector<int64_t>(sector, sector + s->s.rank()));
+  } catch (...) {
+    return -1;  // host allocation failure: reported as not found, nothing crosses the C boundary
+  }
 }
 
 // ---------------------------------------------------------------------------
